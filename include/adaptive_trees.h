@@ -1,0 +1,167 @@
+/*
+ * adaptive_trees.h -- C ABI of libadaptive_trees_hip.so, the MI355X (gfx950)
+ * adaptive octree / binary-tree build and update engine.
+ *
+ * Every entry point is a plain C function over plain pointers and sizes, so
+ * any FFI (ctypes, cgo, JNI, N-API) can bind it. Coordinates are DEVICE
+ * pointers owned by the caller, as in the reference's constructors
+ * (include/adaptive_octree/octree.hpp:36-43, kdtree3d.hpp:28-35): they are not
+ * copied, must outlive the tree and are re-read by every update().
+ *
+ * Status codes replace the reference's exceptions one to one
+ * (include/adaptive_octree/tree.hpp:110-130, src/octree/octree.cu:326-330,
+ * 387-410, 474-475, 187-189): AT_EINVAL <-> std::invalid_argument,
+ * AT_ELOGIC <-> std::logic_error, AT_ELENGTH <-> std::length_error. A HIP
+ * failure returns AT_EHIP instead of the reference's exit(EXIT_FAILURE)
+ * (include/adaptive_octree/config.hpp:164-172). at_last_error() returns the
+ * message of the last failing call on the calling thread.
+ *
+ * Threading: one tree is driven by one host thread at a time, on the stream
+ * named in its config (NULL = the default stream). at_tree_update() and
+ * at_tree_compute_views() launch only asynchronous work and may be recorded
+ * into a HIP graph; build, needs_resize and resize_buffers synchronise.
+ */
+#ifndef ADAPTIVE_TREES_H
+#define ADAPTIVE_TREES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    AT_OK = 0,
+    AT_EINVAL = 1,  /* std::invalid_argument in the reference */
+    AT_ELOGIC = 2,  /* std::logic_error */
+    AT_ELENGTH = 3, /* std::length_error */
+    AT_EHIP = 4,    /* HIP runtime failure */
+    AT_ENOMEM = 5   /* device or host allocation failure */
+};
+
+/* ---- enums (include/adaptive_octree/config.hpp:48-50) ------------------ */
+enum { AT_TREE_OCTREE = 0, AT_TREE_BINARY = 1 };
+enum { AT_LEAF_COUNT = 0, AT_NF_COUNT = 1 };
+enum { AT_MORTON = 0, AT_HILBERT = 1 };
+
+/* Box<Real> (config.hpp:95-101). Values are converted to the tree's Real. */
+typedef struct at_box {
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+} at_box;
+
+/*
+ * Tree configuration: TreeConfig<Real> (config.hpp:143-155) with the backend
+ * defaults of Tree::backendConfig (tree.hpp:108-133).
+ *   max_depth          -1 = backend default (octree: 21 build passes, which
+ *                      the reference does not enforce as a depth,
+ *                      config.hpp:125; binary: depth 63).
+ *   update_iterations  -1 = backend default (octree 2, binary 6).
+ *   views              {treeStructure, particleCountsPerBox, particleMapping}
+ *   stream             hipStream_t of the caller, NULL = default stream.
+ */
+typedef struct at_config {
+    int32_t max_depth;
+    uint32_t max_particles_per_leaf;
+    int32_t update_iterations;
+    int32_t split_criterion;
+    int32_t sfc_kind;
+    uint8_t views[3];
+    uint8_t reserved0;
+    at_box box;
+    void *stream;
+} at_config;
+
+/* Output buffers; names follow the getters of tree.hpp:62-103. */
+enum {
+    AT_BUF_PARTICLE_KEYS = 0,       /* P,    uint64[numParticles]          */
+    AT_BUF_PERM = 1,                /* Perm, uint32[numParticles]          */
+    AT_BUF_CORNERSTONE = 2,         /* K,    uint64[numLeaves + 1]         */
+    AT_BUF_LEAF_COUNTS = 3,         /* N,    uint32[numLeaves]             */
+    AT_BUF_NEAR_FIELD_COUNTS = 4,   /* NF,   uint32[numLeaves] (octree)    */
+    AT_BUF_SFC_BOX_INDEX = 5,       /* uint64[numNodes]                    */
+    AT_BUF_HAS_BOX_SPLIT = 6,       /* uint8[numNodes]                     */
+    AT_BUF_BOX_DEPTH = 7,           /* uint8[numNodes]                     */
+    AT_BUF_PARENT_INDEX = 8,        /* int32[numNodes]                     */
+    AT_BUF_CHILD_INDEX = 9,         /* int32[numNodes]                     */
+    AT_BUF_LEVEL_OFFSET = 10,       /* int32[23] octree, int32[65] binary  */
+    AT_BUF_PARTICLE_BEGIN_INDEX = 11, /* uint32[numNodes]                  */
+    AT_BUF_PARTICLE_COUNTS = 12,    /* uint32[numNodes + 1]                */
+    AT_BUF_SFC_PARTICLE_INDEX = 13, /* uint64[numActiveParticles]          */
+    AT_BUF_COUNT = 14
+};
+
+/* Host-visible sizes from the last build()/needs_resize() (octree.hpp:60-69). */
+typedef struct at_scalars {
+    uint64_t num_particles;
+    uint64_t num_active_particles;
+    uint64_t num_nodes;
+    uint64_t num_leaves;
+    uint64_t leaf_capacity;
+    int32_t tree_type;
+    int32_t split_criterion;
+    int32_t sfc_kind;
+    int32_t real_bytes;
+    int32_t active_buffer;
+    int32_t update_iterations;
+    int32_t max_depth;
+    uint32_t max_particles_per_leaf;
+} at_scalars;
+
+typedef struct at_tree at_tree;
+
+/* Constructor: Octree(x,y,z,n,config) (octree.cu:322-425), KDTree3D(...)
+ * (kdtree3d.cu:184-219), Tree(...) (tree.hpp:24-27). real_bytes is 4 or 8. */
+int at_tree_create(int real_bytes, int tree_type, const void *x, const void *y,
+                   const void *z, size_t count, const at_config *config,
+                   at_tree **out);
+int at_tree_destroy(at_tree *tree);
+/* Octree<Real>::build (octree.cu:428-469), KDTree3D<Real>::build (kdtree3d.cu:227-257). */
+int at_tree_build(at_tree *tree);
+/* update() (octree.cu:472-481, kdtree3d.cu:259-270): asynchronous, capturable. */
+int at_tree_update(at_tree *tree);
+/* computeViews(ViewConfig) (octree.cu:484-497); views == NULL uses config.views. */
+int at_tree_compute_views(at_tree *tree, const uint8_t *views);
+/* doBuffersNeedResize() (octree.cu:499-537, kdtree3d.cu:283-287). */
+int at_tree_needs_resize(at_tree *tree, int *needs_resize);
+/* resizeBuffers() (octree.cu:540-551, kdtree3d.cu:289-297). */
+int at_tree_resize_buffers(at_tree *tree);
+/* Getter of one output buffer: device address, logical size, capacity (elements). */
+int at_tree_buffer(at_tree *tree, int buffer_id, const void **data, size_t *size,
+                   size_t *capacity);
+int at_tree_scalars(at_tree *tree, at_scalars *out);
+/* numLeaves_d(), numNodes_d(), numActiveParticles_d() (octree.hpp:62-66). */
+int at_tree_device_counters(at_tree *tree, const int32_t **num_leaves,
+                            const int32_t **num_nodes, const uint32_t **num_active);
+/* maxAchievedDepth() (octree.cu:666-670, kdtree3d.cu:404-415); synchronises. */
+int at_tree_max_achieved_depth(at_tree *tree, uint32_t *depth);
+/* Last error message of the calling thread ("" if none). */
+const char *at_last_error(void);
+
+/* ---- device plumbing for FFI callers without their own HIP bindings ---- */
+int at_device_count(int *count);
+int at_device_malloc(void **ptr, size_t bytes);
+int at_device_free(void *ptr);
+/* kind: 1 host->device, 2 device->host, 3 device->device; synchronous. */
+int at_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int at_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int at_stream_create(void **stream);
+int at_stream_destroy(void *stream);
+int at_stream_synchronize(void *stream);
+int at_device_synchronize(void);
+int at_event_create(void **event);
+int at_event_destroy(void *event);
+int at_event_record(void *event, void *stream);
+int at_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Record tree->update() on `stream` into an executable HIP graph (the
+ * reference's capture protocol, tests/tree_test.cu:44-63). */
+int at_graph_capture_update(at_tree *tree, void *stream, void **graph_exec);
+int at_graph_launch(void *graph_exec, void *stream);
+int at_graph_destroy(void *graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADAPTIVE_TREES_H */
