@@ -1,0 +1,78 @@
+// buffers.hpp -- device allocations and thrust-style capacity bookkeeping.
+#pragma once
+
+#include <utility>
+
+#include "common.hpp"
+
+namespace at
+{
+// Tracks the logical size and allocation capacity of one of the reference's
+// thrust::device_vector buffers with thrust's growth rules (reserve(n)
+// allocates exactly n; a growing resize allocates max(2*size, n,
+// 2*capacity)). The octree's leafCapacity is derived from these capacities
+// after build (octree.cu:453-457), so they are mirrored exactly; the device
+// memory itself is managed separately.
+struct VecShape
+{
+    std::size_t size = 0, cap = 0;
+    void reserve(std::size_t n)
+    {
+        if (n > cap) cap = n;
+    }
+    void resize(std::size_t n)
+    {
+        if (n > cap)
+        {
+            std::size_t c = 2 * size > n ? 2 * size : n;
+            cap = c > 2 * cap ? c : 2 * cap;
+        }
+        size = n;
+    }
+    void resizeHeadroom(std::size_t req) // resizeWithHeadroom (helpers.cuh:122-141)
+    {
+        if (req > cap) reserve(req + req / 4);
+        resize(req);
+    }
+};
+
+// A device allocation that grows on request.
+template <class T>
+struct DevBuf
+{
+    T *p = nullptr;
+    std::size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    // Grow to at least n elements; the first `keep` elements survive.
+    void ensure(std::size_t n, std::size_t keep, hipStream_t s)
+    {
+        if (n <= cap) return;
+        T *q = nullptr;
+        AT_HIP(hipMalloc(&q, (n ? n : 1) * sizeof(T)));
+        if (p && keep)
+            AT_HIP(hipMemcpyAsync(q, p, (keep < cap ? keep : cap) * sizeof(T), hipMemcpyDeviceToDevice, s));
+        if (p)
+        {
+            AT_HIP(hipStreamSynchronize(s));
+            (void)hipFree(p);
+        }
+        p = q;
+        cap = n;
+    }
+    void swap(DevBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
+};
+
+} // namespace at

@@ -1,0 +1,322 @@
+// common.hpp -- shared device/host definitions of the gfx950 adaptive-tree engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace at
+{
+using Key = std::uint64_t;
+constexpr Key kMaxKey = Key(1) << 63;   // root's exclusive end (config.hpp:45-46)
+constexpr Key kInvalidKey = ~Key(0);    // escaped-slot key (config.hpp:41)
+constexpr unsigned kOctLevels = 21;     // 21 bits per axis, 63-bit keys
+constexpr unsigned kBinLevels = 63;     // binary depth cap
+
+// Error with a C-ABI status code attached.
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char *what, const char *file, int line);
+#define AT_HIP(expr) ::at::hipCheck((expr), #expr, __FILE__, __LINE__)
+
+// Device-resident tree state. The first eight fields are the reference's
+// RebalanceState (helpers.cuh:110-120) in the same order, so numLeaves_d(),
+// numNodes_d() and numActiveParticles_d() point into it.
+struct DevState
+{
+    int changed;
+    int converged;
+    int numLeaves;
+    int newNumLeaves;
+    int needsResize;
+    int numNodes;
+    int activeBuffer;
+    unsigned numActiveParticles;
+};
+
+__host__ __device__ inline int clz64(Key x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return x ? __clzll((long long)x) : 64;
+#else
+    return x ? __builtin_clzll(x) : 64;
+#endif
+}
+__host__ __device__ inline int clz32(std::uint32_t x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return x ? __clz((int)x) : 32;
+#else
+    return x ? __builtin_clz(x) : 32;
+#endif
+}
+
+// ---- octree level helpers: a level-l node spans 2^(3(21-l)) keys ----------
+__host__ __device__ inline Key nodeRange(unsigned level) { return Key(1) << (3u * (kOctLevels - level)); }
+__host__ __device__ inline unsigned treeLevel(Key range) { return unsigned(clz64(range - 1) - 1) / 3u; }
+__host__ __device__ inline unsigned octalDigit(Key code, unsigned pos)
+{
+    return unsigned(code >> (3u * (kOctLevels - pos))) & 7u;
+}
+__host__ __device__ inline Key enclosingBoxCode(Key k, unsigned level) { return k & ~(nodeRange(level) - 1); }
+// Warren-Salmon placeholder-bit box key of a node with `prefix` path bits.
+__host__ __device__ inline Key placeholderKey(Key start, int prefix)
+{
+    return (Key(1) << prefix) | (start >> (63 - prefix));
+}
+
+// ---- Morton: magic-number bit spreading of 21-bit coordinates --------------
+__host__ __device__ inline Key spread3(std::uint32_t v)
+{
+    Key x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+__host__ __device__ inline std::uint32_t gather3(Key x)
+{
+    x &= 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x1f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x1f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x1fffffull;
+    return std::uint32_t(x);
+}
+__host__ __device__ inline Key mortonKey(unsigned x, unsigned y, unsigned z)
+{
+    return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+}
+
+// ---- Hilbert (the reference curve, sfc_keys.cuh:254-331) --------------------
+// Per octal digit: visit order of the Morton octant, then reflect and rotate
+// the remaining coordinate bits into the child's frame.
+__host__ __device__ inline Key hilbertKey(unsigned px, unsigned py, unsigned pz)
+{
+    // visit order {0,1,3,2,7,6,4,5} packed as 3-bit fields
+    constexpr std::uint32_t kVisit = (0u << 0) | (1u << 3) | (3u << 6) | (2u << 9) | (7u << 12) |
+                                     (6u << 15) | (4u << 18) | (5u << 21);
+    Key key = 0;
+#pragma unroll
+    for (int level = int(kOctLevels) - 1; level >= 0; --level)
+    {
+        const unsigned xi = (px >> level) & 1u, yi = (py >> level) & 1u, zi = (pz >> level) & 1u;
+        const unsigned nyi = yi ^ 1u, nzi = zi ^ 1u;
+        const unsigned oct = (xi << 2) | (yi << 1) | zi;
+        key = (key << 3) | Key((kVisit >> (3 * oct)) & 7u);
+        px ^= 0u - (xi & (nyi | zi));
+        py ^= 0u - ((xi & (yi | zi)) | (yi & nzi));
+        pz ^= 0u - ((xi & nyi & nzi) | (yi & nzi));
+        const unsigned ox = px, oy = py, oz = pz;
+        if (zi) { px = oy; py = oz; pz = ox; }
+        else if (!yi) { px = oz; pz = ox; }
+    }
+    return key;
+}
+__host__ __device__ inline void hilbertDecode(Key key, unsigned &ox, unsigned &oy, unsigned &oz)
+{
+    unsigned px = 0, py = 0, pz = 0;
+#pragma unroll
+    for (unsigned level = 0; level < kOctLevels; ++level)
+    {
+        const unsigned oct = unsigned(key >> (3 * level)) & 7u;
+        const unsigned xi = oct >> 2, yi = (oct >> 1) & 1u, zi = oct & 1u;
+        const unsigned nxi = xi ^ 1u, nyi = yi ^ 1u, nzi = zi ^ 1u;
+        const unsigned tx = px, ty = py, tz = pz;
+        if (yi ^ zi) { px = tz; pz = ty; py = tx; }
+        else if ((nxi & nyi & nzi) | (xi & yi & zi)) { px = tz; pz = tx; }
+        const unsigned mask = (1u << level) - 1u;
+        px ^= mask & (0u - (xi & (yi | zi)));
+        py ^= mask & (0u - ((xi & (nyi | nzi)) | (nxi & yi & zi)));
+        pz ^= mask & (0u - ((xi & nyi & nzi) | (yi & zi)));
+        px |= xi << level;
+        py |= (xi ^ yi) << level;
+        pz |= (yi ^ zi) << level;
+    }
+    ox = px; oy = py; oz = pz;
+}
+__host__ __device__ inline Key sfcEncode(unsigned x, unsigned y, unsigned z, bool hilbert)
+{
+    return hilbert ? hilbertKey(x, y, z) : mortonKey(x, y, z);
+}
+__host__ __device__ inline void sfcDecode(Key k, bool hilbert, unsigned &x, unsigned &y, unsigned &z)
+{
+    if (hilbert) hilbertDecode(k, x, y, z);
+    else { x = gather3(k >> 2); y = gather3(k >> 1); z = gather3(k); }
+}
+
+// ---- float box -> integer grid (sfc_keys.cuh:420-496) -------------------------
+template <class T>
+struct GridBox
+{
+    T xmin, ymin, zmin;
+    T mx, my, mz;          // 2^bits * (1/length), computed on the host in T
+    int cx, cy, cz;        // 2^bits per axis
+};
+
+template <class T>
+__host__ __device__ inline bool isEscaped(T x, T y, T z);
+template <>
+__host__ __device__ inline bool isEscaped<float>(float x, float y, float z)
+{
+    constexpr float e = 3.40282346638528859811704183484516925e+38f;
+    return x == e && y == e && z == e;
+}
+template <>
+__host__ __device__ inline bool isEscaped<double>(double x, double y, double z)
+{
+    constexpr double e = 1.79769313486231570814527423731704357e+308;
+    return x == e && y == e && z == e;
+}
+
+// float/double -> int with saturation (NaN -> 0): the device conversion
+// instruction's own semantics, spelled out so the host path agrees.
+template <class T>
+__host__ __device__ inline int toIntSat(T v)
+{
+    if (!(v == v)) return 0;
+    if (v >= T(2147483648.0)) return 2147483647;
+    if (v <= T(-2147483648.0)) return -2147483647 - 1;
+    return int(v);
+}
+
+template <class T>
+__host__ __device__ inline Key gridKey(T x, T y, T z, const GridBox<T> &b, bool hilbert)
+{
+    int ix = toIntSat<T>(fma(-b.xmin, b.mx, floor(x * b.mx)));
+    int iy = toIntSat<T>(fma(-b.ymin, b.my, floor(y * b.my)));
+    int iz = toIntSat<T>(fma(-b.zmin, b.mz, floor(z * b.mz)));
+    ix = ix < b.cx - 1 ? ix : b.cx - 1;
+    iy = iy < b.cy - 1 ? iy : b.cy - 1;
+    iz = iz < b.cz - 1 ? iz : b.cz - 1;
+    return sfcEncode(unsigned(ix), unsigned(iy), unsigned(iz), hilbert);
+}
+
+// ---- binary longest-side keys (binary_node.cuh:43-59) ------------------------
+
+__host__ __device__ inline Key fixed63(double v)
+{
+    const double c1 = (1.0 < v) ? 1.0 : v;
+    const double c0 = (0.0 > c1) ? 0.0 : c1;
+    const Key q = Key(c0 * 9223372036854775808.0);
+    return q < kMaxKey - 1 ? q : kMaxKey - 1;
+}
+
+struct BinaryGeom
+{
+    double lo[3];       // box minimum, widened from Real
+    double len[3];      // box length, computed in double from the Real bounds
+    std::uint8_t axis[kBinLevels];
+    // Fast path: the schedule is a short prefix followed by a period-3 cycle
+    // of three distinct axes; the cycle is a 3-way bit interleave.
+    int periodic;       // 1 if the decomposition below describes axis[]
+    int prefix;         // number of leading depths handled one by one
+    int consumed[3];    // bits each axis gave to the prefix
+    int order[3];       // axes of one cycle, most significant first
+    int triples;        // ceil((63 - prefix) / 3)
+};
+
+// Longest-side key of normalised coordinates v (binary_node.cuh:43-59):
+// depth d takes the next most significant bit of axis[d]'s 63-bit fixed point.
+__host__ __device__ inline Key binaryInterleave(Key f0, Key f1, Key f2, const BinaryGeom &g)
+{
+    Key f[3] = {f0, f1, f2};
+    if (g.periodic)
+    {
+        Key hi = 0;
+        for (int d = 0; d < g.prefix; ++d)
+        {
+            const int a = g.axis[d];
+            const Key fa = a == 0 ? f0 : (a == 1 ? f1 : f2);
+            hi = (hi << 1) | ((fa >> 62) & 1u);
+            if (a == 0) f0 <<= 1; else if (a == 1) f1 <<= 1; else f2 <<= 1;
+        }
+        const int R = 63 - g.prefix;
+        if (R == 0) return hi;
+        const int m = g.triples;
+        const Key fr[3] = {f0, f1, f2};
+        const Key mask = (Key(1) << m) - 1;
+        const Key t0 = (fr[g.order[0]] >> (63 - m)) & mask;
+        const Key t1 = (fr[g.order[1]] >> (63 - m)) & mask;
+        const Key t2 = (fr[g.order[2]] >> (63 - m)) & mask;
+        const Key full = (spread3(std::uint32_t(t0)) << 2) | (spread3(std::uint32_t(t1)) << 1) |
+                         spread3(std::uint32_t(t2));
+        const Key low = full >> (3 * m - R);
+        return (hi << R) | low;
+    }
+    Key key = 0;
+    for (unsigned d = 0; d < kBinLevels; ++d)
+    {
+        Key &c = f[g.axis[d]];
+        key = (key << 1) | ((c >> 62) & 1u);
+        c <<= 1;
+    }
+    return key;
+}
+
+template <class T>
+__host__ __device__ inline Key binaryKey(T x, T y, T z, const BinaryGeom &g)
+{
+    if (isEscaped<T>(x, y, z)) return kInvalidKey;
+    const double v0 = (double(x) - g.lo[0]) / g.len[0];
+    const double v1 = (double(y) - g.lo[1]) / g.len[1];
+    const double v2 = (double(z) - g.lo[2]) / g.len[2];
+    return binaryInterleave(fixed63(v0), fixed63(v1), fixed63(v2), g);
+}
+
+// ---- integer sequences -------------------------------------------------------
+template <class T>
+__host__ __device__ inline std::size_t lowerBound(const T *a, std::size_t n, T v)
+{
+    std::size_t first = 0;
+    while (n > 0)
+    {
+        const std::size_t half = n >> 1;
+        if (a[first + half] < v) { first += half + 1; n -= half + 1; }
+        else n = half;
+    }
+    return first;
+}
+template <class T>
+__host__ __device__ inline std::size_t upperBound(const T *a, std::size_t n, T v)
+{
+    std::size_t first = 0;
+    while (n > 0)
+    {
+        const std::size_t half = n >> 1;
+        if (!(v < a[first + half])) { first += half + 1; n -= half + 1; }
+        else n = half;
+    }
+    return first;
+}
+
+inline unsigned ceilDiv(std::size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
+
+// ---- device-scope atomics used by the chained scans ---------------------------
+__device__ inline std::uint32_t ldAcq(const std::uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void stRel(std::uint32_t *p, std::uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline std::uint64_t ldAcq64(const std::uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void stRel64(std::uint64_t *p, std::uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+} // namespace at

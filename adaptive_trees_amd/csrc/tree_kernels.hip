@@ -1,0 +1,861 @@
+// tree_kernels.hip -- octree and binary-tree rebalance kernels for gfx950.
+//
+// One split/merge pass is three launches that read every size from the
+// device state, so a pass is graph-capturable and grids are sized by
+// capacity:
+//   decideScan  per-leaf decision (octree_rebalance.cuh:174-249 /
+//               kdtree3d_rebalance.cuh:61-81), block scan and a chained scan
+//               across tiles in the same kernel; the tile that holds the
+//               sentinel index applies the capacity rule
+//               (octree_rebalance.cuh:1211-1232).
+//   process     scatter into the inactive A/B half (octree_rebalance.cuh:
+//               253-365, 1234-1265 / kdtree3d_rebalance.cuh:103-117), then a
+//               one-block commit that flips the halves and clears the scan
+//               status for the next pass.
+//   refresh     leaf counts, or near-field counts of dirty leaves and
+//               dirty sibling groups from a compacted work list.
+#include "tree_kernels.hpp"
+
+namespace at
+{
+namespace
+{
+constexpr std::uint64_t kAgg = std::uint64_t(1) << 62;
+constexpr std::uint64_t kPre = std::uint64_t(2) << 62;
+constexpr std::uint64_t kChg = std::uint64_t(1) << 61;
+constexpr std::uint64_t kSumMask = kChg - 1;
+constexpr std::uint64_t kFlagMask = std::uint64_t(3) << 62;
+constexpr int kThreads = 256;
+constexpr int kItems = kPassTile / kThreads;
+
+// Chained scan across tiles (one status word per tile). Called by one thread.
+__device__ void chainedScan(std::uint64_t *status, std::size_t tile, std::uint64_t tileSum, bool tileChg,
+                            std::uint64_t &exclusive, bool &exclChg)
+{
+    std::uint64_t *me = status + tile;
+    if (tile == 0)
+    {
+        stRel64(me, kPre | (tileChg ? kChg : 0) | tileSum);
+        exclusive = 0;
+        exclChg = false;
+        return;
+    }
+    stRel64(me, kAgg | (tileChg ? kChg : 0) | tileSum);
+    std::uint64_t acc = 0;
+    bool chg = false;
+    std::size_t j = tile - 1;
+    for (;;)
+    {
+        const std::uint64_t v = ldAcq64(status + j);
+        const std::uint64_t f = v & kFlagMask;
+        if (!f)
+        {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        acc += v & kSumMask;
+        chg |= (v & kChg) != 0;
+        if (f == kPre) break;
+        --j;
+    }
+    stRel64(me, kPre | ((chg || tileChg) ? kChg : 0) | (acc + tileSum));
+    exclusive = acc;
+    exclChg = chg;
+}
+
+// Block-wide exclusive scan of a 64-bit value and OR of a flag.
+__device__ std::uint64_t blockScan64(std::uint64_t v, bool flag, std::uint64_t *wsum, int *wflag,
+                                     std::uint64_t &total, bool &anyFlag)
+{
+    const unsigned lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    std::uint64_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const std::uint64_t t = __shfl_up(inc, o, 64);
+        if (lane >= unsigned(o)) inc += t;
+    }
+    const std::uint64_t fb = __ballot(flag);
+    if (lane == 63)
+    {
+        wsum[wave] = inc;
+        wflag[wave] = fb != 0;
+    }
+    __syncthreads();
+    std::uint64_t before = 0;
+    total = 0;
+    anyFlag = false;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w)
+    {
+        if (w < wave) before += wsum[w];
+        total += wsum[w];
+        anyFlag |= wflag[w] != 0;
+    }
+    return before + inc - v;
+}
+
+// ---- octree decision (calculateNodeOp, octree_rebalance.cuh:174-249) --------
+__device__ inline int siblingIndex(const Key *tree, int numNodes, int i, unsigned &level)
+{
+    const Key node = tree[i];
+    level = treeLevel(tree[i + 1] - node);
+    if (level == 0) return -1;
+    const int s = int(octalDigit(node, level));
+    const int first = i - s;
+    if (first < 0 || first > numNodes - 8) return -1;
+    return tree[first + 8] == tree[first] + nodeRange(level - 1) ? s : -1;
+}
+
+struct OctDecider
+{
+    const Key *K[2];
+    const std::uint32_t *N[2];
+    const std::uint32_t *NF[2];
+    const std::uint8_t *gcm[2];
+    unsigned bucket;
+    bool nf;
+
+    __device__ int operator()(int ab, int i, int L) const
+    {
+        const Key *tree = K[ab];
+        unsigned level = 0;
+        const int s = siblingIndex(tree, L, i, level);
+        if (!nf)
+        {
+            if (s > 0)
+            {
+                const std::uint32_t *g = N[ab] + i - s;
+                std::uint64_t parent = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) parent += g[k];
+                if (parent <= std::uint64_t(bucket)) return 0;
+            }
+            const std::uint32_t c = N[ab][i];
+            // 32-bit threshold products, as the reference computes them
+            if (c > bucket * 512u && level + 3 < kOctLevels) return 4096;
+            if (c > bucket * 64u && level + 2 < kOctLevels) return 512;
+            if (c > bucket * 8u && level + 1 < kOctLevels) return 64;
+            if (c > bucket && level < kOctLevels) return 8;
+            return 1;
+        }
+        if (s >= 0)
+        {
+            const int first = i - s;
+            if (first + 8 <= L && gcm[ab][first]) return s == 0 ? 1 : 0;
+        }
+        const std::uint64_t c = NF[ab][i];
+        const std::uint64_t b = bucket;
+        if (c > b * 262144u && level + 3 < kOctLevels) return 4096;
+        if (c > b * 4096u && level + 2 < kOctLevels) return 512;
+        if (c > b * 64u && level + 1 < kOctLevels) return 64;
+        if (c > b && level < kOctLevels) return 8;
+        return 1;
+    }
+};
+
+// ---- binary decision (kdtree3d_rebalance.cuh:61-81, binary_node.cuh:91-98) ---
+__device__ inline bool binSiblings(const Key *tree, int count, int first)
+{
+    if (first < 0 || first + 1 >= count) return false;
+    const Key r = tree[first + 1] - tree[first];
+    return !(tree[first] & r) && tree[first + 2] - tree[first + 1] == r;
+}
+
+struct BinDecider
+{
+    const Key *K[2];
+    const std::uint32_t *N[2];
+    unsigned bucket;
+    unsigned maxDepth;
+
+    __device__ int operator()(int ab, int i, int L) const
+    {
+        const Key *tree = K[ab];
+        const std::uint32_t *c = N[ab];
+        const bool merge = i > 0 && binSiblings(tree, L, i - 1) &&
+                           std::uint64_t(c[i - 1]) + c[i] <= std::uint64_t(bucket);
+        if (merge) return 0;
+        return (c[i] > bucket && unsigned(clz64(tree[i + 1] - tree[i])) < maxDepth) ? 2 : 1;
+    }
+};
+
+template <class Dec>
+__global__ __launch_bounds__(kThreads) void decideScanKernel(Dec dec, DevState *st, long capacity,
+                                                             std::int64_t *offs, std::uint64_t *status,
+                                                             std::uint32_t *ticket)
+{
+    __shared__ std::uint32_t tileShared;
+    __shared__ std::uint64_t wsum[kThreads / 64];
+    __shared__ int wflag[kThreads / 64];
+    __shared__ std::uint64_t prefixShared;
+    __shared__ int prefixChgShared;
+    const int tid = threadIdx.x;
+    if (tid == 0) tileShared = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const long tile = tileShared;
+    const long L = st->numLeaves;
+    const long base = tile * kPassTile;
+    if (base > L) return; // tiles past the sentinel index hold no operations
+    const bool skip = st->converged || st->needsResize;
+    const int ab = st->activeBuffer;
+    const long first = base + long(tid) * kItems;
+    int op[kItems];
+    std::uint64_t sum = 0;
+    bool chg = false;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        const long i = first + k;
+        int o = 0;
+        if (!skip && i < L)
+        {
+            o = dec(ab, int(i), int(L));
+            chg |= o != 1;
+        }
+        op[k] = o;
+        sum += std::uint64_t(o);
+    }
+    std::uint64_t tileSum;
+    bool tileChg;
+    const std::uint64_t local = blockScan64(sum, chg, wsum, wflag, tileSum, tileChg);
+    if (tid == 0)
+    {
+        std::uint64_t ex;
+        bool exChg;
+        chainedScan(status, std::size_t(tile), tileSum, tileChg, ex, exChg);
+        prefixShared = ex;
+        prefixChgShared = (exChg || tileChg) ? 1 : 0;
+    }
+    __syncthreads();
+    std::uint64_t run = prefixShared + local;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        const long i = first + k;
+        if (i <= L) offs[i] = std::int64_t(run);
+        if (i == L)
+        {
+            // capacity rule of the pass (graphCheckCapacityKernel)
+            const bool changedAll = prefixChgShared != 0;
+            if (!st->converged)
+            {
+                if (st->needsResize || !changedAll) st->converged = 1;
+                else if (long(run) > capacity)
+                {
+                    st->needsResize = 1;
+                    st->converged = 1;
+                }
+                else
+                    st->newNumLeaves = int(run);
+            }
+            st->changed = changedAll ? 1 : 0;
+        }
+        run += std::uint64_t(op[k]);
+    }
+}
+
+// ---- octree scatter (processNode) -------------------------------------------
+__global__ __launch_bounds__(kThreads) void octProcessKernel(TreeDev t, bool nf)
+{
+    DevState *st = t.st;
+    if (st->converged) return;
+    const int L = st->numLeaves;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= L) return;
+    const int ab = st->activeBuffer;
+    const Key *tree = t.K[ab];
+    Key *out = t.K[ab ^ 1];
+    const std::int64_t *ops = t.offs;
+    if (i == 0) out[st->newNumLeaves] = nodeRange(0);
+    const int code = int(ops[i + 1] - ops[i]);
+    if (code == 0) return;
+    const Key node = tree[i];
+    const std::int64_t dst = ops[i];
+    if (code == 1)
+    {
+        out[dst] = node;
+        if (nf)
+        {
+            const bool leader = i + 1 < L && ops[i + 2] - ops[i + 1] == 0;
+            if (leader)
+            {
+                t.nfDirty[dst] = 1;
+                t.groupDirty[dst] = 1;
+            }
+            else
+            {
+                t.N[ab ^ 1][dst] = t.N[ab][i];
+                t.NF[ab ^ 1][dst] = t.NF[ab][i];
+                t.nfDirty[dst] = 0;
+                unsigned sl = 0;
+                const int s = siblingIndex(tree, L, i, sl);
+                bool reusable = false;
+                if (s == 0 && i + 8 <= L)
+                {
+                    reusable = true;
+                    for (int k = 0; k < 8; ++k)
+                        if (ops[i + k + 1] - ops[i + k] != 1) { reusable = false; break; }
+                }
+                if (reusable)
+                {
+                    t.gcm[ab ^ 1][dst] = t.gcm[ab][i];
+                    t.groupDirty[dst] = 0;
+                }
+                else
+                    t.groupDirty[dst] = 1;
+            }
+        }
+        return;
+    }
+    const unsigned level = treeLevel(tree[i + 1] - node);
+    const unsigned diff = code == 8 ? 1u : (code == 64 ? 2u : (code == 512 ? 3u : 4u));
+    const Key r = nodeRange(level + diff);
+    for (int s = 0; s < code; ++s)
+    {
+        out[dst + s] = node + Key(s) * r;
+        if (nf)
+        {
+            t.nfDirty[dst + s] = 1;
+            t.groupDirty[dst + s] = 1;
+        }
+    }
+}
+
+// ---- binary scatter (binaryProcessNodesKernel) --------------------------------
+__global__ __launch_bounds__(kThreads) void binProcessKernel(TreeDev t)
+{
+    DevState *st = t.st;
+    if (st->converged) return;
+    const int L = st->numLeaves;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= L) return;
+    const int ab = st->activeBuffer;
+    const Key *tree = t.K[ab];
+    Key *out = t.K[ab ^ 1];
+    if (i == 0) out[st->newNumLeaves] = kMaxKey;
+    const int op = int(t.offs[i + 1] - t.offs[i]);
+    if (!op) return;
+    const std::int64_t dst = t.offs[i];
+    out[dst] = tree[i];
+    if (op == 2) out[dst + 1] = tree[i] + (tree[i + 1] - tree[i]) / 2;
+}
+
+// Commit the pass and clear the chained-scan status for the next one.
+__global__ void commitKernel(DevState *st, int binary, std::uint64_t *status, std::size_t words,
+                             std::uint32_t *ticket, std::uint32_t *nfCount)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        if (!st->converged)
+        {
+            st->numLeaves = st->newNumLeaves;
+            if (binary) st->numNodes = (st->numLeaves - 1) * 2 + 1;
+            st->activeBuffer ^= 1;
+        }
+        *ticket = 0;
+        if (nfCount) *nfCount = 0;
+    }
+    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
+         w += std::size_t(gridDim.x) * blockDim.x)
+        status[w] = 0;
+}
+
+// Start of update(): resetRebalanceStateKernel (octree_rebalance.cuh:62-73).
+__global__ void resetKernel(DevState *st, const Key *P, std::size_t n, std::uint64_t *status, std::size_t words,
+                            std::uint32_t *ticket, std::uint32_t *nfCount)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        st->changed = 0;
+        st->converged = 0;
+        st->numActiveParticles = n ? unsigned(lowerBound(P, n, kMaxKey)) : 0u;
+        *ticket = 0;
+        if (nfCount) *nfCount = 0;
+    }
+    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
+         w += std::size_t(gridDim.x) * blockDim.x)
+        status[w] = 0;
+}
+
+// initRootOctreeKernel + resetRebalanceState of the build (octree.cu:24-38, 159-163)
+__global__ void octInitKernel(TreeDev t, std::size_t words)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        Key *K = t.K[0];
+        K[0] = 0;
+        K[1] = nodeRange(0);
+        const std::uint64_t active = t.n ? std::uint64_t(lowerBound(t.P, t.n, kMaxKey)) : 0;
+        t.N[0][0] = unsigned(active);
+        const std::uint64_t sq = active * active;
+        if (t.NF[0]) t.NF[0][0] = sq > 0xffffffffull / 27ull ? 0xffffffffu : unsigned(27ull * sq);
+        DevState s{};
+        s.numLeaves = 1;
+        s.newNumLeaves = 1;
+        s.numActiveParticles = unsigned(active);
+        *t.st = s;
+        *t.scanTicket = 0;
+        if (t.nfCount) *t.nfCount = 0;
+    }
+    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
+         w += std::size_t(gridDim.x) * blockDim.x)
+        t.scanStatus[w] = 0;
+}
+
+// initBinaryRootKernel + refreshCounts (kdtree3d.cu:24-30, 139-146)
+__global__ void binInitKernel(TreeDev t, std::size_t words)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        t.K[0][0] = 0;
+        t.K[0][1] = kMaxKey;
+        const unsigned active = t.n ? unsigned(lowerBound(t.P, t.n, kMaxKey)) : 0u;
+        DevState s{};
+        s.numLeaves = 1;
+        s.newNumLeaves = 1;
+        s.numNodes = 1;
+        s.numActiveParticles = active;
+        *t.st = s;
+        t.N[0][0] = active;
+        *t.scanTicket = 0;
+    }
+    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
+         w += std::size_t(gridDim.x) * blockDim.x)
+        t.scanStatus[w] = 0;
+}
+
+// N[i] = lb(P, K[i+1]) - lb(P, K[i]) over all slots (calculateNodeCount).
+__global__ __launch_bounds__(kThreads) void leafCountsKernel(TreeDev t)
+{
+    DevState *st = t.st;
+    if (st->converged) return;
+    const int L = st->numLeaves;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= L) return;
+    const int ab = st->activeBuffer;
+    const Key *K = t.K[ab];
+    const std::size_t a = lowerBound(t.P, t.n, K[i]);
+    const std::size_t b = lowerBound(t.P, t.n, K[i + 1]);
+    const std::size_t c = b - a;
+    t.N[ab][i] = c > 0xffffffffu ? 0xffffffffu : unsigned(c);
+}
+
+// ---- near-field work list ------------------------------------------------------
+__device__ inline bool dirtyLeader(const Key *tree, const std::uint8_t *gDirty, int L, int i)
+{
+    if (i + 8 > L) return false;
+    if (gDirty && !gDirty[i]) return false;
+    unsigned sl = 0;
+    return siblingIndex(tree, L, i, sl) == 0 && sl > 0;
+}
+
+__global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full)
+{
+    DevState *st = t.st;
+    if (st->converged) return;
+    const int L = st->numLeaves;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int ab = st->activeBuffer;
+    bool own = false, lead = false;
+    if (i < L)
+    {
+        own = full || t.nfDirty[i];
+        lead = dirtyLeader(t.K[ab], full ? nullptr : t.groupDirty, L, i);
+    }
+    const std::uint64_t bo = __ballot(own), bl = __ballot(lead);
+    const unsigned lane = __lane_id();
+    const std::uint32_t no = __popcll(bo), nl = __popcll(bl);
+    std::uint32_t baseIdx = 0;
+    if (lane == 0 && (no + nl)) baseIdx = atomicAdd(t.nfCount, no + nl);
+    baseIdx = __shfl(baseIdx, 0, 64);
+    const std::uint64_t lt = (std::uint64_t(1) << lane) - 1;
+    if (own) t.nfTasks[baseIdx + __popcll(bo & lt)] = std::uint32_t(i) << 1;
+    if (lead) t.nfTasks[baseIdx + no + __popcll(bl & lt)] = (std::uint32_t(i) << 1) | 1u;
+}
+
+// Near-field count of the level-`level` node at `start`: its own population
+// times the population of the 27 same-level cells, periodic at 2^21 per axis
+// (computeNFCooperative, octree_rebalance.cuh:579-634). One lane per cell in
+// groups of 32 lanes.
+__device__ inline void nearField(const Key *P, std::size_t n, Key start, unsigned level, bool hilbert,
+                                 unsigned lane32, std::uint64_t &center, std::uint64_t &sum)
+{
+    std::uint64_t pop = 0;
+    if (lane32 < 27)
+    {
+        Key ck = start;
+        if (lane32 != 13)
+        {
+            unsigned cx, cy, cz;
+            sfcDecode(start, hilbert, cx, cy, cz);
+            const int len = int(1u << (kOctLevels - level));
+            const int R = int(1u << kOctLevels);
+            const int dx = int(lane32 % 3) - 1, dy = int((lane32 / 3) % 3) - 1, dz = int(lane32 / 9) - 1;
+            auto pbc = [](int v) { const int w = v < 0 ? v + R : v; return w >= R ? w - R : w; };
+            ck = enclosingBoxCode(sfcEncode(unsigned(pbc(int(cx) + dx * len)), unsigned(pbc(int(cy) + dy * len)),
+                                            unsigned(pbc(int(cz) + dz * len)), hilbert),
+                                  level);
+        }
+        const std::size_t a = lowerBound(P, n, ck);
+        const std::size_t b = lowerBound(P, n, ck + nodeRange(level));
+        pop = std::uint64_t(b - a);
+    }
+    std::uint64_t s = pop;
+#pragma unroll
+    for (int m = 16; m >= 1; m >>= 1) s += __shfl_xor(s, m, 32);
+    sum = s;
+    center = __shfl(pop, 13, 32);
+}
+
+__global__ __launch_bounds__(kThreads) void nfComputeKernel(TreeDev t, unsigned bucket, int hilbert)
+{
+    DevState *st = t.st;
+    if (st->converged) return;
+    const int ab = st->activeBuffer;
+    const Key *tree = t.K[ab];
+    const std::uint32_t total = *t.nfCount;
+    const unsigned lane32 = threadIdx.x & 31;
+    const std::uint32_t groups = gridDim.x * (blockDim.x / 32);
+    for (std::uint32_t g = blockIdx.x * (blockDim.x / 32) + threadIdx.x / 32; g < total; g += groups)
+    {
+        const std::uint32_t task = t.nfTasks[g];
+        const int leaf = int(task >> 1);
+        const bool parent = task & 1u;
+        const Key start = tree[leaf];
+        unsigned level = treeLevel(tree[leaf + 1] - start);
+        if (parent) level -= 1;
+        std::uint64_t center, sum;
+        nearField(t.P, t.n, start, level, hilbert != 0, lane32, center, sum);
+        if (lane32 == 0)
+        {
+            const std::uint64_t nf = center * sum;
+            if (!parent)
+            {
+                t.N[ab][leaf] = center > 0xffffffffull ? 0xffffffffu : unsigned(center);
+                t.NF[ab][leaf] = nf > 0xffffffffull ? 0xffffffffu : unsigned(nf);
+            }
+            else
+                t.gcm[ab][leaf] = nf <= std::uint64_t(bucket) ? 1 : 0;
+        }
+    }
+}
+
+// ---- one-pass converged binary tree ---------------------------------------------
+// Internal nodes are exactly the nodes that contain a window of bucket+1
+// consecutive sorted particles; a node is counted once, at its first
+// particle j, for the depths d with cpl(P[j-1],P[j]) < d <= D_j, where
+// D_j = min(cpl(P[j], P[j+bucket]), maxDepth-1).
+__device__ inline int cpl63(Key a, Key b) { return clz64(a ^ b) - 1; } // 63 when equal
+
+__global__ __launch_bounds__(kThreads) void binWindowKernel(const Key *P, const DevState *st, unsigned bucket,
+                                                            unsigned maxDepth, std::int8_t *depth,
+                                                            unsigned long long *total)
+{
+    __shared__ unsigned long long part[kThreads / 64];
+    const std::size_t nact = st->numActiveParticles;
+    const std::size_t j = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long local = 0;
+    if (j < nact)
+    {
+        int D = -1;
+        if (maxDepth > 0 && j + std::size_t(bucket) < nact)
+        {
+            const int c = cpl63(P[j], P[j + bucket]);
+            D = c < int(maxDepth) - 1 ? c : int(maxDepth) - 1;
+        }
+        depth[j] = std::int8_t(D);
+        const int cj = j > 0 ? cpl63(P[j - 1], P[j]) : -1;
+        local = D > cj ? (unsigned long long)(D - cj) : 0ull;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) local += __shfl_xor(local, m, 64);
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long s = 0;
+        for (int w = 0; w < kThreads / 64; ++w) s += part[w];
+        if (s) atomicAdd(total, s);
+    }
+}
+
+__device__ inline Key bitsBetween(int lo, int hi) // bits lo..hi inclusive, 0 <= lo, hi <= 62
+{
+    if (hi < lo) return 0;
+    const Key top = (Key(1) << (hi + 1)) - 1;
+    return top & ~((Key(1) << lo) - 1);
+}
+
+// Leaf boundaries between sorted particles P[q-1] and P[q] are the
+// midpoints of (a) internal ancestors of P[q-1] where it lies in the lower
+// half and P[q] is outside, deepest first, (b) the internal common ancestor,
+// (c) internal ancestors of P[q] where it lies in the upper half and P[q-1]
+// is outside, shallowest first. delta(q) is the depth of the deepest
+// internal node holding P[q]: the maximum of D_j over the windows j in
+// [q-bucket, q], computed per tile with a sparse table in LDS.
+__global__ __launch_bounds__(kThreads) void binMidsKernel(const Key *P, const DevState *st, const std::int8_t *depth,
+                                                          unsigned bucket, Key *K, std::uint32_t *bound, long leaves,
+                                                          std::uint64_t *status, std::uint32_t *ticket,
+                                                          unsigned long long *check)
+{
+    constexpr int Q = kPassTile;
+    constexpr int kMaxSpan = Q + int(kDirectMaxBucket) + 2;
+    __shared__ std::int8_t tab[2][kMaxSpan];
+    __shared__ std::int8_t dl[Q + 1];
+    __shared__ std::uint32_t tileShared;
+    __shared__ std::uint64_t wsum[kThreads / 64];
+    __shared__ int wflag[kThreads / 64];
+    __shared__ std::uint64_t prefixShared;
+    const int tid = threadIdx.x;
+    if (tid == 0) tileShared = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const long nact = long(st->numActiveParticles);
+    const long q0 = long(tileShared) * Q;
+    if (q0 > nact) return;
+    const int B = int(bucket);
+    const int span = Q + B + 2;
+    const long origin = q0 - 1 - B;
+    for (int s = tid; s < span; s += kThreads)
+    {
+        const long j = origin + s;
+        tab[0][s] = (j < 0 || j >= nact) ? std::int8_t(-1) : depth[j];
+    }
+    __syncthreads();
+    int lg = 0;
+    while ((2 << lg) <= B + 1) ++lg; // floor(log2(B + 1))
+    int cur = 0;
+    for (int t = 0; t < lg; ++t)
+    {
+        const int step = 1 << t;
+        for (int s = tid; s < span; s += kThreads)
+        {
+            std::int8_t v = tab[cur][s];
+            if (s + step < span)
+            {
+                const std::int8_t w = tab[cur][s + step];
+                v = w > v ? w : v;
+            }
+            tab[cur ^ 1][s] = v;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    const int pw = 1 << lg;
+    for (int u = tid; u <= Q; u += kThreads)
+    {
+        const long q = q0 - 1 + u;
+        std::int8_t d = -1;
+        if (q >= 0 && q < nact)
+        {
+            const std::int8_t a = tab[cur][u], b = tab[cur][u + B - pw + 1];
+            d = a > b ? a : b;
+        }
+        dl[u] = d;
+    }
+    __syncthreads();
+
+    const long qFirst = q0 + long(tid) * kItems;
+    int cnt[kItems];
+    std::uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        const long q = qFirst + k;
+        int c = 0;
+        if (q <= nact)
+        {
+            const bool hp = q > 0, hc = q < nact;
+            const Key prev = hp ? P[q - 1] : 0, curk = hc ? P[q] : 0;
+            const int dp = hp ? dl[q - q0] : -1;
+            const int dc = hc ? dl[q - q0 + 1] : -1;
+            const int cp = (hp && hc) ? cpl63(prev, curk) : -1;
+            const int lca = (hp && hc && cp <= dc) ? 1 : 0;
+            const Key am = dp > cp ? bitsBetween(62 - dp, 61 - cp) : 0;
+            const Key bm = dc > cp ? bitsBetween(62 - dc, 61 - cp) : 0;
+            c = __popcll(~prev & am) + lca + __popcll(curk & bm);
+        }
+        cnt[k] = c;
+        sum += std::uint64_t(c);
+    }
+    std::uint64_t tileSum;
+    bool unused;
+    const std::uint64_t local = blockScan64(sum, false, wsum, wflag, tileSum, unused);
+    if (tid == 0)
+    {
+        std::uint64_t ex;
+        bool exChg;
+        chainedScan(status, std::size_t(tileShared), tileSum, false, ex, exChg);
+        prefixShared = ex;
+    }
+    __syncthreads();
+    std::uint64_t run = prefixShared + local;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        const long q = qFirst + k;
+        if (q > nact) break;
+        long pos = 1 + long(run);
+        if (cnt[k])
+        {
+            const bool hp = q > 0, hc = q < nact;
+            const Key prev = hp ? P[q - 1] : 0, curk = hc ? P[q] : 0;
+            const int dp = hp ? dl[q - q0] : -1;
+            const int dc = hc ? dl[q - q0 + 1] : -1;
+            const int cp = (hp && hc) ? cpl63(prev, curk) : -1;
+            Key m = (dp > cp ? bitsBetween(62 - dp, 61 - cp) : 0) & ~prev;
+            while (m)
+            {
+                const int bit = __ffsll((long long)m) - 1;
+                const Key mid = (prev & ~((Key(2) << bit) - 1)) | (Key(1) << bit);
+                if (pos < leaves) { K[pos] = mid; bound[pos] = std::uint32_t(q); }
+                ++pos;
+                m &= m - 1;
+            }
+            if (hp && hc && cp <= dc)
+            {
+                const int bit = 62 - cp;
+                if (pos < leaves) { K[pos] = curk & ~((Key(1) << bit) - 1); bound[pos] = std::uint32_t(q); }
+                ++pos;
+            }
+            m = (dc > cp ? bitsBetween(62 - dc, 61 - cp) : 0) & curk;
+            while (m)
+            {
+                const int bit = 63 - clz64(m);
+                if (pos < leaves) { K[pos] = curk & ~((Key(1) << bit) - 1); bound[pos] = std::uint32_t(q); }
+                ++pos;
+                m &= ~(Key(1) << bit);
+            }
+        }
+        if (q == nact)
+        {
+            const std::uint64_t total = run + std::uint64_t(cnt[k]);
+            *check = total;
+            K[0] = 0;
+            bound[0] = 0;
+            if (long(total) + 1 == leaves)
+            {
+                K[leaves] = kMaxKey;
+                bound[leaves] = std::uint32_t(nact);
+            }
+        }
+        run += std::uint64_t(cnt[k]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void binBoundsToCountsKernel(const std::uint32_t *bound, std::uint32_t *N,
+                                                                    long leaves)
+{
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < leaves) N[i] = bound[i + 1] - bound[i];
+}
+
+__global__ void binDirectStateKernel(DevState *st, int leaves)
+{
+    st->changed = 0;
+    st->converged = 1;
+    st->numLeaves = leaves;
+    st->newNumLeaves = leaves;
+    st->needsResize = 0;
+    st->numNodes = 2 * leaves - 1;
+    st->activeBuffer = 0;
+}
+
+unsigned clearBlocks(std::size_t words)
+{
+    const unsigned b = ceilDiv(words, 256);
+    return b < 1 ? 1 : (b > 1024 ? 1024 : b);
+}
+} // namespace
+
+std::size_t passTiles(std::size_t cap) { return cap / kPassTile + 1; }
+std::size_t scanStatusWords(std::size_t items) { return items / kPassTile + 2; }
+
+void octInitRoot(hipStream_t s, const TreeDev &t)
+{
+    hipLaunchKernelGGL(octInitKernel, dim3(clearBlocks(t.statusWords)), dim3(256), 0, s, t, t.statusWords);
+}
+void binInitRoot(hipStream_t s, const TreeDev &t)
+{
+    hipLaunchKernelGGL(binInitKernel, dim3(clearBlocks(t.statusWords)), dim3(256), 0, s, t, t.statusWords);
+}
+void updateReset(hipStream_t s, const TreeDev &t)
+{
+    hipLaunchKernelGGL(resetKernel, dim3(clearBlocks(t.statusWords)), dim3(256), 0, s, t.st, t.P, t.n, t.scanStatus,
+                       t.statusWords, t.scanTicket, t.nfCount);
+}
+
+void octDecideScan(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves)
+{
+    OctDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, {t.NF[0], t.NF[1]}, {t.gcm[0], t.gcm[1]}, p.bucket, p.nf};
+    hipLaunchKernelGGL((decideScanKernel<OctDecider>), dim3(unsigned(passTiles(gridLeaves))), dim3(kThreads), 0, s,
+                       d, t.st, capacity, t.offs, t.scanStatus, t.scanTicket);
+}
+void octProcessCommit(hipStream_t s, const TreeDev &t, const OctParams &p, std::size_t gridLeaves)
+{
+    hipLaunchKernelGGL(octProcessKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t, p.nf);
+    hipLaunchKernelGGL(commitKernel, dim3(clearBlocks(t.statusWords)), dim3(256), 0, s, t.st, 0, t.scanStatus,
+                       t.statusWords, t.scanTicket, t.nfCount);
+}
+void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, std::size_t gridLeaves)
+{
+    if (!p.nf)
+    {
+        hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t);
+        return;
+    }
+    hipLaunchKernelGGL(nfCollectKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t, full ? 1 : 0);
+    hipLaunchKernelGGL(nfComputeKernel, dim3(2048), dim3(kThreads), 0, s, t, p.bucket, p.hilbert ? 1 : 0);
+}
+void octPass(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves)
+{
+    octDecideScan(s, t, p, capacity, gridLeaves);
+    octProcessCommit(s, t, p, gridLeaves);
+    octRefresh(s, t, p, false, gridLeaves);
+}
+
+void binDecideScan(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves)
+{
+    BinDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, p.bucket, p.maxDepth};
+    hipLaunchKernelGGL((decideScanKernel<BinDecider>), dim3(unsigned(passTiles(gridLeaves))), dim3(kThreads), 0, s,
+                       d, t.st, capacity, t.offs, t.scanStatus, t.scanTicket);
+}
+void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
+{
+    hipLaunchKernelGGL(binProcessKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t);
+    hipLaunchKernelGGL(commitKernel, dim3(clearBlocks(t.statusWords)), dim3(256), 0, s, t.st, 1, t.scanStatus,
+                       t.statusWords, t.scanTicket, t.nfCount);
+}
+void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
+{
+    hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t);
+}
+void binPass(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves)
+{
+    binDecideScan(s, t, p, capacity, gridLeaves);
+    binProcessCommit(s, t, gridLeaves);
+    leafCounts(s, t, gridLeaves);
+}
+
+void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d)
+{
+    hipLaunchKernelGGL(resetKernel, dim3(clearBlocks(d.statusWords)), dim3(256), 0, s, t.st, t.P, t.n, d.status,
+                       d.statusWords, d.ticket, nullptr);
+    AT_HIP(hipMemsetAsync(d.total, 0, sizeof(unsigned long long), s));
+    if (t.n)
+        hipLaunchKernelGGL(binWindowKernel, dim3(ceilDiv(t.n, kThreads)), dim3(kThreads), 0, s, t.P, t.st, p.bucket,
+                           p.maxDepth, d.depth, d.total);
+}
+void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves)
+{
+    hipLaunchKernelGGL(binMidsKernel, dim3(unsigned(t.n / kPassTile + 1)), dim3(kThreads), 0, s, t.P, t.st, d.depth,
+                       p.bucket, t.K[0], d.bound, leaves, d.status, d.ticket, d.total);
+    hipLaunchKernelGGL(binBoundsToCountsKernel, dim3(ceilDiv(std::size_t(leaves), kThreads)), dim3(kThreads), 0, s,
+                       d.bound, t.N[0], leaves);
+}
+void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves)
+{
+    hipLaunchKernelGGL(binDirectStateKernel, dim3(1), dim3(1), 0, s, t.st, int(leaves));
+}
+} // namespace at

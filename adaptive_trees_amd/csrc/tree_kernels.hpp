@@ -1,0 +1,78 @@
+// tree_kernels.hpp -- launchers of the octree / binary rebalance kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace at
+{
+// Device buffers of one tree; [0]/[1] are the A/B halves selected by
+// DevState::activeBuffer.
+struct TreeDev
+{
+    Key *K[2] = {nullptr, nullptr};              // cornerstone, capacity + 1
+    std::uint32_t *N[2] = {nullptr, nullptr};    // leaf counts
+    std::uint32_t *NF[2] = {nullptr, nullptr};   // near-field counts (octree NF)
+    std::uint8_t *gcm[2] = {nullptr, nullptr};   // groupCanMerge (octree NF)
+    std::uint8_t *nfDirty = nullptr;
+    std::uint8_t *groupDirty = nullptr;
+    std::int64_t *offs = nullptr;                // scanned node ops, capacity + 1
+    std::uint64_t *scanStatus = nullptr;         // chained-scan status words
+    std::uint32_t *scanTicket = nullptr;
+    std::uint32_t *nfTasks = nullptr;            // (leaf << 1 | kind) work items
+    std::uint32_t *nfCount = nullptr;
+    std::size_t statusWords = 0;
+    DevState *st = nullptr;
+    const Key *P = nullptr;
+    std::size_t n = 0;                           // particle slots
+};
+
+constexpr int kPassTile = 2048;                  // leaves per chained-scan tile
+std::size_t passTiles(std::size_t cap);
+// Words of chained-scan status needed for `items` scanned elements.
+std::size_t scanStatusWords(std::size_t items);
+
+struct OctParams
+{
+    unsigned bucket;
+    bool nf;
+    bool hilbert;
+};
+struct BinParams
+{
+    unsigned bucket;
+    unsigned maxDepth;
+};
+
+// ---- octree ----
+void octInitRoot(hipStream_t s, const TreeDev &t);
+void updateReset(hipStream_t s, const TreeDev &t);
+// One split/merge pass: decision + chained scan + capacity check, scatter
+// into the inactive half, commit, then the count refresh.
+void octPass(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves);
+void octDecideScan(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves);
+void octProcessCommit(hipStream_t s, const TreeDev &t, const OctParams &p, std::size_t gridLeaves);
+void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, std::size_t gridLeaves);
+
+// ---- binary ----
+void binInitRoot(hipStream_t s, const TreeDev &t);
+void binPass(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves);
+void binDecideScan(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves);
+void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves);
+void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves);
+// One-pass construction of the converged binary tree into K[0]/N[0]:
+// phase 1 returns the number of internal nodes (the host sizes buffers),
+// phase 2 writes the sorted leaf boundaries and their particle counts.
+struct BinDirectScratch
+{
+    std::int8_t *depth = nullptr;       // per sorted particle
+    std::uint32_t *bound = nullptr;     // per boundary: index of its first particle
+    unsigned long long *total = nullptr;
+    std::uint64_t *status = nullptr;
+    std::uint32_t *ticket = nullptr;
+    std::size_t statusWords = 0;
+};
+constexpr unsigned kDirectMaxBucket = 8192;
+void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d);
+void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves);
+void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves);
+} // namespace at
