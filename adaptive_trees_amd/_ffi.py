@@ -163,6 +163,13 @@ def hip_lib():
             "graph_capture_update": [_P, _P, ctypes.POINTER(_P)],
             "graph_launch": [_P, _P],
             "graph_destroy": [_P],
+            "profile_enable": [_I],
+            "profile_reset": [],
+            "profile_query": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_ulonglong)],
+            "bench_copy": [_P, _P, _SZ, _P],
+            "tune_set": [ctypes.c_char_p, _I],
+            "tune_get": [ctypes.c_char_p, ctypes.POINTER(_I)],
         }.items():
             f = getattr(d, "at_" + name)
             f.argtypes = args

@@ -610,12 +610,18 @@ private:
     // until the final leaf count fits, the same rule as growBuffers.
     void buildDirect(const BinParams &p)
     {
-        direct_.depth.ensure(n_ ? n_ : 1, 0, stream_);
+        const std::size_t tiles = n_ / kPassTile + 2;
+        direct_.delta.ensure(n_ ? n_ : 1, 0, stream_);
+        direct_.cnt.ensure(n_ + 1, 0, stream_);
+        direct_.tileSum.ensure(tiles, 0, stream_);
         direct_.total.ensure(1, 0, stream_);
-        direct_.status.ensure(n_ / kPassTile + 2, 0, stream_);
         direct_.ticket.ensure(1, 0, stream_);
-        BinDirectScratch d{direct_.depth.p, nullptr, direct_.total.p, direct_.status.p, direct_.ticket.p,
-                           direct_.status.cap};
+        BinDirectScratch d;
+        d.delta = direct_.delta.p;
+        d.cnt = direct_.cnt.p;
+        d.tileSum = direct_.tileSum.p;
+        d.total = direct_.total.p;
+        d.ticket = direct_.ticket.p;
         const TreeDev t0 = dev();
         binDirectCount(stream_, t0, p, d);
         unsigned long long internals = 0;
@@ -635,12 +641,9 @@ private:
         const TreeDev t = dev();
         binDirectWrite(stream_, t, p, d, long(L));
         binDirectFinish(stream_, t, long(L));
-        unsigned long long check = 0;
-        AT_HIP(hipMemcpyAsync(&check, direct_.total.p, sizeof(check), hipMemcpyDeviceToHost, stream_));
         syncState();
-        if (check != internals)
-            throw Error(AT_EHIP, "binary construction: boundary count " + std::to_string(check) +
-                                     " differs from internal-node count " + std::to_string(internals));
+        if (std::size_t(sth_->numLeaves) != L)
+            throw Error(AT_EHIP, "binary construction produced an inconsistent leaf count");
     }
 
     // The reference loop (kdtree3d.cu:236-252), used for buckets whose
@@ -722,11 +725,10 @@ private:
 
     struct DirectBufs
     {
-        DevBuf<std::int8_t> depth;
-        DevBuf<std::uint32_t> bound;
+        DevBuf<std::int8_t> delta;
+        DevBuf<std::uint8_t> cnt;
+        DevBuf<std::uint32_t> tileSum, bound, ticket;
         DevBuf<unsigned long long> total;
-        DevBuf<std::uint64_t> status;
-        DevBuf<std::uint32_t> ticket;
     };
 
     const Real *x_ = nullptr, *y_ = nullptr, *z_ = nullptr;
@@ -975,4 +977,25 @@ int at_graph_launch(void *exec, void *stream)
     return hipStatus(hipGraphLaunch(hipGraphExec_t(exec), hipStream_t(stream)));
 }
 int at_graph_destroy(void *exec) { return hipStatus(hipGraphExecDestroy(hipGraphExec_t(exec))); }
+
+int at_tune_set(const char *key, int value)
+{
+    return guarded([&] {
+        const std::string k = key ? key : "";
+        if (k == "sort_items") at::setSortItemsPerThread(value);
+        else if (k == "sort_ranges") at::setSortTargetRanges(value);
+        else if (k == "sort_threads") at::setSortThreads(value);
+        else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
+    });
+}
+int at_tune_get(const char *key, int *value)
+{
+    return guarded([&] {
+        const std::string k = key ? key : "";
+        if (k == "sort_items") *value = at::sortItemsPerThread();
+        else if (k == "sort_ranges") *value = at::sortTargetRanges();
+        else if (k == "sort_threads") *value = at::sortThreads();
+        else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
+    });
+}
 }

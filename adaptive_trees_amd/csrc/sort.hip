@@ -1,38 +1,43 @@
-// sort.hip -- stable LSD radix sort (one sweep per 8-bit digit) for gfx950.
+// sort.hip -- stable LSD radix sort for gfx950, 8-bit digits, 8 passes.
 //
 // Replaces the reference's cub::DeviceRadixSort::SortPairs calls
 // (src/octree/octree.cu:301-304, src/kdtree3d/kdtree3d.cu:168-170,
 // src/common/tree_view.cu:664-689): ascending by all 64 key bits, equal keys
 // keep their input order.
 //
-// Design (wave64, 256 threads = 4 waves, ITEMS keys per thread):
-//  * one up-front kernel builds the digit histograms of all 8 passes (fused
-//    with key generation for particle keys, so keys are read once less);
-//  * each pass is one kernel: a tile takes a ticket (tile order = start
-//    order, which gives the chained scan forward progress), ranks its keys
-//    per wave with 8 ballots per key (match of equal digits, stable in lane
-//    order), publishes its per-digit counts and looks back across earlier
-//    tiles for the exclusive prefix (status word = 2-bit flag | 30-bit count),
-//    then stages keys (and values) through LDS so global writes leave in runs
-//    of equal digits.
+// Each pass is range-partitioned, with no communication between blocks:
+//   upsweep   digit histogram of each contiguous range of keys. Pass 0's
+//             histograms come out of key generation itself; pass k+1 reads a
+//             one-byte digit array that pass k's scatter wrote, not the keys.
+//   scan      two small kernels: per-range offsets inside chunks of 64
+//             ranges, then chunk prefixes and the global digit bases.
+//   scatter   one block per range walks its range tile by tile (wave64,
+//             256 threads, ITEMS keys per thread). Keys are ranked per wave
+//             with 8 ballots each (lanes with equal digits, stable in lane
+//             order), staged in LDS in output order, and written in runs of
+//             equal digits at a per-digit running offset kept in LDS.
+// A chained (look-back) scan is avoided on purpose: with ~768 tiles resident
+// on the 256 CUs the prefix chain made the look-back cost ~45% of a pass.
 #include "sort.hpp"
+
+#include "profile.hpp"
 
 namespace at
 {
 namespace
 {
 constexpr int kWaves = kSortThreads / 64;
-constexpr std::uint32_t kFlagAgg = 1u << 30;
-constexpr std::uint32_t kFlagPre = 2u << 30;
-constexpr std::uint32_t kValMask = (1u << 30) - 1;
+constexpr int kChunk = 64; // ranges per first-level scan chunk
+constexpr std::size_t kHistogramBlocks = 4096;
 
 int g_items = 16;
+int g_threads = 512;
+int g_targetRanges = 1024;
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 
 // Exclusive scan of one value per thread over a 256-thread block.
-__device__ inline std::uint32_t blockExclusiveScan(std::uint32_t v, std::uint32_t *waveTotals,
-                                                   std::uint32_t &total)
+__device__ inline std::uint32_t blockExclusiveScan(std::uint32_t v, std::uint32_t *waveTotals)
 {
     const unsigned lane = laneId();
     const int wave = threadIdx.x >> 6;
@@ -46,299 +51,451 @@ __device__ inline std::uint32_t blockExclusiveScan(std::uint32_t v, std::uint32_
     if (lane == 63) waveTotals[wave] = inc;
     __syncthreads();
     std::uint32_t before = 0;
-    total = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w)
-    {
-        const std::uint32_t t = waveTotals[w];
-        if (w < wave) before += t;
-        total += t;
-    }
+        if (w < wave) before += waveTotals[w];
     __syncthreads();
     return before + inc - v;
 }
 
-template <int ITEMS, bool kVals, bool kIota>
-__global__ __launch_bounds__(kSortThreads) void onesweepKernel(
-    const Key *__restrict__ kin, Key *__restrict__ kout, const std::uint32_t *__restrict__ vin,
-    std::uint32_t *__restrict__ vout, std::size_t n, const std::uint32_t *__restrict__ hist,
-    std::uint32_t *lookback, std::uint32_t *lookbackNext, std::uint32_t *ticket, int shift)
+// Adds a digit to a block histogram. A digit equal on every active lane of
+// the wave (common for the high digits of clustered data) is added once
+// instead of by 64 conflicting LDS atomics.
+__device__ inline void countDigit(std::uint32_t *h, unsigned d)
 {
-    constexpr int kTile = ITEMS * kSortThreads;
+    const std::uint64_t active = __ballot(1);
+    const int leader = __ffsll((long long)active) - 1;
+    const unsigned d0 = __shfl(d, leader, 64);
+    if (__ballot(d != d0) == 0)
+    {
+        if (int(laneId()) == leader) atomicAdd(&h[d0], std::uint32_t(__popcll(active)));
+    }
+    else
+        atomicAdd(&h[d], 1u);
+}
+
+__device__ inline void flushHistogram(const std::uint32_t *h, std::uint32_t *out)
+{
+    __syncthreads();
+    out[threadIdx.x] = h[threadIdx.x];
+}
+
+constexpr int kKeygenUnroll = 4;
+
+template <class T, KeyKind K>
+__device__ inline Key particleKey(const KeygenParams<T> &p, std::size_t i)
+{
+    const T x = p.x[i], y = p.y[i], z = p.z[i];
+    if (K == KeyKind::Binary) return binaryKey<T>(x, y, z, p.geom);
+    return isEscaped<T>(x, y, z) ? kInvalidKey : gridKey<T>(x, y, z, p.grid, K == KeyKind::Hilbert);
+}
+
+// Part `blockIdx.x % parts` of a range: its keys and a partial digit-0
+// histogram; each thread keeps kKeygenUnroll particles in flight.
+template <class T, KeyKind K>
+__global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, std::size_t n, std::size_t rangeKeys,
+                                                             int parts, Key *__restrict__ keys, std::uint32_t *partHist)
+{
+    __shared__ std::uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const std::size_t r = blockIdx.x / unsigned(parts), part = blockIdx.x % unsigned(parts);
+    const std::size_t partKeys = rangeKeys / std::size_t(parts);
+    const std::size_t rEnd = (r + 1) * rangeKeys < n ? (r + 1) * rangeKeys : n;
+    const std::size_t begin = r * rangeKeys + part * partKeys;
+    const std::size_t end0 = begin + partKeys;
+    const std::size_t end = end0 < rEnd ? end0 : rEnd;
+    constexpr std::size_t kStep = std::size_t(kSortThreads) * kKeygenUnroll;
+    std::size_t i = begin + threadIdx.x;
+    for (; i + (kKeygenUnroll - 1) * kSortThreads < end; i += kStep)
+    {
+        Key k[kKeygenUnroll];
+#pragma unroll
+        for (int u = 0; u < kKeygenUnroll; ++u) k[u] = particleKey<T, K>(p, i + std::size_t(u) * kSortThreads);
+#pragma unroll
+        for (int u = 0; u < kKeygenUnroll; ++u)
+        {
+            keys[i + std::size_t(u) * kSortThreads] = k[u];
+            countDigit(h, unsigned(k[u]) & (kRadix - 1));
+        }
+    }
+    for (; i < end; i += kSortThreads)
+    {
+        const Key k = particleKey<T, K>(p, i);
+        keys[i] = k;
+        countDigit(h, unsigned(k) & (kRadix - 1));
+    }
+    flushHistogram(h, partHist + std::size_t(blockIdx.x) * kRadix);
+}
+
+__global__ __launch_bounds__(kSortThreads) void upsweepKeysKernel(const Key *__restrict__ keys, std::size_t n,
+                                                                  std::size_t rangeKeys, int shift,
+                                                                  std::uint32_t *rangeHist)
+{
+    __shared__ std::uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const std::size_t begin = std::size_t(blockIdx.x) * rangeKeys;
+    const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
+    for (std::size_t i = begin + threadIdx.x; i < end; i += kSortThreads)
+        countDigit(h, unsigned(keys[i] >> shift) & (kRadix - 1));
+    flushHistogram(h, rangeHist + std::size_t(blockIdx.x) * kRadix);
+}
+
+// Partial digit histogram of part `blockIdx.x % parts` of a range from the
+// one-byte digit array: 16 digits per load, one LDS histogram per wave.
+__global__ __launch_bounds__(kSortThreads) void upsweepDigitsKernel(const std::uint8_t *__restrict__ digits,
+                                                                    std::size_t n, std::size_t rangeKeys, int parts,
+                                                                    std::uint32_t *partHist)
+{
+    __shared__ std::uint32_t h[kWaves][kRadix];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) h[w][threadIdx.x] = 0;
+    __syncthreads();
+    std::uint32_t *mine = h[threadIdx.x >> 6];
+    const std::size_t r = blockIdx.x / unsigned(parts), part = blockIdx.x % unsigned(parts);
+    const std::size_t partKeys = rangeKeys / std::size_t(parts); // multiple of 16
+    const std::size_t rEnd = (r + 1) * rangeKeys < n ? (r + 1) * rangeKeys : n;
+    const std::size_t begin = r * rangeKeys + part * partKeys;
+    const std::size_t end = begin + partKeys < rEnd ? begin + partKeys : rEnd;
+    const uint4 *quads = reinterpret_cast<const uint4 *>(digits);
+    for (std::size_t i = begin + 16 * threadIdx.x; i < end; i += 16 * kSortThreads)
+    {
+        if (i + 16 <= end)
+        {
+            const uint4 q = quads[i >> 4];
+            const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) countDigit(mine, (w[k] >> (8 * b)) & 0xffu);
+        }
+        else
+            for (std::size_t j = i; j < end; ++j) atomicAdd(&mine[digits[j]], 1u);
+    }
+    __syncthreads();
+    std::uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += h[w][threadIdx.x];
+    partHist[std::size_t(blockIdx.x) * kRadix + threadIdx.x] = t;
+}
+
+// Offsets of each range inside its chunk of kChunk ranges, and chunk totals;
+// a range's histogram is the sum of its `parts` partial histograms. All loads
+// of a thread are issued before the running sum consumes them.
+__global__ __launch_bounds__(kSortThreads) void scanChunksKernel(const std::uint32_t *__restrict__ partHist,
+                                                                 std::size_t ranges, int parts,
+                                                                 std::uint32_t *rangeOff, std::uint32_t *chunkOff)
+{
+    const int d = threadIdx.x;
+    const std::size_t r0 = std::size_t(blockIdx.x) * kChunk;
+    std::uint32_t v[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) v[k] = 0;
+    for (int q = 0; q < parts; ++q)
+    {
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (r0 + k < ranges) v[k] += partHist[((r0 + k) * unsigned(parts) + unsigned(q)) * kRadix + d];
+    }
+    std::uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+    {
+        if (r0 + k < ranges) rangeOff[(r0 + k) * kRadix + d] = run;
+        run += v[k];
+    }
+    chunkOff[std::size_t(blockIdx.x) * kRadix + d] = run;
+}
+
+// Exclusive prefixes of the chunks and the global digit bases.
+__global__ __launch_bounds__(kSortThreads) void scanBaseKernel(std::uint32_t *chunkOff, std::size_t chunks,
+                                                               std::uint32_t *digitBase)
+{
+    __shared__ std::uint32_t waveTotals[kWaves];
+    const int d = threadIdx.x;
+    std::uint32_t run = 0;
+    for (std::size_t c = 0; c < chunks; ++c)
+    {
+        const std::uint32_t t = chunkOff[c * kRadix + d];
+        chunkOff[c * kRadix + d] = run;
+        run += t;
+    }
+    digitBase[d] = blockExclusiveScan(run, waveTotals);
+}
+
+// Exclusive scan of the 256 per-digit values held by threads 0..255 of a
+// block of any size (all threads must call it).
+__device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_t *waveTotals)
+{
+    const unsigned lane = laneId();
+    const int wave = threadIdx.x >> 6;
+    std::uint32_t inc = v;
+    if (wave < kRadix / 64)
+    {
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const std::uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= unsigned(o)) inc += t;
+        }
+        if (lane == 63) waveTotals[wave] = inc;
+    }
+    __syncthreads();
+    std::uint32_t before = 0;
+#pragma unroll
+    for (int w = 0; w < kRadix / 64; ++w)
+        if (w < wave) before += waveTotals[w];
+    __syncthreads();
+    return before + inc - v;
+}
+
+template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits>
+__global__ __launch_bounds__(THREADS) void scatterKernel(
+    const Key *__restrict__ kin, Key *__restrict__ kout, const std::uint32_t *__restrict__ vin,
+    std::uint32_t *__restrict__ vout, std::uint8_t *__restrict__ digitsOut, std::size_t n, std::size_t rangeKeys,
+    const std::uint32_t *__restrict__ rangeOff, const std::uint32_t *__restrict__ chunkOff,
+    const std::uint32_t *__restrict__ digitBase, int shift, int nextShift)
+{
+    constexpr int kW = THREADS / 64;
+    constexpr int kTile = ITEMS * THREADS;
     __shared__ union
     {
         Key keys[kTile];
         std::uint32_t vals[kTile];
     } stage;
-    __shared__ std::uint32_t waveCount[kWaves][kRadix];
+    __shared__ std::uint16_t waveCount[kW][kRadix]; // counts and offsets < 2^16
+    __shared__ std::uint32_t running[kRadix];
     __shared__ std::uint32_t adjust[kRadix];
-    __shared__ std::uint32_t scratch[2 * kWaves];
-    __shared__ std::uint32_t tileShared;
+    __shared__ std::uint32_t scratch[kRadix / 64];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
-    if (tid == 0) tileShared = atomicAdd(ticket, 1u);
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) waveCount[w][tid] = 0;
-    __syncthreads();
-    const std::size_t tile = tileShared;
-    const std::size_t base = tile * kTile;
-    // clear this tile's status row of the next pass
-    lookbackNext[tile * kRadix + tid] = 0;
-
-    // ---- load (wave-striped: item j of lane l is element j*64 + l of the wave's chunk)
-    Key key[ITEMS];
-    std::uint32_t val[ITEMS];
-    std::uint32_t digit[ITEMS];
-    const std::size_t waveBase = base + std::size_t(wave) * (ITEMS * 64);
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
-    {
-        const std::size_t idx = waveBase + std::size_t(j) * 64 + lane;
-        const bool ok = idx < n;
-        key[j] = ok ? kin[idx] : kInvalidKey;
-        if (kVals) val[j] = kIota ? std::uint32_t(idx) : (ok ? vin[idx] : 0u);
-        digit[j] = ok ? unsigned(key[j] >> shift) & (kRadix - 1) : kRadix - 1;
-    }
-
-    // ---- rank within the wave: lanes with equal digits via 8 ballots
+    const std::size_t r = blockIdx.x;
+    if (tid < kRadix) running[tid] = digitBase[tid] + chunkOff[(r / kChunk) * kRadix + tid] + rangeOff[r * kRadix + tid];
+    const std::size_t rBegin = r * rangeKeys;
+    const std::size_t rEnd = rBegin + rangeKeys < n ? rBegin + rangeKeys : n;
     const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
-    std::uint32_t rank[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
+
+    for (std::size_t base = rBegin; base < rEnd; base += kTile)
     {
-        std::uint64_t match = ~std::uint64_t(0);
+        for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
+        // ---- load keys (wave-striped: item j of lane l is element j*64 + l of the wave's chunk)
+        Key key[ITEMS];
+        const std::size_t waveBase = base + std::size_t(wave) * (ITEMS * 64);
 #pragma unroll
-        for (int b = 0; b < kRadixBits; ++b)
+        for (int j = 0; j < ITEMS; ++j)
         {
-            const bool bit = (digit[j] >> b) & 1u;
-            const std::uint64_t ballot = __ballot(bit);
-            match &= bit ? ballot : ~ballot;
+            const std::size_t idx = waveBase + std::size_t(j) * 64 + lane;
+            key[j] = idx < rEnd ? kin[idx] : kInvalidKey; // padding ranks last with digit 255
         }
-        const std::uint32_t lower = __popcll(match & ltMask);
-        const std::uint32_t cnt = __popcll(match);
-        const std::uint32_t before = waveCount[wave][digit[j]];
-        rank[j] = before + lower;
-        if (lower == 0) waveCount[wave][digit[j]] = before + cnt;
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // ---- per digit (thread tid owns digit tid): wave offsets and tile count
-    const int d = tid;
-    std::uint32_t tileCount = 0;
+        // ---- rank within the wave: lanes with equal digits via 8 ballots
+        std::uint32_t rank[ITEMS];
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w)
-    {
-        const std::uint32_t c = waveCount[w][d];
-        waveCount[w][d] = tileCount;
-        tileCount += c;
-    }
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
+            std::uint64_t match = ~std::uint64_t(0);
+#pragma unroll
+            for (int b = 0; b < kRadixBits; ++b)
+            {
+                const bool bit = (dg >> b) & 1u;
+                const std::uint64_t ballot = __ballot(bit);
+                match &= bit ? ballot : ~ballot;
+            }
+            const std::uint32_t lower = __popcll(match & ltMask);
+            const std::uint32_t before = waveCount[wave][dg];
+            rank[j] = before + lower;
+            if (lower == 0) waveCount[wave][dg] = std::uint16_t(before + __popcll(match));
+        }
+        __syncthreads();
 
-    // ---- chained scan across tiles for this digit
-    std::uint32_t *status = lookback + tile * kRadix + d;
-    std::uint32_t exclusive = 0;
-    if (tile == 0)
-        stRel(status, kFlagPre | tileCount);
+        // ---- per digit: wave offsets, tile count, tile-local digit start
+        std::uint32_t tileCount = 0;
+        if (tid < kRadix)
+        {
+#pragma unroll
+            for (int w = 0; w < kW; ++w)
+            {
+                const std::uint32_t c = waveCount[w][tid];
+                waveCount[w][tid] = std::uint16_t(tileCount);
+                tileCount += c;
+            }
+        }
+        const std::uint32_t tileStart = digitExclusiveScan(tileCount, scratch);
+        if (tid < kRadix)
+        {
+#pragma unroll
+            for (int w = 0; w < kW; ++w) waveCount[w][tid] = std::uint16_t(waveCount[w][tid] + tileStart);
+            adjust[tid] = running[tid] - tileStart;
+        }
+        __syncthreads();
+
+        // ---- stage keys in output order, then write runs of equal digits
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            rank[j] += waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)]; // local position
+            stage.keys[rank[j]] = key[j];
+        }
+        __syncthreads();
+        const std::size_t remaining = rEnd - base;
+        const int valid = remaining < std::size_t(kTile) ? int(remaining) : kTile;
+        std::uint32_t dst[ITEMS];
+#pragma unroll
+        for (int q = 0; q < ITEMS; ++q)
+        {
+            const int i = q * THREADS + tid;
+            if (i < valid)
+            {
+                const Key k = stage.keys[i];
+                dst[q] = adjust[unsigned(k >> shift) & (kRadix - 1)] + std::uint32_t(i);
+                kout[dst[q]] = k;
+                if (kDigits) digitsOut[dst[q]] = std::uint8_t(k >> nextShift);
+            }
+        }
+        if (kVals)
+        {
+            std::uint32_t val[ITEMS];
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+            {
+                const std::size_t idx = waveBase + std::size_t(j) * 64 + lane;
+                val[j] = kIota ? std::uint32_t(idx) : (idx < rEnd ? vin[idx] : 0u);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j) stage.vals[rank[j]] = val[j];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < ITEMS; ++q)
+            {
+                const int i = q * THREADS + tid;
+                if (i < valid) vout[dst[q]] = stage.vals[i];
+            }
+        }
+        if (tid < kRadix) running[tid] += tileCount;
+        __syncthreads();
+    }
+}
+
+template <int THREADS, int ITEMS>
+void launchScatter(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, const Key *kin, Key *kout,
+                   const std::uint32_t *vin, std::uint32_t *vout, bool vals, bool iota, bool digits, int shift)
+{
+    const dim3 grid(unsigned(pl.ranges)), block(THREADS);
+    const int next = shift + kRadixBits;
+#define AT_SCATTER(V, I, D)                                                                                         \
+    hipLaunchKernelGGL((scatterKernel<THREADS, ITEMS, V, I, D>), grid, block, 0, s, kin, kout, vin, vout, ws.digits, \
+                       pl.n, pl.rangeKeys, ws.rangeOff, ws.chunkOff, ws.digitBase, shift, next)
+    if (!vals)
+    {
+        if (digits) AT_SCATTER(false, false, true);
+        else AT_SCATTER(false, false, false);
+    }
+    else if (iota)
+    {
+        if (digits) AT_SCATTER(true, true, true);
+        else AT_SCATTER(true, true, false);
+    }
     else
     {
-        stRel(status, kFlagAgg | tileCount);
-        std::size_t j = tile - 1;
-        for (;;)
-        {
-            const std::uint32_t v = ldAcq(lookback + j * kRadix + d);
-            const std::uint32_t flag = v & ~kValMask;
-            if (flag == 0)
-            {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            exclusive += v & kValMask;
-            if (flag == kFlagPre) break;
-            --j;
-        }
-        stRel(status, kFlagPre | (exclusive + tileCount));
+        if (digits) AT_SCATTER(true, false, true);
+        else AT_SCATTER(true, false, false);
     }
-
-    // ---- global digit base (exclusive scan of this pass's histogram) and the
-    //      tile-local digit start (exclusive scan of the tile counts)
-    std::uint32_t total;
-    const std::uint32_t digitBase = blockExclusiveScan(hist[d], scratch, total);
-    const std::uint32_t tileStart = blockExclusiveScan(tileCount, scratch + kWaves, total);
-    adjust[d] = digitBase + exclusive - tileStart;
-    // waveCount now holds wave offsets; fold the tile start in
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) waveCount[w][d] += tileStart;
-    __syncthreads();
-
-    // ---- stage keys in tile order of the output
-    std::uint32_t local[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; ++j)
-    {
-        local[j] = waveCount[wave][digit[j]] + rank[j];
-        stage.keys[local[j]] = key[j];
-    }
-    __syncthreads();
-    const std::size_t remaining = n - base;
-    const int valid = remaining < std::size_t(kTile) ? int(remaining) : kTile;
-    std::uint32_t dst[ITEMS];
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r)
-    {
-        const int i = r * kSortThreads + tid;
-        if (i < valid)
-        {
-            const Key k = stage.keys[i];
-            const unsigned dg = unsigned(k >> shift) & (kRadix - 1);
-            dst[r] = adjust[dg] + std::uint32_t(i);
-            kout[dst[r]] = k;
-        }
-    }
-    if (kVals)
-    {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) stage.vals[local[j]] = val[j];
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < ITEMS; ++r)
-        {
-            const int i = r * kSortThreads + tid;
-            if (i < valid) vout[dst[r]] = stage.vals[i];
-        }
-    }
+#undef AT_SCATTER
 }
 
-// Digit histograms of all passes from keys in memory; also clears the first
-// pass's chained-scan status rows.
-__global__ __launch_bounds__(256) void histogramKernel(const Key *__restrict__ keys, std::size_t n,
-                                                       std::uint32_t *hist, std::uint32_t *status0,
-                                                       std::size_t statusWords)
+void scanOffsets(hipStream_t s, const SortPlan &pl, SortWorkspace &ws)
 {
-    __shared__ std::uint32_t h[kSortPasses][kRadix];
-    for (int i = threadIdx.x; i < kSortPasses * kRadix; i += blockDim.x) (&h[0][0])[i] = 0;
-    __syncthreads();
-    const std::size_t stride = std::size_t(gridDim.x) * blockDim.x;
-    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    {
-        const Key k = keys[i];
-#pragma unroll
-        for (int p = 0; p < kSortPasses; ++p) atomicAdd(&h[p][(k >> (p * kRadixBits)) & (kRadix - 1)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kSortPasses * kRadix; i += blockDim.x)
-    {
-        const std::uint32_t c = (&h[0][0])[i];
-        if (c) atomicAdd(&hist[i], c);
-    }
-    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < statusWords; i += stride)
-        status0[i] = 0;
+    hipLaunchKernelGGL(scanChunksKernel, dim3(unsigned(pl.chunks)), dim3(kSortThreads), 0, s, ws.rangeHist, pl.ranges,
+                       pl.parts, ws.rangeOff, ws.chunkOff);
+    hipLaunchKernelGGL(scanBaseKernel, dim3(1), dim3(kSortThreads), 0, s, ws.chunkOff, pl.chunks, ws.digitBase);
 }
 
-template <class T, KeyKind K>
-__global__ __launch_bounds__(256) void keygenKernel(KeygenParams<T> p, std::size_t n, Key *__restrict__ keys,
-                                                    std::uint32_t *hist, std::uint32_t *status0,
-                                                    std::size_t statusWords)
+// Passes 0..7 given pass 0's range histograms in ws.rangeHist. Keys ping-pong
+// between kA (input, and output after the even number of passes) and kB.
+void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Key *kB, std::uint32_t *vA,
+               std::uint32_t *vB, bool vals, bool iota, bool firstHistWhole)
 {
-    __shared__ std::uint32_t h[kSortPasses][kRadix];
-    for (int i = threadIdx.x; i < kSortPasses * kRadix; i += blockDim.x) (&h[0][0])[i] = 0;
-    __syncthreads();
-    const std::size_t stride = std::size_t(gridDim.x) * blockDim.x;
-    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    {
-        const T x = p.x[i], y = p.y[i], z = p.z[i];
-        Key k;
-        if (K == KeyKind::Binary)
-            k = binaryKey<T>(x, y, z, p.geom);
-        else
-            k = isEscaped<T>(x, y, z) ? kInvalidKey : gridKey<T>(x, y, z, p.grid, K == KeyKind::Hilbert);
-        keys[i] = k;
-#pragma unroll
-        for (int q = 0; q < kSortPasses; ++q) atomicAdd(&h[q][(k >> (q * kRadixBits)) & (kRadix - 1)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kSortPasses * kRadix; i += blockDim.x)
-    {
-        const std::uint32_t c = (&h[0][0])[i];
-        if (c) atomicAdd(&hist[i], c);
-    }
-    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < statusWords; i += stride)
-        status0[i] = 0;
-}
-
-__global__ void clearKernel(std::uint32_t *a, std::size_t n)
-{
-    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += std::size_t(gridDim.x) * blockDim.x)
-        a[i] = 0;
-}
-
-unsigned histogramBlocks(std::size_t n)
-{
-    const unsigned want = ceilDiv(n, 256 * 8);
-    return want < 1 ? 1 : (want > 2048 ? 2048 : want);
-}
-
-template <int ITEMS>
-void runPasses(hipStream_t s, Key *kA, Key *kB, std::uint32_t *vA, std::uint32_t *vB, bool vals, bool iota,
-               std::size_t n, SortWorkspace &ws)
-{
-    constexpr std::size_t tileKeys = std::size_t(ITEMS) * kSortThreads;
-    const std::size_t tiles = (n + tileKeys - 1) / tileKeys;
     for (int pass = 0; pass < kSortPasses; ++pass)
     {
         const Key *src = (pass & 1) ? kB : kA;
         Key *dst = (pass & 1) ? kA : kB;
         const std::uint32_t *vsrc = (pass & 1) ? vB : vA;
         std::uint32_t *vdst = (pass & 1) ? vA : vB;
-        std::uint32_t *lb = ws.lookback + std::size_t(pass & 1) * ws.maxTiles * kRadix;
-        std::uint32_t *lbNext = ws.lookback + std::size_t((pass + 1) & 1) * ws.maxTiles * kRadix;
-        const std::uint32_t *hist = ws.hist + pass * kRadix;
-        std::uint32_t *ticket = ws.tickets + pass;
         const int shift = pass * kRadixBits;
-        if (!vals)
-            hipLaunchKernelGGL((onesweepKernel<ITEMS, false, false>), dim3(unsigned(tiles)), dim3(kSortThreads), 0,
-                               s, src, dst, nullptr, nullptr, n, hist, lb, lbNext, ticket, shift);
-        else if (iota && pass == 0)
-            hipLaunchKernelGGL((onesweepKernel<ITEMS, true, true>), dim3(unsigned(tiles)), dim3(kSortThreads), 0, s,
-                               src, dst, nullptr, vdst, n, hist, lb, lbNext, ticket, shift);
-        else
-            hipLaunchKernelGGL((onesweepKernel<ITEMS, true, false>), dim3(unsigned(tiles)), dim3(kSortThreads), 0,
-                               s, src, dst, vsrc, vdst, n, hist, lb, lbNext, ticket, shift);
+        if (pass > 0)
+        {
+            prof::Scope scope("sort_upsweep", s);
+            hipLaunchKernelGGL(upsweepDigitsKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s,
+                               ws.digits, pl.n, pl.rangeKeys, 1, ws.rangeHist);
+        }
+        // key generation writes `parts` partial histograms per range;
+        // every other histogram has one row per range
+        SortPlan scanPlan = pl;
+        if (pass > 0 || firstHistWhole) scanPlan.parts = 1;
+        scanOffsets(s, scanPlan, ws);
+        const bool digits = pass + 1 < kSortPasses;
+        prof::Scope scope("sort_pass", s);
+        const bool first = iota && pass == 0;
+        switch (pl.threads * 100 + pl.items)
+        {
+        case 25608: launchScatter<256, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 25616: launchScatter<256, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 51208: launchScatter<512, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 51216: launchScatter<512, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 102408: launchScatter<1024, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        default: launchScatter<1024, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        }
     }
     AT_HIP(hipGetLastError());
-}
-
-void dispatchPasses(hipStream_t s, Key *kA, Key *kB, std::uint32_t *vA, std::uint32_t *vB, bool vals, bool iota,
-                    std::size_t n, SortWorkspace &ws)
-{
-    switch (g_items)
-    {
-    case 8: runPasses<8>(s, kA, kB, vA, vB, vals, iota, n, ws); break;
-    case 12: runPasses<12>(s, kA, kB, vA, vB, vals, iota, n, ws); break;
-    default: runPasses<16>(s, kA, kB, vA, vB, vals, iota, n, ws); break;
-    }
-}
-
-void clearCounters(hipStream_t s, SortWorkspace &ws)
-{
-    hipLaunchKernelGGL(clearKernel, dim3(1), dim3(256), 0, s, ws.hist, std::size_t(kSortPasses * kRadix));
-    hipLaunchKernelGGL(clearKernel, dim3(1), dim3(64), 0, s, ws.tickets, std::size_t(kSortPasses + 1));
 }
 } // namespace
 
 int sortItemsPerThread() { return g_items; }
 void setSortItemsPerThread(int items)
 {
-    if (items != 8 && items != 12 && items != 16) throw Error(1, "sort items per thread must be 8, 12 or 16");
+    if (items != 8 && items != 16) throw Error(1, "sort items per thread must be 8 or 16");
     g_items = items;
 }
-
-std::size_t sortTiles(std::size_t n)
+int sortThreads() { return g_threads; }
+void setSortThreads(int threads)
 {
-    const std::size_t tile = std::size_t(8) * kSortThreads; // smallest tile: upper bound on the tile count
-    return (n + tile - 1) / tile + 1;
+    if (threads != 256 && threads != 512 && threads != 1024) throw Error(1, "sort threads must be 256, 512 or 1024");
+    g_threads = threads;
+}
+int sortTargetRanges() { return g_targetRanges; }
+void setSortTargetRanges(int ranges)
+{
+    if (ranges < 64 || ranges > 65536) throw Error(1, "sort target ranges must be in [64, 65536]");
+    g_targetRanges = ranges;
+}
+
+SortPlan makeSortPlan(std::size_t n)
+{
+    SortPlan p;
+    p.n = n;
+    p.items = g_items;
+    p.threads = g_threads;
+    p.tileKeys = std::size_t(p.items) * std::size_t(p.threads);
+    const std::size_t tiles = (n + p.tileKeys - 1) / p.tileKeys;
+    p.rangeTiles = (tiles + std::size_t(g_targetRanges) - 1) / std::size_t(g_targetRanges);
+    if (p.rangeTiles < 1) p.rangeTiles = 1;
+    p.rangeKeys = p.rangeTiles * p.tileKeys;
+    p.ranges = (n + p.rangeKeys - 1) / p.rangeKeys;
+    if (p.ranges < 1) p.ranges = 1;
+    p.chunks = (p.ranges + kChunk - 1) / kChunk;
+    // enough histogram blocks to fill the GPU: parts per range, each a
+    // multiple of 4 keys
+    p.parts = 1;
+    while (p.ranges * std::size_t(p.parts) < kHistogramBlocks && p.parts < 16 &&
+           (p.rangeKeys / std::size_t(p.parts * 2)) % 4 == 0 && p.rangeKeys / std::size_t(p.parts * 2) >= 1024)
+        p.parts *= 2;
+    return p;
 }
 
 void SortWorkspace::allocate(std::size_t cap, bool withValues)
@@ -346,24 +503,29 @@ void SortWorkspace::allocate(std::size_t cap, bool withValues)
     if (cap <= capacity && (!withValues || values)) return;
     release();
     capacity = cap < 1 ? 1 : cap;
-    maxTiles = sortTiles(capacity);
+    // the smallest tile gives the largest range count for any tuning
+    maxRanges = (capacity + 8 * kSortThreads - 1) / (8 * kSortThreads);
+    if (maxRanges < 1) maxRanges = 1;
+    const std::size_t histRows = maxRanges > kHistogramBlocks * 2 ? maxRanges : kHistogramBlocks * 2;
+    const std::size_t maxChunks = (maxRanges + kChunk - 1) / kChunk;
     values = withValues;
     AT_HIP(hipMalloc(&keysAlt, capacity * sizeof(Key)));
     if (withValues) AT_HIP(hipMalloc(&valsAlt, capacity * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&hist, kSortPasses * kRadix * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&lookback, 2 * maxTiles * kRadix * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&tickets, (kSortPasses + 1) * sizeof(std::uint32_t)));
+    AT_HIP(hipMalloc(&digits, (capacity + 16) * sizeof(std::uint8_t)));
+    AT_HIP(hipMalloc(&rangeHist, histRows * kRadix * sizeof(std::uint32_t)));
+    AT_HIP(hipMalloc(&rangeOff, maxRanges * kRadix * sizeof(std::uint32_t)));
+    AT_HIP(hipMalloc(&chunkOff, maxChunks * kRadix * sizeof(std::uint32_t)));
+    AT_HIP(hipMalloc(&digitBase, kRadix * sizeof(std::uint32_t)));
 }
 
 void SortWorkspace::release()
 {
-    if (keysAlt) (void)hipFree(keysAlt);
-    if (valsAlt) (void)hipFree(valsAlt);
-    if (hist) (void)hipFree(hist);
-    if (lookback) (void)hipFree(lookback);
-    if (tickets) (void)hipFree(tickets);
-    keysAlt = nullptr; valsAlt = nullptr; hist = nullptr; lookback = nullptr; tickets = nullptr;
-    capacity = 0; maxTiles = 0; values = false;
+    for (void *p : {(void *)keysAlt, (void *)valsAlt, (void *)digits, (void *)rangeHist, (void *)rangeOff,
+                    (void *)chunkOff, (void *)digitBase})
+        if (p) (void)hipFree(p);
+    keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
+    chunkOff = nullptr; digitBase = nullptr;
+    capacity = 0; maxRanges = 0; values = false;
 }
 
 template <class T>
@@ -371,52 +533,51 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
                    std::uint32_t *valsOut, SortWorkspace &ws)
 {
     if (n == 0) return;
-    if (n >= (std::size_t(1) << 30)) throw Error(3, "radix sort supports fewer than 2^30 keys per tree");
+    if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys per tree");
     ws.allocate(n, true);
-    clearCounters(s, ws);
-    const std::size_t statusWords = ws.maxTiles * kRadix;
-    const unsigned blocks = histogramBlocks(n);
-    switch (kind)
+    const SortPlan pl = makeSortPlan(n);
     {
-    case KeyKind::Morton:
-        hipLaunchKernelGGL((keygenKernel<T, KeyKind::Morton>), dim3(blocks), dim3(256), 0, s, p, n, keysOut,
-                           ws.hist, ws.lookback, statusWords);
-        break;
-    case KeyKind::Hilbert:
-        hipLaunchKernelGGL((keygenKernel<T, KeyKind::Hilbert>), dim3(blocks), dim3(256), 0, s, p, n, keysOut,
-                           ws.hist, ws.lookback, statusWords);
-        break;
-    case KeyKind::Binary:
-        hipLaunchKernelGGL((keygenKernel<T, KeyKind::Binary>), dim3(blocks), dim3(256), 0, s, p, n, keysOut,
-                           ws.hist, ws.lookback, statusWords);
-        break;
+        prof::Scope scope("keygen", s);
+        const dim3 grid(unsigned(pl.ranges * pl.parts)), block(kSortThreads);
+        switch (kind)
+        {
+        case KeyKind::Morton:
+            hipLaunchKernelGGL((keygenKernel<T, KeyKind::Morton>), grid, block, 0, s, p, n, pl.rangeKeys, pl.parts,
+                               keysOut, ws.rangeHist);
+            break;
+        case KeyKind::Hilbert:
+            hipLaunchKernelGGL((keygenKernel<T, KeyKind::Hilbert>), grid, block, 0, s, p, n, pl.rangeKeys, pl.parts,
+                               keysOut, ws.rangeHist);
+            break;
+        case KeyKind::Binary:
+            hipLaunchKernelGGL((keygenKernel<T, KeyKind::Binary>), grid, block, 0, s, p, n, pl.rangeKeys, pl.parts,
+                               keysOut, ws.rangeHist);
+            break;
+        }
     }
-    // an even number of ping-pong passes starting from keysOut ends in keysOut;
-    // pass 0 generates the slot indices as values
-    dispatchPasses(s, keysOut, ws.keysAlt, valsOut, ws.valsAlt, true, true, n, ws);
+    runPasses(s, pl, ws, keysOut, ws.keysAlt, valsOut, ws.valsAlt, true, true, false);
 }
 
 void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws, bool iota)
 {
     if (n == 0) return;
-    if (n >= (std::size_t(1) << 30)) throw Error(3, "radix sort supports fewer than 2^30 keys");
+    if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
     ws.allocate(n, true);
-    clearCounters(s, ws);
-    hipLaunchKernelGGL(histogramKernel, dim3(histogramBlocks(n)), dim3(256), 0, s, keys, n, ws.hist, ws.lookback,
-                       ws.maxTiles * kRadix);
-    // ping-pong (keys, keysAlt): pass 0 keys -> alt, pass 7 alt -> keys
-    dispatchPasses(s, keys, ws.keysAlt, vals, ws.valsAlt, true, iota, n, ws);
+    const SortPlan pl = makeSortPlan(n);
+    hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
+                       0, ws.rangeHist);
+    runPasses(s, pl, ws, keys, ws.keysAlt, vals, ws.valsAlt, true, iota, true);
 }
 
 void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws)
 {
     if (n == 0) return;
-    if (n >= (std::size_t(1) << 30)) throw Error(3, "radix sort supports fewer than 2^30 keys");
+    if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
     ws.allocate(n, false);
-    clearCounters(s, ws);
-    hipLaunchKernelGGL(histogramKernel, dim3(histogramBlocks(n)), dim3(256), 0, s, keys, n, ws.hist, ws.lookback,
-                       ws.maxTiles * kRadix);
-    dispatchPasses(s, keys, ws.keysAlt, nullptr, nullptr, false, false, n, ws);
+    const SortPlan pl = makeSortPlan(n);
+    hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
+                       0, ws.rangeHist);
+    runPasses(s, pl, ws, keys, ws.keysAlt, nullptr, nullptr, false, false, true);
 }
 
 template void keygenAndSort<float>(hipStream_t, KeyKind, const KeygenParams<float> &, std::size_t, Key *,

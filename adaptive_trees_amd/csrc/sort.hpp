@@ -11,20 +11,43 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortPasses = 64 / kRadixBits;
 constexpr int kSortThreads = 256;
 
-// Tile shape chosen by the autotuner (items per thread); 16 by default.
+// Tile shape (threads per block, keys per thread) and target number of
+// ranges, chosen by the autotuner; defaults 512, 16 and 1024.
 int sortItemsPerThread();
 void setSortItemsPerThread(int items);
+int sortTargetRanges();
+void setSortTargetRanges(int ranges);
+int sortThreads();
+void setSortThreads(int threads);
+
+// Partition of n keys into contiguous ranges of whole tiles; each scatter
+// block owns one range and walks it tile by tile.
+struct SortPlan
+{
+    std::size_t n = 0;
+    int items = 16;
+    int threads = 256;
+    std::size_t tileKeys = 0;
+    std::size_t rangeTiles = 1;
+    std::size_t rangeKeys = 0;
+    std::size_t ranges = 0;
+    std::size_t chunks = 0; // groups of 64 ranges for the two-level offset scan
+    int parts = 1;          // histogram blocks per range
+};
+SortPlan makeSortPlan(std::size_t n);
 
 // Device scratch of one sort instance, sized for `capacity` keys.
 struct SortWorkspace
 {
-    Key *keysAlt = nullptr;        // ping-pong key buffer [capacity]
-    std::uint32_t *valsAlt = nullptr;
-    std::uint32_t *hist = nullptr; // [kSortPasses][kRadix] digit counts
-    std::uint32_t *lookback = nullptr; // [2][maxTiles][kRadix] chained-scan status
-    std::uint32_t *tickets = nullptr;  // [kSortPasses + 1] tile tickets
+    Key *keysAlt = nullptr;            // ping-pong key buffer [capacity]
+    std::uint32_t *valsAlt = nullptr;  // ping-pong value buffer
+    std::uint8_t *digits = nullptr;    // next-pass digit per key [capacity]
+    std::uint32_t *rangeHist = nullptr; // [maxRanges][kRadix] digit counts per range
+    std::uint32_t *rangeOff = nullptr;  // exclusive offsets inside a chunk
+    std::uint32_t *chunkOff = nullptr;  // [maxChunks][kRadix]
+    std::uint32_t *digitBase = nullptr; // [kRadix]
     std::size_t capacity = 0;
-    std::size_t maxTiles = 0;
+    std::size_t maxRanges = 0;
     bool values = false;
 
     void allocate(std::size_t cap, bool withValues);
@@ -32,9 +55,7 @@ struct SortWorkspace
     ~SortWorkspace() { release(); }
 };
 
-std::size_t sortTiles(std::size_t n);
-
-// Key generation kinds fused with the digit histogram.
+// Key generation kinds fused with the first pass's range histograms.
 enum class KeyKind { Morton, Hilbert, Binary };
 
 template <class T>
@@ -45,15 +66,15 @@ struct KeygenParams
     BinaryGeom geom;   // binary keys
 };
 
-// Writes keys of n particles into ws.keysAlt (perm is implicit: slot index)
-// and builds all digit histograms; then sorts into (keysOut, valsOut) with
-// valsOut[p] = original slot. Stable: equal keys keep ascending slots.
+// Writes keys of n particles into keysOut (values are the slot indices) and
+// sorts (keysOut, valsOut) in place with valsOut[p] = original slot. Stable:
+// equal keys keep ascending slots.
 template <class T>
 void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::size_t n,
                    Key *keysOut, std::uint32_t *valsOut, SortWorkspace &ws);
 
-// Stable sort of existing keys (and optional values) in place in keys/vals.
-// valsIn == nullptr: values become the input positions 0..n-1.
+// Stable sort of existing keys (and values) in place. valuesAreIndices:
+// values become the input positions 0..n-1 (vals is output only).
 void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws,
                bool valuesAreIndices);
 void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws);

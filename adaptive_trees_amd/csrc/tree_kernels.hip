@@ -16,6 +16,10 @@
 //               dirty sibling groups from a compacted work list.
 #include "tree_kernels.hpp"
 
+#include "profile.hpp"
+
+#include <algorithm>
+
 namespace at
 {
 namespace
@@ -544,42 +548,19 @@ __global__ __launch_bounds__(kThreads) void nfComputeKernel(TreeDev t, unsigned 
 
 // ---- one-pass converged binary tree ---------------------------------------------
 // Internal nodes are exactly the nodes that contain a window of bucket+1
-// consecutive sorted particles; a node is counted once, at its first
-// particle j, for the depths d with cpl(P[j-1],P[j]) < d <= D_j, where
-// D_j = min(cpl(P[j], P[j+bucket]), maxDepth-1).
+// consecutive sorted particles (count > bucket) above depth maxDepth. With
+// D_j = min(cpl(P[j], P[j+bucket]), maxDepth-1) for each window j, the
+// deepest internal node holding particle q sits at depth
+//   delta(q) = max D_j over the windows j in [q-bucket, q].
+// The leaf boundaries are {0} and the midpoints of the internal nodes; those
+// lying in (P[q-1], P[q]] are, in ascending order,
+//   (a) internal ancestors of P[q-1] below the common ancestor where P[q-1]
+//       lies in the lower half, deepest first,
+//   (b) the internal common ancestor of P[q-1] and P[q],
+//   (c) internal ancestors of P[q] below the common ancestor where P[q] lies
+//       in the upper half, shallowest first,
+// and every boundary written for q has lower_bound(P, boundary) == q.
 __device__ inline int cpl63(Key a, Key b) { return clz64(a ^ b) - 1; } // 63 when equal
-
-__global__ __launch_bounds__(kThreads) void binWindowKernel(const Key *P, const DevState *st, unsigned bucket,
-                                                            unsigned maxDepth, std::int8_t *depth,
-                                                            unsigned long long *total)
-{
-    __shared__ unsigned long long part[kThreads / 64];
-    const std::size_t nact = st->numActiveParticles;
-    const std::size_t j = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    unsigned long long local = 0;
-    if (j < nact)
-    {
-        int D = -1;
-        if (maxDepth > 0 && j + std::size_t(bucket) < nact)
-        {
-            const int c = cpl63(P[j], P[j + bucket]);
-            D = c < int(maxDepth) - 1 ? c : int(maxDepth) - 1;
-        }
-        depth[j] = std::int8_t(D);
-        const int cj = j > 0 ? cpl63(P[j - 1], P[j]) : -1;
-        local = D > cj ? (unsigned long long)(D - cj) : 0ull;
-    }
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) local += __shfl_xor(local, m, 64);
-    if (__lane_id() == 0) part[threadIdx.x >> 6] = local;
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        unsigned long long s = 0;
-        for (int w = 0; w < kThreads / 64; ++w) s += part[w];
-        if (s) atomicAdd(total, s);
-    }
-}
 
 __device__ inline Key bitsBetween(int lo, int hi) // bits lo..hi inclusive, 0 <= lo, hi <= 62
 {
@@ -588,53 +569,58 @@ __device__ inline Key bitsBetween(int lo, int hi) // bits lo..hi inclusive, 0 <=
     return top & ~((Key(1) << lo) - 1);
 }
 
-// Leaf boundaries between sorted particles P[q-1] and P[q] are the
-// midpoints of (a) internal ancestors of P[q-1] where it lies in the lower
-// half and P[q] is outside, deepest first, (b) the internal common ancestor,
-// (c) internal ancestors of P[q] where it lies in the upper half and P[q-1]
-// is outside, shallowest first. delta(q) is the depth of the deepest
-// internal node holding P[q]: the maximum of D_j over the windows j in
-// [q-bucket, q], computed per tile with a sparse table in LDS.
-__global__ __launch_bounds__(kThreads) void binMidsKernel(const Key *P, const DevState *st, const std::int8_t *depth,
-                                                          unsigned bucket, Key *K, std::uint32_t *bound, long leaves,
-                                                          std::uint64_t *status, std::uint32_t *ticket,
-                                                          unsigned long long *check)
+constexpr int kQ = kPassTile; // particles per tile of the binary construction
+
+// Per tile of kQ particles: delta(q) via a sparse table over the D window in
+// LDS, the number of boundaries of each interval q, and the tile's total.
+__global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const DevState *st, unsigned bucket,
+                                                           unsigned maxDepth, std::int8_t *delta, std::uint8_t *cnt,
+                                                           std::uint32_t *tileSum)
 {
-    constexpr int Q = kPassTile;
-    constexpr int kMaxSpan = Q + int(kDirectMaxBucket) + 2;
-    __shared__ std::int8_t tab[2][kMaxSpan];
-    __shared__ std::int8_t dl[Q + 1];
-    __shared__ std::uint32_t tileShared;
-    __shared__ std::uint64_t wsum[kThreads / 64];
-    __shared__ int wflag[kThreads / 64];
-    __shared__ std::uint64_t prefixShared;
+    constexpr int kMaxP = kQ + 2 * int(kDirectMaxBucket) + 2;
+    constexpr int kMaxD = kQ + int(kDirectMaxBucket) + 1;
+    __shared__ Key pw[kMaxP];
+    __shared__ int tab[2][kMaxD];
+    __shared__ int dl[kQ + 1];
+    __shared__ std::uint32_t part[kThreads / 64];
     const int tid = threadIdx.x;
-    if (tid == 0) tileShared = atomicAdd(ticket, 1u);
-    __syncthreads();
     const long nact = long(st->numActiveParticles);
-    const long q0 = long(tileShared) * Q;
+    const long q0 = long(blockIdx.x) * kQ;
     if (q0 > nact) return;
     const int B = int(bucket);
-    const int span = Q + B + 2;
-    const long origin = q0 - 1 - B;
-    for (int s = tid; s < span; s += kThreads)
+    const long origin = q0 - 1 - B; // particle index of pw[0] and tab[.][0]
+    const int spanP = kQ + 2 * B + 2;
+    const int spanD = kQ + B + 1;
+    for (int s = tid; s < spanP; s += kThreads)
     {
         const long j = origin + s;
-        tab[0][s] = (j < 0 || j >= nact) ? std::int8_t(-1) : depth[j];
+        pw[s] = (j >= 0 && j < nact) ? P[j] : Key(0);
+    }
+    __syncthreads();
+    for (int s = tid; s < spanD; s += kThreads)
+    {
+        const long j = origin + s;
+        int D = -1;
+        if (maxDepth > 0 && j >= 0 && j + B < nact)
+        {
+            const int c = cpl63(pw[s], pw[s + B]);
+            D = c < int(maxDepth) - 1 ? c : int(maxDepth) - 1;
+        }
+        tab[0][s] = D;
     }
     __syncthreads();
     int lg = 0;
-    while ((2 << lg) <= B + 1) ++lg; // floor(log2(B + 1))
+    while ((2 << lg) <= B + 1) ++lg; // floor(log2(bucket + 1))
     int cur = 0;
     for (int t = 0; t < lg; ++t)
     {
         const int step = 1 << t;
-        for (int s = tid; s < span; s += kThreads)
+        for (int s = tid; s < spanD; s += kThreads)
         {
-            std::int8_t v = tab[cur][s];
-            if (s + step < span)
+            int v = tab[cur][s];
+            if (s + step < spanD)
             {
-                const std::int8_t w = tab[cur][s + step];
+                const int w = tab[cur][s + step];
                 v = w > v ? w : v;
             }
             tab[cur ^ 1][s] = v;
@@ -642,67 +628,128 @@ __global__ __launch_bounds__(kThreads) void binMidsKernel(const Key *P, const De
         __syncthreads();
         cur ^= 1;
     }
-    const int pw = 1 << lg;
-    for (int u = tid; u <= Q; u += kThreads)
+    const int pw2 = 1 << lg;
+    // dl[u] = delta(q0 - 1 + u): windows [q-B, q] are tab indices [u, u+B]
+    for (int u = tid; u <= kQ; u += kThreads)
     {
         const long q = q0 - 1 + u;
-        std::int8_t d = -1;
+        int d = -1;
         if (q >= 0 && q < nact)
         {
-            const std::int8_t a = tab[cur][u], b = tab[cur][u + B - pw + 1];
+            const int a = tab[cur][u], b = tab[cur][u + B - pw2 + 1];
             d = a > b ? a : b;
         }
         dl[u] = d;
     }
     __syncthreads();
+    std::uint32_t sum = 0;
+    for (int u = tid; u < kQ; u += kThreads)
+    {
+        const long q = q0 + u;
+        if (q > nact) break;
+        const bool hp = q > 0, hc = q < nact;
+        const Key prev = pw[u + B], curk = pw[u + B + 1]; // P[q-1], P[q]
+        const int dp = hp ? dl[u] : -1;
+        const int dc = hc ? dl[u + 1] : -1;
+        const int cp = (hp && hc) ? cpl63(prev, curk) : -1;
+        const int lca = (hp && hc && cp <= dc) ? 1 : 0;
+        const Key am = dp > cp ? bitsBetween(62 - dp, 61 - cp) : 0;
+        const Key bm = dc > cp ? bitsBetween(62 - dc, 61 - cp) : 0;
+        const int c = __popcll(~prev & am) + lca + __popcll(curk & bm);
+        cnt[q] = std::uint8_t(c);
+        if (hc) delta[q] = std::int8_t(dc);
+        sum += std::uint32_t(c);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) sum += __shfl_xor(sum, m, 64);
+    if (__lane_id() == 0) part[tid >> 6] = sum;
+    __syncthreads();
+    if (tid == 0)
+    {
+        std::uint32_t s = 0;
+        for (int w = 0; w < kThreads / 64; ++w) s += part[w];
+        tileSum[blockIdx.x] = s;
+    }
+}
 
+// Exclusive scan of the tile sums (one block) and the total boundary count.
+__global__ __launch_bounds__(1024) void binScanTilesKernel(const DevState *st, std::uint32_t *tileSum,
+                                                          unsigned long long *total)
+{
+    __shared__ unsigned long long wsum[16];
+    const long tiles = long(st->numActiveParticles) / kQ + 1;
+    const int tid = threadIdx.x;
+    const long per = (tiles + 1023) / 1024;
+    const long b = long(tid) * per, e = b + per < tiles ? b + per : tiles;
+    unsigned long long local = 0;
+    for (long i = b; i < e; ++i) local += tileSum[i];
+    const unsigned lane = __lane_id();
+    const int wave = tid >> 6;
+    unsigned long long inc = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const unsigned long long t = __shfl_up(inc, o, 64);
+        if (lane >= unsigned(o)) inc += t;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    unsigned long long run = inc - local;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+    for (long i = b; i < e; ++i)
+    {
+        const std::uint32_t v = tileSum[i];
+        tileSum[i] = std::uint32_t(run);
+        run += v;
+    }
+    if (tid == 1023) *total = run;
+}
+
+// Writes the boundaries of each interval q at 1 + (boundaries before q).
+__global__ __launch_bounds__(kThreads) void binEmitKernel(const Key *P, const DevState *st, const std::int8_t *delta,
+                                                          const std::uint8_t *cnt, const std::uint32_t *tileOff,
+                                                          Key *K, std::uint32_t *bound, long leaves)
+{
+    __shared__ std::uint32_t wsum[kThreads / 64];
+    const int tid = threadIdx.x;
+    const long nact = long(st->numActiveParticles);
+    const long q0 = long(blockIdx.x) * kQ;
+    if (q0 > nact) return;
     const long qFirst = q0 + long(tid) * kItems;
-    int cnt[kItems];
-    std::uint64_t sum = 0;
+    std::uint32_t c[kItems];
+    std::uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
     {
         const long q = qFirst + k;
-        int c = 0;
-        if (q <= nact)
-        {
-            const bool hp = q > 0, hc = q < nact;
-            const Key prev = hp ? P[q - 1] : 0, curk = hc ? P[q] : 0;
-            const int dp = hp ? dl[q - q0] : -1;
-            const int dc = hc ? dl[q - q0 + 1] : -1;
-            const int cp = (hp && hc) ? cpl63(prev, curk) : -1;
-            const int lca = (hp && hc && cp <= dc) ? 1 : 0;
-            const Key am = dp > cp ? bitsBetween(62 - dp, 61 - cp) : 0;
-            const Key bm = dc > cp ? bitsBetween(62 - dc, 61 - cp) : 0;
-            c = __popcll(~prev & am) + lca + __popcll(curk & bm);
-        }
-        cnt[k] = c;
-        sum += std::uint64_t(c);
+        c[k] = q <= nact ? cnt[q] : 0u;
+        sum += c[k];
     }
-    std::uint64_t tileSum;
-    bool unused;
-    const std::uint64_t local = blockScan64(sum, false, wsum, wflag, tileSum, unused);
-    if (tid == 0)
+    const unsigned lane = __lane_id();
+    const int wave = tid >> 6;
+    std::uint32_t inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
     {
-        std::uint64_t ex;
-        bool exChg;
-        chainedScan(status, std::size_t(tileShared), tileSum, false, ex, exChg);
-        prefixShared = ex;
+        const std::uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= unsigned(o)) inc += t;
     }
+    if (lane == 63) wsum[wave] = inc;
     __syncthreads();
-    std::uint64_t run = prefixShared + local;
+    std::uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    long pos = 1 + long(tileOff[blockIdx.x]) + long(before + inc - sum);
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
     {
         const long q = qFirst + k;
         if (q > nact) break;
-        long pos = 1 + long(run);
-        if (cnt[k])
+        if (c[k])
         {
             const bool hp = q > 0, hc = q < nact;
             const Key prev = hp ? P[q - 1] : 0, curk = hc ? P[q] : 0;
-            const int dp = hp ? dl[q - q0] : -1;
-            const int dc = hc ? dl[q - q0 + 1] : -1;
+            const int dp = hp ? delta[q - 1] : -1;
+            const int dc = hc ? delta[q] : -1;
             const int cp = (hp && hc) ? cpl63(prev, curk) : -1;
             Key m = (dp > cp ? bitsBetween(62 - dp, 61 - cp) : 0) & ~prev;
             while (m)
@@ -730,17 +777,14 @@ __global__ __launch_bounds__(kThreads) void binMidsKernel(const Key *P, const De
         }
         if (q == nact)
         {
-            const std::uint64_t total = run + std::uint64_t(cnt[k]);
-            *check = total;
             K[0] = 0;
             bound[0] = 0;
-            if (long(total) + 1 == leaves)
+            if (pos == leaves)
             {
                 K[leaves] = kMaxKey;
                 bound[leaves] = std::uint32_t(nact);
             }
         }
-        run += std::uint64_t(cnt[k]);
     }
 }
 
@@ -789,6 +833,7 @@ void updateReset(hipStream_t s, const TreeDev &t)
 void octDecideScan(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves)
 {
     OctDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, {t.NF[0], t.NF[1]}, {t.gcm[0], t.gcm[1]}, p.bucket, p.nf};
+    prof::Scope scope("decide_scan", s);
     hipLaunchKernelGGL((decideScanKernel<OctDecider>), dim3(unsigned(passTiles(gridLeaves))), dim3(kThreads), 0, s,
                        d, t.st, capacity, t.offs, t.scanStatus, t.scanTicket);
 }
@@ -802,9 +847,11 @@ void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, 
 {
     if (!p.nf)
     {
+        prof::Scope scope("leaf_counts", s);
         hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t);
         return;
     }
+    prof::Scope scope("near_field", s);
     hipLaunchKernelGGL(nfCollectKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t, full ? 1 : 0);
     hipLaunchKernelGGL(nfComputeKernel, dim3(2048), dim3(kThreads), 0, s, t, p.bucket, p.hilbert ? 1 : 0);
 }
@@ -818,6 +865,7 @@ void octPass(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity,
 void binDecideScan(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves)
 {
     BinDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, p.bucket, p.maxDepth};
+    prof::Scope scope("decide_scan", s);
     hipLaunchKernelGGL((decideScanKernel<BinDecider>), dim3(unsigned(passTiles(gridLeaves))), dim3(kThreads), 0, s,
                        d, t.st, capacity, t.offs, t.scanStatus, t.scanTicket);
 }
@@ -829,6 +877,7 @@ void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 }
 void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 {
+    prof::Scope scope("leaf_counts", s);
     hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t);
 }
 void binPass(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves)
@@ -840,17 +889,19 @@ void binPass(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity,
 
 void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d)
 {
-    hipLaunchKernelGGL(resetKernel, dim3(clearBlocks(d.statusWords)), dim3(256), 0, s, t.st, t.P, t.n, d.status,
-                       d.statusWords, d.ticket, nullptr);
-    AT_HIP(hipMemsetAsync(d.total, 0, sizeof(unsigned long long), s));
-    if (t.n)
-        hipLaunchKernelGGL(binWindowKernel, dim3(ceilDiv(t.n, kThreads)), dim3(kThreads), 0, s, t.P, t.st, p.bucket,
-                           p.maxDepth, d.depth, d.total);
+    hipLaunchKernelGGL(resetKernel, dim3(1), dim3(64), 0, s, t.st, t.P, t.n, d.status, std::size_t(0), d.ticket,
+                       nullptr);
+    prof::Scope scope("bin_direct", s);
+    const unsigned tiles = unsigned(t.n / kQ + 1);
+    hipLaunchKernelGGL(binCountKernel, dim3(tiles), dim3(kThreads), 0, s, t.P, t.st, p.bucket, p.maxDepth, d.delta,
+                       d.cnt, d.tileSum);
+    hipLaunchKernelGGL(binScanTilesKernel, dim3(1), dim3(1024), 0, s, t.st, d.tileSum, d.total);
 }
 void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves)
 {
-    hipLaunchKernelGGL(binMidsKernel, dim3(unsigned(t.n / kPassTile + 1)), dim3(kThreads), 0, s, t.P, t.st, d.depth,
-                       p.bucket, t.K[0], d.bound, leaves, d.status, d.ticket, d.total);
+    prof::Scope scope("bin_direct", s);
+    hipLaunchKernelGGL(binEmitKernel, dim3(unsigned(t.n / kQ + 1)), dim3(kThreads), 0, s, t.P, t.st, d.delta, d.cnt,
+                       d.tileSum, t.K[0], d.bound, leaves);
     hipLaunchKernelGGL(binBoundsToCountsKernel, dim3(ceilDiv(std::size_t(leaves), kThreads)), dim3(kThreads), 0, s,
                        d.bound, t.N[0], leaves);
 }

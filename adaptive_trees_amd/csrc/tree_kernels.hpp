@@ -64,14 +64,15 @@ void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves);
 // phase 2 writes the sorted leaf boundaries and their particle counts.
 struct BinDirectScratch
 {
-    std::int8_t *depth = nullptr;       // per sorted particle
+    std::int8_t *delta = nullptr;       // per sorted particle: depth of its deepest internal node
+    std::uint8_t *cnt = nullptr;        // per interval q in [0, nact]: boundaries in (P[q-1], P[q]]
+    std::uint32_t *tileSum = nullptr;   // per tile: boundary count, then its exclusive prefix
     std::uint32_t *bound = nullptr;     // per boundary: index of its first particle
     unsigned long long *total = nullptr;
     std::uint64_t *status = nullptr;
     std::uint32_t *ticket = nullptr;
-    std::size_t statusWords = 0;
 };
-constexpr unsigned kDirectMaxBucket = 8192;
+constexpr unsigned kDirectMaxBucket = 1024;
 void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d);
 void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves);
 void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves);

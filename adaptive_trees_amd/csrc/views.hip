@@ -19,6 +19,7 @@
 #include "views.hpp"
 
 #include "buffers.hpp"
+#include "profile.hpp"
 
 namespace at
 {
@@ -467,6 +468,7 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
         return;
     }
     Impl &m = *impl_;
+    prof::Scope scope("views", s);
     const int cap = int(leafCap_);
     const std::size_t icap = (leafCap_ - 1) / ((1u << bits_) - 1);
     const unsigned leafBlocks = ceilDiv(leafCap_, kT);

@@ -1,0 +1,255 @@
+"""Benchmark of the adaptive-tree build/update path on one MI355X.
+
+Workload (the 32M-particle build the headline roofline target is quoted on):
+binary longest-side tree, N = 2^25 FP32 particles in a 4:1:1 box, 40% + 40%
+isotropic Gaussian clusters (sigma = 0.02 of the long side, centres at 1/4
+and 3/4 of x) + 20% uniform background, LeafCount 32.
+
+One step = build() followed by one hipGraph-replayed update() of the same
+resident particles; value = 2 N / step time (particles through the path per
+second). Coordinates are generated on the host, uploaded once, and stay in
+HBM for the whole timed region.
+
+Output: one JSON line (metric, value, roofline of the dominant kernel, CPU
+baseline of the oracle on a bounded sample, per-phase timings).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from adaptive_trees_amd import BINARY, DeviceArray, Tree, make_config  # noqa: E402
+from adaptive_trees_amd import _ffi, workloads  # noqa: E402
+
+METRIC = "Mparticles/s build+update, % HBM roofline; batched vs plain speedup; p99 latency"
+PEAK_HBM_GBS = 8000.0
+SORT_PASS_BYTES = 24  # read + write of one (8-byte key, 4-byte value) pair per pass
+
+
+def _lib():
+    return _ffi.hip_lib()
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _lib().call("stream_create", ctypes.byref(p))
+        self.p = p.value
+
+    def sync(self):
+        _lib().call("stream_synchronize", self.p)
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _lib().call("event_create", ctypes.byref(p))
+        self.p = p.value
+
+    def record(self, stream):
+        _lib().call("event_record", self.p, stream.p)
+
+    def ms_since(self, start):
+        v = ctypes.c_float()
+        _lib().call("event_elapsed_ms", start.p, self.p, ctypes.byref(v))
+        return v.value
+
+
+def profile_query(label):
+    ms, n = ctypes.c_double(), ctypes.c_ulonglong()
+    status = _lib().fn("profile_query")(label.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return (ms.value, n.value) if status == 0 else (0.0, 0)
+
+
+def algorithmic_build_bytes(n, leaves, s=4, passes=8):
+    """Bytes a build must move: keygen N(3s+8) ... per the path's definition
+    (keygen reads 3s and writes 8; each sort pass moves 24 with an 8-byte
+    histogram read; one streaming read of P; K and N written)."""
+    return n * (3 * s + 12) + n * (24 * passes + 8) + 8 * n + 12 * leaves
+
+
+def cpu_baseline(sample):
+    """Oracle build + update of a bounded sample of the same workload."""
+    box = workloads.ANISO_BOX
+    x, y, z = workloads.two_clusters(sample, seed=3, dtype=np.float32)
+    x1, y1, z1 = workloads.reflect_step((x, y, z), 1e-3, seed=33, box=box)
+    hx, hy, hz = x.copy(), y.copy(), z.copy()
+    cfg = make_config(max_particles_per_leaf=32, box=box)
+    t = Tree(hx, hy, hz, sample, cfg, real=np.float32, kind=BINARY, backend="oracle")
+    t0 = time.perf_counter()
+    t.build()
+    hx[:], hy[:], hz[:] = x1, y1, z1
+    t.update()
+    t.doBuffersNeedResize()
+    dt = time.perf_counter() - t0
+    threads = _ffi.oracle_lib().dll.orc_num_threads()
+    return {"value": round(2 * sample / dt / 1e6, 3), "unit": "Mparticles/s", "cores": int(threads),
+            "kind": "port",
+            "sample": "oracle build + one update of %d particles of the same workload "
+                      "(two Gaussian clusters + uniform, 4:1:1 box, LeafCount 32), %.2f s wall" % (sample, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 25)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run the known-size copy kernel (for hardware counter scaling)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (requested %d)" % max(args.gpus, world),
+              file=sys.stderr)
+        sys.exit(2)
+
+    n = args.n
+    box = workloads.ANISO_BOX
+    x, y, z = workloads.two_clusters(n, seed=3, dtype=np.float32)
+    dx, dy, dz = DeviceArray(x), DeviceArray(y), DeviceArray(z)
+    stream = Stream()
+    cfg = make_config(max_particles_per_leaf=32, box=box, stream=stream.p)
+    tree = Tree(dx, dy, dz, n, cfg, real=np.float32, kind=BINARY, backend="hip")
+    tree.build()
+    graph = ctypes.c_void_p()
+    _lib().call("graph_capture_update", tree.handle, stream.p, ctypes.byref(graph))
+    addr = tree.buffer_info("cornerstone_d")[0]
+
+    def step():
+        tree.build()
+        _lib().call("graph_launch", graph.value, stream.p)
+
+    for _ in range(args.warmup):
+        step()
+    stream.sync()
+    assert tree.buffer_info("cornerstone_d")[0] == addr, "build moved the captured buffers"
+
+    # phase timings (outside the timed region)
+    e0, e1, e2 = Event(), Event(), Event()
+    tree.build()
+    e0.record(stream)
+    _lib().call("graph_launch", graph.value, stream.p)
+    e1.record(stream)
+    stream.sync()
+    update_ms = e1.ms_since(e0)
+    t0 = time.perf_counter()
+    tree.build()
+    stream.sync()
+    build_wall_ms = (time.perf_counter() - t0) * 1e3
+
+    _lib().call("profile_reset")
+    _lib().call("profile_enable", 1)
+    stream.sync()
+    e0.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    e2.record(stream)
+    stream.sync()
+    wall = time.perf_counter() - t0
+    _lib().call("profile_enable", 0)
+    ms_per_step = wall * 1e3 / args.steps
+    event_ms = e2.ms_since(e0) / args.steps
+
+    leaves = int(tree.scalars().num_leaves)
+    sort_ms, sort_n = profile_query("sort_pass")
+    key_ms, key_n = profile_query("keygen")
+    dir_ms, dir_n = profile_query("bin_direct")
+    pass_ms = sort_ms / max(sort_n, 1)
+    achieved = SORT_PASS_BYTES * n / (pass_ms * 1e-3) / 1e9 if sort_n else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("sort_pass_bytes_per_launch")
+
+    # an update that rebalances: reflected Gaussian steps (sigma = 1e-3 of the box)
+    x1, y1, z1 = workloads.reflect_step((x, y, z), 1e-3, seed=33, box=box)
+    moved_ms = []
+    for r in range(3):
+        tree.build()
+        for d, a in ((dx, x1), (dy, y1), (dz, z1)):
+            d.upload(a)
+        stream.sync()
+        e0.record(stream)
+        _lib().call("graph_launch", graph.value, stream.p)
+        e1.record(stream)
+        stream.sync()
+        moved_ms.append(e1.ms_since(e0))
+        for d, a in ((dx, x), (dy, y), (dz, z)):
+            d.upload(a)
+    tree.doBuffersNeedResize()
+
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        src, dst = DeviceArray(nbytes), DeviceArray(nbytes)
+        for _ in range(3):
+            _lib().call("bench_copy", dst.ptr, src.ptr, nbytes, stream.p)
+        stream.sync()
+
+    build_bytes = algorithmic_build_bytes(n, leaves)
+    out = {
+        "metric": METRIC,
+        "value": round(2 * n / (ms_per_step * 1e-3) / 1e6, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "binary longest-side tree build() + graph-replayed update(), N=2^25 FP32, "
+                        "4:1:1 box, two Gaussian clusters + 20% uniform, LeafCount 32",
+            "particles": n,
+            "leaves": leaves,
+            "tree": "binary",
+            "bucket": 32,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "onesweep radix pass (8 per sort)",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": SORT_PASS_BYTES * n,
+            "avg_launch_ms": round(pass_ms, 5),
+        },
+        "phases_ms": {
+            "step_event": round(event_ms, 4),
+            "build_wall": round(build_wall_ms, 4),
+            "update_static_graph": round(update_ms, 4),
+            "update_moved_graph": round(float(np.median(moved_ms)), 4),
+            "keygen": round(key_ms / max(key_n, 1), 4),
+            "sort_pass": round(pass_ms, 4),
+            "binary_construction": round(dir_ms / max(dir_n, 1) * 2, 4),
+        },
+        "build_roofline": {
+            "algorithmic_bytes": int(build_bytes),
+            "frac_of_peak": round(build_bytes / (build_wall_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4),
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

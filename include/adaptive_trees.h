@@ -160,6 +160,24 @@ int at_graph_capture_update(at_tree *tree, void *stream, void **graph_exec);
 int at_graph_launch(void *graph_exec, void *stream);
 int at_graph_destroy(void *graph_exec);
 
+/* ---- measurement ------------------------------------------------------ */
+/* Event timing of labelled launches ("keygen", "sort_pass", "decide_scan",
+ * "leaf_counts", "near_field", "bin_direct", "views", "copy") outside graph
+ * capture; query returns the summed milliseconds and the launch count. */
+int at_profile_enable(int on);
+int at_profile_reset(void);
+int at_profile_query(const char *label, double *total_ms, unsigned long long *launches);
+/* Streaming device copy of `bytes` (multiple of 16): the known-traffic
+ * kernel that calibrates hardware byte counters. */
+int at_bench_copy(void *dst, const void *src, size_t bytes, void *stream);
+
+/* ---- tuning (process-wide; set before building trees) ------------------ */
+/* "sort_threads": threads per radix-sort block (256, 512 or 1024);
+ * "sort_items":  keys per thread of a radix-sort tile (8 or 16);
+ * "sort_ranges": target number of key ranges per sort pass (64..65536). */
+int at_tune_set(const char *key, int value);
+int at_tune_get(const char *key, int *value);
+
 #ifdef __cplusplus
 }
 #endif
