@@ -1,8 +1,8 @@
 """ctypes bindings of the C ABI declared in include/adaptive_trees.h.
 
-The same binding describes the HIP library (prefix ``at_``) and, for tests,
-the CPU oracle (prefix ``orc_``), whose tree entry points have identical
-signatures over host pointers.
+``Lib`` binds the tree entry points of a library by prefix; ``hip_lib()``
+loads the HIP engine and raises if it is not built (there is no CPU
+fallback).
 """
 import ctypes
 import os
@@ -10,7 +10,6 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIP_LIB = os.path.join(HERE, "libadaptive_trees_hip.so")
-ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 
 AT_OK, AT_EINVAL, AT_ELOGIC, AT_ELENGTH, AT_EHIP, AT_ENOMEM = range(6)
 OCTREE, BINARY = 0, 1
@@ -121,15 +120,16 @@ class Lib:
             fn = getattr(self.dll, prefix + name)
             fn.argtypes = args
             fn.restype = _I
-        self.dll[prefix + "last_error"].restype = ctypes.c_char_p
-        self.dll[prefix + "last_error"].argtypes = []
+        last = getattr(self.dll, prefix + "last_error")
+        last.restype = ctypes.c_char_p
+        last.argtypes = []
 
     def fn(self, name):
         return getattr(self.dll, self.prefix + name)
 
     def check(self, status):
         if status != AT_OK:
-            msg = self.dll[self.prefix + "last_error"]().decode(errors="replace")
+            msg = (getattr(self.dll, self.prefix + "last_error")() or b"").decode(errors="replace")
             raise _ERRORS.get(status, TreeError)(status, msg)
 
     def call(self, name, *args):
@@ -137,7 +137,6 @@ class Lib:
 
 
 _hip = None
-_oracle = None
 
 
 def hip_lib():
@@ -176,31 +175,3 @@ def hip_lib():
             f.restype = _I
         _hip = lib
     return _hip
-
-
-def oracle_lib():
-    """The CPU oracle (test infrastructure and CPU baseline only)."""
-    global _oracle
-    if _oracle is None:
-        lib = Lib(ORACLE_LIB, "orc_")
-        d = lib.dll
-        d.orc_sfc_keys.argtypes = [_I, _P, _P, _P, _SZ, ctypes.POINTER(Box), _I, _P]
-        d.orc_sfc_keys.restype = _I
-        d.orc_binary_keys.argtypes = [_I, _P, _P, _P, _SZ, ctypes.POINTER(Box), ctypes.c_uint32, _P]
-        d.orc_binary_keys.restype = _I
-        for n in ("orc_morton", "orc_hilbert"):
-            getattr(d, n).argtypes = [ctypes.c_uint32] * 3
-            getattr(d, n).restype = ctypes.c_uint64
-        for n in ("orc_hilbert_decode", "orc_morton_decode"):
-            getattr(d, n).argtypes = [ctypes.c_uint64, _P]
-            getattr(d, n).restype = None
-        d.orc_sort_pairs.argtypes = [_P, _P, _SZ]
-        d.orc_sort_pairs.restype = _I
-        d.orc_near_field.argtypes = [_P, _SZ, ctypes.c_uint64, ctypes.c_uint32, _I, _P]
-        d.orc_near_field.restype = ctypes.c_uint64
-        d.orc_binary_geometry.argtypes = [ctypes.POINTER(Box), ctypes.c_uint32, _P]
-        d.orc_binary_geometry.restype = _I
-        d.orc_num_threads.argtypes = []
-        d.orc_num_threads.restype = _I
-        _oracle = lib
-    return _oracle

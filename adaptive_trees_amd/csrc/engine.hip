@@ -612,7 +612,7 @@ private:
     {
         const std::size_t tiles = n_ / kPassTile + 2;
         direct_.delta.ensure(n_ ? n_ : 1, 0, stream_);
-        direct_.cnt.ensure(n_ + 1, 0, stream_);
+        direct_.cnt.ensure((n_ / kPassTile + 1) * kPassTile, 0, stream_); // whole tiles: 8-byte loads
         direct_.tileSum.ensure(tiles, 0, stream_);
         direct_.total.ensure(1, 0, stream_);
         direct_.ticket.ensure(1, 0, stream_);
@@ -841,6 +841,20 @@ struct at_tree
     std::unique_ptr<at::TreeBase> impl;
 };
 
+namespace
+{
+at::TreeBase &treeOf(at_tree *t)
+{
+    if (!t || !t->impl) throw at::Error(AT_EINVAL, "null tree");
+    return *t->impl;
+}
+template <class... P>
+void needOutputs(P *...p)
+{
+    if (((p == nullptr) || ...)) throw at::Error(AT_EINVAL, "null output pointer");
+}
+} // namespace
+
 extern "C" {
 int at_tree_create(int realBytes, int type, const void *x, const void *y, const void *z, size_t count,
                    const at_config *config, at_tree **out)
@@ -858,11 +872,11 @@ int at_tree_destroy(at_tree *t)
 }
 int at_tree_build(at_tree *t)
 {
-    return guarded([&] { t->impl->build(); });
+    return guarded([&] { treeOf(t).build(); });
 }
 int at_tree_update(at_tree *t)
 {
-    return guarded([&] { t->impl->update(); });
+    return guarded([&] { treeOf(t).update(); });
 }
 int at_tree_compute_views(at_tree *t, const uint8_t *v)
 {
@@ -870,32 +884,42 @@ int at_tree_compute_views(at_tree *t, const uint8_t *v)
         if (v)
         {
             at::ViewFlags f{v[0] != 0, v[1] != 0, v[2] != 0};
-            t->impl->computeViews(&f);
+            treeOf(t).computeViews(&f);
         }
         else
-            t->impl->computeViews(nullptr);
+            treeOf(t).computeViews(nullptr);
     });
 }
 int at_tree_needs_resize(at_tree *t, int *o)
 {
-    return guarded([&] { *o = t->impl->needsResize() ? 1 : 0; });
+    return guarded([&] {
+        needOutputs(o);
+        *o = treeOf(t).needsResize() ? 1 : 0;
+    });
 }
 int at_tree_resize_buffers(at_tree *t)
 {
-    return guarded([&] { t->impl->resizeBuffers(); });
+    return guarded([&] { treeOf(t).resizeBuffers(); });
 }
 int at_tree_buffer(at_tree *t, int id, const void **d, size_t *s, size_t *c)
 {
-    return guarded([&] { t->impl->buffer(id, d, s, c); });
+    return guarded([&] {
+        needOutputs(d, s, c);
+        treeOf(t).buffer(id, d, s, c);
+    });
 }
 int at_tree_scalars(at_tree *t, at_scalars *o)
 {
-    return guarded([&] { t->impl->scalars(o); });
+    return guarded([&] {
+        needOutputs(o);
+        treeOf(t).scalars(o);
+    });
 }
 int at_tree_device_counters(at_tree *t, const int32_t **l, const int32_t **nn, const uint32_t **a)
 {
     return guarded([&] {
-        at::DevState *s = t->impl->deviceState();
+        needOutputs(l, nn, a);
+        at::DevState *s = treeOf(t).deviceState();
         *l = &s->numLeaves;
         *nn = &s->numNodes;
         *a = &s->numActiveParticles;
@@ -903,7 +927,10 @@ int at_tree_device_counters(at_tree *t, const int32_t **l, const int32_t **nn, c
 }
 int at_tree_max_achieved_depth(at_tree *t, uint32_t *d)
 {
-    return guarded([&] { *d = t->impl->maxAchievedDepth(); });
+    return guarded([&] {
+        needOutputs(d);
+        *d = treeOf(t).maxAchievedDepth();
+    });
 }
 const char *at_last_error(void) { return g_error.c_str(); }
 
@@ -948,7 +975,8 @@ int at_event_elapsed_ms(void *a, void *b, float *ms)
 int at_graph_capture_update(at_tree *t, void *stream, void **exec)
 {
     return guarded([&] {
-        hipStream_t s = t->impl->stream();
+        needOutputs(exec);
+        hipStream_t s = treeOf(t).stream();
         if (!s) throw at::Error(AT_EINVAL, "graph capture needs a tree created on a non-default stream");
         if (stream && hipStream_t(stream) != s)
             throw at::Error(AT_EINVAL, "capture stream must be the tree's configured stream");
@@ -956,7 +984,7 @@ int at_graph_capture_update(at_tree *t, void *stream, void **exec)
         AT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
         try
         {
-            t->impl->update();
+            treeOf(t).update();
         }
         catch (...)
         {
@@ -991,6 +1019,7 @@ int at_tune_set(const char *key, int value)
 int at_tune_get(const char *key, int *value)
 {
     return guarded([&] {
+        needOutputs(value);
         const std::string k = key ? key : "";
         if (k == "sort_items") *value = at::sortItemsPerThread();
         else if (k == "sort_ranges") *value = at::sortTargetRanges();

@@ -9,8 +9,10 @@
 //   upsweep   digit histogram of each contiguous range of keys. Pass 0's
 //             histograms come out of key generation itself; pass k+1 reads a
 //             one-byte digit array that pass k's scatter wrote, not the keys.
-//   scan      two small kernels: per-range offsets inside chunks of 64
-//             ranges, then chunk prefixes and the global digit bases.
+//   scan      one block per digit: the digit's exclusive prefix over the
+//             ranges and its total. Histograms are stored digit-major
+//             (hist[digit][row]), so each block reads one contiguous row set;
+//             every scatter block forms the digit bases from the 256 totals.
 //   scatter   one block per range walks its range tile by tile (wave64,
 //             256 threads, ITEMS keys per thread). Keys are ranked per wave
 //             with 8 ballots each (lanes with equal digits, stable in lane
@@ -27,7 +29,6 @@ namespace at
 namespace
 {
 constexpr int kWaves = kSortThreads / 64;
-constexpr int kChunk = 64; // ranges per first-level scan chunk
 constexpr std::size_t kHistogramBlocks = 4096;
 
 int g_items = 16;
@@ -74,10 +75,12 @@ __device__ inline void countDigit(std::uint32_t *h, unsigned d)
         atomicAdd(&h[d], 1u);
 }
 
-__device__ inline void flushHistogram(const std::uint32_t *h, std::uint32_t *out)
+// Block histogram -> row blockIdx.x of the digit-major histogram table
+// (hist[digit * gridDim.x + row]).
+__device__ inline void flushHistogram(const std::uint32_t *h, std::uint32_t *hist)
 {
     __syncthreads();
-    out[threadIdx.x] = h[threadIdx.x];
+    hist[std::size_t(threadIdx.x) * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
 constexpr int kKeygenUnroll = 4;
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
         keys[i] = k;
         countDigit(h, unsigned(k) & (kRadix - 1));
     }
-    flushHistogram(h, partHist + std::size_t(blockIdx.x) * kRadix);
+    flushHistogram(h, partHist);
 }
 
 __global__ __launch_bounds__(kSortThreads) void upsweepKeysKernel(const Key *__restrict__ keys, std::size_t n,
@@ -139,25 +142,23 @@ __global__ __launch_bounds__(kSortThreads) void upsweepKeysKernel(const Key *__r
     const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
     for (std::size_t i = begin + threadIdx.x; i < end; i += kSortThreads)
         countDigit(h, unsigned(keys[i] >> shift) & (kRadix - 1));
-    flushHistogram(h, rangeHist + std::size_t(blockIdx.x) * kRadix);
+    flushHistogram(h, rangeHist);
 }
 
-// Partial digit histogram of part `blockIdx.x % parts` of a range from the
-// one-byte digit array: 16 digits per load, one LDS histogram per wave.
+// Digit histogram of range blockIdx.x from the one-byte digit array: 16
+// digits per load (ranges start at multiples of the tile size), one LDS
+// histogram per wave.
 __global__ __launch_bounds__(kSortThreads) void upsweepDigitsKernel(const std::uint8_t *__restrict__ digits,
-                                                                    std::size_t n, std::size_t rangeKeys, int parts,
-                                                                    std::uint32_t *partHist)
+                                                                    std::size_t n, std::size_t rangeKeys,
+                                                                    std::uint32_t *rangeHist)
 {
     __shared__ std::uint32_t h[kWaves][kRadix];
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) h[w][threadIdx.x] = 0;
     __syncthreads();
     std::uint32_t *mine = h[threadIdx.x >> 6];
-    const std::size_t r = blockIdx.x / unsigned(parts), part = blockIdx.x % unsigned(parts);
-    const std::size_t partKeys = rangeKeys / std::size_t(parts); // multiple of 16
-    const std::size_t rEnd = (r + 1) * rangeKeys < n ? (r + 1) * rangeKeys : n;
-    const std::size_t begin = r * rangeKeys + part * partKeys;
-    const std::size_t end = begin + partKeys < rEnd ? begin + partKeys : rEnd;
+    const std::size_t begin = std::size_t(blockIdx.x) * rangeKeys;
+    const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
     const uint4 *quads = reinterpret_cast<const uint4 *>(digits);
     for (std::size_t i = begin + 16 * threadIdx.x; i < end; i += 16 * kSortThreads)
     {
@@ -177,51 +178,54 @@ __global__ __launch_bounds__(kSortThreads) void upsweepDigitsKernel(const std::u
     std::uint32_t t = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) t += h[w][threadIdx.x];
-    partHist[std::size_t(blockIdx.x) * kRadix + threadIdx.x] = t;
+    rangeHist[std::size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
 
-// Offsets of each range inside its chunk of kChunk ranges, and chunk totals;
-// a range's histogram is the sum of its `parts` partial histograms. All loads
-// of a thread are issued before the running sum consumes them.
-__global__ __launch_bounds__(kSortThreads) void scanChunksKernel(const std::uint32_t *__restrict__ partHist,
-                                                                 std::size_t ranges, int parts,
-                                                                 std::uint32_t *rangeOff, std::uint32_t *chunkOff)
+// Block digit blockIdx.x: exclusive prefix of the digit's count over the
+// ranges (a range's count is the sum of its `parts` rows) and the total.
+// Ranges are taken 256 at a time, one per thread, with a running carry.
+__global__ __launch_bounds__(kSortThreads) void scanDigitKernel(const std::uint32_t *__restrict__ hist,
+                                                                std::size_t ranges, int parts,
+                                                                std::uint32_t *rangeOff, std::uint32_t *digitTotal)
 {
-    const int d = threadIdx.x;
-    const std::size_t r0 = std::size_t(blockIdx.x) * kChunk;
-    std::uint32_t v[kChunk];
-#pragma unroll
-    for (int k = 0; k < kChunk; ++k) v[k] = 0;
-    for (int q = 0; q < parts; ++q)
+    __shared__ std::uint32_t waveTotals[2][kWaves];
+    const std::size_t d = blockIdx.x;
+    const std::size_t rows = ranges * std::size_t(parts);
+    const std::uint32_t *h = hist + d * rows;
+    std::uint32_t carry = 0;
+    int buf = 0;
+    for (std::size_t r0 = 0; r0 < ranges; r0 += kSortThreads)
     {
+        const std::size_t r = r0 + threadIdx.x;
+        std::uint32_t v = 0;
+        if (r < ranges)
+            for (int q = 0; q < parts; ++q) v += h[r * unsigned(parts) + unsigned(q)];
+        // wave-inclusive scan, then the wave totals (double-buffered so one
+        // barrier per group suffices)
+        const unsigned lane = laneId();
+        const int wave = threadIdx.x >> 6;
+        std::uint32_t inc = v;
 #pragma unroll
-        for (int k = 0; k < kChunk; ++k)
-            if (r0 + k < ranges) v[k] += partHist[((r0 + k) * unsigned(parts) + unsigned(q)) * kRadix + d];
-    }
-    std::uint32_t run = 0;
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const std::uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= unsigned(o)) inc += t;
+        }
+        if (lane == 63) waveTotals[buf][wave] = inc;
+        __syncthreads();
+        std::uint32_t before = 0, total = 0;
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k)
-    {
-        if (r0 + k < ranges) rangeOff[(r0 + k) * kRadix + d] = run;
-        run += v[k];
+        for (int w = 0; w < kWaves; ++w)
+        {
+            const std::uint32_t t = waveTotals[buf][w];
+            before += w < wave ? t : 0u;
+            total += t;
+        }
+        if (r < ranges) rangeOff[d * ranges + r] = carry + before + inc - v;
+        carry += total;
+        buf ^= 1;
     }
-    chunkOff[std::size_t(blockIdx.x) * kRadix + d] = run;
-}
-
-// Exclusive prefixes of the chunks and the global digit bases.
-__global__ __launch_bounds__(kSortThreads) void scanBaseKernel(std::uint32_t *chunkOff, std::size_t chunks,
-                                                               std::uint32_t *digitBase)
-{
-    __shared__ std::uint32_t waveTotals[kWaves];
-    const int d = threadIdx.x;
-    std::uint32_t run = 0;
-    for (std::size_t c = 0; c < chunks; ++c)
-    {
-        const std::uint32_t t = chunkOff[c * kRadix + d];
-        chunkOff[c * kRadix + d] = run;
-        run += t;
-    }
-    digitBase[d] = blockExclusiveScan(run, waveTotals);
+    if (threadIdx.x == 0) digitTotal[d] = carry;
 }
 
 // Exclusive scan of the 256 per-digit values held by threads 0..255 of a
@@ -254,8 +258,8 @@ template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits>
 __global__ __launch_bounds__(THREADS) void scatterKernel(
     const Key *__restrict__ kin, Key *__restrict__ kout, const std::uint32_t *__restrict__ vin,
     std::uint32_t *__restrict__ vout, std::uint8_t *__restrict__ digitsOut, std::size_t n, std::size_t rangeKeys,
-    const std::uint32_t *__restrict__ rangeOff, const std::uint32_t *__restrict__ chunkOff,
-    const std::uint32_t *__restrict__ digitBase, int shift, int nextShift)
+    const std::uint32_t *__restrict__ rangeOff, const std::uint32_t *__restrict__ digitTotal, std::size_t ranges,
+    int shift, int nextShift)
 {
     constexpr int kW = THREADS / 64;
     constexpr int kTile = ITEMS * THREADS;
@@ -273,7 +277,11 @@ __global__ __launch_bounds__(THREADS) void scatterKernel(
     const int wave = tid >> 6;
     const unsigned lane = laneId();
     const std::size_t r = blockIdx.x;
-    if (tid < kRadix) running[tid] = digitBase[tid] + chunkOff[(r / kChunk) * kRadix + tid] + rangeOff[r * kRadix + tid];
+    {
+        const std::uint32_t total = tid < kRadix ? digitTotal[tid] : 0u;
+        const std::uint32_t digitBase = digitExclusiveScan(total, scratch);
+        if (tid < kRadix) running[tid] = digitBase + rangeOff[std::size_t(tid) * ranges + r];
+    }
     const std::size_t rBegin = r * rangeKeys;
     const std::size_t rEnd = rBegin + rangeKeys < n ? rBegin + rangeKeys : n;
     const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
@@ -390,7 +398,7 @@ void launchScatter(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, const K
     const int next = shift + kRadixBits;
 #define AT_SCATTER(V, I, D)                                                                                         \
     hipLaunchKernelGGL((scatterKernel<THREADS, ITEMS, V, I, D>), grid, block, 0, s, kin, kout, vin, vout, ws.digits, \
-                       pl.n, pl.rangeKeys, ws.rangeOff, ws.chunkOff, ws.digitBase, shift, next)
+                       pl.n, pl.rangeKeys, ws.rangeOff, ws.digitTotal, pl.ranges, shift, next)
     if (!vals)
     {
         if (digits) AT_SCATTER(false, false, true);
@@ -411,9 +419,8 @@ void launchScatter(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, const K
 
 void scanOffsets(hipStream_t s, const SortPlan &pl, SortWorkspace &ws)
 {
-    hipLaunchKernelGGL(scanChunksKernel, dim3(unsigned(pl.chunks)), dim3(kSortThreads), 0, s, ws.rangeHist, pl.ranges,
-                       pl.parts, ws.rangeOff, ws.chunkOff);
-    hipLaunchKernelGGL(scanBaseKernel, dim3(1), dim3(kSortThreads), 0, s, ws.chunkOff, pl.chunks, ws.digitBase);
+    hipLaunchKernelGGL(scanDigitKernel, dim3(kRadix), dim3(kSortThreads), 0, s, ws.rangeHist, pl.ranges, pl.parts,
+                       ws.rangeOff, ws.digitTotal);
 }
 
 // Passes 0..7 given pass 0's range histograms in ws.rangeHist. Keys ping-pong
@@ -432,7 +439,7 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         {
             prof::Scope scope("sort_upsweep", s);
             hipLaunchKernelGGL(upsweepDigitsKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s,
-                               ws.digits, pl.n, pl.rangeKeys, 1, ws.rangeHist);
+                               ws.digits, pl.n, pl.rangeKeys, ws.rangeHist);
         }
         // key generation writes `parts` partial histograms per range;
         // every other histogram has one row per range
@@ -488,7 +495,6 @@ SortPlan makeSortPlan(std::size_t n)
     p.rangeKeys = p.rangeTiles * p.tileKeys;
     p.ranges = (n + p.rangeKeys - 1) / p.rangeKeys;
     if (p.ranges < 1) p.ranges = 1;
-    p.chunks = (p.ranges + kChunk - 1) / kChunk;
     // enough histogram blocks to fill the GPU: parts per range, each a
     // multiple of 4 keys
     p.parts = 1;
@@ -507,24 +513,22 @@ void SortWorkspace::allocate(std::size_t cap, bool withValues)
     maxRanges = (capacity + 8 * kSortThreads - 1) / (8 * kSortThreads);
     if (maxRanges < 1) maxRanges = 1;
     const std::size_t histRows = maxRanges > kHistogramBlocks * 2 ? maxRanges : kHistogramBlocks * 2;
-    const std::size_t maxChunks = (maxRanges + kChunk - 1) / kChunk;
     values = withValues;
     AT_HIP(hipMalloc(&keysAlt, capacity * sizeof(Key)));
     if (withValues) AT_HIP(hipMalloc(&valsAlt, capacity * sizeof(std::uint32_t)));
     AT_HIP(hipMalloc(&digits, (capacity + 16) * sizeof(std::uint8_t)));
     AT_HIP(hipMalloc(&rangeHist, histRows * kRadix * sizeof(std::uint32_t)));
     AT_HIP(hipMalloc(&rangeOff, maxRanges * kRadix * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&chunkOff, maxChunks * kRadix * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&digitBase, kRadix * sizeof(std::uint32_t)));
+    AT_HIP(hipMalloc(&digitTotal, kRadix * sizeof(std::uint32_t)));
 }
 
 void SortWorkspace::release()
 {
     for (void *p : {(void *)keysAlt, (void *)valsAlt, (void *)digits, (void *)rangeHist, (void *)rangeOff,
-                    (void *)chunkOff, (void *)digitBase})
+                    (void *)digitTotal})
         if (p) (void)hipFree(p);
     keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
-    chunkOff = nullptr; digitBase = nullptr;
+    digitTotal = nullptr;
     capacity = 0; maxRanges = 0; values = false;
 }
 

@@ -31,7 +31,6 @@ struct SortPlan
     std::size_t rangeTiles = 1;
     std::size_t rangeKeys = 0;
     std::size_t ranges = 0;
-    std::size_t chunks = 0; // groups of 64 ranges for the two-level offset scan
     int parts = 1;          // histogram blocks per range
 };
 SortPlan makeSortPlan(std::size_t n);
@@ -42,10 +41,9 @@ struct SortWorkspace
     Key *keysAlt = nullptr;            // ping-pong key buffer [capacity]
     std::uint32_t *valsAlt = nullptr;  // ping-pong value buffer
     std::uint8_t *digits = nullptr;    // next-pass digit per key [capacity]
-    std::uint32_t *rangeHist = nullptr; // [maxRanges][kRadix] digit counts per range
-    std::uint32_t *rangeOff = nullptr;  // exclusive offsets inside a chunk
-    std::uint32_t *chunkOff = nullptr;  // [maxChunks][kRadix]
-    std::uint32_t *digitBase = nullptr; // [kRadix]
+    std::uint32_t *rangeHist = nullptr;  // [kRadix][rows] digit counts per range (or range part)
+    std::uint32_t *rangeOff = nullptr;   // [kRadix][ranges] exclusive offset of each range within its digit
+    std::uint32_t *digitTotal = nullptr; // [kRadix]
     std::size_t capacity = 0;
     std::size_t maxRanges = 0;
     bool values = false;

@@ -577,11 +577,9 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
                                                            unsigned maxDepth, std::int8_t *delta, std::uint8_t *cnt,
                                                            std::uint32_t *tileSum)
 {
-    constexpr int kMaxP = kQ + 2 * int(kDirectMaxBucket) + 2;
-    constexpr int kMaxD = kQ + int(kDirectMaxBucket) + 1;
-    __shared__ Key pw[kMaxP];
-    __shared__ int tab[2][kMaxD];
-    __shared__ int dl[kQ + 1];
+    // dynamic LDS sized for this bucket (binCountLdsBytes): keys P[q0-1-B ..
+    // q0+kQ+B], two rows of the sparse table, delta per interval
+    extern __shared__ __align__(16) unsigned char lds[];
     __shared__ std::uint32_t part[kThreads / 64];
     const int tid = threadIdx.x;
     const long nact = long(st->numActiveParticles);
@@ -591,6 +589,10 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
     const long origin = q0 - 1 - B; // particle index of pw[0] and tab[.][0]
     const int spanP = kQ + 2 * B + 2;
     const int spanD = kQ + B + 1;
+    Key *pw = reinterpret_cast<Key *>(lds);
+    std::int8_t *tab[2] = {reinterpret_cast<std::int8_t *>(pw + spanP),
+                           reinterpret_cast<std::int8_t *>(pw + spanP) + spanD};
+    std::int8_t *dl = tab[1] + spanD;
     for (int s = tid; s < spanP; s += kThreads)
     {
         const long j = origin + s;
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
             const int c = cpl63(pw[s], pw[s + B]);
             D = c < int(maxDepth) - 1 ? c : int(maxDepth) - 1;
         }
-        tab[0][s] = D;
+        tab[0][s] = std::int8_t(D);
     }
     __syncthreads();
     int lg = 0;
@@ -617,10 +619,10 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
         const int step = 1 << t;
         for (int s = tid; s < spanD; s += kThreads)
         {
-            int v = tab[cur][s];
+            std::int8_t v = tab[cur][s];
             if (s + step < spanD)
             {
-                const int w = tab[cur][s + step];
+                const std::int8_t w = tab[cur][s + step];
                 v = w > v ? w : v;
             }
             tab[cur ^ 1][s] = v;
@@ -639,7 +641,7 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
             const int a = tab[cur][u], b = tab[cur][u + B - pw2 + 1];
             d = a > b ? a : b;
         }
-        dl[u] = d;
+        dl[u] = std::int8_t(d);
     }
     __syncthreads();
     std::uint32_t sum = 0;
@@ -715,14 +717,18 @@ __global__ __launch_bounds__(kThreads) void binEmitKernel(const Key *P, const De
     const long nact = long(st->numActiveParticles);
     const long q0 = long(blockIdx.x) * kQ;
     if (q0 > nact) return;
+    static_assert(kItems == 8, "one 8-byte load of interval counts per thread");
     const long qFirst = q0 + long(tid) * kItems;
+    // cnt is allocated to whole tiles; entries past nact are masked
+    const uint2 raw = *reinterpret_cast<const uint2 *>(cnt + qFirst);
     std::uint32_t c[kItems];
     std::uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
     {
         const long q = qFirst + k;
-        c[k] = q <= nact ? cnt[q] : 0u;
+        const std::uint32_t v = ((k < 4 ? raw.x : raw.y) >> (8 * (k & 3))) & 0xffu;
+        c[k] = q <= nact ? v : 0u;
         sum += c[k];
     }
     const unsigned lane = __lane_id();
@@ -893,7 +899,9 @@ void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDire
                        nullptr);
     prof::Scope scope("bin_direct", s);
     const unsigned tiles = unsigned(t.n / kQ + 1);
-    hipLaunchKernelGGL(binCountKernel, dim3(tiles), dim3(kThreads), 0, s, t.P, t.st, p.bucket, p.maxDepth, d.delta,
+    const int B = int(p.bucket);
+    const std::size_t lds = std::size_t(kQ + 2 * B + 2) * sizeof(Key) + 2 * std::size_t(kQ + B + 1) + kQ + 1;
+    hipLaunchKernelGGL(binCountKernel, dim3(tiles), dim3(kThreads), lds, s, t.P, t.st, p.bucket, p.maxDepth, d.delta,
                        d.cnt, d.tileSum);
     hipLaunchKernelGGL(binScanTilesKernel, dim3(1), dim3(1024), 0, s, t.st, d.tileSum, d.total);
 }

@@ -96,13 +96,12 @@ def _ptr(a):
 
 
 class Tree:
-    """``Tree<Real, Kind>`` on the HIP engine (backend="hip") or on the CPU
-    oracle (backend="oracle", host coordinate arrays; tests only)."""
+    """``Tree<Real, Kind>`` on the HIP engine. Coordinates are device
+    pointers (DeviceArray or int addresses) that the tree borrows and
+    re-reads on every update()."""
 
-    def __init__(self, x, y, z, count, config=None, real=np.float64, kind=OCTREE,
-                 backend="hip"):
-        self.backend = backend
-        self.lib = _ffi.hip_lib() if backend == "hip" else _ffi.oracle_lib()
+    def __init__(self, x, y, z, count, config=None, real=np.float64, kind=OCTREE):
+        self.lib = self._open_lib()
         self.real = np.dtype(real)
         self.kind = kind
         self._coords = (x, y, z)  # keep alive: the tree borrows them
@@ -111,6 +110,13 @@ class Tree:
         self.lib.call("tree_create", self.real.itemsize, kind, _ptr(x), _ptr(y), _ptr(z),
                       int(count), ctypes.byref(self.config), ctypes.byref(h))
         self.handle = h.value
+
+    def _open_lib(self):
+        return _ffi.hip_lib()
+
+    def _copy_out(self, dst, src, nbytes):
+        """Device-to-host copy of a tree buffer."""
+        self.lib.call("memcpy", dst, src, nbytes, 2)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -180,10 +186,7 @@ class Tree:
         vals = []
         for p, dt in ((pl, np.int32), (pn, np.int32), (pa, np.uint32)):
             out = np.zeros(1, dtype=dt)
-            if self.backend == "hip":
-                self.lib.call("memcpy", out.ctypes.data, p.value, 4, 2)
-            else:
-                ctypes.memmove(out.ctypes.data, p.value, 4)
+            self._copy_out(out.ctypes.data, p.value, 4)
             vals.append(int(out[0]))
         return tuple(vals)
 
@@ -209,10 +212,7 @@ class Tree:
         dt = np.dtype(_ffi.BUFFER_DTYPES[_ffi.BUFFERS[name]])
         out = np.empty(n, dtype=dt)
         if n and ptr:
-            if self.backend == "hip":
-                self.lib.call("memcpy", out.ctypes.data, ptr, out.nbytes, 2)
-            else:
-                ctypes.memmove(out.ctypes.data, ptr, out.nbytes)
+            self._copy_out(out.ctypes.data, ptr, out.nbytes)
         return out
 
     def __getattr__(self, name):

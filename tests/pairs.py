@@ -5,6 +5,7 @@ Every comparison is exact (integer arrays, bit for bit).
 import numpy as np
 
 from adaptive_trees_amd import BINARY, OCTREE, DeviceArray, Tree, make_config
+from oracle.binding import OracleTree
 
 ARRAYS = ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d", "nearFieldCounts_d",
           "sfcBoxIndex_d", "hasBoxSplit_d", "boxDepth_d", "parentIndex_d", "childIndex_d",
@@ -21,8 +22,8 @@ class Pair:
         self.n = len(x)
         self.host = [x.copy(), y.copy(), z.copy()]
         self.dev = [DeviceArray(a) for a in self.host]
-        self.gpu = Tree(*self.dev, self.n, make_config(**cfg), real=self.real, kind=kind, backend="hip")
-        self.cpu = Tree(*self.host, self.n, make_config(**cfg), real=self.real, kind=kind, backend="oracle")
+        self.gpu = Tree(*self.dev, self.n, make_config(**cfg), real=self.real, kind=kind)
+        self.cpu = OracleTree(*self.host, self.n, make_config(**cfg), real=self.real, kind=kind)
 
     def set_coords(self, x, y, z):
         for h, d, a in zip(self.host, self.dev, (x, y, z)):
