@@ -1013,6 +1013,7 @@ int at_tune_set(const char *key, int value)
         if (k == "sort_items") at::setSortItemsPerThread(value);
         else if (k == "sort_ranges") at::setSortTargetRanges(value);
         else if (k == "sort_threads") at::setSortThreads(value);
+        else if (k == "sort_digit_array") at::setSortDigitArray(value);
         else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
     });
 }
@@ -1024,6 +1025,7 @@ int at_tune_get(const char *key, int *value)
         if (k == "sort_items") *value = at::sortItemsPerThread();
         else if (k == "sort_ranges") *value = at::sortTargetRanges();
         else if (k == "sort_threads") *value = at::sortThreads();
+        else if (k == "sort_digit_array") *value = at::sortDigitArray();
         else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
     });
 }

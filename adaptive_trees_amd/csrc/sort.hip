@@ -31,9 +31,10 @@ namespace
 constexpr int kWaves = kSortThreads / 64;
 constexpr std::size_t kHistogramBlocks = 4096;
 
-int g_items = 16;
+int g_items = 12;
 int g_threads = 512;
-int g_targetRanges = 1024;
+int g_targetRanges = 2048;
+int g_digitArray = 1; // scatter writes the next pass's digit bytes for the histogram
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 
@@ -131,18 +132,51 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     flushHistogram(h, partHist);
 }
 
+// Digit histogram of range blockIdx.x read from the keys: two keys per
+// 16-byte load when the array is 16-byte aligned (ranges start at even
+// indices), one LDS histogram per wave.
 __global__ __launch_bounds__(kSortThreads) void upsweepKeysKernel(const Key *__restrict__ keys, std::size_t n,
                                                                   std::size_t rangeKeys, int shift,
                                                                   std::uint32_t *rangeHist)
 {
-    __shared__ std::uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
+    __shared__ std::uint32_t h[kWaves][kRadix];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) h[w][threadIdx.x] = 0;
     __syncthreads();
+    std::uint32_t *mine = h[threadIdx.x >> 6];
     const std::size_t begin = std::size_t(blockIdx.x) * rangeKeys;
     const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
-    for (std::size_t i = begin + threadIdx.x; i < end; i += kSortThreads)
-        countDigit(h, unsigned(keys[i] >> shift) & (kRadix - 1));
-    flushHistogram(h, rangeHist);
+    if ((reinterpret_cast<std::uintptr_t>(keys) & 15) == 0)
+    {
+        const ulonglong2 *pairs = reinterpret_cast<const ulonglong2 *>(keys);
+        constexpr int kU = 2;
+        std::size_t i = begin + 2 * threadIdx.x;
+        for (; i + 2 * kSortThreads * (kU - 1) + 1 < end; i += 2 * kSortThreads * kU)
+        {
+            ulonglong2 q[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) q[u] = pairs[(i >> 1) + std::size_t(u) * kSortThreads];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+            {
+                countDigit(mine, unsigned(q[u].x >> shift) & (kRadix - 1));
+                countDigit(mine, unsigned(q[u].y >> shift) & (kRadix - 1));
+            }
+        }
+        for (; i < end; i += 2 * kSortThreads)
+        {
+            atomicAdd(&mine[unsigned(keys[i] >> shift) & (kRadix - 1)], 1u);
+            if (i + 1 < end) atomicAdd(&mine[unsigned(keys[i + 1] >> shift) & (kRadix - 1)], 1u);
+        }
+    }
+    else
+        for (std::size_t i = begin + threadIdx.x; i < end; i += kSortThreads)
+            atomicAdd(&mine[unsigned(keys[i] >> shift) & (kRadix - 1)], 1u);
+    __syncthreads();
+    std::uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += h[w][threadIdx.x];
+    rangeHist[std::size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
 
 // Digit histogram of range blockIdx.x from the one-byte digit array: 16
@@ -254,8 +288,10 @@ __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_
     return before + inc - v;
 }
 
+// Second launch bound = minimum waves per SIMD: two blocks of 512 (or one
+// of 1024) per CU, i.e. at most 128 VGPRs.
 template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits>
-__global__ __launch_bounds__(THREADS) void scatterKernel(
+__global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel(
     const Key *__restrict__ kin, Key *__restrict__ kout, const std::uint32_t *__restrict__ vin,
     std::uint32_t *__restrict__ vout, std::uint8_t *__restrict__ digitsOut, std::size_t n, std::size_t rangeKeys,
     const std::uint32_t *__restrict__ rangeOff, const std::uint32_t *__restrict__ digitTotal, std::size_t ranges,
@@ -268,7 +304,8 @@ __global__ __launch_bounds__(THREADS) void scatterKernel(
         Key keys[kTile];
         std::uint32_t vals[kTile];
     } stage;
-    __shared__ std::uint16_t waveCount[kW][kRadix]; // counts and offsets < 2^16
+    __shared__ std::uint8_t digitAt[kVals ? kTile : 1]; // digit of each staged position
+    __shared__ std::uint16_t waveCount[kW][kRadix];     // counts and offsets < 2^16
     __shared__ std::uint32_t running[kRadix];
     __shared__ std::uint32_t adjust[kRadix];
     __shared__ std::uint32_t scratch[kRadix / 64];
@@ -282,21 +319,23 @@ __global__ __launch_bounds__(THREADS) void scatterKernel(
         const std::uint32_t digitBase = digitExclusiveScan(total, scratch);
         if (tid < kRadix) running[tid] = digitBase + rangeOff[std::size_t(tid) * ranges + r];
     }
+    // range-relative 32-bit offsets (n < 2^32)
     const std::size_t rBegin = r * rangeKeys;
-    const std::size_t rEnd = rBegin + rangeKeys < n ? rBegin + rangeKeys : n;
+    const std::uint32_t rLen = std::uint32_t((rBegin + rangeKeys < n ? rBegin + rangeKeys : n) - rBegin);
+    const Key *kr = kin + rBegin;
     const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
 
-    for (std::size_t base = rBegin; base < rEnd; base += kTile)
+    for (std::uint32_t base = 0; base < rLen; base += kTile)
     {
         for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
         // ---- load keys (wave-striped: item j of lane l is element j*64 + l of the wave's chunk)
         Key key[ITEMS];
-        const std::size_t waveBase = base + std::size_t(wave) * (ITEMS * 64);
+        const std::uint32_t waveFirst = base + std::uint32_t(wave) * (ITEMS * 64) + lane;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
         {
-            const std::size_t idx = waveBase + std::size_t(j) * 64 + lane;
-            key[j] = idx < rEnd ? kin[idx] : kInvalidKey; // padding ranks last with digit 255
+            const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
+            key[j] = idx < rLen ? kr[idx] : kInvalidKey; // padding ranks last with digit 255
         }
         __syncthreads();
 
@@ -350,9 +389,8 @@ __global__ __launch_bounds__(THREADS) void scatterKernel(
             stage.keys[rank[j]] = key[j];
         }
         __syncthreads();
-        const std::size_t remaining = rEnd - base;
-        const int valid = remaining < std::size_t(kTile) ? int(remaining) : kTile;
-        std::uint32_t dst[ITEMS];
+        const std::uint32_t remaining = rLen - base;
+        const int valid = remaining < std::uint32_t(kTile) ? int(remaining) : kTile;
 #pragma unroll
         for (int q = 0; q < ITEMS; ++q)
         {
@@ -360,29 +398,29 @@ __global__ __launch_bounds__(THREADS) void scatterKernel(
             if (i < valid)
             {
                 const Key k = stage.keys[i];
-                dst[q] = adjust[unsigned(k >> shift) & (kRadix - 1)] + std::uint32_t(i);
-                kout[dst[q]] = k;
-                if (kDigits) digitsOut[dst[q]] = std::uint8_t(k >> nextShift);
+                const unsigned dg = unsigned(k >> shift) & (kRadix - 1);
+                if (kVals) digitAt[i] = std::uint8_t(dg);
+                const std::uint32_t dst = adjust[dg] + std::uint32_t(i);
+                kout[dst] = k;
+                if (kDigits) digitsOut[dst] = std::uint8_t(k >> nextShift);
             }
         }
         if (kVals)
         {
-            std::uint32_t val[ITEMS];
+            const std::uint32_t *vr = vin + rBegin;
+            __syncthreads(); // staged keys are consumed before values overwrite them
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j)
             {
-                const std::size_t idx = waveBase + std::size_t(j) * 64 + lane;
-                val[j] = kIota ? std::uint32_t(idx) : (idx < rEnd ? vin[idx] : 0u);
+                const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
+                stage.vals[rank[j]] = kIota ? std::uint32_t(rBegin) + idx : (idx < rLen ? vr[idx] : 0u);
             }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < ITEMS; ++j) stage.vals[rank[j]] = val[j];
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < ITEMS; ++q)
             {
                 const int i = q * THREADS + tid;
-                if (i < valid) vout[dst[q]] = stage.vals[i];
+                if (i < valid) vout[adjust[digitAt[i]] + std::uint32_t(i)] = stage.vals[i];
             }
         }
         if (tid < kRadix) running[tid] += tileCount;
@@ -438,24 +476,31 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         if (pass > 0)
         {
             prof::Scope scope("sort_upsweep", s);
-            hipLaunchKernelGGL(upsweepDigitsKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s,
-                               ws.digits, pl.n, pl.rangeKeys, ws.rangeHist);
+            if (g_digitArray)
+                hipLaunchKernelGGL(upsweepDigitsKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s,
+                                   ws.digits, pl.n, pl.rangeKeys, ws.rangeHist);
+            else
+                hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, src,
+                                   pl.n, pl.rangeKeys, shift, ws.rangeHist);
         }
         // key generation writes `parts` partial histograms per range;
         // every other histogram has one row per range
         SortPlan scanPlan = pl;
         if (pass > 0 || firstHistWhole) scanPlan.parts = 1;
         scanOffsets(s, scanPlan, ws);
-        const bool digits = pass + 1 < kSortPasses;
+        const bool digits = g_digitArray && pass + 1 < kSortPasses;
         prof::Scope scope("sort_pass", s);
         const bool first = iota && pass == 0;
         switch (pl.threads * 100 + pl.items)
         {
         case 25608: launchScatter<256, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 25612: launchScatter<256, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 25616: launchScatter<256, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 51208: launchScatter<512, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 51212: launchScatter<512, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 51216: launchScatter<512, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 102408: launchScatter<1024, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 102412: launchScatter<1024, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         default: launchScatter<1024, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         }
     }
@@ -466,7 +511,7 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
 int sortItemsPerThread() { return g_items; }
 void setSortItemsPerThread(int items)
 {
-    if (items != 8 && items != 16) throw Error(1, "sort items per thread must be 8 or 16");
+    if (items != 8 && items != 12 && items != 16) throw Error(1, "sort items per thread must be 8, 12 or 16");
     g_items = items;
 }
 int sortThreads() { return g_threads; }
@@ -474,6 +519,12 @@ void setSortThreads(int threads)
 {
     if (threads != 256 && threads != 512 && threads != 1024) throw Error(1, "sort threads must be 256, 512 or 1024");
     g_threads = threads;
+}
+int sortDigitArray() { return g_digitArray; }
+void setSortDigitArray(int on)
+{
+    if (on != 0 && on != 1) throw Error(1, "sort_digit_array must be 0 or 1");
+    g_digitArray = on;
 }
 int sortTargetRanges() { return g_targetRanges; }
 void setSortTargetRanges(int ranges)

@@ -12,13 +12,17 @@ constexpr int kSortPasses = 64 / kRadixBits;
 constexpr int kSortThreads = 256;
 
 // Tile shape (threads per block, keys per thread) and target number of
-// ranges, chosen by the autotuner; defaults 512, 16 and 1024.
+// ranges, chosen by the autotuner; defaults 512, 12 and 2048.
 int sortItemsPerThread();
 void setSortItemsPerThread(int items);
 int sortTargetRanges();
 void setSortTargetRanges(int ranges);
 int sortThreads();
 void setSortThreads(int threads);
+// 1: each scatter also writes the next pass's digit bytes and the next
+// histogram reads those; 0: the histogram reads the keys.
+int sortDigitArray();
+void setSortDigitArray(int on);
 
 // Partition of n keys into contiguous ranges of whole tiles; each scatter
 // block owns one range and walks it tile by tile.

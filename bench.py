@@ -31,6 +31,7 @@ from adaptive_trees_amd import _ffi, workloads  # noqa: E402
 METRIC = "Mparticles/s build+update, % HBM roofline; batched vs plain speedup; p99 latency"
 PEAK_HBM_GBS = 8000.0
 SORT_PASS_BYTES = 24  # read + write of one (8-byte key, 4-byte value) pair per pass
+COPY_BYTES = 1 << 30  # calibration copy: reads and writes this many bytes
 
 
 def _lib():
@@ -101,6 +102,39 @@ def cpu_baseline(x, y, z, x1, y1, z1, box, min_seconds):
                       "median of %d repeats (%.2f s each, %.1f s total)" % (n, len(times), dt, sum(times))}
 
 
+def _counter_rows(path):
+    """(kernel name, counter value) per dispatch from a rocprofv3 counter_collection.csv."""
+    import csv
+    import glob
+    files = [path] if os.path.isfile(path) else glob.glob(os.path.join(path, "**", "*counter_collection.csv"),
+                                                            recursive=True)
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                rows.append((r.get("Kernel_Name", ""), float(r.get("Counter_Value", "nan"))))
+    return rows
+
+
+def traffic_from_counters(fetch_csv, write_csv, copy_bytes):
+    """HBM bytes per launch of the radix scatter from FETCH_SIZE / WRITE_SIZE runs (KiB per
+    dispatch), each scaled by the same counter's reading for the copy kernel of copy_bytes."""
+    out = {}
+    for label, path in (("fetch", fetch_csv), ("write", write_csv)):
+        rows = _counter_rows(path)
+        copy = [v for k, v in rows if "copyKernel" in k]
+        scat = [v for k, v in rows if "scatterKernel" in k and "true, false, true" in k]
+        if not copy or not scat:
+            raise SystemExit("no copyKernel / scatterKernel dispatches in %s" % path)
+        factor = copy_bytes / (float(np.median(copy)) * 1024.0)
+        out[label + "_kib_per_dispatch"] = float(np.median(scat))
+        out[label + "_copy_kib"] = float(np.median(copy))
+        out[label + "_scale"] = factor
+        out[label + "_bytes"] = float(np.median(scat)) * 1024.0 * factor
+    out["sort_pass_bytes_per_launch"] = int(out["fetch_bytes"] + out["write_bytes"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,7 +145,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the known-size copy kernel (for hardware counter scaling)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.traffic:
+        t = traffic_from_counters(args.traffic[0], args.traffic[1], COPY_BYTES)
+        with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+            json.dump(t, f, indent=1)
+        print(json.dumps(t))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested %d)" % max(args.gpus, world),
@@ -196,7 +238,7 @@ def main():
     tree.doBuffersNeedResize()
 
     if args.copy_calibration:
-        nbytes = 1 << 30
+        nbytes = COPY_BYTES
         src, dst = DeviceArray(nbytes), DeviceArray(nbytes)
         for _ in range(3):
             _lib().call("bench_copy", dst.ptr, src.ptr, nbytes, stream.p)
@@ -226,7 +268,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "onesweep radix pass (8 per sort)",
+            "kernel": "radix scatter pass (scatterKernel, 8 per sort)",
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

@@ -173,8 +173,11 @@ int at_bench_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ---- tuning (process-wide; set before building trees) ------------------ */
 /* "sort_threads": threads per radix-sort block (256, 512 or 1024);
- * "sort_items":  keys per thread of a radix-sort tile (8 or 16);
- * "sort_ranges": target number of key ranges per sort pass (64..65536). */
+ * "sort_items":  keys per thread of a radix-sort tile (8, 12 or 16);
+ * "sort_ranges": target number of key ranges per sort pass (64..65536);
+ * "sort_digit_array": 1 = each scatter pass writes the next digit byte per
+ *                     key for the next histogram, 0 = the histogram reads
+ *                     the keys. */
 int at_tune_set(const char *key, int value);
 int at_tune_get(const char *key, int *value);
 

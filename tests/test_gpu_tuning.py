@@ -1,0 +1,63 @@
+"""Every radix-sort tuning the autotuner may pick gives the same trees.
+
+The tile shape (threads x keys per thread), the number of key ranges and
+the histogram source (digit bytes or keys) change only the schedule of the
+sort; P, Perm, K and N must stay bit-exact against the oracle.
+"""
+import numpy as np
+import pytest
+
+from adaptive_trees_amd import BINARY, HILBERT, MORTON, OCTREE, _ffi
+from pairs import Pair, plummer
+
+TUNINGS = [
+    (512, 12, 2048, 1), (512, 12, 2048, 0), (256, 8, 64, 1), (256, 16, 4096, 0),
+    (1024, 16, 512, 1), (1024, 8, 65536, 0), (512, 16, 1024, 1),
+]
+KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array")
+
+
+@pytest.fixture
+def tuning():
+    lib = _ffi.hip_lib()
+    saved = {}
+    for k in KEYS:
+        import ctypes
+        v = ctypes.c_int()
+        lib.call("tune_get", k.encode(), ctypes.byref(v))
+        saved[k] = v.value
+
+    def apply(values):
+        for k, v in zip(KEYS, values):
+            lib.call("tune_set", k.encode(), int(v))
+
+    yield apply
+    apply([saved[k] for k in KEYS])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg", TUNINGS, ids=lambda c: "%dx%d_r%d_d%d" % c)
+def test_tunings_are_bit_exact(tuning, cfg):
+    tuning(cfg)
+    print("tuning", cfg, flush=True)
+    x, y, z = plummer(300_000, seed=71)
+    # duplicates: ties must keep ascending slots
+    x[::97], y[::97], z[::97] = 0.25, 0.5, 0.75
+    p = Pair(x, y, z, kind=OCTREE, real=np.float32, max_particles_per_leaf=64, sfc_kind=HILBERT)
+    p.build()
+    p.compare("octree build %s" % (cfg,))
+    b = Pair(x, y, z, kind=BINARY, real=np.float64, max_particles_per_leaf=16, sfc_kind=MORTON,
+             views=(1, 1, 1))
+    b.build()
+    b.gpu.computeViews()
+    b.cpu.computeViews()
+    b.update()
+    b.compare("binary build+update+views %s" % (cfg,))
+
+
+def test_invalid_tunings_are_rejected():
+    lib = _ffi.hip_lib()
+    for k, v in (("sort_threads", 300), ("sort_items", 7), ("sort_ranges", 1), ("sort_digit_array", 2),
+                 ("no_such_key", 1)):
+        with pytest.raises(_ffi.InvalidArgument):
+            lib.call("tune_set", k.encode(), v)
