@@ -7,3 +7,4 @@ ctypes binding and a Python mirror of the reference facade.
 from ._ffi import (BINARY, HILBERT, LEAF_COUNT, MORTON, NF_COUNT, OCTREE,  # noqa: F401
                    InvalidArgument, LengthError, LogicError, TreeError)
 from .tree import DeviceArray, Tree, make_config  # noqa: F401
+from .batch import Batch  # noqa: F401

@@ -169,6 +169,16 @@ def hip_lib():
             "bench_copy": [_P, _P, _SZ, _P],
             "tune_set": [ctypes.c_char_p, _I],
             "tune_get": [ctypes.c_char_p, ctypes.POINTER(_I)],
+            "batch_create": [_I, _I, _SZ, _P, _P, _P, _P, ctypes.POINTER(Config), ctypes.POINTER(_P)],
+            "batch_destroy": [_P],
+            "batch_build": [_P],
+            "batch_update": [_P],
+            "batch_needs_resize": [_P, _P, ctypes.POINTER(_SZ)],
+            "batch_resize_buffers": [_P],
+            "batch_size": [_P, ctypes.POINTER(_SZ)],
+            "batch_tree_buffer": [_P, _SZ, _I, ctypes.POINTER(_P), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)],
+            "batch_tree_scalars": [_P, _SZ, ctypes.POINTER(Scalars)],
+            "batch_graph_capture_update": [_P, _P, ctypes.POINTER(_P)],
         }.items():
             f = getattr(d, "at_" + name)
             f.argtypes = args

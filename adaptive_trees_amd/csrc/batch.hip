@@ -1,0 +1,454 @@
+// batch.hip -- hundreds of independent octrees built and updated together.
+//
+// A batch owns T member trees (the same Octree<Real> objects as the single
+// tree API, with the same capacity rules) and drives them through every
+// stage with one launch for all members:
+//   keys + sort   one segmented key generation and radix sort over the
+//                 concatenated particles (SegmentedSort): each member's
+//                 P/Perm are its segment of the batch arrays;
+//   passes        multi-tree kernels (tree_kernels.hip) over a TreeSet: a
+//                 table of member TreeDevs plus, per launch shape, the
+//                 prefix of per-member block counts, so small and large
+//                 trees share one grid with no idle blocks.
+// The build runs the members' pass loops in lock step (a member leaves the
+// loop when its pass changes nothing, exactly where its own build() would
+// stop); update() has fixed capacity-sized tables and is graph-capturable.
+// Results are those of each member's own build()/update(): the per-tree
+// arithmetic is unchanged, only the launches are shared.
+#include "batch.hpp"
+#include "tree_impl.hpp"
+
+#include <climits>
+
+namespace at
+{
+namespace
+{
+// Launch shapes of one TreeSet table.
+enum Shape { kClear = 0, kTiles, kLeaves, kNf, kShapes };
+
+// Device + pinned host staging of a TreeDev table and its block maps.
+struct SetTables
+{
+    std::size_t cap = 0;
+    TreeDev *dTrees = nullptr, *hTrees = nullptr;
+    std::uint32_t *dMaps = nullptr, *hMaps = nullptr;
+    ~SetTables() { release(); }
+    void release()
+    {
+        if (dTrees) (void)hipFree(dTrees);
+        if (dMaps) (void)hipFree(dMaps);
+        if (hTrees) (void)hipHostFree(hTrees);
+        if (hMaps) (void)hipHostFree(hMaps);
+        dTrees = hTrees = nullptr;
+        dMaps = hMaps = nullptr;
+        cap = 0;
+    }
+    void reserve(std::size_t n)
+    {
+        if (n <= cap) return;
+        release();
+        cap = n;
+        AT_HIP(hipMalloc(&dTrees, n * sizeof(TreeDev)));
+        AT_HIP(hipMalloc(&dMaps, kShapes * (n + 1) * sizeof(std::uint32_t)));
+        AT_HIP(hipHostMalloc(&hTrees, n * sizeof(TreeDev)));
+        AT_HIP(hipHostMalloc(&hMaps, kShapes * (n + 1) * sizeof(std::uint32_t)));
+    }
+    // hTrees[0..count) and the per-member block counts are filled by the
+    // caller through `blocks`; writes the prefixes and copies both tables.
+    template <class F>
+    void upload(hipStream_t s, std::size_t count, F blocks, TreeSet out[kShapes])
+    {
+        for (int sh = 0; sh < kShapes; ++sh)
+        {
+            std::uint32_t *m = hMaps + sh * (cap + 1);
+            m[0] = 0;
+            for (std::size_t k = 0; k < count; ++k) m[k + 1] = m[k] + blocks(Shape(sh), k);
+            out[sh].trees = dTrees;
+            out[sh].start = dMaps + sh * (cap + 1);
+            out[sh].count = int(count);
+            out[sh].blocks = count ? m[count] : 0;
+        }
+        if (!count) return;
+        AT_HIP(hipMemcpyAsync(dTrees, hTrees, count * sizeof(TreeDev), hipMemcpyHostToDevice, s));
+        AT_HIP(hipMemcpyAsync(dMaps, hMaps, kShapes * (cap + 1) * sizeof(std::uint32_t), hipMemcpyHostToDevice, s));
+    }
+};
+
+template <class Real>
+class OctreeBatch final : public BatchBase
+{
+    using Member = impl::OctreeImpl<Real>;
+
+public:
+    OctreeBatch(std::size_t T, const Real *const *x, const Real *const *y, const Real *const *z,
+                const std::size_t *counts, const at_config &c, unsigned maxDepth, unsigned iterations)
+        : stream_(hipStream_t(c.stream)), maxDepth_(maxDepth), iterations_(iterations)
+    {
+        counts_.assign(counts, counts + T);
+        sort_.plan(counts_, stream_);
+        const std::size_t total = sort_.total();
+        P_.ensure(total ? total : 1, 0, stream_);
+        perm_.ensure(total ? total : 1, 0, stream_);
+        AT_HIP(hipMalloc(&st_, (T ? T : 1) * sizeof(DevState)));
+        AT_HIP(hipHostMalloc(&sth_, (T ? T : 1) * sizeof(DevState)));
+        AT_HIP(hipMemsetAsync(st_, 0, (T ? T : 1) * sizeof(DevState), stream_));
+        std::vector<SegCoords<Real>> coords(T);
+        trees_.reserve(T);
+        for (std::size_t k = 0; k < T; ++k)
+        {
+            coords[k] = SegCoords<Real>{x[k], y[k], z[k]};
+            MemberSlot slot{st_ + k, sth_ + k, P_.p + sort_.segBegin(k), perm_.p + sort_.segBegin(k)};
+            trees_.emplace_back(new Member(x[k], y[k], z[k], counts[k], c, maxDepth, iterations, &slot));
+        }
+        coords_.ensure(T ? T : 1, 0, stream_);
+        AT_HIP(hipStreamSynchronize(stream_));
+        if (T) AT_HIP(hipMemcpy(coords_.p, coords.data(), T * sizeof(SegCoords<Real>), hipMemcpyHostToDevice));
+        kp_ = KeygenParams<Real>{};
+        kp_.grid = impl::makeGridBox<Real>(c.box);
+        kind_ = c.sfc_kind == AT_HILBERT ? KeyKind::Hilbert : KeyKind::Morton;
+        params_ = OctParams{c.max_particles_per_leaf, c.split_criterion == AT_NF_COUNT, c.sfc_kind == AT_HILBERT};
+        build_.reserve(T ? T : 1);
+        aux_.reserve(T ? T : 1);
+        upd_.reserve(T ? T : 1);
+    }
+
+    ~OctreeBatch() override
+    {
+        (void)hipStreamSynchronize(stream_);
+        trees_.clear();
+        if (st_) (void)hipFree(st_);
+        if (sth_) (void)hipHostFree(sth_);
+    }
+
+    void build() override
+    {
+        AT_HIP(hipStreamSynchronize(stream_));
+        built_ = false;
+        const std::size_t T = trees_.size();
+        sort_.keygenAndSort<Real>(stream_, kind_, kp_, coords_.p, P_.p, perm_.p);
+        std::vector<std::size_t> active(T);
+        for (std::size_t k = 0; k < T; ++k)
+        {
+            active[k] = k;
+            trees_[k]->buildBegin();
+        }
+        TreeSet set[kShapes];
+        stage(build_, active, true, [&](Shape sh, std::size_t k) { return blocksFor(sh, k, 1); }, set);
+        multiOctInit(stream_, set[kClear]);
+        for (unsigned pass = 0; pass < maxDepth_ && !active.empty(); ++pass)
+        {
+            for (std::size_t k : active) trees_[k]->buildPrePass();
+            stage(build_, active, true,
+                  [&](Shape sh, std::size_t i) { return blocksFor(sh, active[i], trees_[active[i]]->buildLeaves()); },
+                  set);
+            multiClearScan(stream_, set[kClear]);
+            multiOctDecideScan(stream_, set[kTiles], params_);
+            readStates();
+            std::vector<std::size_t> next;
+            for (std::size_t k : active)
+                if (trees_[k]->buildAfterDecide(sth_[k])) next.push_back(k);
+            if (next.empty()) break;
+            // the process grid covers the old leaves, the refresh the new ones
+            TreeSet proc[kShapes];
+            stage(build_, next, true,
+                  [&](Shape sh, std::size_t i) { return blocksFor(sh, next[i], trees_[next[i]]->buildLeaves()); },
+                  proc);
+            multiOctProcess(stream_, proc[kLeaves], params_);
+            multiCommit(stream_, proc[kClear], false);
+            TreeSet fresh[kShapes];
+            stage(aux_, next, true,
+                  [&](Shape sh, std::size_t i) { return blocksFor(sh, next[i], trees_[next[i]]->buildNewLeaves()); },
+                  fresh);
+            multiOctRefresh(stream_, fresh[kLeaves], fresh[kNf], params_, false);
+            for (std::size_t k : next) trees_[k]->buildAfterProcess();
+            active.swap(next);
+        }
+        for (auto &t : trees_) t->buildEnd(false);
+        readStates();
+        for (std::size_t k = 0; k < T; ++k) Member::finishState(sth_[k], trees_[k]->buildLeaves());
+        if (T) AT_HIP(hipMemcpyAsync(st_, sth_, T * sizeof(DevState), hipMemcpyHostToDevice, stream_));
+        AT_HIP(hipStreamSynchronize(stream_));
+        built_ = true;
+        refreshUpdateTables();
+        for (auto &t : trees_)
+            if (t->configuredViews().any()) t->buildViews(t->configuredViews());
+        (void)needsResize(nullptr);
+    }
+
+    void update() override
+    {
+        if (!built_) throw Error(AT_ELOGIC, "adaptive_octree: call build() before update()");
+        sort_.keygenAndSort<Real>(stream_, kind_, kp_, coords_.p, P_.p, perm_.p);
+        multiReset(stream_, updSet_[kClear]);
+        multiOctRefresh(stream_, updSet_[kLeaves], updSet_[kNf], params_, true);
+        for (unsigned i = 0; i < iterations_; ++i)
+        {
+            multiOctDecideScan(stream_, updSet_[kTiles], params_);
+            multiOctProcess(stream_, updSet_[kLeaves], params_);
+            multiCommit(stream_, updSet_[kClear], false);
+            multiOctRefresh(stream_, updSet_[kLeaves], updSet_[kNf], params_, false);
+        }
+        for (auto &t : trees_)
+            if (t->configuredViews().any()) t->buildViews(t->configuredViews());
+        AT_HIP(hipGetLastError());
+    }
+
+    std::size_t needsResize(int *flags) override
+    {
+        if (!built_) return 0;
+        readStates();
+        std::size_t n = 0;
+        for (std::size_t k = 0; k < trees_.size(); ++k)
+        {
+            const bool r = trees_[k]->applyState(sth_[k]);
+            if (flags) flags[k] = r ? 1 : 0;
+            n += r ? 1 : 0;
+        }
+        return n;
+    }
+
+    void resizeBuffers() override
+    {
+        if (!built_) throw Error(AT_ELOGIC, "adaptive_octree: call build() before resizeBuffers()");
+        readStates();
+        for (std::size_t k = 0; k < trees_.size(); ++k)
+            if (trees_[k]->applyState(sth_[k]))
+            {
+                trees_[k]->growForResize();
+                AT_HIP(hipMemsetAsync(&st_[k].needsResize, 0, sizeof(int), stream_));
+            }
+        AT_HIP(hipStreamSynchronize(stream_));
+        refreshUpdateTables();
+    }
+
+    std::size_t size() const override { return trees_.size(); }
+    TreeBase &tree(std::size_t k) override
+    {
+        if (k >= trees_.size()) throw Error(AT_EINVAL, "batch tree index out of range");
+        return *trees_[k];
+    }
+    hipStream_t stream() const override { return stream_; }
+
+private:
+    std::uint32_t blocksFor(Shape sh, std::size_t k, std::size_t leaves) const
+    {
+        switch (sh)
+        {
+        case kClear: return clearBlocksFor(trees_[k]->device().statusWords);
+        case kTiles: return tileBlocksFor(leaves);
+        case kLeaves: return leafBlocksFor(leaves);
+        default: return params_.nf ? nfBlocksFor(leaves) : 0u;
+        }
+    }
+
+    // Stages the TreeDevs of `members` (build: capacity unlimited) and the
+    // block maps; one copy of each table.
+    template <class F>
+    void stage(SetTables &tab, const std::vector<std::size_t> &members, bool building, F blocks, TreeSet out[kShapes])
+    {
+        AT_HIP(hipStreamSynchronize(stream_)); // the staging buffers are reused
+        for (std::size_t i = 0; i < members.size(); ++i)
+        {
+            TreeDev t = trees_[members[i]]->device();
+            if (building) t.capacity = LONG_MAX;
+            tab.hTrees[i] = t;
+        }
+        tab.upload(stream_, members.size(), blocks, out);
+    }
+
+    void refreshUpdateTables()
+    {
+        std::vector<std::size_t> all(trees_.size());
+        for (std::size_t k = 0; k < all.size(); ++k) all[k] = k;
+        stage(upd_, all, false,
+              [&](Shape sh, std::size_t k) { return blocksFor(sh, k, std::size_t(trees_[k]->device().capacity)); },
+              updSet_);
+        AT_HIP(hipStreamSynchronize(stream_));
+    }
+
+    void readStates()
+    {
+        if (!trees_.empty())
+            AT_HIP(hipMemcpyAsync(sth_, st_, trees_.size() * sizeof(DevState), hipMemcpyDeviceToHost, stream_));
+        AT_HIP(hipStreamSynchronize(stream_));
+    }
+
+    hipStream_t stream_ = nullptr;
+    unsigned maxDepth_ = 21, iterations_ = 2;
+    bool built_ = false;
+    std::vector<std::size_t> counts_;
+    std::vector<std::unique_ptr<Member>> trees_;
+    SegmentedSort sort_;
+    DevBuf<Key> P_;
+    DevBuf<std::uint32_t> perm_;
+    DevBuf<SegCoords<Real>> coords_;
+    DevState *st_ = nullptr, *sth_ = nullptr;
+    KeygenParams<Real> kp_{};
+    KeyKind kind_ = KeyKind::Morton;
+    OctParams params_{};
+    SetTables build_, aux_, upd_;
+    TreeSet updSet_[kShapes];
+};
+
+template <class Real>
+BatchBase *makeBatch(int type, std::size_t T, const void *const *x, const void *const *y, const void *const *z,
+                     const std::size_t *counts, const at_config &c)
+{
+    if (type != AT_TREE_OCTREE) throw Error(AT_EINVAL, "batches hold octrees");
+    if (c.max_depth < -1 || c.update_iterations < -1) throw Error(AT_EINVAL, "Tree: use -1 for backend defaults");
+    if (c.max_particles_per_leaf == 0) throw Error(AT_EINVAL, "adaptive_octree: particle limit must be positive");
+    const unsigned it = c.update_iterations >= 0 ? unsigned(c.update_iterations) : 2u;
+    if (it == 0) throw Error(AT_EINVAL, "adaptive_octree: updateIterations must be positive");
+    if (T && (!x || !y || !z || !counts)) throw Error(AT_EINVAL, "null argument");
+    for (std::size_t k = 0; k < T; ++k)
+    {
+        if (counts[k] > 0 && (!x[k] || !y[k] || !z[k]))
+            throw Error(AT_EINVAL, "adaptive_octree: coordinate device pointers must not be null");
+        if (counts[k] > std::size_t(std::numeric_limits<std::uint32_t>::max()))
+            throw Error(AT_EINVAL, "adaptive_octree: particle count exceeds 32-bit permutation index range");
+    }
+    const unsigned md = c.max_depth >= 0 ? unsigned(c.max_depth) : kOctLevels;
+    return new OctreeBatch<Real>(T, reinterpret_cast<const Real *const *>(x), reinterpret_cast<const Real *const *>(y),
+                                 reinterpret_cast<const Real *const *>(z), counts, c, md, it);
+}
+} // namespace
+
+BatchBase::~BatchBase() = default;
+
+BatchBase *createBatch(int realBytes, int type, std::size_t T, const void *const *x, const void *const *y,
+                       const void *const *z, const std::size_t *counts, const at_config &c)
+{
+    if (realBytes == 4) return makeBatch<float>(type, T, x, y, z, counts, c);
+    if (realBytes == 8) return makeBatch<double>(type, T, x, y, z, counts, c);
+    throw Error(AT_EINVAL, "real_bytes must be 4 or 8");
+}
+} // namespace at
+
+// ---------------------------------------------------------------------------
+// C ABI of batches
+// ---------------------------------------------------------------------------
+struct at_batch
+{
+    std::unique_ptr<at::BatchBase> impl;
+};
+
+namespace
+{
+template <class F>
+int guard(F &&f)
+{
+    try
+    {
+        f();
+        return AT_OK;
+    }
+    catch (const at::Error &e)
+    {
+        at::lastError() = e.what();
+        return e.code;
+    }
+    catch (const std::bad_alloc &e)
+    {
+        at::lastError() = e.what();
+        return AT_ENOMEM;
+    }
+    catch (const std::exception &e)
+    {
+        at::lastError() = e.what();
+        return AT_EINVAL;
+    }
+}
+
+at::BatchBase &batchOf(at_batch *b)
+{
+    if (!b || !b->impl) throw at::Error(AT_EINVAL, "null batch");
+    return *b->impl;
+}
+} // namespace
+
+extern "C" {
+int at_batch_create(int realBytes, int type, size_t numTrees, const void *const *x, const void *const *y,
+                    const void *const *z, const size_t *counts, const at_config *config, at_batch **out)
+{
+    return guard([&] {
+        if (!config || !out) throw at::Error(AT_EINVAL, "null argument");
+        auto b = std::make_unique<at_batch>();
+        b->impl.reset(at::createBatch(realBytes, type, numTrees, x, y, z, counts, *config));
+        *out = b.release();
+    });
+}
+int at_batch_destroy(at_batch *b)
+{
+    return guard([&] { delete b; });
+}
+int at_batch_build(at_batch *b)
+{
+    return guard([&] { batchOf(b).build(); });
+}
+int at_batch_update(at_batch *b)
+{
+    return guard([&] { batchOf(b).update(); });
+}
+int at_batch_needs_resize(at_batch *b, int *flags, size_t *count)
+{
+    return guard([&] {
+        const std::size_t n = batchOf(b).needsResize(flags);
+        if (count) *count = n;
+    });
+}
+int at_batch_resize_buffers(at_batch *b)
+{
+    return guard([&] { batchOf(b).resizeBuffers(); });
+}
+int at_batch_size(at_batch *b, size_t *n)
+{
+    return guard([&] {
+        if (!n) throw at::Error(AT_EINVAL, "null output pointer");
+        *n = batchOf(b).size();
+    });
+}
+int at_batch_tree_buffer(at_batch *b, size_t tree, int id, const void **d, size_t *s, size_t *c)
+{
+    return guard([&] {
+        if (!d || !s || !c) throw at::Error(AT_EINVAL, "null output pointer");
+        batchOf(b).tree(tree).buffer(id, d, s, c);
+    });
+}
+int at_batch_tree_scalars(at_batch *b, size_t tree, at_scalars *o)
+{
+    return guard([&] {
+        if (!o) throw at::Error(AT_EINVAL, "null output pointer");
+        batchOf(b).tree(tree).scalars(o);
+    });
+}
+int at_batch_graph_capture_update(at_batch *b, void *stream, void **exec)
+{
+    return guard([&] {
+        if (!exec) throw at::Error(AT_EINVAL, "null output pointer");
+        at::BatchBase &batch = batchOf(b);
+        hipStream_t s = batch.stream();
+        if (!s) throw at::Error(AT_EINVAL, "graph capture needs a batch created on a non-default stream");
+        if (stream && hipStream_t(stream) != s)
+            throw at::Error(AT_EINVAL, "capture stream must be the batch's configured stream");
+        hipGraph_t g = nullptr;
+        AT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal));
+        try
+        {
+            batch.update();
+        }
+        catch (...)
+        {
+            hipGraph_t dummy = nullptr;
+            (void)hipStreamEndCapture(s, &dummy);
+            if (dummy) (void)hipGraphDestroy(dummy);
+            throw;
+        }
+        AT_HIP(hipStreamEndCapture(s, &g));
+        hipGraphExec_t e = nullptr;
+        AT_HIP(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        *exec = e;
+    });
+}
+}

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../../include/adaptive_trees.h"
@@ -34,13 +35,29 @@ public:
     DevState *deviceState() const { return st_; }
 
 protected:
-    explicit TreeBase(hipStream_t s);
+    // st/sth: device and pinned host state slots owned by a batch (the tree
+    // then allocates none of its own).
+    explicit TreeBase(hipStream_t s, DevState *st = nullptr, DevState *sth = nullptr);
     void readState(); // synchronises the stream and copies the state to the host
     hipStream_t stream_ = nullptr;
     DevState *st_ = nullptr;  // device
     DevState *sth_ = nullptr; // pinned host copy
+    bool ownsState_ = true;
     SortWorkspace sort_;
 };
+
+// Resources a batch lends to each member tree: its state slots and its
+// segments of the batch's sorted keys and permutation.
+struct MemberSlot
+{
+    DevState *st = nullptr;
+    DevState *sth = nullptr;
+    Key *P = nullptr;
+    std::uint32_t *perm = nullptr;
+};
+
+// Message of the last failing C-ABI call on this thread.
+std::string &lastError();
 
 TreeBase *createTree(int realBytes, int type, const void *x, const void *y, const void *z, std::size_t n,
                      const at_config &c);

@@ -290,12 +290,16 @@ __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_
 
 // Second launch bound = minimum waves per SIMD: two blocks of 512 (or one
 // of 1024) per CU, i.e. at most 128 VGPRs.
-template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits>
+//
+// kSeg: segmented sort. The range comes from segRanges[blockIdx.x] and
+// rangeOff holds its absolute per-digit start ([range][digit]); values of the
+// first pass are slots within the segment.
+template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits, bool kSeg = false>
 __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel(
     const Key *__restrict__ kin, Key *__restrict__ kout, const std::uint32_t *__restrict__ vin,
     std::uint32_t *__restrict__ vout, std::uint8_t *__restrict__ digitsOut, std::size_t n, std::size_t rangeKeys,
     const std::uint32_t *__restrict__ rangeOff, const std::uint32_t *__restrict__ digitTotal, std::size_t ranges,
-    int shift, int nextShift)
+    int shift, int nextShift, const SegRange *__restrict__ segRanges = nullptr)
 {
     constexpr int kW = THREADS / 64;
     constexpr int kTile = ITEMS * THREADS;
@@ -314,14 +318,25 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
     const int wave = tid >> 6;
     const unsigned lane = laneId();
     const std::size_t r = blockIdx.x;
+    std::size_t rBegin;
+    std::uint32_t rLen, valueBase; // range-relative 32-bit offsets (n < 2^32)
+    if constexpr (kSeg)
+    {
+        const SegRange d = segRanges[r];
+        rBegin = d.begin;
+        rLen = d.len;
+        valueBase = d.localBase;
+        if (tid < kRadix) running[tid] = rangeOff[r * kRadix + tid];
+    }
+    else
     {
         const std::uint32_t total = tid < kRadix ? digitTotal[tid] : 0u;
         const std::uint32_t digitBase = digitExclusiveScan(total, scratch);
         if (tid < kRadix) running[tid] = digitBase + rangeOff[std::size_t(tid) * ranges + r];
+        rBegin = r * rangeKeys;
+        rLen = std::uint32_t((rBegin + rangeKeys < n ? rBegin + rangeKeys : n) - rBegin);
+        valueBase = std::uint32_t(rBegin);
     }
-    // range-relative 32-bit offsets (n < 2^32)
-    const std::size_t rBegin = r * rangeKeys;
-    const std::uint32_t rLen = std::uint32_t((rBegin + rangeKeys < n ? rBegin + rangeKeys : n) - rBegin);
     const Key *kr = kin + rBegin;
     const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
 
@@ -413,7 +428,7 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
             for (int j = 0; j < ITEMS; ++j)
             {
                 const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
-                stage.vals[rank[j]] = kIota ? std::uint32_t(rBegin) + idx : (idx < rLen ? vr[idx] : 0u);
+                stage.vals[rank[j]] = kIota ? valueBase + idx : (idx < rLen ? vr[idx] : 0u);
             }
             __syncthreads();
 #pragma unroll
@@ -426,6 +441,72 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
         if (tid < kRadix) running[tid] += tileCount;
         __syncthreads();
     }
+}
+
+// ---- segmented sort kernels --------------------------------------------------
+constexpr int kSegThreads = 512, kSegItems = 12; // tile shape of the segmented scatter
+
+template <class T, KeyKind K>
+__global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> p, const SegCoords<T> *coords,
+                                                                const SegRange *ranges, Key *keys,
+                                                                std::uint32_t *hist)
+{
+    __shared__ std::uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const SegRange d = ranges[blockIdx.x];
+    const SegCoords<T> c = coords[d.seg];
+    p.x = c.x;
+    p.y = c.y;
+    p.z = c.z;
+    for (std::uint32_t j = threadIdx.x; j < d.len; j += kSortThreads)
+    {
+        const Key k = particleKey<T, K>(p, std::size_t(d.localBase) + j);
+        keys[d.begin + j] = k;
+        countDigit(h, unsigned(k) & (kRadix - 1));
+    }
+    __syncthreads();
+    hist[std::size_t(blockIdx.x) * kRadix + threadIdx.x] = h[threadIdx.x];
+}
+
+// Digit histogram of one range from the digit bytes, [range][digit] layout.
+__global__ __launch_bounds__(kSortThreads) void segUpsweepDigitsKernel(const std::uint8_t *__restrict__ digits,
+                                                                       const SegRange *ranges, std::uint32_t *hist)
+{
+    __shared__ std::uint32_t h[kWaves][kRadix];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) h[w][threadIdx.x] = 0;
+    __syncthreads();
+    std::uint32_t *mine = h[threadIdx.x >> 6];
+    const SegRange d = ranges[blockIdx.x];
+    const std::uint8_t *src = digits + d.begin;
+    for (std::uint32_t j = threadIdx.x; j < d.len; j += kSortThreads) atomicAdd(&mine[src[j]], 1u);
+    __syncthreads();
+    std::uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += h[w][threadIdx.x];
+    hist[std::size_t(blockIdx.x) * kRadix + threadIdx.x] = t;
+}
+
+// One block per segment, one thread per digit: the absolute start of every
+// (range, digit) of the segment = segment begin + keys of smaller digits in
+// the segment + keys of this digit in earlier ranges of the segment.
+__global__ __launch_bounds__(kSortThreads) void segScanKernel(const std::uint32_t *__restrict__ hist,
+                                                              const std::uint32_t *segRange,
+                                                              const std::uint64_t *segBegin, std::uint32_t *off)
+{
+    __shared__ std::uint32_t waveTotals[kWaves];
+    const unsigned d = threadIdx.x;
+    const std::uint32_t r0 = segRange[blockIdx.x], r1 = segRange[blockIdx.x + 1];
+    std::uint32_t run = 0;
+    for (std::uint32_t r = r0; r < r1; ++r)
+    {
+        const std::uint32_t c = hist[std::size_t(r) * kRadix + d];
+        off[std::size_t(r) * kRadix + d] = run;
+        run += c;
+    }
+    const std::uint32_t base = std::uint32_t(segBegin[blockIdx.x]) + blockExclusiveScan(run, waveTotals);
+    for (std::uint32_t r = r0; r < r1; ++r) off[std::size_t(r) * kRadix + d] += base;
 }
 
 template <int THREADS, int ITEMS>
@@ -634,6 +715,105 @@ void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws)
                        0, ws.rangeHist);
     runPasses(s, pl, ws, keys, ws.keysAlt, nullptr, nullptr, false, false, true);
 }
+
+// ---- SegmentedSort -------------------------------------------------------------
+void SegmentedSort::plan(const std::vector<std::size_t> &counts, hipStream_t s)
+{
+    segments_ = counts.size();
+    segBegin_.assign(segments_ + 1, 0);
+    for (std::size_t i = 0; i < segments_; ++i) segBegin_[i + 1] = segBegin_[i] + counts[i];
+    total_ = segBegin_[segments_];
+    if (total_ >= (std::size_t(1) << 32)) throw Error(3, "segmented sort supports fewer than 2^32 keys in total");
+    constexpr std::size_t tile = std::size_t(kSegThreads) * kSegItems;
+    const std::size_t target = (total_ + 2047) / 2048;
+    rangeKeys_ = ((target + tile - 1) / tile) * tile;
+    if (rangeKeys_ < tile) rangeKeys_ = tile;
+    std::vector<SegRange> rs;
+    std::vector<std::uint32_t> first(segments_ + 1, 0);
+    for (std::size_t g = 0; g < segments_; ++g)
+    {
+        first[g] = std::uint32_t(rs.size());
+        for (std::size_t b = 0; b < counts[g]; b += rangeKeys_)
+        {
+            const std::size_t len = counts[g] - b < rangeKeys_ ? counts[g] - b : rangeKeys_;
+            rs.push_back(SegRange{std::uint64_t(segBegin_[g] + b), std::uint32_t(len), std::uint32_t(b),
+                                  std::uint32_t(g), 0u});
+        }
+    }
+    first[segments_] = std::uint32_t(rs.size());
+    ranges_ = rs.size();
+    std::vector<std::uint64_t> begins(segBegin_.begin(), segBegin_.end());
+    range_.ensure(ranges_ ? ranges_ : 1, 0, s);
+    segRange_.ensure(segments_ + 1, 0, s);
+    segBeginD_.ensure(segments_ + 1, 0, s);
+    hist_.ensure((ranges_ ? ranges_ : 1) * kRadix, 0, s);
+    off_.ensure((ranges_ ? ranges_ : 1) * kRadix, 0, s);
+    keysAlt_.ensure(total_ ? total_ : 1, 0, s);
+    valsAlt_.ensure(total_ ? total_ : 1, 0, s);
+    digits_.ensure(total_ ? total_ : 1, 0, s);
+    AT_HIP(hipStreamSynchronize(s));
+    if (ranges_) AT_HIP(hipMemcpy(range_.p, rs.data(), ranges_ * sizeof(SegRange), hipMemcpyHostToDevice));
+    AT_HIP(hipMemcpy(segRange_.p, first.data(), first.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice));
+    AT_HIP(hipMemcpy(segBeginD_.p, begins.data(), begins.size() * sizeof(std::uint64_t), hipMemcpyHostToDevice));
+}
+
+template <class T>
+void SegmentedSort::keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &shared,
+                                  const SegCoords<T> *coords, Key *keys, std::uint32_t *vals)
+{
+    if (!ranges_) return;
+    const dim3 rgrid{unsigned(ranges_)}, block{unsigned(kSortThreads)};
+    {
+        prof::Scope scope("keygen", s);
+        switch (kind)
+        {
+        case KeyKind::Morton:
+            hipLaunchKernelGGL((segKeygenKernel<T, KeyKind::Morton>), rgrid, block, 0, s, shared, coords, range_.p,
+                               keys, hist_.p);
+            break;
+        case KeyKind::Hilbert:
+            hipLaunchKernelGGL((segKeygenKernel<T, KeyKind::Hilbert>), rgrid, block, 0, s, shared, coords, range_.p,
+                               keys, hist_.p);
+            break;
+        case KeyKind::Binary:
+            hipLaunchKernelGGL((segKeygenKernel<T, KeyKind::Binary>), rgrid, block, 0, s, shared, coords, range_.p,
+                               keys, hist_.p);
+            break;
+        }
+    }
+    Key *kA = keys, *kB = keysAlt_.p;
+    std::uint32_t *vA = vals, *vB = valsAlt_.p;
+    for (int pass = 0; pass < kSortPasses; ++pass)
+    {
+        const Key *src = (pass & 1) ? kB : kA;
+        Key *dst = (pass & 1) ? kA : kB;
+        const std::uint32_t *vsrc = (pass & 1) ? vB : vA;
+        std::uint32_t *vdst = (pass & 1) ? vA : vB;
+        const int shift = pass * kRadixBits, next = shift + kRadixBits;
+        if (pass > 0)
+        {
+            prof::Scope scope("sort_upsweep", s);
+            hipLaunchKernelGGL(segUpsweepDigitsKernel, rgrid, block, 0, s, digits_.p, range_.p, hist_.p);
+        }
+        hipLaunchKernelGGL(segScanKernel, dim3(unsigned(segments_)), block, 0, s, hist_.p, segRange_.p,
+                           segBeginD_.p, off_.p);
+        prof::Scope scope("sort_pass", s);
+        const dim3 sgrid{unsigned(ranges_)}, sblock{unsigned(kSegThreads)};
+#define AT_SEG_SCATTER(I, D)                                                                                           \
+    hipLaunchKernelGGL((scatterKernel<kSegThreads, kSegItems, true, I, D, true>), sgrid, sblock, 0, s, src, dst, vsrc, \
+                       vdst, digits_.p, total_, rangeKeys_, off_.p, nullptr, ranges_, shift, next, range_.p)
+        if (pass == 0) AT_SEG_SCATTER(true, true);
+        else if (pass + 1 < kSortPasses) AT_SEG_SCATTER(false, true);
+        else AT_SEG_SCATTER(false, false);
+#undef AT_SEG_SCATTER
+    }
+    AT_HIP(hipGetLastError());
+}
+
+template void SegmentedSort::keygenAndSort<float>(hipStream_t, KeyKind, const KeygenParams<float> &,
+                                                  const SegCoords<float> *, Key *, std::uint32_t *);
+template void SegmentedSort::keygenAndSort<double>(hipStream_t, KeyKind, const KeygenParams<double> &,
+                                                   const SegCoords<double> *, Key *, std::uint32_t *);
 
 template void keygenAndSort<float>(hipStream_t, KeyKind, const KeygenParams<float> &, std::size_t, Key *,
                                    std::uint32_t *, SortWorkspace &);

@@ -1,6 +1,9 @@
 // sort.hpp -- stable LSD radix sort of 64-bit keys (+ 32-bit values) for gfx950.
 #pragma once
 
+#include <vector>
+
+#include "buffers.hpp"
 #include "common.hpp"
 
 namespace at
@@ -80,4 +83,51 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
 void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws,
                bool valuesAreIndices);
 void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws);
+
+// ---- segmented sort: many independent key arrays in one set of launches ----
+// Segment s holds counts[s] particles at [segBegin[s], segBegin[s+1]) of one
+// concatenated key array and is sorted on its own (stable, values = the
+// particle's slot within its segment). Every stage is one launch over all
+// segments: ranges never straddle a segment, the per-segment offset scan
+// runs one block per segment.
+struct SegRange
+{
+    std::uint64_t begin;    // first key of the range in the concatenated array
+    std::uint32_t len;      // keys in the range
+    std::uint32_t localBase; // slot of the first key within its segment
+    std::uint32_t seg;
+    std::uint32_t pad;
+};
+
+template <class T>
+struct SegCoords
+{
+    const T *x, *y, *z;
+};
+
+class SegmentedSort
+{
+public:
+    // Plans ranges for the given segment sizes and allocates scratch.
+    void plan(const std::vector<std::size_t> &counts, hipStream_t s);
+    std::size_t total() const { return total_; }
+    std::size_t segBegin(std::size_t seg) const { return segBegin_[seg]; }
+    std::size_t ranges() const { return ranges_; }
+    // Keys of every segment (particle j of segment s from coords[s]) into
+    // keys[segBegin[s] + ...], sorted per segment with vals = slot in segment.
+    template <class T>
+    void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &shared, const SegCoords<T> *coords,
+                       Key *keys, std::uint32_t *vals);
+
+private:
+    std::size_t total_ = 0, ranges_ = 0, segments_ = 0, rangeKeys_ = 0;
+    std::vector<std::size_t> segBegin_;
+    DevBuf<SegRange> range_;
+    DevBuf<std::uint32_t> segRange_;   // first range of each segment [segments + 1]
+    DevBuf<std::uint64_t> segBeginD_;  // [segments + 1]
+    DevBuf<std::uint32_t> hist_, off_; // [ranges][kRadix]
+    DevBuf<Key> keysAlt_;
+    DevBuf<std::uint32_t> valsAlt_;
+    DevBuf<std::uint8_t> digits_;
+};
 } // namespace at

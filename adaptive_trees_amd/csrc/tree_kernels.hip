@@ -186,9 +186,8 @@ struct BinDecider
 };
 
 template <class Dec>
-__global__ __launch_bounds__(kThreads) void decideScanKernel(Dec dec, DevState *st, long capacity,
-                                                             std::int64_t *offs, std::uint64_t *status,
-                                                             std::uint32_t *ticket)
+__device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std::int64_t *offs,
+                               std::uint64_t *status, std::uint32_t *ticket)
 {
     __shared__ std::uint32_t tileShared;
     __shared__ std::uint64_t wsum[kThreads / 64];
@@ -260,13 +259,21 @@ __global__ __launch_bounds__(kThreads) void decideScanKernel(Dec dec, DevState *
     }
 }
 
+template <class Dec>
+__global__ __launch_bounds__(kThreads) void decideScanKernel(Dec dec, DevState *st, long capacity,
+                                                             std::int64_t *offs, std::uint64_t *status,
+                                                             std::uint32_t *ticket)
+{
+    decideScanBody(dec, st, capacity, offs, status, ticket);
+}
+
 // ---- octree scatter (processNode) -------------------------------------------
-__global__ __launch_bounds__(kThreads) void octProcessKernel(TreeDev t, bool nf)
+__device__ void octProcessBody(const TreeDev &t, bool nf, unsigned block)
 {
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = int(block * blockDim.x + threadIdx.x);
     if (i >= L) return;
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
@@ -327,13 +334,15 @@ __global__ __launch_bounds__(kThreads) void octProcessKernel(TreeDev t, bool nf)
     }
 }
 
+__global__ __launch_bounds__(kThreads) void octProcessKernel(TreeDev t, bool nf) { octProcessBody(t, nf, blockIdx.x); }
+
 // ---- binary scatter (binaryProcessNodesKernel) --------------------------------
-__global__ __launch_bounds__(kThreads) void binProcessKernel(TreeDev t)
+__device__ void binProcessBody(const TreeDev &t, unsigned block)
 {
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = int(block * blockDim.x + threadIdx.x);
     if (i >= L) return;
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
@@ -346,11 +355,13 @@ __global__ __launch_bounds__(kThreads) void binProcessKernel(TreeDev t)
     if (op == 2) out[dst + 1] = tree[i] + (tree[i + 1] - tree[i]) / 2;
 }
 
+__global__ __launch_bounds__(kThreads) void binProcessKernel(TreeDev t) { binProcessBody(t, blockIdx.x); }
+
 // Commit the pass and clear the chained-scan status for the next one.
-__global__ void commitKernel(DevState *st, int binary, std::uint64_t *status, std::size_t words,
-                             std::uint32_t *ticket, std::uint32_t *nfCount)
+__device__ void commitBody(DevState *st, int binary, std::uint64_t *status, std::size_t words, std::uint32_t *ticket,
+                           std::uint32_t *nfCount, unsigned block, unsigned blocks)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (block == 0 && threadIdx.x == 0)
     {
         if (!st->converged)
         {
@@ -361,16 +372,20 @@ __global__ void commitKernel(DevState *st, int binary, std::uint64_t *status, st
         *ticket = 0;
         if (nfCount) *nfCount = 0;
     }
-    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
-         w += std::size_t(gridDim.x) * blockDim.x)
+    for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < words; w += std::size_t(blocks) * blockDim.x)
         status[w] = 0;
+}
+__global__ void commitKernel(DevState *st, int binary, std::uint64_t *status, std::size_t words,
+                             std::uint32_t *ticket, std::uint32_t *nfCount)
+{
+    commitBody(st, binary, status, words, ticket, nfCount, blockIdx.x, gridDim.x);
 }
 
 // Start of update(): resetRebalanceStateKernel (octree_rebalance.cuh:62-73).
-__global__ void resetKernel(DevState *st, const Key *P, std::size_t n, std::uint64_t *status, std::size_t words,
-                            std::uint32_t *ticket, std::uint32_t *nfCount)
+__device__ void resetBody(DevState *st, const Key *P, std::size_t n, std::uint64_t *status, std::size_t words,
+                          std::uint32_t *ticket, std::uint32_t *nfCount, unsigned block, unsigned blocks)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (block == 0 && threadIdx.x == 0)
     {
         st->changed = 0;
         st->converged = 0;
@@ -378,15 +393,19 @@ __global__ void resetKernel(DevState *st, const Key *P, std::size_t n, std::uint
         *ticket = 0;
         if (nfCount) *nfCount = 0;
     }
-    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
-         w += std::size_t(gridDim.x) * blockDim.x)
+    for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < words; w += std::size_t(blocks) * blockDim.x)
         status[w] = 0;
+}
+__global__ void resetKernel(DevState *st, const Key *P, std::size_t n, std::uint64_t *status, std::size_t words,
+                            std::uint32_t *ticket, std::uint32_t *nfCount)
+{
+    resetBody(st, P, n, status, words, ticket, nfCount, blockIdx.x, gridDim.x);
 }
 
 // initRootOctreeKernel + resetRebalanceState of the build (octree.cu:24-38, 159-163)
-__global__ void octInitKernel(TreeDev t, std::size_t words)
+__device__ void octInitBody(const TreeDev &t, std::size_t words, unsigned block, unsigned blocks)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (block == 0 && threadIdx.x == 0)
     {
         Key *K = t.K[0];
         K[0] = 0;
@@ -403,15 +422,15 @@ __global__ void octInitKernel(TreeDev t, std::size_t words)
         *t.scanTicket = 0;
         if (t.nfCount) *t.nfCount = 0;
     }
-    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
-         w += std::size_t(gridDim.x) * blockDim.x)
+    for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < words; w += std::size_t(blocks) * blockDim.x)
         t.scanStatus[w] = 0;
 }
+__global__ void octInitKernel(TreeDev t, std::size_t words) { octInitBody(t, words, blockIdx.x, gridDim.x); }
 
 // initBinaryRootKernel + refreshCounts (kdtree3d.cu:24-30, 139-146)
-__global__ void binInitKernel(TreeDev t, std::size_t words)
+__device__ void binInitBody(const TreeDev &t, std::size_t words, unsigned block, unsigned blocks)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (block == 0 && threadIdx.x == 0)
     {
         t.K[0][0] = 0;
         t.K[0][1] = kMaxKey;
@@ -425,18 +444,18 @@ __global__ void binInitKernel(TreeDev t, std::size_t words)
         t.N[0][0] = active;
         *t.scanTicket = 0;
     }
-    for (std::size_t w = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
-         w += std::size_t(gridDim.x) * blockDim.x)
+    for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < words; w += std::size_t(blocks) * blockDim.x)
         t.scanStatus[w] = 0;
 }
+__global__ void binInitKernel(TreeDev t, std::size_t words) { binInitBody(t, words, blockIdx.x, gridDim.x); }
 
 // N[i] = lb(P, K[i+1]) - lb(P, K[i]) over all slots (calculateNodeCount).
-__global__ __launch_bounds__(kThreads) void leafCountsKernel(TreeDev t)
+__device__ void leafCountsBody(const TreeDev &t, unsigned block)
 {
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = int(block * blockDim.x + threadIdx.x);
     if (i >= L) return;
     const int ab = st->activeBuffer;
     const Key *K = t.K[ab];
@@ -445,6 +464,7 @@ __global__ __launch_bounds__(kThreads) void leafCountsKernel(TreeDev t)
     const std::size_t c = b - a;
     t.N[ab][i] = c > 0xffffffffu ? 0xffffffffu : unsigned(c);
 }
+__global__ __launch_bounds__(kThreads) void leafCountsKernel(TreeDev t) { leafCountsBody(t, blockIdx.x); }
 
 // ---- near-field work list ------------------------------------------------------
 __device__ inline bool dirtyLeader(const Key *tree, const std::uint8_t *gDirty, int L, int i)
@@ -455,12 +475,12 @@ __device__ inline bool dirtyLeader(const Key *tree, const std::uint8_t *gDirty, 
     return siblingIndex(tree, L, i, sl) == 0 && sl > 0;
 }
 
-__global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full)
+__device__ void nfCollectBody(const TreeDev &t, int full, unsigned block)
 {
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = int(block * blockDim.x + threadIdx.x);
     const int ab = st->activeBuffer;
     bool own = false, lead = false;
     if (i < L)
@@ -478,6 +498,7 @@ __global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full)
     if (own) t.nfTasks[baseIdx + __popcll(bo & lt)] = std::uint32_t(i) << 1;
     if (lead) t.nfTasks[baseIdx + no + __popcll(bl & lt)] = (std::uint32_t(i) << 1) | 1u;
 }
+__global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full) { nfCollectBody(t, full, blockIdx.x); }
 
 // Near-field count of the level-`level` node at `start`: its own population
 // times the population of the 27 same-level cells, periodic at 2^21 per axis
@@ -513,7 +534,7 @@ __device__ inline void nearField(const Key *P, std::size_t n, Key start, unsigne
     center = __shfl(pop, 13, 32);
 }
 
-__global__ __launch_bounds__(kThreads) void nfComputeKernel(TreeDev t, unsigned bucket, int hilbert)
+__device__ void nfComputeBody(const TreeDev &t, unsigned bucket, int hilbert, unsigned block, unsigned blocks)
 {
     DevState *st = t.st;
     if (st->converged) return;
@@ -521,8 +542,8 @@ __global__ __launch_bounds__(kThreads) void nfComputeKernel(TreeDev t, unsigned 
     const Key *tree = t.K[ab];
     const std::uint32_t total = *t.nfCount;
     const unsigned lane32 = threadIdx.x & 31;
-    const std::uint32_t groups = gridDim.x * (blockDim.x / 32);
-    for (std::uint32_t g = blockIdx.x * (blockDim.x / 32) + threadIdx.x / 32; g < total; g += groups)
+    const std::uint32_t groups = blocks * (blockDim.x / 32);
+    for (std::uint32_t g = block * (blockDim.x / 32) + threadIdx.x / 32; g < total; g += groups)
     {
         const std::uint32_t task = t.nfTasks[g];
         const int leaf = int(task >> 1);
@@ -544,6 +565,10 @@ __global__ __launch_bounds__(kThreads) void nfComputeKernel(TreeDev t, unsigned 
                 t.gcm[ab][leaf] = nf <= std::uint64_t(bucket) ? 1 : 0;
         }
     }
+}
+__global__ __launch_bounds__(kThreads) void nfComputeKernel(TreeDev t, unsigned bucket, int hilbert)
+{
+    nfComputeBody(t, bucket, hilbert, blockIdx.x, gridDim.x);
 }
 
 // ---- one-pass converged binary tree ---------------------------------------------
@@ -817,7 +842,136 @@ unsigned clearBlocks(std::size_t words)
     const unsigned b = ceilDiv(words, 256);
     return b < 1 ? 1 : (b > 1024 ? 1024 : b);
 }
+
+// ---- multi-tree kernels: block -> (tree, block within the tree) ------------------
+__device__ inline int treeOfBlock(const TreeSet &s, unsigned b)
+{
+    int lo = 0, hi = s.count - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s.start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define AT_MULTI_PROLOGUE                                                                                              \
+    const int k = treeOfBlock(s, blockIdx.x);                                                                          \
+    const TreeDev t = s.trees[k];                                                                                      \
+    const unsigned block = blockIdx.x - s.start[k];                                                                    \
+    const unsigned blocks = s.start[k + 1] - s.start[k];                                                               \
+    (void)block;                                                                                                       \
+    (void)blocks
+
+__global__ void multiOctInitKernel(TreeSet s)
+{
+    AT_MULTI_PROLOGUE;
+    octInitBody(t, t.statusWords, block, blocks);
+}
+__global__ void multiResetKernel(TreeSet s)
+{
+    AT_MULTI_PROLOGUE;
+    resetBody(t.st, t.P, t.n, t.scanStatus, t.statusWords, t.scanTicket, t.nfCount, block, blocks);
+}
+// Clears the chained-scan status words and ticket of each tree (the build
+// loop's per-pass reset, without a memset per tree).
+__global__ void multiClearScanKernel(TreeSet s)
+{
+    AT_MULTI_PROLOGUE;
+    if (block == 0 && threadIdx.x == 0) *t.scanTicket = 0;
+    for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < t.statusWords;
+         w += std::size_t(blocks) * blockDim.x)
+        t.scanStatus[w] = 0;
+}
+__global__ __launch_bounds__(kThreads) void multiOctDecideScanKernel(TreeSet s, unsigned bucket, int nf)
+{
+    AT_MULTI_PROLOGUE;
+    const OctDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, {t.NF[0], t.NF[1]}, {t.gcm[0], t.gcm[1]}, bucket, nf != 0};
+    decideScanBody(d, t.st, t.capacity, t.offs, t.scanStatus, t.scanTicket);
+}
+__global__ __launch_bounds__(kThreads) void multiOctProcessKernel(TreeSet s, int nf)
+{
+    AT_MULTI_PROLOGUE;
+    octProcessBody(t, nf != 0, block);
+}
+__global__ void multiCommitKernel(TreeSet s, int binary)
+{
+    AT_MULTI_PROLOGUE;
+    commitBody(t.st, binary, t.scanStatus, t.statusWords, t.scanTicket, t.nfCount, block, blocks);
+}
+__global__ __launch_bounds__(kThreads) void multiLeafCountsKernel(TreeSet s)
+{
+    AT_MULTI_PROLOGUE;
+    leafCountsBody(t, block);
+}
+__global__ __launch_bounds__(kThreads) void multiNfCollectKernel(TreeSet s, int full)
+{
+    AT_MULTI_PROLOGUE;
+    nfCollectBody(t, full, block);
+}
+__global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsigned bucket, int hilbert)
+{
+    AT_MULTI_PROLOGUE;
+    nfComputeBody(t, bucket, hilbert, block, blocks);
+}
+#undef AT_MULTI_PROLOGUE
 } // namespace
+
+unsigned clearBlocksFor(std::size_t statusWords) { return clearBlocks(statusWords); }
+unsigned tileBlocksFor(std::size_t gridLeaves) { return unsigned(gridLeaves / kPassTile + 1); }
+unsigned leafBlocksFor(std::size_t gridLeaves)
+{
+    const unsigned b = ceilDiv(gridLeaves, kThreads);
+    return b < 1 ? 1 : b;
+}
+unsigned nfBlocksFor(std::size_t gridLeaves)
+{
+    const unsigned b = ceilDiv(gridLeaves, 64);
+    return b < 1 ? 1 : (b > 2048 ? 2048 : b);
+}
+
+void multiOctInit(hipStream_t s, const TreeSet &clear)
+{
+    if (clear.blocks) hipLaunchKernelGGL(multiOctInitKernel, dim3(clear.blocks), dim3(256), 0, s, clear);
+}
+void multiReset(hipStream_t s, const TreeSet &clear)
+{
+    if (clear.blocks) hipLaunchKernelGGL(multiResetKernel, dim3(clear.blocks), dim3(256), 0, s, clear);
+}
+void multiClearScan(hipStream_t s, const TreeSet &clear)
+{
+    if (clear.blocks) hipLaunchKernelGGL(multiClearScanKernel, dim3(clear.blocks), dim3(256), 0, s, clear);
+}
+void multiOctDecideScan(hipStream_t s, const TreeSet &tiles, const OctParams &p)
+{
+    prof::Scope scope("decide_scan", s);
+    if (tiles.blocks)
+        hipLaunchKernelGGL(multiOctDecideScanKernel, dim3(tiles.blocks), dim3(kThreads), 0, s, tiles, p.bucket,
+                           p.nf ? 1 : 0);
+}
+void multiOctProcess(hipStream_t s, const TreeSet &leaves, const OctParams &p)
+{
+    if (leaves.blocks)
+        hipLaunchKernelGGL(multiOctProcessKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, p.nf ? 1 : 0);
+}
+void multiCommit(hipStream_t s, const TreeSet &clear, bool binary)
+{
+    if (clear.blocks) hipLaunchKernelGGL(multiCommitKernel, dim3(clear.blocks), dim3(256), 0, s, clear, binary ? 1 : 0);
+}
+void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, const OctParams &p, bool full)
+{
+    if (!leaves.blocks) return;
+    if (!p.nf)
+    {
+        prof::Scope scope("leaf_counts", s);
+        hipLaunchKernelGGL(multiLeafCountsKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves);
+        return;
+    }
+    prof::Scope scope("near_field", s);
+    hipLaunchKernelGGL(multiNfCollectKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, full ? 1 : 0);
+    hipLaunchKernelGGL(multiNfComputeKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.bucket, p.hilbert ? 1 : 0);
+}
 
 std::size_t passTiles(std::size_t cap) { return cap / kPassTile + 1; }
 std::size_t scanStatusWords(std::size_t items) { return items / kPassTile + 2; }

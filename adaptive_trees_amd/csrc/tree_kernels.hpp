@@ -24,7 +24,25 @@ struct TreeDev
     DevState *st = nullptr;
     const Key *P = nullptr;
     std::size_t n = 0;                           // particle slots
+    long capacity = 0;                           // leaf capacity of a pass (multi-tree launches)
 };
+
+// Trees driven by one launch. Block b belongs to the tree k with
+// start[k] <= b < start[k+1]; start is a host-computed prefix of per-tree
+// block counts (one table per launch shape), copied to the device.
+struct TreeSet
+{
+    const TreeDev *trees = nullptr;       // device array [count]
+    const std::uint32_t *start = nullptr; // device array [count + 1]
+    int count = 0;
+    unsigned blocks = 0;                  // start[count]
+};
+
+// Per-tree block counts of each multi-tree launch shape.
+unsigned clearBlocksFor(std::size_t statusWords); // status-word clearing (init, reset, commit)
+unsigned tileBlocksFor(std::size_t gridLeaves);   // decision + chained scan tiles
+unsigned leafBlocksFor(std::size_t gridLeaves);   // one thread per leaf
+unsigned nfBlocksFor(std::size_t gridLeaves);     // near-field work groups
 
 constexpr int kPassTile = 2048;                  // leaves per chained-scan tile
 std::size_t passTiles(std::size_t cap);
@@ -76,4 +94,13 @@ constexpr unsigned kDirectMaxBucket = 1024;
 void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d);
 void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves);
 void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves);
+
+// ---- multi-tree launches (segmented over a TreeSet) ----
+void multiOctInit(hipStream_t s, const TreeSet &clear);
+void multiReset(hipStream_t s, const TreeSet &clear);
+void multiClearScan(hipStream_t s, const TreeSet &clear);
+void multiOctDecideScan(hipStream_t s, const TreeSet &tiles, const OctParams &p);
+void multiOctProcess(hipStream_t s, const TreeSet &leaves, const OctParams &p);
+void multiCommit(hipStream_t s, const TreeSet &clear, bool binary);
+void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, const OctParams &p, bool full);
 } // namespace at

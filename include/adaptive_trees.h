@@ -171,6 +171,34 @@ int at_profile_query(const char *label, double *total_ms, unsigned long long *la
  * kernel that calibrates hardware byte counters. */
 int at_bench_copy(void *dst, const void *src, size_t bytes, void *stream);
 
+/* ---- batches: many independent octrees, shared launches ----------------
+ * Batched build/update of independent trees (no reference counterpart: the
+ * reference drives one Tree per call; this replaces a loop of
+ * Tree<Real,Octree>::build()/update() calls, tree.hpp:35-41, with one
+ * segmented launch per stage). All members share `config` (octree only;
+ * LeafCount or NFCount, Morton or Hilbert); member k borrows the device
+ * coordinates x[k], y[k], z[k] of counts[k] particles. Each member's
+ * results equal those of its own Tree after the same calls.
+ * at_batch_update() is capturable; at_batch_graph_capture_update() records
+ * it on the batch's (non-NULL) stream. Member buffers are read with
+ * at_batch_tree_buffer (same ids as at_tree_buffer). */
+typedef struct at_batch at_batch;
+int at_batch_create(int real_bytes, int tree_type, size_t num_trees, const void *const *x,
+                    const void *const *y, const void *const *z, const size_t *counts,
+                    const at_config *config, at_batch **out);
+int at_batch_destroy(at_batch *batch);
+int at_batch_build(at_batch *batch);
+int at_batch_update(at_batch *batch);
+/* flags: num_trees ints (may be NULL); count: members needing a resize. */
+int at_batch_needs_resize(at_batch *batch, int *flags, size_t *count);
+/* Grows the members whose last update was rejected for capacity. */
+int at_batch_resize_buffers(at_batch *batch);
+int at_batch_size(at_batch *batch, size_t *num_trees);
+int at_batch_tree_buffer(at_batch *batch, size_t tree, int buffer_id, const void **data,
+                         size_t *size, size_t *capacity);
+int at_batch_tree_scalars(at_batch *batch, size_t tree, at_scalars *out);
+int at_batch_graph_capture_update(at_batch *batch, void *stream, void **graph_exec);
+
 /* ---- tuning (process-wide; set before building trees) ------------------ */
 /* "sort_threads": threads per radix-sort block (256, 512 or 1024);
  * "sort_items":  keys per thread of a radix-sort tile (8, 12 or 16);
