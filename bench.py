@@ -25,7 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from adaptive_trees_amd import BINARY, DeviceArray, Tree, make_config  # noqa: E402
+from adaptive_trees_amd import BINARY, OCTREE, Batch, DeviceArray, Tree, make_config  # noqa: E402
 from adaptive_trees_amd import _ffi, workloads  # noqa: E402
 
 METRIC = "Mparticles/s build+update, % HBM roofline; batched vs plain speedup; p99 latency"
@@ -36,6 +36,10 @@ COPY_BYTES = 1 << 30  # calibration copy: reads and writes this many bytes
 
 def _lib():
     return _ffi.hip_lib()
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
 class Stream:
@@ -102,6 +106,114 @@ def cpu_baseline(x, y, z, x1, y1, z1, box, min_seconds):
                       "median of %d repeats (%.2f s each, %.1f s total)" % (n, len(times), dt, sum(times))}
 
 
+def batch_workload(trees, seed):
+    """512 octree inputs of the batched workload: sizes log-uniform in [1e3, 2e6], uniform or
+    Plummer per tree by a seed bit, FP32, plus one reflected-Gaussian moved snapshot
+    (sigma = 1e-3 of the unit box). Returns sizes and two concatenated (x, y, z) snapshots."""
+    sizes = workloads.batch_sizes(trees, seed=seed)
+    rng = np.random.default_rng(seed)
+    kinds = rng.integers(0, 2, size=trees)
+    first, moved = [[], [], []], [[], [], []]
+    for k, n in enumerate(sizes):
+        gen = workloads.plummer if kinds[k] else workloads.uniform_cube
+        c = gen(int(n), seed=seed * 100003 + k, dtype=np.float32)
+        m = workloads.reflect_step(c, 1e-3, seed=seed * 100019 + k)
+        for a in range(3):
+            first[a].append(c[a])
+            moved[a].append(m[a])
+    return sizes, [np.concatenate(v) for v in first], [np.concatenate(v) for v in moved]
+
+
+def batch_benchmark(trees=512, rounds=5, seed=4):
+    """Batched path (one Batch: segmented build, then `rounds` graph-replayed updates) against
+    the plain path (one Tree per input on one stream: build() + rounds x (update() +
+    doBuffersNeedResize())) on the same device-resident particles. Each round first copies the
+    next snapshot into the borrowed coordinate arrays (one device copy per axis for all trees,
+    the same in both paths)."""
+    lib = _lib()
+    sizes, first, moved = batch_workload(trees, seed)
+    progress("batched workload generated: %d trees, %d particles" % (trees, int(sizes.sum())))
+    total = int(sizes.sum())
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    cur = [DeviceArray(a) for a in first]
+    snaps = [[DeviceArray(a) for a in first], [DeviceArray(a) for a in moved]]
+    nbytes = total * 4
+    stream = Stream()
+    cfg = dict(max_particles_per_leaf=64)
+
+    def ptrs(a):
+        return [cur[a].ptr + 4 * int(off[k]) for k in range(trees)]
+
+    def load(r):
+        for a in range(3):
+            lib.call("memcpy_async", cur[a].ptr, snaps[r % 2][a].ptr, nbytes, 3, stream.p)
+
+    def resize_plain(ts):
+        for t in ts:
+            while t.doBuffersNeedResize():
+                t.resizeBuffers()
+                t.update()
+
+    # plain: 512 independent trees on one stream
+    plain = [Tree(*(ptrs(a)[k] for a in range(3)), int(sizes[k]), make_config(stream=stream.p, **cfg),
+                  real=np.float32, kind=OCTREE) for k in range(trees)]
+
+    def run_plain():
+        load(0)
+        for t in plain:
+            t.build()
+        for r in range(1, rounds + 1):
+            load(r)
+            for t in plain:
+                t.update()
+            resize_plain(plain)
+        stream.sync()
+
+    batch = Batch(ptrs(0), ptrs(1), ptrs(2), sizes, make_config(stream=stream.p, **cfg), real=np.float32)
+    graph = [None]
+
+    def run_batch():
+        load(0)
+        batch.build()
+        graph[0] = graph[0] or batch.capture_update(stream.p)
+        for r in range(1, rounds + 1):
+            load(r)
+            lib.call("graph_launch", graph[0], stream.p)
+            while batch.needsResize().any():
+                batch.resizeBuffers()
+                lib.call("graph_destroy", graph[0])
+                graph[0] = batch.capture_update(stream.p)
+                lib.call("graph_launch", graph[0], stream.p)
+        stream.sync()
+
+    run_plain()  # warm-up: allocations reach their steady state
+    run_batch()
+    progress("batched/plain warm-up done")
+    t0 = time.perf_counter()
+    run_plain()
+    plain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    run_batch()
+    batch_s = time.perf_counter() - t0
+    sample = list(range(0, trees, max(1, trees // 32)))
+    exact = all(np.array_equal(batch.tree(k).get(n), plain[k].get(n))
+                for k in sample for n in ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d"))
+    work = total * (rounds + 1)
+    return {
+        "workload": "%d octrees, sizes log-uniform [1e3, 2e6] (total %d), FP32 uniform/Plummer, Morton, "
+                    "LeafCount 64; build + %d graph-replayed updates (sigma 1e-3 reflected steps)"
+                    % (trees, total, rounds),
+        "particles": total,
+        "plain_s": round(plain_s, 4),
+        "batched_s": round(batch_s, 4),
+        "speedup": round(plain_s / batch_s, 3),
+        "batched_Mparticles_per_s": round(work / batch_s / 1e6, 1),
+        "plain_Mparticles_per_s": round(work / plain_s / 1e6, 1),
+        "bit_exact_sample_trees": len(sample),
+        "bit_exact": bool(exact),
+    }
+
+
 def _counter_rows(path):
     """(kernel name, counter value) per dispatch from a rocprofv3 counter_collection.csv."""
     import csv
@@ -145,6 +257,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the known-size copy kernel (for hardware counter scaling)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
+    ap.add_argument("--only-batch", action="store_true", help="run only the batched-vs-plain measurement")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
@@ -159,6 +273,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (requested %d)" % max(args.gpus, world),
               file=sys.stderr)
         sys.exit(2)
+
+    if args.only_batch:
+        print(json.dumps({"batched": batch_benchmark()}))
+        return
 
     n = args.n
     box = workloads.ANISO_BOX
@@ -291,6 +409,9 @@ def main():
             "frac_of_peak": round(build_bytes / (build_wall_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4),
         },
     }
+    if not args.no_batch:
+        tree.close()
+        out["batched"] = batch_benchmark()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, z, x1, y1, z1, box, args.cpu_seconds)
     else:
