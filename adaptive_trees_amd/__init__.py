@@ -8,3 +8,4 @@ from ._ffi import (BINARY, HILBERT, LEAF_COUNT, MORTON, NF_COUNT, OCTREE,  # noq
                    InvalidArgument, LengthError, LogicError, TreeError)
 from .tree import DeviceArray, Tree, make_config  # noqa: F401
 from .batch import Batch  # noqa: F401
+from .server import Server  # noqa: F401

@@ -85,6 +85,16 @@ class LengthError(TreeError):
     pass
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("workers", ctypes.c_int32), ("max_batch", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("bytes_in_use", "bytes_peak", "hbm_budget", "resident_trees",
+                                                "waiting_builds", "completed", "batched_updates",
+                                                "update_batches")]
+
+
 _ERRORS = {AT_EINVAL: InvalidArgument, AT_ELOGIC: LogicError, AT_ELENGTH: LengthError}
 
 _P = ctypes.c_void_p
@@ -179,6 +189,20 @@ def hip_lib():
             "batch_tree_buffer": [_P, _SZ, _I, ctypes.POINTER(_P), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)],
             "batch_tree_scalars": [_P, _SZ, ctypes.POINTER(Scalars)],
             "batch_graph_capture_update": [_P, _P, ctypes.POINTER(_P)],
+            "server_create": [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)],
+            "server_destroy": [_P],
+            "server_submit_build": [_P, _I, _I, _P, _P, _P, _SZ, ctypes.POINTER(Config),
+                                    ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
+            "server_submit_update": [_P, ctypes.c_uint64, _P, _P, _P, ctypes.POINTER(ctypes.c_uint64)],
+            "server_submit_release": [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)],
+            "server_wait": [_P, ctypes.c_uint64, ctypes.POINTER(_I)],
+            "server_poll": [_P, ctypes.c_uint64, ctypes.POINTER(_I)],
+            "server_request_times": [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
+            "server_tree_buffer": [_P, ctypes.c_uint64, _I, ctypes.POINTER(_P), ctypes.POINTER(_SZ),
+                                   ctypes.POINTER(_SZ)],
+            "server_tree_scalars": [_P, ctypes.c_uint64, ctypes.POINTER(Scalars)],
+            "server_get_stats": [_P, ctypes.POINTER(ServerStats)],
         }.items():
             f = getattr(d, "at_" + name)
             f.argtypes = args

@@ -1,0 +1,544 @@
+// server.hip -- a request loop that keeps trees resident in HBM and serves
+// mixed build / update / release requests with continuous batching.
+//
+//   admission  build requests wait in arrival order until their estimated
+//              device bytes fit the HBM budget; a release returns its bytes
+//              and admits the waiting builds that now fit.
+//   placement  each admitted tree is pinned to one worker (a host thread
+//              owning one HIP stream), the worker with the least queued
+//              particle work; all requests of a tree run in submission
+//              order on that worker.
+//   batching   a worker takes the next request and, if it is an update, the
+//              updates of other trees queued behind it (up to max_batch):
+//              their new positions are copied in and their update() work is
+//              launched back to back on the stream before one
+//              synchronisation, then each tree's resize protocol
+//              (doBuffersNeedResize / resizeBuffers / update) completes it.
+//              Workers run concurrently, so the work of requests admitted
+//              while others are on the GPU overlaps on the device.
+// Every request runs the same tree operations it would run alone, on a tree
+// that only its own requests touch, so results do not depend on the mix.
+#include "engine.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+namespace at
+{
+namespace
+{
+using Clock = std::chrono::steady_clock;
+
+enum Kind { kBuild, kUpdate, kRelease };
+enum State { kQueued, kRunning, kDone };
+
+struct Request
+{
+    std::uint64_t id = 0, tree = 0;
+    Kind kind = kBuild;
+    // build
+    int realBytes = 8, type = AT_TREE_OCTREE;
+    const void *x = nullptr, *y = nullptr, *z = nullptr;
+    std::size_t n = 0;
+    at_config cfg{};
+    // update: new positions copied into the tree's coordinates (may be null)
+    const void *nx = nullptr, *ny = nullptr, *nz = nullptr;
+    State state = kQueued;
+    int status = AT_OK;
+    std::string error;
+    Clock::time_point submit, start, done;
+};
+
+struct Resident
+{
+    std::unique_ptr<TreeBase> tree;
+    int worker = -1;
+    bool admitted = false;
+    std::size_t bytes = 0, n = 0;
+    int realBytes = 8;
+    const void *x = nullptr, *y = nullptr, *z = nullptr;
+    std::deque<Request *> held; // requests submitted before the build was admitted
+};
+
+std::size_t estimateBytes(std::size_t n, int realBytes, const at_config &c)
+{
+    // P + Perm + sort scratch (keys, values, digit bytes) + leaves (~n/8
+    // leaves x two halves of K, N and the scan) + views when configured
+    const bool views = c.views[0] || c.views[1] || c.views[2];
+    (void)realBytes;
+    return n * (8 + 4 + 13) + (n / 8 + 64) * 48 + (views ? n * 8 + (n / 8) * 26 : 0) + (std::size_t(1) << 20);
+}
+} // namespace
+
+class Server
+{
+public:
+    explicit Server(const at_server_config &c)
+        : budget_(c.hbm_budget_bytes), maxBatch_(c.max_batch > 0 ? c.max_batch : 16), t0_(Clock::now())
+    {
+        const int w = c.workers > 0 ? c.workers : 4;
+        if (w > 32) throw Error(AT_EINVAL, "at most 32 server workers");
+        workers_.resize(std::size_t(w));
+        for (auto &wk : workers_) AT_HIP(hipStreamCreateWithFlags(&wk.stream, hipStreamNonBlocking));
+        for (int i = 0; i < w; ++i) threads_.emplace_back([this, i] { loop(i); });
+    }
+
+    ~Server()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stopping_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+        trees_.clear();
+        for (auto &wk : workers_) (void)hipStreamDestroy(wk.stream);
+        for (auto &kv : requests_) delete kv.second;
+    }
+
+    std::uint64_t submitBuild(int realBytes, int type, const void *x, const void *y, const void *z, std::size_t n,
+                              const at_config &cfg, std::uint64_t *treeId)
+    {
+        if (realBytes != 4 && realBytes != 8) throw Error(AT_EINVAL, "real_bytes must be 4 or 8");
+        if (n > 0 && (!x || !y || !z))
+            throw Error(AT_EINVAL, "adaptive_octree: coordinate device pointers must not be null");
+        const std::size_t bytes = estimateBytes(n, realBytes, cfg);
+        std::lock_guard<std::mutex> g(m_);
+        if (bytes > budget_) throw Error(AT_ENOMEM, "tree exceeds the server's HBM budget");
+        Request *r = newRequest(kBuild);
+        r->tree = ++lastTree_;
+        r->realBytes = realBytes;
+        r->type = type;
+        r->x = x, r->y = y, r->z = z;
+        r->n = n;
+        r->cfg = cfg;
+        Resident &t = trees_[r->tree];
+        t.n = n;
+        t.realBytes = realBytes;
+        t.x = x, t.y = y, t.z = z;
+        t.bytes = bytes;
+        pendingBuilds_.push_back(r);
+        admit();
+        *treeId = r->tree;
+        return r->id;
+    }
+
+    std::uint64_t submitUpdate(std::uint64_t tree, const void *nx, const void *ny, const void *nz)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        Resident &t = residentOf(tree);
+        Request *r = newRequest(kUpdate);
+        r->tree = tree;
+        r->nx = nx, r->ny = ny, r->nz = nz;
+        route(t, r);
+        return r->id;
+    }
+
+    std::uint64_t submitRelease(std::uint64_t tree)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        Resident &t = residentOf(tree);
+        Request *r = newRequest(kRelease);
+        r->tree = tree;
+        route(t, r);
+        return r->id;
+    }
+
+    int wait(std::uint64_t id)
+    {
+        std::unique_lock<std::mutex> g(m_);
+        Request *r = requestOf(id);
+        doneCv_.wait(g, [&] { return r->state == kDone; });
+        if (r->status != AT_OK) lastError() = r->error;
+        return r->status;
+    }
+
+    bool poll(std::uint64_t id)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        return requestOf(id)->state == kDone;
+    }
+
+    void times(std::uint64_t id, double *submit, double *start, double *done)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        Request *r = requestOf(id);
+        auto ms = [&](Clock::time_point t) { return std::chrono::duration<double, std::milli>(t - t0_).count(); };
+        if (submit) *submit = ms(r->submit);
+        if (start) *start = r->state == kQueued ? -1.0 : ms(r->start);
+        if (done) *done = r->state == kDone ? ms(r->done) : -1.0;
+    }
+
+    // The tree must have no request in flight (wait for the last one first).
+    TreeBase &tree(std::uint64_t id)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        Resident &t = residentOf(id);
+        if (!t.tree) throw Error(AT_ELOGIC, "tree is not built yet");
+        return *t.tree;
+    }
+
+    void stats(at_server_stats *o)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        o->bytes_in_use = used_;
+        o->bytes_peak = peak_;
+        o->hbm_budget = budget_;
+        o->resident_trees = trees_.size();
+        o->waiting_builds = pendingBuilds_.size();
+        o->completed = completed_;
+        o->batched_updates = batchedUpdates_;
+        o->update_batches = updateBatches_;
+    }
+
+private:
+    struct Worker
+    {
+        hipStream_t stream = nullptr;
+        std::deque<Request *> queue;
+        std::size_t work = 0; // queued particles
+    };
+
+    Request *newRequest(Kind k)
+    {
+        Request *r = new Request;
+        r->id = ++lastRequest_;
+        r->kind = k;
+        r->submit = Clock::now();
+        requests_[r->id] = r;
+        return r;
+    }
+    Request *requestOf(std::uint64_t id)
+    {
+        auto it = requests_.find(id);
+        if (it == requests_.end()) throw Error(AT_EINVAL, "unknown request id");
+        return it->second;
+    }
+    Resident &residentOf(std::uint64_t id)
+    {
+        auto it = trees_.find(id);
+        if (it == trees_.end()) throw Error(AT_EINVAL, "unknown or released tree id");
+        return it->second;
+    }
+
+    // Requests of an admitted tree go to its worker; otherwise they wait
+    // behind the tree's build.
+    void route(Resident &t, Request *r)
+    {
+        if (!t.admitted)
+        {
+            t.held.push_back(r);
+            return;
+        }
+        enqueue(t.worker, r, t.n);
+    }
+    void enqueue(int w, Request *r, std::size_t n)
+    {
+        workers_[std::size_t(w)].queue.push_back(r);
+        workers_[std::size_t(w)].work += n;
+        cv_.notify_all();
+    }
+
+    // Admits waiting builds in arrival order while their bytes fit.
+    void admit()
+    {
+        while (!pendingBuilds_.empty())
+        {
+            Request *r = pendingBuilds_.front();
+            Resident &t = trees_[r->tree];
+            if (used_ + t.bytes > budget_) break;
+            pendingBuilds_.pop_front();
+            used_ += t.bytes;
+            peak_ = std::max(peak_, used_);
+            int best = 0;
+            for (int w = 1; w < int(workers_.size()); ++w)
+                if (workers_[std::size_t(w)].work < workers_[std::size_t(best)].work) best = w;
+            t.worker = best;
+            t.admitted = true;
+            enqueue(best, r, t.n);
+            while (!t.held.empty())
+            {
+                enqueue(best, t.held.front(), t.n);
+                t.held.pop_front();
+            }
+        }
+    }
+
+    void loop(int w)
+    {
+        Worker &wk = workers_[std::size_t(w)];
+        for (;;)
+        {
+            std::vector<Request *> batch;
+            std::vector<TreeBase *> trees;
+            std::vector<Resident *> res;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return stopping_ || !wk.queue.empty(); });
+                if (wk.queue.empty()) return; // stopping and drained
+                batch.push_back(wk.queue.front());
+                wk.queue.pop_front();
+                if (batch[0]->kind == kUpdate)
+                {
+                    // later updates of other trees join the batch
+                    for (auto it = wk.queue.begin(); it != wk.queue.end() && int(batch.size()) < maxBatch_;)
+                    {
+                        Request *q = *it;
+                        bool distinct = q->kind == kUpdate;
+                        for (Request *b : batch) distinct = distinct && b->tree != q->tree;
+                        if (!distinct) break; // keep per-tree order: stop at the first conflict
+                        batch.push_back(q);
+                        it = wk.queue.erase(it);
+                    }
+                }
+                const auto now = Clock::now();
+                for (Request *r : batch)
+                {
+                    r->state = kRunning;
+                    r->start = now;
+                    Resident &t = trees_[r->tree];
+                    res.push_back(&t);
+                    trees.push_back(t.tree.get());
+                    wk.work -= std::min(wk.work, t.n);
+                }
+                if (batch[0]->kind == kUpdate && batch.size() > 1)
+                {
+                    ++updateBatches_;
+                    batchedUpdates_ += batch.size();
+                }
+            }
+            std::vector<int> status(batch.size(), AT_OK);
+            std::vector<std::string> errors(batch.size());
+            auto fail = [&](std::size_t i, int code, const char *what) {
+                status[i] = code;
+                errors[i] = what;
+            };
+            Request *head = batch[0];
+            std::unique_ptr<TreeBase> built;
+            if (head->kind == kBuild)
+            {
+                try
+                {
+                    at_config c = head->cfg;
+                    c.stream = wk.stream;
+                    built.reset(createTree(head->realBytes, head->type, head->x, head->y, head->z, head->n, c));
+                    built->build();
+                    (void)built->needsResize();
+                }
+                catch (const Error &e) { fail(0, e.code, e.what()); }
+                catch (const std::exception &e) { fail(0, AT_ENOMEM, e.what()); }
+            }
+            else if (head->kind == kRelease)
+            {
+                // the tree is destroyed under the lock below
+            }
+            else
+            {
+                // copy the new positions and launch every update, then sync once
+                std::vector<bool> launched(batch.size(), false);
+                for (std::size_t i = 0; i < batch.size(); ++i)
+                {
+                    Request *r = batch[i];
+                    try
+                    {
+                        if (!trees[i]) throw Error(AT_ELOGIC, "tree failed to build");
+                        const std::size_t bytes = res[i]->n * std::size_t(res[i]->realBytes);
+                        if (r->nx && bytes)
+                        {
+                            AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->x), r->nx, bytes,
+                                                  hipMemcpyDeviceToDevice, wk.stream));
+                            AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->y), r->ny, bytes,
+                                                  hipMemcpyDeviceToDevice, wk.stream));
+                            AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->z), r->nz, bytes,
+                                                  hipMemcpyDeviceToDevice, wk.stream));
+                        }
+                        trees[i]->update();
+                        launched[i] = true;
+                    }
+                    catch (const Error &e) { fail(i, e.code, e.what()); }
+                    catch (const std::exception &e) { fail(i, AT_ENOMEM, e.what()); }
+                }
+                (void)hipStreamSynchronize(wk.stream);
+                for (std::size_t i = 0; i < batch.size(); ++i)
+                {
+                    if (!launched[i]) continue;
+                    try
+                    {
+                        int attempts = 0;
+                        while (trees[i]->needsResize())
+                        {
+                            if (++attempts > 64) throw Error(AT_ELENGTH, "update did not fit after 64 resizes");
+                            trees[i]->resizeBuffers();
+                            trees[i]->update();
+                        }
+                    }
+                    catch (const Error &e) { fail(i, e.code, e.what()); }
+                    catch (const std::exception &e) { fail(i, AT_ENOMEM, e.what()); }
+                }
+            }
+            std::unique_ptr<TreeBase> dropped;
+            {
+                std::lock_guard<std::mutex> g(m_);
+                const auto now = Clock::now();
+                if (head->kind == kBuild) res[0]->tree = std::move(built);
+                if (head->kind == kRelease)
+                {
+                    auto it = trees_.find(head->tree);
+                    used_ -= std::min(used_, it->second.bytes);
+                    dropped = std::move(it->second.tree);
+                    trees_.erase(it);
+                }
+                for (std::size_t i = 0; i < batch.size(); ++i)
+                {
+                    batch[i]->status = status[i];
+                    batch[i]->error = errors[i];
+                    batch[i]->state = kDone;
+                    batch[i]->done = now;
+                    ++completed_;
+                }
+                admit();
+            }
+            dropped.reset(); // free device memory outside the lock
+            doneCv_.notify_all();
+        }
+    }
+
+    std::size_t budget_ = 0, used_ = 0, peak_ = 0;
+    int maxBatch_ = 16;
+    Clock::time_point t0_;
+    std::mutex m_;
+    std::condition_variable cv_, doneCv_;
+    bool stopping_ = false;
+    std::vector<Worker> workers_;
+    std::vector<std::thread> threads_;
+    std::map<std::uint64_t, Resident> trees_;
+    std::map<std::uint64_t, Request *> requests_;
+    std::deque<Request *> pendingBuilds_;
+    std::uint64_t lastTree_ = 0, lastRequest_ = 0;
+    std::uint64_t completed_ = 0, batchedUpdates_ = 0, updateBatches_ = 0;
+};
+} // namespace at
+
+// ---------------------------------------------------------------------------
+// C ABI of the server
+// ---------------------------------------------------------------------------
+struct at_server
+{
+    std::unique_ptr<at::Server> impl;
+};
+
+namespace
+{
+template <class F>
+int guardServer(F &&f)
+{
+    try
+    {
+        f();
+        return AT_OK;
+    }
+    catch (const at::Error &e)
+    {
+        at::lastError() = e.what();
+        return e.code;
+    }
+    catch (const std::bad_alloc &e)
+    {
+        at::lastError() = e.what();
+        return AT_ENOMEM;
+    }
+    catch (const std::exception &e)
+    {
+        at::lastError() = e.what();
+        return AT_EINVAL;
+    }
+}
+at::Server &serverOf(at_server *s)
+{
+    if (!s || !s->impl) throw at::Error(AT_EINVAL, "null server");
+    return *s->impl;
+}
+} // namespace
+
+extern "C" {
+int at_server_create(const at_server_config *config, at_server **out)
+{
+    return guardServer([&] {
+        if (!config || !out) throw at::Error(AT_EINVAL, "null argument");
+        auto s = std::make_unique<at_server>();
+        s->impl = std::make_unique<at::Server>(*config);
+        *out = s.release();
+    });
+}
+int at_server_destroy(at_server *s)
+{
+    return guardServer([&] { delete s; });
+}
+int at_server_submit_build(at_server *s, int realBytes, int type, const void *x, const void *y, const void *z,
+                           size_t count, const at_config *config, uint64_t *tree, uint64_t *request)
+{
+    return guardServer([&] {
+        if (!config || !tree || !request) throw at::Error(AT_EINVAL, "null argument");
+        *request = serverOf(s).submitBuild(realBytes, type, x, y, z, count, *config, tree);
+    });
+}
+int at_server_submit_update(at_server *s, uint64_t tree, const void *nx, const void *ny, const void *nz,
+                            uint64_t *request)
+{
+    return guardServer([&] {
+        if (!request) throw at::Error(AT_EINVAL, "null argument");
+        if ((nx || ny || nz) && !(nx && ny && nz)) throw at::Error(AT_EINVAL, "give all three new coordinate arrays");
+        *request = serverOf(s).submitUpdate(tree, nx, ny, nz);
+    });
+}
+int at_server_submit_release(at_server *s, uint64_t tree, uint64_t *request)
+{
+    return guardServer([&] {
+        if (!request) throw at::Error(AT_EINVAL, "null argument");
+        *request = serverOf(s).submitRelease(tree);
+    });
+}
+int at_server_wait(at_server *s, uint64_t request, int *status)
+{
+    return guardServer([&] {
+        const int st = serverOf(s).wait(request);
+        if (status) *status = st;
+    });
+}
+int at_server_poll(at_server *s, uint64_t request, int *done)
+{
+    return guardServer([&] {
+        if (!done) throw at::Error(AT_EINVAL, "null argument");
+        *done = serverOf(s).poll(request) ? 1 : 0;
+    });
+}
+int at_server_request_times(at_server *s, uint64_t request, double *submit_ms, double *start_ms, double *done_ms)
+{
+    return guardServer([&] { serverOf(s).times(request, submit_ms, start_ms, done_ms); });
+}
+int at_server_tree_buffer(at_server *s, uint64_t tree, int id, const void **d, size_t *size, size_t *cap)
+{
+    return guardServer([&] {
+        if (!d || !size || !cap) throw at::Error(AT_EINVAL, "null output pointer");
+        serverOf(s).tree(tree).buffer(id, d, size, cap);
+    });
+}
+int at_server_tree_scalars(at_server *s, uint64_t tree, at_scalars *o)
+{
+    return guardServer([&] {
+        if (!o) throw at::Error(AT_EINVAL, "null output pointer");
+        serverOf(s).tree(tree).scalars(o);
+    });
+}
+int at_server_get_stats(at_server *s, at_server_stats *o)
+{
+    return guardServer([&] {
+        if (!o) throw at::Error(AT_EINVAL, "null output pointer");
+        serverOf(s).stats(o);
+    });
+}
+}

@@ -199,6 +199,48 @@ int at_batch_tree_buffer(at_batch *batch, size_t tree, int buffer_id, const void
 int at_batch_tree_scalars(at_batch *batch, size_t tree, at_scalars *out);
 int at_batch_graph_capture_update(at_batch *batch, void *stream, void **graph_exec);
 
+/* ---- serving: resident trees, admission by HBM budget, continuous batching
+ * No reference counterpart (the reference has no request loop); for callers
+ * that issue many independent build/update requests and want them to
+ * overlap on one GPU. Requests return at once with an id; at_server_wait
+ * blocks until one is done. A tree's requests run in submission order; an
+ * update may bring new positions (device arrays copied into the tree's
+ * coordinates before its update) or NULLs when they were changed in place
+ * (only safe when no request of that tree is in flight). Build requests are
+ * admitted in arrival order while their estimated bytes fit hbm_budget
+ * (a release frees them); updates of different trees queued on one worker
+ * are launched together (up to max_batch) before one synchronisation. Tree
+ * buffers may be read once the tree's last request is done. */
+typedef struct at_server at_server;
+typedef struct at_server_config {
+    uint64_t hbm_budget_bytes;
+    int32_t workers;   /* worker threads, one HIP stream each (default 4) */
+    int32_t max_batch; /* updates launched per synchronisation (default 16) */
+} at_server_config;
+typedef struct at_server_stats {
+    uint64_t bytes_in_use, bytes_peak, hbm_budget;
+    uint64_t resident_trees, waiting_builds, completed;
+    uint64_t batched_updates, update_batches;
+} at_server_stats;
+int at_server_create(const at_server_config *config, at_server **out);
+int at_server_destroy(at_server *server); /* finishes queued requests first */
+int at_server_submit_build(at_server *server, int real_bytes, int tree_type, const void *x,
+                           const void *y, const void *z, size_t count, const at_config *config,
+                           uint64_t *tree_id, uint64_t *request_id);
+int at_server_submit_update(at_server *server, uint64_t tree_id, const void *new_x,
+                            const void *new_y, const void *new_z, uint64_t *request_id);
+int at_server_submit_release(at_server *server, uint64_t tree_id, uint64_t *request_id);
+/* *status receives the request's own status code (AT_OK on success). */
+int at_server_wait(at_server *server, uint64_t request_id, int *status);
+int at_server_poll(at_server *server, uint64_t request_id, int *done);
+/* Milliseconds since server creation; -1 while not started / not done. */
+int at_server_request_times(at_server *server, uint64_t request_id, double *submit_ms,
+                            double *start_ms, double *done_ms);
+int at_server_tree_buffer(at_server *server, uint64_t tree_id, int buffer_id, const void **data,
+                          size_t *size, size_t *capacity);
+int at_server_tree_scalars(at_server *server, uint64_t tree_id, at_scalars *out);
+int at_server_get_stats(at_server *server, at_server_stats *out);
+
 /* ---- tuning (process-wide; set before building trees) ------------------ */
 /* "sort_threads": threads per radix-sort block (256, 512 or 1024);
  * "sort_items":  keys per thread of a radix-sort tile (8, 12 or 16);

@@ -1,0 +1,120 @@
+"""The serving loop: results alone equal results in the mix; admission by budget.
+
+Trees of every kind (octree LeafCount / NFCount, Morton / Hilbert, binary;
+FP32 / FP64) are built and updated through one Server with all requests in
+flight at once, then each tree's request sequence is replayed alone on a
+plain Tree: every exported array must be identical (tolerance 0).
+"""
+import numpy as np
+import pytest
+
+from adaptive_trees_amd import (BINARY, HILBERT, LEAF_COUNT, MORTON, NF_COUNT, OCTREE, DeviceArray, Server, Tree,
+                                TreeError, make_config)
+from pairs import ARRAYS, jitter, plummer, uniform
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [
+    (OCTREE, np.float32, dict(sfc_kind=MORTON, max_particles_per_leaf=64), 50_000),
+    (OCTREE, np.float64, dict(sfc_kind=HILBERT, split_criterion=NF_COUNT, max_particles_per_leaf=4096), 30_000),
+    (BINARY, np.float32, dict(max_particles_per_leaf=32, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)), 80_000),
+    (BINARY, np.float64, dict(max_particles_per_leaf=16, views=(1, 1, 1)), 20_000),
+    (OCTREE, np.float32, dict(sfc_kind=HILBERT, max_particles_per_leaf=16, views=(1, 1, 1)), 5_000),
+    (OCTREE, np.float64, dict(sfc_kind=MORTON, max_particles_per_leaf=64), 100_000),
+    (OCTREE, np.float32, dict(max_particles_per_leaf=8), 100),
+    (BINARY, np.float32, dict(max_particles_per_leaf=4), 7),
+]
+UPDATES = 3
+
+
+def snapshots(k, n, real):
+    base = (plummer if k % 2 else uniform)(n, seed=300 + k)
+    snaps = [base]
+    for u in range(UPDATES):
+        snaps.append(jitter(snaps[-1], 2e-3, seed=400 + 10 * k + u))
+    return [[DeviceArray(np.ascontiguousarray(a, dtype=real)) for a in s] for s in snaps]
+
+
+def copy_into(dst, src):
+    from adaptive_trees_amd import _ffi
+    for d, s in zip(dst, src):
+        _ffi.hip_lib().call("memcpy", d.ptr, s.ptr, s.nbytes, 3)
+
+
+def test_results_alone_equal_results_in_the_mix():
+    server = Server(hbm_budget_bytes=200 << 30, workers=4, max_batch=8)
+    trees = []
+    for k, (kind, real, cfg, n) in enumerate(SPECS):
+        snaps = snapshots(k, n, real)
+        coords = [DeviceArray(max(n, 1) * np.dtype(real).itemsize) for _ in range(3)]
+        copy_into(coords, snaps[0])
+        trees.append((kind, real, cfg, n, snaps, coords))
+    ids, reqs = [], []
+    for kind, real, cfg, n, snaps, coords in trees:
+        tid, rid = server.submit_build(*coords, n, make_config(**cfg), real=real, kind=kind)
+        ids.append(tid)
+        reqs.append(rid)
+    for u in range(1, UPDATES + 1):  # every update is queued before any result is read
+        for tid, t in zip(ids, trees):
+            reqs.append(server.submit_update(tid, new=t[4][u]))
+    print("submitted", len(reqs), "requests", flush=True)
+    for r in reqs:
+        assert server.wait(r) == 0
+        s, b, d = server.times(r)
+        assert 0 <= s <= b <= d
+    st = server.stats()
+    assert st["completed"] == len(reqs)
+    for tid, (kind, real, cfg, n, snaps, coords) in zip(ids, trees):
+        alone_coords = [DeviceArray(max(n, 1) * np.dtype(real).itemsize) for _ in range(3)]
+        copy_into(alone_coords, snaps[0])
+        alone = Tree(*alone_coords, n, make_config(**cfg), real=real, kind=kind)
+        alone.build()
+        for u in range(1, UPDATES + 1):
+            copy_into(alone_coords, snaps[u])
+            alone.update()
+            while alone.doBuffersNeedResize():
+                alone.resizeBuffers()
+                alone.update()
+        mine = server.tree(tid)
+        sa, sb = mine.scalars(), alone.scalars()
+        for f in ("num_particles", "num_active_particles", "num_nodes", "num_leaves", "leaf_capacity"):
+            assert getattr(sa, f) == getattr(sb, f), (tid, f)
+        for name in ARRAYS:
+            if name == "nearFieldCounts_d" and (kind == BINARY or sa.split_criterion == LEAF_COUNT):
+                continue
+            a, b = mine.get(name), alone.get(name)
+            assert a.shape == b.shape and np.array_equal(a, b), (tid, name)
+    server.close()
+
+
+def test_admission_waits_for_budget():
+    n = 100_000
+    est = n * 25 + (n // 8 + 64) * 48 + (1 << 20)  # the server's per-tree estimate (no views)
+    server = Server(hbm_budget_bytes=2 * est + 1000, workers=2)
+    coords = [[DeviceArray(np.ascontiguousarray(a, dtype=np.float32)) for a in uniform(n, seed=500 + k)]
+              for k in range(5)]
+    builds = [server.submit_build(*c, n, make_config(max_particles_per_leaf=32), real=np.float32) for c in coords]
+    assert server.stats()["waiting_builds"] >= 3
+    done = []
+    for tid, rid in builds:
+        assert server.wait(rid) == 0
+        done.append(tid)
+        if len(done) >= 2:  # release the oldest to admit the next build
+            assert server.wait(server.submit_release(done.pop(0))) == 0
+    st = server.stats()
+    assert st["bytes_peak"] <= 2 * est + 1000
+    assert st["waiting_builds"] == 0
+    server.close()
+
+
+def test_request_errors():
+    server = Server(hbm_budget_bytes=1 << 30, workers=1)
+    with pytest.raises(TreeError):
+        server.submit_update(12345)
+    x = DeviceArray(np.zeros(10))
+    tid, rid = server.submit_build(x, x, x, 10, make_config(max_particles_per_leaf=0))
+    assert server.wait(rid) == 1  # AT_EINVAL from the tree constructor
+    with pytest.raises(TreeError):
+        big = DeviceArray(8)
+        server.submit_build(big, big, big, 10 ** 9, make_config())  # exceeds the budget
+    server.close()
