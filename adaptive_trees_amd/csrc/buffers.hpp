@@ -36,6 +36,15 @@ struct VecShape
     }
 };
 
+// Device memory of the engine comes from the device's default stream-ordered
+// pool; its release threshold is raised once per process so that freed
+// blocks stay cached for reuse. Allocating and freeing then never
+// synchronises the device (hipFree does), which matters when trees are
+// created and released while others compute. Frees on the null stream are
+// for owners that have synchronised their own stream first.
+void *poolAlloc(std::size_t bytes, hipStream_t s);
+void poolFree(void *p, hipStream_t s);
+
 // A device allocation that grows on request.
 template <class T>
 struct DevBuf
@@ -48,7 +57,7 @@ struct DevBuf
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        poolFree(p, nullptr);
         p = nullptr;
         cap = 0;
     }
@@ -56,15 +65,10 @@ struct DevBuf
     void ensure(std::size_t n, std::size_t keep, hipStream_t s)
     {
         if (n <= cap) return;
-        T *q = nullptr;
-        AT_HIP(hipMalloc(&q, (n ? n : 1) * sizeof(T)));
+        T *q = static_cast<T *>(poolAlloc((n ? n : 1) * sizeof(T), s));
         if (p && keep)
             AT_HIP(hipMemcpyAsync(q, p, (keep < cap ? keep : cap) * sizeof(T), hipMemcpyDeviceToDevice, s));
-        if (p)
-        {
-            AT_HIP(hipStreamSynchronize(s));
-            (void)hipFree(p);
-        }
+        poolFree(p, s); // ordered after the copy and every earlier use on s
         p = q;
         cap = n;
     }

@@ -2,6 +2,8 @@
 #include "engine.hpp"
 #include "tree_impl.hpp"
 
+#include <mutex>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -18,6 +20,33 @@ void hipCheck(hipError_t e, const char *what, const char *file, int line)
                                  hipGetErrorString(e));
 }
 
+namespace
+{
+std::once_flag g_poolOnce[64];
+}
+
+void *poolAlloc(std::size_t bytes, hipStream_t s)
+{
+    int dev = 0;
+    AT_HIP(hipGetDevice(&dev));
+    std::call_once(g_poolOnce[dev & 63], [dev] {
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool)
+        {
+            std::uint64_t threshold = ~std::uint64_t(0);
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        }
+    });
+    void *p = nullptr;
+    AT_HIP(hipMallocAsync(&p, bytes ? bytes : 1, s));
+    return p;
+}
+
+void poolFree(void *p, hipStream_t s)
+{
+    if (p) (void)hipFreeAsync(p, s);
+}
+
 TreeBase::TreeBase(hipStream_t s, DevState *st, DevState *sth) : stream_(s)
 {
     if (st && sth)
@@ -27,18 +56,17 @@ TreeBase::TreeBase(hipStream_t s, DevState *st, DevState *sth) : stream_(s)
         ownsState_ = false;
         return;
     }
-    AT_HIP(hipMalloc(&st_, sizeof(DevState)));
-    AT_HIP(hipHostMalloc(&sth_, sizeof(DevState)));
+    st_ = static_cast<DevState *>(poolAlloc(sizeof(DevState), stream_));
+    sth_ = new DevState{};
     AT_HIP(hipMemsetAsync(st_, 0, sizeof(DevState), stream_));
-    std::memset(sth_, 0, sizeof(DevState));
 }
 
 TreeBase::~TreeBase()
 {
     (void)hipStreamSynchronize(stream_);
     if (!ownsState_) return;
-    if (st_) (void)hipFree(st_);
-    if (sth_) (void)hipHostFree(sth_);
+    poolFree(st_, nullptr);
+    delete sth_;
 }
 
 void TreeBase::readState()

@@ -636,7 +636,7 @@ SortPlan makeSortPlan(std::size_t n)
     return p;
 }
 
-void SortWorkspace::allocate(std::size_t cap, bool withValues)
+void SortWorkspace::allocate(std::size_t cap, bool withValues, hipStream_t s)
 {
     if (cap <= capacity && (!withValues || values)) return;
     release();
@@ -646,19 +646,20 @@ void SortWorkspace::allocate(std::size_t cap, bool withValues)
     if (maxRanges < 1) maxRanges = 1;
     const std::size_t histRows = maxRanges > kHistogramBlocks * 2 ? maxRanges : kHistogramBlocks * 2;
     values = withValues;
-    AT_HIP(hipMalloc(&keysAlt, capacity * sizeof(Key)));
-    if (withValues) AT_HIP(hipMalloc(&valsAlt, capacity * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&digits, (capacity + 16) * sizeof(std::uint8_t)));
-    AT_HIP(hipMalloc(&rangeHist, histRows * kRadix * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&rangeOff, maxRanges * kRadix * sizeof(std::uint32_t)));
-    AT_HIP(hipMalloc(&digitTotal, kRadix * sizeof(std::uint32_t)));
+    keysAlt = static_cast<Key *>(poolAlloc(capacity * sizeof(Key), s));
+    if (withValues) valsAlt = static_cast<std::uint32_t *>(poolAlloc(capacity * sizeof(std::uint32_t), s));
+    digits = static_cast<std::uint8_t *>(poolAlloc((capacity + 16) * sizeof(std::uint8_t), s));
+    rangeHist = static_cast<std::uint32_t *>(poolAlloc(histRows * kRadix * sizeof(std::uint32_t), s));
+    rangeOff = static_cast<std::uint32_t *>(poolAlloc(maxRanges * kRadix * sizeof(std::uint32_t), s));
+    digitTotal = static_cast<std::uint32_t *>(poolAlloc(kRadix * sizeof(std::uint32_t), s));
 }
 
 void SortWorkspace::release()
 {
+    // owners synchronise their stream before the workspace goes away
     for (void *p : {(void *)keysAlt, (void *)valsAlt, (void *)digits, (void *)rangeHist, (void *)rangeOff,
                     (void *)digitTotal})
-        if (p) (void)hipFree(p);
+        poolFree(p, nullptr);
     keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
     digitTotal = nullptr;
     capacity = 0; maxRanges = 0; values = false;
@@ -670,7 +671,7 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
 {
     if (n == 0) return;
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys per tree");
-    ws.allocate(n, true);
+    ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
     {
         prof::Scope scope("keygen", s);
@@ -698,7 +699,7 @@ void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, Sor
 {
     if (n == 0) return;
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
-    ws.allocate(n, true);
+    ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
                        0, ws.rangeHist);
@@ -709,7 +710,7 @@ void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws)
 {
     if (n == 0) return;
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
-    ws.allocate(n, false);
+    ws.allocate(n, false, s);
     const SortPlan pl = makeSortPlan(n);
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
                        0, ws.rangeHist);

@@ -55,7 +55,7 @@ struct SortWorkspace
     std::size_t maxRanges = 0;
     bool values = false;
 
-    void allocate(std::size_t cap, bool withValues);
+    void allocate(std::size_t cap, bool withValues, hipStream_t s);
     void release();
     ~SortWorkspace() { release(); }
 };

@@ -126,7 +126,7 @@ public:
         {
             P_.ensure(n ? n : 1, 0, stream_);
             perm_.ensure(n ? n : 1, 0, stream_);
-            sort_.allocate(n ? n : 1, true);
+            sort_.allocate(n ? n : 1, true, stream_);
             Pp_ = P_.p;
             permp_ = perm_.p;
         }
@@ -134,6 +134,9 @@ public:
         ensurePhysical(8, 0);
         AT_HIP(hipStreamSynchronize(stream_));
     }
+
+    // members free their memory on the null stream: finish this tree's work first
+    ~OctreeImpl() override { (void)hipStreamSynchronize(stream_); }
 
     void build() override
     {
@@ -516,9 +519,11 @@ public:
         geom_ = makeBinaryGeom<Real>(box_, maxDepth_);
         P_.ensure(n ? n : 1, 0, stream_);
         perm_.ensure(n ? n : 1, 0, stream_);
-        sort_.allocate(n ? n : 1, true);
+        sort_.allocate(n ? n : 1, true, stream_);
         AT_HIP(hipStreamSynchronize(stream_));
     }
+
+    ~BinaryImpl() override { (void)hipStreamSynchronize(stream_); }
 
     void build() override
     {

@@ -380,7 +380,7 @@ ViewEngine::ViewEngine(unsigned levelBits) : impl_(new Impl), bits_(levelBits), 
 ViewEngine::~ViewEngine()
 {
     delete impl_;
-    if (depth_) (void)hipFree(depth_);
+    poolFree(depth_, nullptr);
 }
 
 void ViewEngine::prepare(std::size_t leafCapacity, std::size_t numParticles, bool structure, bool counts,
@@ -398,7 +398,7 @@ void ViewEngine::prepare(std::size_t leafCapacity, std::size_t numParticles, boo
     scratch_ = true;
     if (!depth_)
     {
-        AT_HIP(hipMalloc(&depth_, 1));
+        depth_ = static_cast<std::uint8_t *>(poolAlloc(1, s));
         AT_HIP(hipMemsetAsync(depth_, 0, 1, s));
     }
     auto grow = [&](Out o, std::size_t n) { m.shape[o].resizeHeadroom(n); };
@@ -441,7 +441,7 @@ void ViewEngine::prepare(std::size_t leafCapacity, std::size_t numParticles, boo
     m.candCount.ensure(1, 0, s);
     m.scanTicket.ensure(1, 0, s);
     m.scanStatus.ensure(leafCap_ / (kT * 8) + 2, 0, s);
-    if (internals > 1) m.sort.allocate(internals, true);
+    if (internals > 1) m.sort.allocate(internals, true, s);
 }
 
 void ViewEngine::syncSizes(std::size_t numLeaves, std::size_t numActive)

@@ -25,7 +25,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from adaptive_trees_amd import BINARY, OCTREE, Batch, DeviceArray, Tree, make_config  # noqa: E402
+from adaptive_trees_amd import (BINARY, HILBERT, MORTON, NF_COUNT, OCTREE, Batch, DeviceArray, Server,  # noqa: E402
+                                Tree, make_config)
 from adaptive_trees_amd import _ffi, workloads  # noqa: E402
 
 METRIC = "Mparticles/s build+update, % HBM roofline; batched vs plain speedup; p99 latency"
@@ -214,6 +215,155 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
     }
 
 
+def serving_trace(rate, requests, pool, seed):
+    """Poisson arrivals at `rate`/s. The first 8 requests build resident trees; afterwards 30%
+    build a new tree (the oldest resident tree is released beyond 16) and 70% update a random
+    resident tree with its next snapshot. Tree inputs come from `pool` (sizes log-uniform in
+    [1e4, 8e6]); each build draws the type (70% octree / 30% binary), the precision (50/50) and,
+    for octrees, Hilbert/Morton and LeafCount 64 / NFCount 4096 (50/50 each)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=requests))
+    trace, resident, next_id = [], [], 0
+    for i in range(requests):
+        if i < 8 or rng.random() < 0.3 or not resident:
+            kind = OCTREE if rng.random() < 0.7 else BINARY
+            real = np.float32 if rng.random() < 0.5 else np.float64
+            cfg = dict(max_particles_per_leaf=64)
+            if kind == OCTREE:
+                cfg["sfc_kind"] = HILBERT if rng.random() < 0.5 else MORTON
+                if rng.random() < 0.5:
+                    cfg.update(split_criterion=NF_COUNT, max_particles_per_leaf=4096)
+            trace.append(("build", t[i], next_id, int(rng.integers(len(pool))), kind, real, cfg))
+            resident.append(next_id)
+            next_id += 1
+            if len(resident) > 16:
+                trace.append(("release", t[i], resident.pop(0)))
+        else:
+            trace.append(("update", t[i], resident[int(rng.integers(len(resident)))]))
+    return trace
+
+
+def serving_benchmark(rates=(50, 200, 1000, 2000), requests=120, seed=5):
+    """The serving loop (4 worker streams, 200 GB budget) against serial plain calls on the
+    same trace; latency = result ready - arrival, per request."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(1e4), np.log(8e6), size=20))).astype(np.int64)
+    pool = []  # per entry: two snapshots, host float64
+    for k, n in enumerate(sizes):
+        a = (workloads.plummer if k % 2 else workloads.uniform_cube)(int(n), seed=700 + k, dtype=np.float64)
+        pool.append((a, workloads.reflect_step(a, 1e-3, seed=800 + k)))
+    progress("serving pool generated: %d inputs, %d particles" % (len(pool), int(sizes.sum())))
+    dev_pool = {}
+
+    def snapshot(p, real, which):
+        key = (p, np.dtype(real).name, which)
+        if key not in dev_pool:
+            dev_pool[key] = [DeviceArray(np.ascontiguousarray(v, dtype=real)) for v in pool[p][which]]
+        return dev_pool[key]
+
+    results = []
+    for rate in rates:
+        trace = serving_trace(rate, requests, pool, seed + rate)
+        # inputs of every request, prepared before the clock starts
+        prep = []
+        for r in trace:
+            if r[0] == "build":
+                _, _, tid, p, kind, real, cfg = r
+                n = len(pool[p][0][0])
+                bufs = [[DeviceArray(n * np.dtype(real).itemsize) for _ in range(3)] for _ in range(2)]
+                for b in bufs:
+                    for d, s in zip(b, snapshot(p, real, 0)):
+                        _lib().call("memcpy", d.ptr, s.ptr, s.nbytes, 3)
+                prep.append(bufs)
+                for w in (0, 1):
+                    snapshot(p, real, w)
+            else:
+                prep.append(None)
+        info = {}
+        for r in trace:
+            if r[0] == "build":
+                info[r[2]] = r
+        # served
+        server = Server(hbm_budget_bytes=200 << 30, workers=4, max_batch=16)
+        ids, turn, reqs = {}, {}, []
+        start = time.perf_counter()
+        for r, bufs in zip(trace, prep):
+            delay = r[1] - (time.perf_counter() - start)
+            if delay > 0:
+                time.sleep(delay)
+            if r[0] == "build":
+                _, _, tid, p, kind, real, cfg = r
+                ids[tid], rid = server.submit_build(*bufs[0], len(pool[p][0][0]), make_config(**cfg), real=real,
+                                                    kind=kind)
+                turn[tid] = 0
+                reqs.append((rid, len(pool[p][0][0])))
+            elif r[0] == "update":
+                tid = r[2]
+                _, _, _, p, kind, real, cfg = info[tid]
+                turn[tid] += 1
+                reqs.append((server.submit_update(ids[tid], new=snapshot(p, real, turn[tid] % 2)),
+                             len(pool[p][0][0])))
+            else:
+                reqs.append((server.submit_release(ids[r[2]]), 0))
+        lat, first, last, work = [], None, 0.0, 0
+        for rid, n in reqs:
+            assert server.wait(rid) == 0
+            s_ms, b_ms, d_ms = server.times(rid)
+            lat.append(d_ms - s_ms)
+            first = s_ms if first is None else min(first, s_ms)
+            last = max(last, d_ms)
+            work += n
+        stats = server.stats()
+        server.close()
+        served_s = (last - first) / 1e3
+        # serial plain calls: one Tree per build, requests in arrival order on one stream
+        plain, turn2, slat = {}, {}, []
+        start = time.perf_counter()
+        for r, bufs in zip(trace, prep):
+            arrival = r[1]
+            now = time.perf_counter() - start
+            if now < arrival:
+                time.sleep(arrival - now)
+            if r[0] == "build":
+                _, _, tid, p, kind, real, cfg = r
+                t = Tree(*bufs[1], len(pool[p][0][0]), make_config(**cfg), real=real, kind=kind)
+                t.build()
+                t.doBuffersNeedResize()
+                plain[tid], turn2[tid] = (t, bufs[1]), 0
+            elif r[0] == "update":
+                tid = r[2]
+                t, coords = plain[tid]
+                _, _, _, p, kind, real, cfg = info[tid]
+                turn2[tid] += 1
+                for d, s in zip(coords, snapshot(p, real, turn2[tid] % 2)):
+                    _lib().call("memcpy", d.ptr, s.ptr, s.nbytes, 3)
+                t.update()
+                while t.doBuffersNeedResize():
+                    t.resizeBuffers()
+                    t.update()
+            else:
+                plain.pop(r[2])[0].close()
+            slat.append((time.perf_counter() - start - arrival) * 1e3)
+        serial_s = time.perf_counter() - start - trace[0][1]
+        for t, _ in plain.values():
+            t.close()
+        results.append({
+            "rate_per_s": rate, "requests": len(reqs),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "serial_p50_ms": round(float(np.percentile(slat, 50)), 3),
+            "serial_p99_ms": round(float(np.percentile(slat, 99)), 3),
+            "requests_per_s": round(len(reqs) / served_s, 1),
+            "Mparticles_per_s": round(work / served_s / 1e6, 1),
+            "serial_requests_per_s": round(len(reqs) / serial_s, 1),
+            "batched_updates": int(stats["batched_updates"]), "peak_GB": round(stats["bytes_peak"] / 1e9, 3),
+        })
+        progress("serving rate %d/s done" % rate)
+    return {"workload": "Poisson trace, %d requests per rate, sizes log-uniform [1e4, 8e6], 70%% octree / 30%% "
+                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; 4 worker streams, "
+                        "200 GB budget; serial = plain Tree calls in arrival order" % requests,
+            "rates": results}
+
+
 def _counter_rows(path):
     """(kernel name, counter value) per dispatch from a rocprofv3 counter_collection.csv."""
     import csv
@@ -259,6 +409,8 @@ def main():
                     help="also run the known-size copy kernel (for hardware counter scaling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched-vs-plain measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--only-serve", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
@@ -276,6 +428,9 @@ def main():
 
     if args.only_batch:
         print(json.dumps({"batched": batch_benchmark()}))
+        return
+    if args.only_serve:
+        print(json.dumps({"serving": serving_benchmark()}))
         return
 
     n = args.n
@@ -412,6 +567,8 @@ def main():
     if not args.no_batch:
         tree.close()
         out["batched"] = batch_benchmark()
+    if not args.no_serve:
+        out["serving"] = serving_benchmark()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, z, x1, y1, z1, box, args.cpu_seconds)
     else:
