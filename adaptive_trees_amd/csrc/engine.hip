@@ -315,6 +315,18 @@ int at_tune_set(const char *key, int value)
         else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
     });
 }
+int at_tune_auto(size_t n, void *stream, char *report, size_t report_len)
+{
+    return guarded([&] {
+        const std::string r = at::autotuneSort(n, hipStream_t(stream));
+        if (report && report_len)
+        {
+            const std::size_t m = r.size() < report_len - 1 ? r.size() : report_len - 1;
+            std::memcpy(report, r.data(), m);
+            report[m] = 0;
+        }
+    });
+}
 int at_tune_get(const char *key, int *value)
 {
     return guarded([&] {

@@ -22,6 +22,8 @@
 // on the 256 CUs the prefix chain made the look-back cost ~45% of a pass.
 #include "sort.hpp"
 
+#include <string>
+
 #include "profile.hpp"
 
 namespace at
@@ -715,6 +717,90 @@ void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws)
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
                        0, ws.rangeHist);
     runPasses(s, pl, ws, keys, ws.keysAlt, nullptr, nullptr, false, false, true);
+}
+
+// ---- autotuner ------------------------------------------------------------------
+namespace
+{
+// Keys with full 64-bit entropy from a counter hash (splitmix64 finaliser).
+__global__ void fillTuneKeysKernel(Key *keys, std::size_t n, std::uint64_t seed)
+{
+    for (std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += std::size_t(gridDim.x) * blockDim.x)
+    {
+        std::uint64_t z = seed + (i + 1) * 0x9e3779b97f4a7c15ull;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        keys[i] = (z ^ (z >> 31)) >> 1; // 63-bit keys, like every SFC key kind
+    }
+}
+} // namespace
+
+std::string autotuneSort(std::size_t n, hipStream_t s)
+{
+    if (n < 1024) n = 1024;
+    if (n >= (std::size_t(1) << 32)) throw Error(1, "autotune size must be below 2^32 keys");
+    DevBuf<Key> src, keys;
+    DevBuf<std::uint32_t> vals;
+    src.ensure(n, 0, s);
+    keys.ensure(n, 0, s);
+    vals.ensure(n, 0, s);
+    SortWorkspace ws;
+    ws.allocate(n, true, s);
+    hipLaunchKernelGGL(fillTuneKeysKernel, dim3(2048), dim3(256), 0, s, src.p, n, 0x5eedull);
+    hipEvent_t e0, e1;
+    AT_HIP(hipEventCreate(&e0));
+    AT_HIP(hipEventCreate(&e1));
+    auto timeOne = [&]() {
+        float best = 1e30f;
+        for (int rep = 0; rep < 4; ++rep)
+        {
+            AT_HIP(hipMemcpyAsync(keys.p, src.p, n * sizeof(Key), hipMemcpyDeviceToDevice, s));
+            AT_HIP(hipEventRecord(e0, s));
+            sortPairs(s, keys.p, vals.p, n, ws, true);
+            AT_HIP(hipEventRecord(e1, s));
+            AT_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            AT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0 && ms < best) best = ms; // the first run warms up
+        }
+        return best;
+    };
+    // coordinate descent: tile shape, then ranges, then the histogram source
+    const int shapes[][2] = {{256, 8}, {256, 12}, {256, 16}, {512, 8}, {512, 12}, {512, 16}, {1024, 8}, {1024, 12}};
+    float best = 1e30f;
+    int bt = g_threads, bi = g_items;
+    for (const auto &sh : shapes)
+    {
+        g_threads = sh[0];
+        g_items = sh[1];
+        const float t = timeOne();
+        if (t < best) best = t, bt = sh[0], bi = sh[1];
+    }
+    g_threads = bt;
+    g_items = bi;
+    int br = g_targetRanges;
+    for (int r : {512, 1024, 2048, 4096})
+    {
+        g_targetRanges = r;
+        const float t = timeOne();
+        if (t < best) best = t, br = r;
+    }
+    g_targetRanges = br;
+    int bd = g_digitArray;
+    for (int d : {0, 1})
+    {
+        g_digitArray = d;
+        const float t = timeOne();
+        if (t < best) best = t, bd = d;
+    }
+    g_digitArray = bd;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    AT_HIP(hipStreamSynchronize(s));
+    return "{\"sort_threads\": " + std::to_string(bt) + ", \"sort_items\": " + std::to_string(bi) +
+           ", \"sort_ranges\": " + std::to_string(br) + ", \"sort_digit_array\": " + std::to_string(bd) +
+           ", \"keys\": " + std::to_string(n) + ", \"sort_ms\": " + std::to_string(best) + "}";
 }
 
 // ---- SegmentedSort -------------------------------------------------------------

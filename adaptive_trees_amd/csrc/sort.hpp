@@ -1,6 +1,7 @@
 // sort.hpp -- stable LSD radix sort of 64-bit keys (+ 32-bit values) for gfx950.
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "buffers.hpp"
@@ -26,6 +27,11 @@ void setSortThreads(int threads);
 // histogram reads those; 0: the histogram reads the keys.
 int sortDigitArray();
 void setSortDigitArray(int on);
+
+// Times every tile shape, then range count, then histogram source on a
+// stable sort of n random 63-bit (key, index) pairs and keeps the fastest
+// settings (process-wide). Returns them as a JSON object.
+std::string autotuneSort(std::size_t n, hipStream_t s);
 
 // Partition of n keys into contiguous ranges of whole tiles; each scatter
 // block owns one range and walks it tile by tile.

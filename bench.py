@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 from adaptive_trees_amd import (BINARY, HILBERT, MORTON, NF_COUNT, OCTREE, Batch, DeviceArray, Server,  # noqa: E402
                                 Tree, make_config)
-from adaptive_trees_amd import _ffi, workloads  # noqa: E402
+from adaptive_trees_amd import _ffi, tune, workloads  # noqa: E402
 
 METRIC = "Mparticles/s build+update, % HBM roofline; batched vs plain speedup; p99 latency"
 PEAK_HBM_GBS = 8000.0
@@ -408,6 +408,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the known-size copy kernel (for hardware counter scaling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default sort tuning")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving-loop measurement")
@@ -435,6 +436,8 @@ def main():
 
     n = args.n
     box = workloads.ANISO_BOX
+    tuning = tune.get() if args.no_autotune else tune.autotune(n)
+    progress("sort tuning: %s" % json.dumps(tuning))
     x, y, z = workloads.two_clusters(n, seed=3, dtype=np.float32)
     dx, dy, dz = DeviceArray(x), DeviceArray(y), DeviceArray(z)
     stream = Stream()
@@ -538,6 +541,7 @@ def main():
             "leaves": leaves,
             "tree": "binary",
             "bucket": 32,
+            "sort_tuning": tuning,
         },
         "roofline": {
             "bound": "hbm",

@@ -250,6 +250,11 @@ int at_server_get_stats(at_server *server, at_server_stats *out);
  *                     the keys. */
 int at_tune_set(const char *key, int value);
 int at_tune_get(const char *key, int *value);
+/* Autotuner: times the tile shapes, then the range counts, then the
+ * histogram source on a sort of n random keys (on `stream`), keeps the
+ * fastest settings and writes them as JSON into report (may be NULL). Every
+ * setting gives identical results; only the schedule changes. */
+int at_tune_auto(size_t n, void *stream, char *report, size_t report_len);
 
 #ifdef __cplusplus
 }

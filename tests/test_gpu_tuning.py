@@ -61,3 +61,23 @@ def test_invalid_tunings_are_rejected():
                  ("no_such_key", 1)):
         with pytest.raises(_ffi.InvalidArgument):
             lib.call("tune_set", k.encode(), v)
+
+
+@pytest.mark.gpu
+def test_autotuner_picks_a_valid_setting(tuning):
+    from adaptive_trees_amd import tune
+    saved = tune.get()
+    print("autotune", flush=True)
+    r = tune.autotune(1 << 20)
+    assert r["sort_threads"] in (256, 512, 1024)
+    assert r["sort_items"] in (8, 12, 16)
+    assert r["sort_ranges"] in (512, 1024, 2048, 4096)
+    assert r["sort_digit_array"] in (0, 1)
+    assert r["sort_ms"] > 0
+    assert {k: r[k] for k in tune.KEYS} == tune.get()
+    # the picked setting is bit-exact as well
+    x, y, z = plummer(200_000, seed=72)
+    p = Pair(x, y, z, kind=OCTREE, real=np.float32, max_particles_per_leaf=32)
+    p.build()
+    p.compare("autotuned")
+    tune.set(**saved)
