@@ -86,7 +86,8 @@ class LengthError(TreeError):
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("workers", ctypes.c_int32), ("max_batch", ctypes.c_int32)]
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("workers", ctypes.c_int32), ("max_batch", ctypes.c_int32),
+                ("update_graphs", ctypes.c_int32), ("reserved0", ctypes.c_int32)]
 
 
 class ServerStats(ctypes.Structure):

@@ -16,6 +16,9 @@
 //              (doBuffersNeedResize / resizeBuffers / update) completes it.
 //              Workers run concurrently, so the work of requests admitted
 //              while others are on the GPU overlaps on the device.
+//   graphs     with update_graphs set, each tree's update() is captured
+//              once (thread-local capture on its worker's stream) after its
+//              build and after every resize, and replayed as one launch.
 // Every request runs the same tree operations it would run alone, on a tree
 // that only its own requests touch, so results do not depend on the mix.
 #include "engine.hpp"
@@ -53,9 +56,52 @@ struct Request
     Clock::time_point submit, start, done;
 };
 
+// An instantiated update graph, destroyed with its tree.
+struct GraphExec
+{
+    hipGraphExec_t g = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec &) = delete;
+    GraphExec &operator=(const GraphExec &) = delete;
+    GraphExec(GraphExec &&o) noexcept : g(o.g) { o.g = nullptr; }
+    GraphExec &operator=(GraphExec &&o) noexcept
+    {
+        std::swap(g, o.g);
+        return *this;
+    }
+    ~GraphExec() { reset(); }
+    void reset()
+    {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+};
+
+// Records tree.update() on stream s; null when the capture is refused.
+hipGraphExec_t captureUpdate(TreeBase &tree, hipStream_t s)
+{
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    bool ok = true;
+    try
+    {
+        tree.update();
+    }
+    catch (...)
+    {
+        ok = false;
+    }
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(s, &g) != hipSuccess) ok = false;
+    hipGraphExec_t e = nullptr;
+    if (ok && g && hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess) e = nullptr;
+    if (g) (void)hipGraphDestroy(g);
+    return ok ? e : nullptr;
+}
+
 struct Resident
 {
     std::unique_ptr<TreeBase> tree;
+    GraphExec graph; // touched only by the tree's worker
     int worker = -1;
     bool admitted = false;
     std::size_t bytes = 0, n = 0;
@@ -78,7 +124,8 @@ class Server
 {
 public:
     explicit Server(const at_server_config &c)
-        : budget_(c.hbm_budget_bytes), maxBatch_(c.max_batch > 0 ? c.max_batch : 16), t0_(Clock::now())
+        : budget_(c.hbm_budget_bytes), maxBatch_(c.max_batch > 0 ? c.max_batch : 16), graphs_(c.update_graphs != 0),
+          t0_(Clock::now())
     {
         const int w = c.workers > 0 ? c.workers : 4;
         if (w > 32) throw Error(AT_EINVAL, "at most 32 server workers");
@@ -319,6 +366,7 @@ private:
             };
             Request *head = batch[0];
             std::unique_ptr<TreeBase> built;
+            GraphExec builtGraph;
             if (head->kind == kBuild)
             {
                 try
@@ -328,6 +376,7 @@ private:
                     built.reset(createTree(head->realBytes, head->type, head->x, head->y, head->z, head->n, c));
                     built->build();
                     (void)built->needsResize();
+                    if (graphs_) builtGraph.g = captureUpdate(*built, wk.stream);
                 }
                 catch (const Error &e) { fail(0, e.code, e.what()); }
                 catch (const std::exception &e) { fail(0, AT_ENOMEM, e.what()); }
@@ -356,7 +405,10 @@ private:
                             AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->z), r->nz, bytes,
                                                   hipMemcpyDeviceToDevice, wk.stream));
                         }
-                        trees[i]->update();
+                        if (res[i]->graph.g)
+                            AT_HIP(hipGraphLaunch(res[i]->graph.g, wk.stream));
+                        else
+                            trees[i]->update();
                         launched[i] = true;
                     }
                     catch (const Error &e) { fail(i, e.code, e.what()); }
@@ -372,8 +424,14 @@ private:
                         while (trees[i]->needsResize())
                         {
                             if (++attempts > 64) throw Error(AT_ELENGTH, "update did not fit after 64 resizes");
+                            res[i]->graph.reset(); // the resize moves the captured buffers
                             trees[i]->resizeBuffers();
                             trees[i]->update();
+                        }
+                        if (attempts && graphs_)
+                        {
+                            (void)trees[i]->needsResize(); // refresh host sizes before capturing
+                            res[i]->graph.g = captureUpdate(*trees[i], wk.stream);
                         }
                     }
                     catch (const Error &e) { fail(i, e.code, e.what()); }
@@ -384,11 +442,16 @@ private:
             {
                 std::lock_guard<std::mutex> g(m_);
                 const auto now = Clock::now();
-                if (head->kind == kBuild) res[0]->tree = std::move(built);
+                if (head->kind == kBuild)
+                {
+                    res[0]->tree = std::move(built);
+                    res[0]->graph = std::move(builtGraph);
+                }
                 if (head->kind == kRelease)
                 {
                     auto it = trees_.find(head->tree);
                     used_ -= std::min(used_, it->second.bytes);
+                    it->second.graph.reset();
                     dropped = std::move(it->second.tree);
                     trees_.erase(it);
                 }
@@ -409,6 +472,7 @@ private:
 
     std::size_t budget_ = 0, used_ = 0, peak_ = 0;
     int maxBatch_ = 16;
+    bool graphs_ = true;
     Clock::time_point t0_;
     std::mutex m_;
     std::condition_variable cv_, doneCv_;

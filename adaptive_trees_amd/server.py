@@ -36,9 +36,9 @@ class ServerTree:
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes, workers=4, max_batch=16):
+    def __init__(self, hbm_budget_bytes, workers=4, max_batch=16, update_graphs=True):
         self.lib = _ffi.hip_lib()
-        cfg = _ffi.ServerConfig(int(hbm_budget_bytes), int(workers), int(max_batch))
+        cfg = _ffi.ServerConfig(int(hbm_budget_bytes), int(workers), int(max_batch), 1 if update_graphs else 0, 0)
         h = ctypes.c_void_p()
         self.lib.call("server_create", ctypes.byref(cfg), ctypes.byref(h))
         self.handle = h.value

@@ -243,7 +243,7 @@ def serving_trace(rate, requests, pool, seed):
     return trace
 
 
-def serving_benchmark(rates=(50, 200, 1000, 2000), requests=120, seed=5):
+def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5):
     """The serving loop (4 worker streams, 200 GB budget) against serial plain calls on the
     same trace; latency = result ready - arrival, per request."""
     rng = np.random.default_rng(seed)
@@ -359,8 +359,9 @@ def serving_benchmark(rates=(50, 200, 1000, 2000), requests=120, seed=5):
         })
         progress("serving rate %d/s done" % rate)
     return {"workload": "Poisson trace, %d requests per rate, sizes log-uniform [1e4, 8e6], 70%% octree / 30%% "
-                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; 4 worker streams, "
-                        "200 GB budget; serial = plain Tree calls in arrival order" % requests,
+                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; 4 worker streams with "
+                        "per-tree update graphs, 200 GB budget; serial = plain Tree calls in arrival order; "
+                        "10000/s is an overload probe of capacity" % requests,
             "rates": results}
 
 

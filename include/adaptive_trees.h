@@ -214,8 +214,10 @@ int at_batch_graph_capture_update(at_batch *batch, void *stream, void **graph_ex
 typedef struct at_server at_server;
 typedef struct at_server_config {
     uint64_t hbm_budget_bytes;
-    int32_t workers;   /* worker threads, one HIP stream each (default 4) */
-    int32_t max_batch; /* updates launched per synchronisation (default 16) */
+    int32_t workers;       /* worker threads, one HIP stream each (default 4) */
+    int32_t max_batch;     /* updates launched per synchronisation (default 16) */
+    int32_t update_graphs; /* nonzero: replay each tree's update from a HIP graph */
+    int32_t reserved0;
 } at_server_config;
 typedef struct at_server_stats {
     uint64_t bytes_in_use, bytes_peak, hbm_budget;

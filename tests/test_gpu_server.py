@@ -41,8 +41,9 @@ def copy_into(dst, src):
         _ffi.hip_lib().call("memcpy", d.ptr, s.ptr, s.nbytes, 3)
 
 
-def test_results_alone_equal_results_in_the_mix():
-    server = Server(hbm_budget_bytes=200 << 30, workers=4, max_batch=8)
+@pytest.mark.parametrize("graphs", [True, False])
+def test_results_alone_equal_results_in_the_mix(graphs):
+    server = Server(hbm_budget_bytes=200 << 30, workers=4, max_batch=8, update_graphs=graphs)
     trees = []
     for k, (kind, real, cfg, n) in enumerate(SPECS):
         snaps = snapshots(k, n, real)
