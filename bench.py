@@ -215,6 +215,71 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
     }
 
 
+def other_configs():
+    """The other single-tree workloads: 65,536 FP64 uniform octree (Morton, LeafCount 64) build,
+    checked bit for bit against the oracle; and a 2^24-particle FP32 Plummer octree (a = 0.05,
+    Hilbert, NFCount 4096) build followed by 10 updates of reflected Gaussian steps (sigma 1e-3)."""
+    from oracle.binding import OracleTree
+    lib = _lib()
+    stream = Stream()
+    e0, e1 = Event(), Event()
+    out = {}
+    # 65k FP64 uniform octree
+    n = 65536
+    x, y, z = workloads.uniform_cube(n, seed=1, dtype=np.float64)
+    dev = [DeviceArray(a) for a in (x, y, z)]
+    t = Tree(*dev, n, make_config(max_particles_per_leaf=64, stream=stream.p), real=np.float64, kind=OCTREE)
+    times = []
+    for _ in range(25):
+        e0.record(stream)
+        t.build()
+        e1.record(stream)
+        stream.sync()
+        times.append(e1.ms_since(e0))
+    o = OracleTree(x.copy(), y.copy(), z.copy(), n, make_config(max_particles_per_leaf=64), real=np.float64,
+                   kind=OCTREE)
+    o.build()
+    exact = all(np.array_equal(t.get(k), o.get(k)) for k in ("particleKeys_d", "perm_d", "cornerstone_d",
+                                                              "leafCounts_d"))
+    out["octree_65k_fp64_uniform_build"] = {"build_ms_median": round(float(np.median(times[5:])), 4),
+                                            "leaves": int(t.scalars().num_leaves), "bit_exact_vs_oracle": exact}
+    t.close()
+    progress("65k octree done")
+    # 16M FP32 Plummer Hilbert NFCount octree, build + 10 updates
+    n = 1 << 24
+    pts = workloads.plummer(n, seed=2, dtype=np.float32)
+    dev = [DeviceArray(a) for a in pts]
+    t = Tree(*dev, n, make_config(max_particles_per_leaf=4096, split_criterion=NF_COUNT, sfc_kind=HILBERT,
+                                  stream=stream.p), real=np.float32, kind=OCTREE)
+    e0.record(stream)
+    t.build()
+    e1.record(stream)
+    stream.sync()
+    build_ms = e1.ms_since(e0)
+    upd, resizes, cur = [], 0, pts
+    for step in range(10):
+        cur = workloads.reflect_step(cur, 1e-3, seed=200 + step)
+        for d, a in zip(dev, cur):
+            d.upload(a)
+        e0.record(stream)
+        t.update()
+        e1.record(stream)
+        stream.sync()
+        upd.append(e1.ms_since(e0))
+        while t.doBuffersNeedResize():
+            resizes += 1
+            t.resizeBuffers()
+            t.update()
+    out["octree_16M_fp32_plummer_hilbert_nf4096"] = {
+        "build_ms": round(build_ms, 3), "update_ms_mean": round(float(np.mean(upd)), 3),
+        "update_ms_max": round(float(np.max(upd)), 3), "resizes": resizes,
+        "leaves": int(t.scalars().num_leaves),
+        "Mparticles_per_s": round(11 * n / ((build_ms + sum(upd)) * 1e-3) / 1e6, 1)}
+    t.close()
+    progress("16M NF octree done")
+    return out
+
+
 def serving_trace(rate, requests, pool, seed):
     """Poisson arrivals at `rate`/s. The first 8 requests build resident trees; afterwards 30%
     build a new tree (the oldest resident tree is released beyond 16) and 70% update a random
@@ -412,6 +477,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default sort tuning")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other single-tree workloads")
+    ap.add_argument("--only-configs", action="store_true", help="run only the other single-tree workloads")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
@@ -433,6 +500,9 @@ def main():
         return
     if args.only_serve:
         print(json.dumps({"serving": serving_benchmark()}))
+        return
+    if args.only_configs:
+        print(json.dumps({"other_configs": other_configs()}))
         return
 
     n = args.n
@@ -569,8 +639,10 @@ def main():
             "frac_of_peak": round(build_bytes / (build_wall_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4),
         },
     }
+    tree.close()
+    if not args.no_configs:
+        out["other_configs"] = other_configs()
     if not args.no_batch:
-        tree.close()
         out["batched"] = batch_benchmark()
     if not args.no_serve:
         out["serving"] = serving_benchmark()
