@@ -25,7 +25,7 @@ namespace at
 namespace
 {
 // Launch shapes of one TreeSet table.
-enum Shape { kClear = 0, kTiles, kLeaves, kNf, kShapes };
+enum Shape { kClear = 0, kTiles, kLeaves, kNf, kIndex, kShapes };
 
 // Device + pinned host staging of a TreeDev table and its block maps.
 struct SetTables
@@ -135,6 +135,7 @@ public:
         }
         TreeSet set[kShapes];
         stage(build_, active, true, [&](Shape sh, std::size_t k) { return blocksFor(sh, k, 1); }, set);
+        multiBuildSearchIndex(stream_, set[kIndex]);
         multiOctInit(stream_, set[kClear]);
         for (unsigned pass = 0; pass < maxDepth_ && !active.empty(); ++pass)
         {
@@ -180,6 +181,7 @@ public:
     {
         if (!built_) throw Error(AT_ELOGIC, "adaptive_octree: call build() before update()");
         sort_.keygenAndSort<Real>(stream_, kind_, kp_, coords_.p, P_.p, perm_.p);
+        multiBuildSearchIndex(stream_, updSet_[kIndex]);
         multiReset(stream_, updSet_[kClear]);
         multiOctRefresh(stream_, updSet_[kLeaves], updSet_[kNf], params_, true);
         for (unsigned i = 0; i < iterations_; ++i)
@@ -238,6 +240,7 @@ private:
         case kClear: return clearBlocksFor(trees_[k]->device().statusWords);
         case kTiles: return tileBlocksFor(leaves);
         case kLeaves: return leafBlocksFor(leaves);
+        case kIndex: return indexBlocksFor(counts_[k]);
         default: return params_.nf ? nfBlocksFor(leaves) : 0u;
         }
     }

@@ -302,6 +302,67 @@ __host__ __device__ inline std::size_t upperBound(const T *a, std::size_t n, T v
 
 inline unsigned ceilDiv(std::size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
 
+// ---- sampled search index of the sorted keys -----------------------------------
+// Q[j] = P[j * kIndexStride]. A lower bound first searches Q (small enough to
+// stay in L2 / MALL), then one stride window of P, so a search over tens of
+// millions of keys touches only a few lines of P. Two searches run in lock
+// step (their loads overlap); the branch-free form has a data-independent
+// step count. Results equal lowerBound over P.
+constexpr std::size_t kIndexStride = 128;
+__host__ __device__ inline std::size_t indexSize(std::size_t n) { return (n + kIndexStride - 1) / kIndexStride; }
+
+__device__ inline void lowerBoundPair(const Key *a, std::size_t n, Key v0, Key v1, std::size_t &r0, std::size_t &r1)
+{
+    if (n == 0)
+    {
+        r0 = r1 = 0;
+        return;
+    }
+    std::size_t b0 = 0, b1 = 0;
+    while (n > 1)
+    {
+        const std::size_t half = n >> 1;
+        const Key x0 = a[b0 + half], x1 = a[b1 + half];
+        b0 = x0 < v0 ? b0 + half : b0;
+        b1 = x1 < v1 ? b1 + half : b1;
+        n -= half;
+    }
+    r0 = b0 + (a[b0] < v0 ? 1 : 0);
+    r1 = b1 + (a[b1] < v1 ? 1 : 0);
+}
+
+// lower_bound(P, n, v0) and lower_bound(P, n, v1) through the index Q.
+__device__ inline void lowerBoundIndexedPair(const Key *P, std::size_t n, const Key *Q, Key v0, Key v1,
+                                             std::size_t &r0, std::size_t &r1)
+{
+    const std::size_t nq = indexSize(n);
+    std::size_t j0, j1;
+    lowerBoundPair(Q, nq, v0, v1, j0, j1);
+    // the answer lies in ((j-1) S, j S] (clipped to [0, n])
+    std::size_t b0 = j0 ? (j0 - 1) * kIndexStride + 1 : 0, b1 = j1 ? (j1 - 1) * kIndexStride + 1 : 0;
+    std::size_t l0 = (j0 < nq ? j0 * kIndexStride : n) - b0, l1 = (j1 < nq ? j1 * kIndexStride : n) - b1;
+    if (j0 == 0) l0 = 0;
+    if (j1 == 0) l1 = 0;
+    // lower bound inside [b, b + l]: branch-free halving while the window is > 1
+    while (l0 > 1 || l1 > 1)
+    {
+        if (l0 > 1)
+        {
+            const std::size_t h = l0 >> 1;
+            b0 = P[b0 + h] < v0 ? b0 + h : b0;
+            l0 -= h;
+        }
+        if (l1 > 1)
+        {
+            const std::size_t h = l1 >> 1;
+            b1 = P[b1 + h] < v1 ? b1 + h : b1;
+            l1 -= h;
+        }
+    }
+    r0 = l0 ? b0 + (P[b0] < v0 ? 1 : 0) : b0;
+    r1 = l1 ? b1 + (P[b1] < v1 ? 1 : 0) : b1;
+}
+
 // ---- device-scope atomics used by the chained scans ---------------------------
 __device__ inline std::uint32_t ldAcq(const std::uint32_t *p)
 {

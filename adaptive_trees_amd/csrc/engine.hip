@@ -47,6 +47,47 @@ void poolFree(void *p, hipStream_t s)
     if (p) (void)hipFreeAsync(p, s);
 }
 
+namespace
+{
+// Pinned host slots for the per-tree state read-backs, carved from slabs that
+// live until the process ends: trees are created and destroyed without
+// hipHostMalloc / hipHostFree (which synchronise the device) and read back
+// through pinned memory.
+class PinnedStates
+{
+public:
+    DevState *take()
+    {
+        std::lock_guard<std::mutex> g(m_);
+        if (free_.empty())
+        {
+            constexpr std::size_t kSlab = 1024;
+            DevState *slab = nullptr;
+            AT_HIP(hipHostMalloc(&slab, kSlab * sizeof(DevState)));
+            for (std::size_t i = 0; i < kSlab; ++i) free_.push_back(slab + i);
+        }
+        DevState *s = free_.back();
+        free_.pop_back();
+        *s = DevState{};
+        return s;
+    }
+    void give(DevState *s)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        free_.push_back(s);
+    }
+
+private:
+    std::mutex m_;
+    std::vector<DevState *> free_;
+};
+PinnedStates &pinnedStates()
+{
+    static PinnedStates *p = new PinnedStates; // never destroyed: slabs stay valid at exit
+    return *p;
+}
+} // namespace
+
 TreeBase::TreeBase(hipStream_t s, DevState *st, DevState *sth) : stream_(s)
 {
     if (st && sth)
@@ -57,7 +98,7 @@ TreeBase::TreeBase(hipStream_t s, DevState *st, DevState *sth) : stream_(s)
         return;
     }
     st_ = static_cast<DevState *>(poolAlloc(sizeof(DevState), stream_));
-    sth_ = new DevState{};
+    sth_ = pinnedStates().take();
     AT_HIP(hipMemsetAsync(st_, 0, sizeof(DevState), stream_));
 }
 
@@ -66,7 +107,7 @@ TreeBase::~TreeBase()
     (void)hipStreamSynchronize(stream_);
     if (!ownsState_) return;
     poolFree(st_, nullptr);
-    delete sth_;
+    pinnedStates().give(sth_);
 }
 
 void TreeBase::readState()

@@ -130,6 +130,7 @@ public:
             Pp_ = P_.p;
             permp_ = perm_.p;
         }
+        idx_.ensure(indexSize(n) + 1, 0, stream_);
         vK_.resize(2);
         ensurePhysical(8, 0);
         AT_HIP(hipStreamSynchronize(stream_));
@@ -346,6 +347,7 @@ private:
         t.statusWords = status_.cap;
         t.st = st_;
         t.P = Pp_;
+        t.Pidx = idx_.p;
         t.n = n_;
         t.capacity = long(leafCapacity_);
         return t;
@@ -357,6 +359,7 @@ private:
         kp.x = x_; kp.y = y_; kp.z = z_;
         kp.grid = grid_;
         keygenAndSort<Real>(stream_, hilbert_ ? KeyKind::Hilbert : KeyKind::Morton, kp, n_, Pp_, permp_, sort_);
+        buildSearchIndex(stream_, dev());
     }
 
     // physical buffers: both halves hold `leaves` entries (+1 for K)
@@ -487,6 +490,7 @@ private:
     VecShape vK_, vTmp_;
     DevBuf<Key> P_;
     DevBuf<std::uint32_t> perm_;
+    DevBuf<Key> idx_;               // sampled search index of P
     Key *Pp_ = nullptr;             // own P_ or the batch's segment
     std::uint32_t *permp_ = nullptr;
     DevBuf<Key> K_[2];
@@ -520,6 +524,7 @@ public:
         P_.ensure(n ? n : 1, 0, stream_);
         perm_.ensure(n ? n : 1, 0, stream_);
         sort_.allocate(n ? n : 1, true, stream_);
+        idx_.ensure(indexSize(n) + 1, 0, stream_);
         AT_HIP(hipStreamSynchronize(stream_));
     }
 
@@ -634,7 +639,9 @@ private:
         t.statusWords = status_.cap;
         t.st = st_;
         t.P = P_.p;
+        t.Pidx = idx_.p;
         t.n = n_;
+        t.capacity = long(leafCapacity_);
         return t;
     }
 
@@ -644,6 +651,7 @@ private:
         kp.x = x_; kp.y = y_; kp.z = z_;
         kp.geom = geom_;
         keygenAndSort<Real>(stream_, KeyKind::Binary, kp, n_, P_.p, perm_.p, sort_);
+        buildSearchIndex(stream_, dev());
     }
 
     // The converged tree is unique (internal <=> count > bucket and depth <
@@ -790,6 +798,7 @@ private:
     DevBuf<std::int64_t> offs_;
     DevBuf<std::uint64_t> status_;
     DevBuf<std::uint32_t> ticket_;
+    DevBuf<Key> idx_;
     DirectBufs direct_;
     ViewEngine view_;
 };

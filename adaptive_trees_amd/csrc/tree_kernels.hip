@@ -459,8 +459,8 @@ __device__ void leafCountsBody(const TreeDev &t, unsigned block)
     if (i >= L) return;
     const int ab = st->activeBuffer;
     const Key *K = t.K[ab];
-    const std::size_t a = lowerBound(t.P, t.n, K[i]);
-    const std::size_t b = lowerBound(t.P, t.n, K[i + 1]);
+    std::size_t a, b;
+    lowerBoundIndexedPair(t.P, t.n, t.Pidx, K[i], K[i + 1], a, b);
     const std::size_t c = b - a;
     t.N[ab][i] = c > 0xffffffffu ? 0xffffffffu : unsigned(c);
 }
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full)
 // times the population of the 27 same-level cells, periodic at 2^21 per axis
 // (computeNFCooperative, octree_rebalance.cuh:579-634). One lane per cell in
 // groups of 32 lanes.
-__device__ inline void nearField(const Key *P, std::size_t n, Key start, unsigned level, bool hilbert,
+__device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, Key start, unsigned level, bool hilbert,
                                  unsigned lane32, std::uint64_t &center, std::uint64_t &sum)
 {
     std::uint64_t pop = 0;
@@ -523,8 +523,8 @@ __device__ inline void nearField(const Key *P, std::size_t n, Key start, unsigne
                                             unsigned(pbc(int(cz) + dz * len)), hilbert),
                                   level);
         }
-        const std::size_t a = lowerBound(P, n, ck);
-        const std::size_t b = lowerBound(P, n, ck + nodeRange(level));
+        std::size_t a, b;
+        lowerBoundIndexedPair(P, n, Q, ck, ck + nodeRange(level), a, b);
         pop = std::uint64_t(b - a);
     }
     std::uint64_t s = pop;
@@ -552,7 +552,7 @@ __device__ void nfComputeBody(const TreeDev &t, unsigned bucket, int hilbert, un
         unsigned level = treeLevel(tree[leaf + 1] - start);
         if (parent) level -= 1;
         std::uint64_t center, sum;
-        nearField(t.P, t.n, start, level, hilbert != 0, lane32, center, sum);
+        nearField(t.P, t.n, t.Pidx, start, level, hilbert != 0, lane32, center, sum);
         if (lane32 == 0)
         {
             const std::uint64_t nf = center * sum;
@@ -837,6 +837,14 @@ __global__ void binDirectStateKernel(DevState *st, int leaves)
     st->activeBuffer = 0;
 }
 
+__device__ void buildIndexBody(const Key *P, std::size_t n, Key *Q, unsigned block, unsigned blocks)
+{
+    const std::size_t nq = indexSize(n);
+    for (std::size_t j = std::size_t(block) * blockDim.x + threadIdx.x; j < nq; j += std::size_t(blocks) * blockDim.x)
+        Q[j] = P[j * kIndexStride];
+}
+__global__ void buildIndexKernel(const Key *P, std::size_t n, Key *Q) { buildIndexBody(P, n, Q, blockIdx.x, gridDim.x); }
+
 unsigned clearBlocks(std::size_t words)
 {
     const unsigned b = ceilDiv(words, 256);
@@ -868,6 +876,11 @@ __global__ void multiOctInitKernel(TreeSet s)
 {
     AT_MULTI_PROLOGUE;
     octInitBody(t, t.statusWords, block, blocks);
+}
+__global__ void multiBuildIndexKernel(TreeSet s)
+{
+    AT_MULTI_PROLOGUE;
+    buildIndexBody(t.P, t.n, const_cast<Key *>(t.Pidx), block, blocks);
 }
 __global__ void multiResetKernel(TreeSet s)
 {
@@ -919,6 +932,20 @@ __global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsi
 } // namespace
 
 unsigned clearBlocksFor(std::size_t statusWords) { return clearBlocks(statusWords); }
+unsigned indexBlocksFor(std::size_t n)
+{
+    const unsigned b = ceilDiv(indexSize(n), 256);
+    return b < 1 ? 1 : (b > 1024 ? 1024 : b);
+}
+void buildSearchIndex(hipStream_t s, const TreeDev &t)
+{
+    if (t.n) hipLaunchKernelGGL(buildIndexKernel, dim3(indexBlocksFor(t.n)), dim3(256), 0, s, t.P, t.n,
+                                const_cast<Key *>(t.Pidx));
+}
+void multiBuildSearchIndex(hipStream_t s, const TreeSet &index)
+{
+    if (index.blocks) hipLaunchKernelGGL(multiBuildIndexKernel, dim3(index.blocks), dim3(256), 0, s, index);
+}
 unsigned tileBlocksFor(std::size_t gridLeaves) { return unsigned(gridLeaves / kPassTile + 1); }
 unsigned leafBlocksFor(std::size_t gridLeaves)
 {

@@ -23,6 +23,7 @@ struct TreeDev
     std::size_t statusWords = 0;
     DevState *st = nullptr;
     const Key *P = nullptr;
+    const Key *Pidx = nullptr;                   // sampled search index of P (indexSize(n) keys)
     std::size_t n = 0;                           // particle slots
     long capacity = 0;                           // leaf capacity of a pass (multi-tree launches)
 };
@@ -95,7 +96,12 @@ void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDire
 void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves);
 void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves);
 
+// Builds t.Pidx from t.P (after every sort).
+void buildSearchIndex(hipStream_t s, const TreeDev &t);
+unsigned indexBlocksFor(std::size_t n);
+
 // ---- multi-tree launches (segmented over a TreeSet) ----
+void multiBuildSearchIndex(hipStream_t s, const TreeSet &index);
 void multiOctInit(hipStream_t s, const TreeSet &clear);
 void multiReset(hipStream_t s, const TreeSet &clear);
 void multiClearScan(hipStream_t s, const TreeSet &clear);

@@ -308,7 +308,7 @@ def serving_trace(rate, requests, pool, seed):
     return trace
 
 
-def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5):
+def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5, workers=4):
     """The serving loop (4 worker streams, 200 GB budget) against serial plain calls on the
     same trace; latency = result ready - arrival, per request."""
     rng = np.random.default_rng(seed)
@@ -349,7 +349,7 @@ def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5):
             if r[0] == "build":
                 info[r[2]] = r
         # served
-        server = Server(hbm_budget_bytes=200 << 30, workers=4, max_batch=16)
+        server = Server(hbm_budget_bytes=200 << 30, workers=workers, max_batch=16)
         ids, turn, reqs = {}, {}, []
         start = time.perf_counter()
         for r, bufs in zip(trace, prep):
@@ -424,9 +424,9 @@ def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5):
         })
         progress("serving rate %d/s done" % rate)
     return {"workload": "Poisson trace, %d requests per rate, sizes log-uniform [1e4, 8e6], 70%% octree / 30%% "
-                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; 4 worker streams with "
+                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; %d worker streams with "
                         "per-tree update graphs, 200 GB budget; serial = plain Tree calls in arrival order; "
-                        "10000/s is an overload probe of capacity" % requests,
+                        "10000/s is an overload probe of capacity" % (requests, workers),
             "rates": results}
 
 
@@ -478,6 +478,7 @@ def main():
     ap.add_argument("--only-batch", action="store_true", help="run only the batched-vs-plain measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-tree workloads")
+    ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the serving loop")
     ap.add_argument("--only-configs", action="store_true", help="run only the other single-tree workloads")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -499,7 +500,7 @@ def main():
         print(json.dumps({"batched": batch_benchmark()}))
         return
     if args.only_serve:
-        print(json.dumps({"serving": serving_benchmark()}))
+        print(json.dumps({"serving": serving_benchmark(workers=args.serve_workers)}))
         return
     if args.only_configs:
         print(json.dumps({"other_configs": other_configs()}))
@@ -645,7 +646,7 @@ def main():
     if not args.no_batch:
         out["batched"] = batch_benchmark()
     if not args.no_serve:
-        out["serving"] = serving_benchmark()
+        out["serving"] = serving_benchmark(workers=args.serve_workers)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(x, y, z, x1, y1, z1, box, args.cpu_seconds)
     else:
