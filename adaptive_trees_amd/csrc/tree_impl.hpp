@@ -343,6 +343,7 @@ private:
         t.scanStatus = status_.p;
         t.scanTicket = ticket_.p;
         t.nfTasks = nf_ ? tasks_.p : nullptr;
+        t.nfGroups = nf_ ? groups_.p : nullptr;
         t.nfCount = nf_ ? taskCount_.p : nullptr;
         t.statusWords = status_.cap;
         t.st = st_;
@@ -380,7 +381,8 @@ private:
             nfDirty_.ensure(leaves, 0, stream_);
             groupDirty_.ensure(leaves, 0, stream_);
             tasks_.ensure(2 * leaves, 0, stream_);
-            taskCount_.ensure(1, 0, stream_);
+            groups_.ensure(leaves / 8 + 1, 0, stream_);
+            taskCount_.ensure(2, 0, stream_);
         }
         ensureOps(leaves + 1);
         ticket_.ensure(1, 0, stream_);
@@ -403,6 +405,7 @@ private:
             nfDirty_.ensure(newK, 0, stream_);
             groupDirty_.ensure(newK, 0, stream_);
             tasks_.ensure(2 * newK, 0, stream_);
+            groups_.ensure(newK / 8 + 1, 0, stream_);
         }
     }
     void swapHalves()
@@ -442,6 +445,7 @@ private:
             nfDirty_.ensure(capacity, 0, stream_);
             groupDirty_.ensure(capacity, 0, stream_);
             tasks_.ensure(2 * capacity, 0, stream_);
+            groups_.ensure(capacity / 8 + 1, 0, stream_);
         }
         ensureOps(capacity + 1);
         view_.prepare(capacity, n_, prepared_.structure, prepared_.counts, prepared_.mapping, stream_);
@@ -498,7 +502,7 @@ private:
     DevBuf<std::uint8_t> gcm_[2], nfDirty_, groupDirty_;
     DevBuf<std::int64_t> offs_;
     DevBuf<std::uint64_t> status_;
-    DevBuf<std::uint32_t> ticket_, tasks_, taskCount_;
+    DevBuf<std::uint32_t> ticket_, tasks_, groups_, taskCount_;
     ViewEngine view_;
 };
 

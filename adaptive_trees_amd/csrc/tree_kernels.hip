@@ -370,7 +370,7 @@ __device__ void commitBody(DevState *st, int binary, std::uint64_t *status, std:
             st->activeBuffer ^= 1;
         }
         *ticket = 0;
-        if (nfCount) *nfCount = 0;
+        if (nfCount) nfCount[0] = nfCount[1] = 0;
     }
     for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < words; w += std::size_t(blocks) * blockDim.x)
         status[w] = 0;
@@ -391,7 +391,7 @@ __device__ void resetBody(DevState *st, const Key *P, std::size_t n, std::uint64
         st->converged = 0;
         st->numActiveParticles = n ? unsigned(lowerBound(P, n, kMaxKey)) : 0u;
         *ticket = 0;
-        if (nfCount) *nfCount = 0;
+        if (nfCount) nfCount[0] = nfCount[1] = 0;
     }
     for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < words; w += std::size_t(blocks) * blockDim.x)
         status[w] = 0;
@@ -420,7 +420,7 @@ __device__ void octInitBody(const TreeDev &t, std::size_t words, unsigned block,
         s.numActiveParticles = unsigned(active);
         *t.st = s;
         *t.scanTicket = 0;
-        if (t.nfCount) *t.nfCount = 0;
+        if (t.nfCount) t.nfCount[0] = t.nfCount[1] = 0;
     }
     for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < words; w += std::size_t(blocks) * blockDim.x)
         t.scanStatus[w] = 0;
@@ -475,28 +475,64 @@ __device__ inline bool dirtyLeader(const Key *tree, const std::uint8_t *gDirty, 
     return siblingIndex(tree, L, i, sl) == 0 && sl > 0;
 }
 
+// Work lists of one refresh: a complete sibling group whose 8 leaves all need
+// their own NF becomes one group item (its 8 stencils share a 4x4x4 block of
+// cells); other dirty leaves are single items, dirty group leaders parent
+// items. Items are appended with one atomic per list per block.
 __device__ void nfCollectBody(const TreeDev &t, int full, unsigned block)
 {
+    __shared__ std::uint32_t waveCnt[kThreads / 64][2];
+    __shared__ std::uint32_t blockBase[2];
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
     const int i = int(block * blockDim.x + threadIdx.x);
     const int ab = st->activeBuffer;
-    bool own = false, lead = false;
+    const Key *tree = t.K[ab];
+    bool own = false, lead = false, grp = false;
     if (i < L)
     {
         own = full || t.nfDirty[i];
-        lead = dirtyLeader(t.K[ab], full ? nullptr : t.groupDirty, L, i);
+        lead = dirtyLeader(tree, full ? nullptr : t.groupDirty, L, i);
+        unsigned level = 0;
+        const int s = own ? siblingIndex(tree, L, i, level) : -1;
+        if (s >= 0)
+        {
+            const int g = i - s;
+            bool all = true;
+            if (!full)
+                for (int k = 0; k < 8; ++k) all = all && t.nfDirty[g + k] != 0;
+            if (all)
+            {
+                own = false;
+                grp = s == 0;
+            }
+        }
     }
-    const std::uint64_t bo = __ballot(own), bl = __ballot(lead);
+    const std::uint64_t bo = __ballot(own), bl = __ballot(lead), bg = __ballot(grp);
     const unsigned lane = __lane_id();
-    const std::uint32_t no = __popcll(bo), nl = __popcll(bl);
-    std::uint32_t baseIdx = 0;
-    if (lane == 0 && (no + nl)) baseIdx = atomicAdd(t.nfCount, no + nl);
-    baseIdx = __shfl(baseIdx, 0, 64);
+    const int wave = threadIdx.x >> 6;
+    const std::uint32_t no = __popcll(bo), nl = __popcll(bl), ng = __popcll(bg);
+    if (lane == 0)
+    {
+        waveCnt[wave][0] = no + nl;
+        waveCnt[wave][1] = ng;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        std::uint32_t a = 0, b = 0;
+        for (int w = 0; w < kThreads / 64; ++w) a += waveCnt[w][0], b += waveCnt[w][1];
+        blockBase[0] = a ? atomicAdd(&t.nfCount[0], a) : 0u;
+        blockBase[1] = b ? atomicAdd(&t.nfCount[1], b) : 0u;
+    }
+    __syncthreads();
+    std::uint32_t baseIdx = blockBase[0], baseGrp = blockBase[1];
+    for (int w = 0; w < wave; ++w) baseIdx += waveCnt[w][0], baseGrp += waveCnt[w][1];
     const std::uint64_t lt = (std::uint64_t(1) << lane) - 1;
     if (own) t.nfTasks[baseIdx + __popcll(bo & lt)] = std::uint32_t(i) << 1;
     if (lead) t.nfTasks[baseIdx + no + __popcll(bl & lt)] = (std::uint32_t(i) << 1) | 1u;
+    if (grp) t.nfGroups[baseGrp + __popcll(bg & lt)] = std::uint32_t(i);
 }
 __global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full) { nfCollectBody(t, full, blockIdx.x); }
 
@@ -534,13 +570,73 @@ __device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, Key 
     center = __shfl(pop, 13, 32);
 }
 
+// One wave per complete sibling group at level l: lane c counts the cell
+// (c%4 - 1, c/4%4 - 1, c/16 - 1) of level l around the parent's corner (the
+// 4x4x4 block holding the 27-cell stencils of all 8 children, periodic at
+// 2^21 per axis as nearField), then lanes 0..7 sum their child's 27 cells.
+// Same cells and the same u64 products as nearField per leaf.
+__device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned block, unsigned blocks)
+{
+    DevState *st = t.st;
+    if (st->converged) return;
+    const int ab = st->activeBuffer;
+    const Key *tree = t.K[ab];
+    const std::uint32_t total = t.nfCount[1];
+    const unsigned lane = __lane_id();
+    const std::uint32_t waves = blocks * (blockDim.x / 64);
+    constexpr int R = int(1u << kOctLevels);
+    auto pbc = [](int v) { const int w = v < 0 ? v + R : v; return w >= R ? w - R : w; };
+    for (std::uint32_t g = block * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += waves)
+    {
+        const int leaf0 = int(t.nfGroups[g]);
+        const Key start = tree[leaf0];
+        const unsigned level = treeLevel(tree[leaf0 + 1] - start);
+        const int w = int(1u << (kOctLevels - level));
+        unsigned cx, cy, cz;
+        sfcDecode(start, hilbert != 0, cx, cy, cz);
+        const unsigned corner = ~(2u * unsigned(w) - 1u);
+        const int bx = int(cx & corner), by = int(cy & corner), bz = int(cz & corner);
+        const int a = int(lane & 3u) - 1, b = int((lane >> 2) & 3u) - 1, c = int(lane >> 4) - 1;
+        const Key ck = enclosingBoxCode(
+            sfcEncode(unsigned(pbc(bx + a * w)), unsigned(pbc(by + b * w)), unsigned(pbc(bz + c * w)), hilbert != 0),
+            level);
+        std::size_t lo, hi;
+        lowerBoundIndexedPair(t.P, t.n, t.Pidx, ck, ck + nodeRange(level), lo, hi);
+        const std::uint32_t pop = std::uint32_t(hi - lo);
+        // child `lane` (lanes 0..7) of the group: its offset inside the parent
+        const int leaf = leaf0 + int(lane & 7u);
+        unsigned x, y, z;
+        sfcDecode(tree[leaf], hilbert != 0, x, y, z);
+        const int rx = int(x) - bx >= w ? 1 : 0, ry = int(y) - by >= w ? 1 : 0, rz = int(z) - bz >= w ? 1 : 0;
+        std::uint64_t sum = 0;
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+                    sum += __shfl(pop, (rx + dx) + 4 * (ry + dy) + 16 * (rz + dz), 64);
+        const std::uint64_t own = __shfl(pop, (rx + 1) + 4 * (ry + 1) + 16 * (rz + 1), 64);
+        if (lane < 8)
+        {
+            const std::uint64_t nf = own * sum;
+            t.N[ab][leaf] = own > 0xffffffffull ? 0xffffffffu : unsigned(own);
+            t.NF[ab][leaf] = nf > 0xffffffffull ? 0xffffffffu : unsigned(nf);
+        }
+    }
+}
+__global__ __launch_bounds__(kThreads) void nfGroupKernel(TreeDev t, int hilbert)
+{
+    nfGroupBody(t, hilbert, blockIdx.x, gridDim.x);
+}
+
 __device__ void nfComputeBody(const TreeDev &t, unsigned bucket, int hilbert, unsigned block, unsigned blocks)
 {
     DevState *st = t.st;
     if (st->converged) return;
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
-    const std::uint32_t total = *t.nfCount;
+    const std::uint32_t total = t.nfCount[0];
     const unsigned lane32 = threadIdx.x & 31;
     const std::uint32_t groups = blocks * (blockDim.x / 32);
     for (std::uint32_t g = block * (blockDim.x / 32) + threadIdx.x / 32; g < total; g += groups)
@@ -928,6 +1024,11 @@ __global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsi
     AT_MULTI_PROLOGUE;
     nfComputeBody(t, bucket, hilbert, block, blocks);
 }
+__global__ __launch_bounds__(kThreads) void multiNfGroupKernel(TreeSet s, int hilbert)
+{
+    AT_MULTI_PROLOGUE;
+    nfGroupBody(t, hilbert, block, blocks);
+}
 #undef AT_MULTI_PROLOGUE
 } // namespace
 
@@ -998,6 +1099,7 @@ void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, co
     prof::Scope scope("near_field", s);
     hipLaunchKernelGGL(multiNfCollectKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, full ? 1 : 0);
     hipLaunchKernelGGL(multiNfComputeKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.bucket, p.hilbert ? 1 : 0);
+    hipLaunchKernelGGL(multiNfGroupKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.hilbert ? 1 : 0);
 }
 
 std::size_t passTiles(std::size_t cap) { return cap / kPassTile + 1; }
@@ -1041,6 +1143,7 @@ void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, 
     prof::Scope scope("near_field", s);
     hipLaunchKernelGGL(nfCollectKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t, full ? 1 : 0);
     hipLaunchKernelGGL(nfComputeKernel, dim3(2048), dim3(kThreads), 0, s, t, p.bucket, p.hilbert ? 1 : 0);
+    hipLaunchKernelGGL(nfGroupKernel, dim3(2048), dim3(kThreads), 0, s, t, p.hilbert ? 1 : 0);
 }
 void octPass(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves)
 {

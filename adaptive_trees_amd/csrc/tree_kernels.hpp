@@ -19,7 +19,8 @@ struct TreeDev
     std::uint64_t *scanStatus = nullptr;         // chained-scan status words
     std::uint32_t *scanTicket = nullptr;
     std::uint32_t *nfTasks = nullptr;            // (leaf << 1 | kind) work items
-    std::uint32_t *nfCount = nullptr;
+    std::uint32_t *nfGroups = nullptr;           // first leaf of each complete sibling group to refresh
+    std::uint32_t *nfCount = nullptr;            // [0] nfTasks, [1] nfGroups
     std::size_t statusWords = 0;
     DevState *st = nullptr;
     const Key *P = nullptr;
