@@ -37,6 +37,7 @@ int g_items = 12;
 int g_threads = 512;
 int g_targetRanges = 2048;
 int g_digitArray = 1; // scatter writes the next pass's digit bytes for the histogram
+int g_prefetch = 0;   // scatter loads the next tile while working on the current one
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 
@@ -296,7 +297,10 @@ __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_
 // kSeg: segmented sort. The range comes from segRanges[blockIdx.x] and
 // rangeOff holds its absolute per-digit start ([range][digit]); values of the
 // first pass are slots within the segment.
-template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits, bool kSeg = false>
+//
+// kPre: the keys (and values) of the next tile are loaded while the current
+// tile is ranked and written (register double buffering).
+template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits, bool kSeg = false, bool kPre = false>
 __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel(
     const Key *__restrict__ kin, Key *__restrict__ kout, const std::uint32_t *__restrict__ vin,
     std::uint32_t *__restrict__ vout, std::uint8_t *__restrict__ digitsOut, std::size_t n, std::size_t rangeKeys,
@@ -340,19 +344,57 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
         valueBase = std::uint32_t(rBegin);
     }
     const Key *kr = kin + rBegin;
+    const std::uint32_t *vr = vin + rBegin;
     const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
+    constexpr bool kPreVals = kPre && kVals && !kIota;
+    Key preKey[kPre ? ITEMS : 1];
+    std::uint32_t preVal[kPreVals ? ITEMS : 1];
+    if constexpr (kPre)
+    {
+        const std::uint32_t first = std::uint32_t(wave) * (ITEMS * 64) + lane;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j)
+        {
+            const std::uint32_t idx = first + std::uint32_t(j) * 64;
+            preKey[j] = idx < rLen ? kr[idx] : kInvalidKey;
+            if constexpr (kPreVals) preVal[j] = idx < rLen ? vr[idx] : 0u;
+        }
+    }
 
     for (std::uint32_t base = 0; base < rLen; base += kTile)
     {
         for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
         // ---- load keys (wave-striped: item j of lane l is element j*64 + l of the wave's chunk)
         Key key[ITEMS];
+        std::uint32_t curVal[kPreVals ? ITEMS : 1];
         const std::uint32_t waveFirst = base + std::uint32_t(wave) * (ITEMS * 64) + lane;
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
+        if constexpr (kPre)
         {
-            const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
-            key[j] = idx < rLen ? kr[idx] : kInvalidKey; // padding ranks last with digit 255
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+            {
+                key[j] = preKey[j];
+                if constexpr (kPreVals) curVal[j] = preVal[j];
+            }
+            if (base + kTile < rLen)
+            {
+#pragma unroll
+                for (int j = 0; j < ITEMS; ++j)
+                {
+                    const std::uint32_t idx = waveFirst + kTile + std::uint32_t(j) * 64;
+                    preKey[j] = idx < rLen ? kr[idx] : kInvalidKey;
+                    if constexpr (kPreVals) preVal[j] = idx < rLen ? vr[idx] : 0u;
+                }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+            {
+                const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
+                key[j] = idx < rLen ? kr[idx] : kInvalidKey; // padding ranks last with digit 255
+            }
         }
         __syncthreads();
 
@@ -424,13 +466,16 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
         }
         if (kVals)
         {
-            const std::uint32_t *vr = vin + rBegin;
             __syncthreads(); // staged keys are consumed before values overwrite them
 #pragma unroll
             for (int j = 0; j < ITEMS; ++j)
             {
                 const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
-                stage.vals[rank[j]] = kIota ? valueBase + idx : (idx < rLen ? vr[idx] : 0u);
+                std::uint32_t v;
+                if constexpr (kIota) v = valueBase + idx;
+                else if constexpr (kPreVals) v = curVal[j];
+                else v = idx < rLen ? vr[idx] : 0u;
+                stage.vals[rank[j]] = v;
             }
             __syncthreads();
 #pragma unroll
@@ -511,15 +556,15 @@ __global__ __launch_bounds__(kSortThreads) void segScanKernel(const std::uint32_
     for (std::uint32_t r = r0; r < r1; ++r) off[std::size_t(r) * kRadix + d] += base;
 }
 
-template <int THREADS, int ITEMS>
+template <int THREADS, int ITEMS, bool PRE = false>
 void launchScatter(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, const Key *kin, Key *kout,
                    const std::uint32_t *vin, std::uint32_t *vout, bool vals, bool iota, bool digits, int shift)
 {
     const dim3 grid(unsigned(pl.ranges)), block(THREADS);
     const int next = shift + kRadixBits;
-#define AT_SCATTER(V, I, D)                                                                                         \
-    hipLaunchKernelGGL((scatterKernel<THREADS, ITEMS, V, I, D>), grid, block, 0, s, kin, kout, vin, vout, ws.digits, \
-                       pl.n, pl.rangeKeys, ws.rangeOff, ws.digitTotal, pl.ranges, shift, next)
+#define AT_SCATTER(V, I, D)                                                                                          \
+    hipLaunchKernelGGL((scatterKernel<THREADS, ITEMS, V, I, D, false, PRE>), grid, block, 0, s, kin, kout, vin, vout, \
+                       ws.digits, pl.n, pl.rangeKeys, ws.rangeOff, ws.digitTotal, pl.ranges, shift, next)
     if (!vals)
     {
         if (digits) AT_SCATTER(false, false, true);
@@ -574,8 +619,12 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         const bool digits = g_digitArray && pass + 1 < kSortPasses;
         prof::Scope scope("sort_pass", s);
         const bool first = iota && pass == 0;
-        switch (pl.threads * 100 + pl.items)
+        switch (pl.threads * 100 + pl.items + (g_prefetch ? 1000000 : 0))
         {
+        case 1025608: launchScatter<256, 8, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 1025612: launchScatter<256, 12, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 1051208: launchScatter<512, 8, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        case 1051212: launchScatter<512, 12, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 25608: launchScatter<256, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 25612: launchScatter<256, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 25616: launchScatter<256, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
@@ -602,6 +651,12 @@ void setSortThreads(int threads)
 {
     if (threads != 256 && threads != 512 && threads != 1024) throw Error(1, "sort threads must be 256, 512 or 1024");
     g_threads = threads;
+}
+int sortPrefetch() { return g_prefetch; }
+void setSortPrefetch(int on)
+{
+    if (on != 0 && on != 1) throw Error(1, "sort_prefetch must be 0 or 1");
+    g_prefetch = on;
 }
 int sortDigitArray() { return g_digitArray; }
 void setSortDigitArray(int on)
@@ -766,19 +821,23 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
         }
         return best;
     };
-    // coordinate descent: tile shape, then ranges, then the histogram source
-    const int shapes[][2] = {{256, 8}, {256, 12}, {256, 16}, {512, 8}, {512, 12}, {512, 16}, {1024, 8}, {1024, 12}};
+    // coordinate descent: tile shape (with / without next-tile prefetch),
+    // then ranges, then the histogram source
+    const int shapes[][3] = {{256, 8, 0},  {256, 12, 0}, {256, 16, 0}, {512, 8, 0},  {512, 12, 0}, {512, 16, 0},
+                             {1024, 8, 0}, {1024, 12, 0}, {256, 8, 1}, {256, 12, 1}, {512, 8, 1}, {512, 12, 1}};
     float best = 1e30f;
-    int bt = g_threads, bi = g_items;
+    int bt = g_threads, bi = g_items, bp = g_prefetch;
     for (const auto &sh : shapes)
     {
         g_threads = sh[0];
         g_items = sh[1];
+        g_prefetch = sh[2];
         const float t = timeOne();
-        if (t < best) best = t, bt = sh[0], bi = sh[1];
+        if (t < best) best = t, bt = sh[0], bi = sh[1], bp = sh[2];
     }
     g_threads = bt;
     g_items = bi;
+    g_prefetch = bp;
     int br = g_targetRanges;
     for (int r : {512, 1024, 2048, 4096})
     {
@@ -799,6 +858,7 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
     (void)hipEventDestroy(e1);
     AT_HIP(hipStreamSynchronize(s));
     return "{\"sort_threads\": " + std::to_string(bt) + ", \"sort_items\": " + std::to_string(bi) +
+           ", \"sort_prefetch\": " + std::to_string(bp) +
            ", \"sort_ranges\": " + std::to_string(br) + ", \"sort_digit_array\": " + std::to_string(bd) +
            ", \"keys\": " + std::to_string(n) + ", \"sort_ms\": " + std::to_string(best) + "}";
 }

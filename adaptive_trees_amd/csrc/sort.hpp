@@ -27,6 +27,10 @@ void setSortThreads(int threads);
 // histogram reads those; 0: the histogram reads the keys.
 int sortDigitArray();
 void setSortDigitArray(int on);
+// 1: the scatter loads the next tile during the current one (tile shapes
+// 256/512 x 8/12; others ignore it).
+int sortPrefetch();
+void setSortPrefetch(int on);
 
 // Times every tile shape, then range count, then histogram source on a
 // stable sort of n random 63-bit (key, index) pairs and keeps the fastest

@@ -8,7 +8,7 @@ import json
 
 from . import _ffi
 
-KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array")
+KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch")
 
 
 def get():

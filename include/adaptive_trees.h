@@ -249,7 +249,9 @@ int at_server_get_stats(at_server *server, at_server_stats *out);
  * "sort_ranges": target number of key ranges per sort pass (64..65536);
  * "sort_digit_array": 1 = each scatter pass writes the next digit byte per
  *                     key for the next histogram, 0 = the histogram reads
- *                     the keys. */
+ *                     the keys;
+ * "sort_prefetch":    1 = the scatter loads the next tile while it works on
+ *                     the current one (tiles 256/512 x 8/12). */
 int at_tune_set(const char *key, int value);
 int at_tune_get(const char *key, int *value);
 /* Autotuner: times the tile shapes, then the range counts, then the

@@ -11,10 +11,11 @@ from adaptive_trees_amd import BINARY, HILBERT, MORTON, OCTREE, _ffi
 from pairs import Pair, plummer
 
 TUNINGS = [
-    (512, 12, 2048, 1), (512, 12, 2048, 0), (256, 8, 64, 1), (256, 16, 4096, 0),
-    (1024, 16, 512, 1), (1024, 8, 65536, 0), (512, 16, 1024, 1),
+    (512, 12, 2048, 1, 0), (512, 12, 2048, 0, 0), (256, 8, 64, 1, 0), (256, 16, 4096, 0, 0),
+    (1024, 16, 512, 1, 0), (1024, 8, 65536, 0, 0), (512, 16, 1024, 1, 0),
+    (512, 8, 2048, 1, 1), (256, 12, 64, 0, 1), (512, 12, 4096, 1, 1), (256, 8, 100, 0, 1),
 ]
-KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array")
+KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch")
 
 
 @pytest.fixture
@@ -36,7 +37,7 @@ def tuning():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cfg", TUNINGS, ids=lambda c: "%dx%d_r%d_d%d" % c)
+@pytest.mark.parametrize("cfg", TUNINGS, ids=lambda c: "%dx%d_r%d_d%d_p%d" % c)
 def test_tunings_are_bit_exact(tuning, cfg):
     tuning(cfg)
     print("tuning", cfg, flush=True)
@@ -58,7 +59,7 @@ def test_tunings_are_bit_exact(tuning, cfg):
 def test_invalid_tunings_are_rejected():
     lib = _ffi.hip_lib()
     for k, v in (("sort_threads", 300), ("sort_items", 7), ("sort_ranges", 1), ("sort_digit_array", 2),
-                 ("no_such_key", 1)):
+                 ("sort_prefetch", 3), ("no_such_key", 1)):
         with pytest.raises(_ffi.InvalidArgument):
             lib.call("tune_set", k.encode(), v)
 
