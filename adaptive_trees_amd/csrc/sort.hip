@@ -103,9 +103,13 @@ template <class T, KeyKind K>
 __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, std::size_t n, std::size_t rangeKeys,
                                                              int parts, Key *__restrict__ keys, std::uint32_t *partHist)
 {
-    __shared__ std::uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
+    // one histogram per wave: the lowest digit is close to uniform, so plain
+    // LDS atomics on a wave-private table collide least
+    __shared__ std::uint32_t hw[kWaves][kRadix];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) hw[w][threadIdx.x] = 0;
     __syncthreads();
+    std::uint32_t *h = hw[threadIdx.x >> 6];
     const std::size_t r = blockIdx.x / unsigned(parts), part = blockIdx.x % unsigned(parts);
     const std::size_t partKeys = rangeKeys / std::size_t(parts);
     const std::size_t rEnd = (r + 1) * rangeKeys < n ? (r + 1) * rangeKeys : n;
@@ -123,16 +127,20 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
         for (int u = 0; u < kKeygenUnroll; ++u)
         {
             keys[i + std::size_t(u) * kSortThreads] = k[u];
-            countDigit(h, unsigned(k[u]) & (kRadix - 1));
+            atomicAdd(&h[unsigned(k[u]) & (kRadix - 1)], 1u);
         }
     }
     for (; i < end; i += kSortThreads)
     {
         const Key k = particleKey<T, K>(p, i);
         keys[i] = k;
-        countDigit(h, unsigned(k) & (kRadix - 1));
+        atomicAdd(&h[unsigned(k) & (kRadix - 1)], 1u);
     }
-    flushHistogram(h, partHist);
+    __syncthreads();
+    std::uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += hw[w][threadIdx.x];
+    partHist[std::size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
 
 // Digit histogram of range blockIdx.x read from the keys: two keys per

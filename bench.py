@@ -17,6 +17,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -451,7 +452,7 @@ def traffic_from_counters(fetch_csv, write_csv, copy_bytes):
     for label, path in (("fetch", fetch_csv), ("write", write_csv)):
         rows = _counter_rows(path)
         copy = [v for k, v in rows if "copyKernel" in k]
-        scat = [v for k, v in rows if "scatterKernel" in k and "true, false, true" in k]
+        scat = [v for k, v in rows if re.search(r"scatterKernel<\d+, \d+, true, false, true", k)]
         if not copy or not scat:
             raise SystemExit("no copyKernel / scatterKernel dispatches in %s" % path)
         factor = copy_bytes / (float(np.median(copy)) * 1024.0)
