@@ -33,11 +33,11 @@ namespace
 constexpr int kWaves = kSortThreads / 64;
 constexpr std::size_t kHistogramBlocks = 4096;
 
-int g_items = 12;
+int g_items = 8;
 int g_threads = 512;
 int g_targetRanges = 2048;
 int g_digitArray = 1; // scatter writes the next pass's digit bytes for the histogram
-int g_prefetch = 0;   // scatter loads the next tile while working on the current one
+int g_prefetch = 1;   // scatter loads the next tile while working on the current one
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 

@@ -16,7 +16,7 @@ constexpr int kSortPasses = 64 / kRadixBits;
 constexpr int kSortThreads = 256;
 
 // Tile shape (threads per block, keys per thread) and target number of
-// ranges, chosen by the autotuner; defaults 512, 12 and 2048.
+// ranges, chosen by the autotuner; defaults 512, 8 (with next-tile prefetch) and 2048.
 int sortItemsPerThread();
 void setSortItemsPerThread(int items);
 int sortTargetRanges();
