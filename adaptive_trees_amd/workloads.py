@@ -4,12 +4,12 @@ All generators draw from numpy's PCG64 (np.random.default_rng(seed)) in
 float64 and cast to the requested precision at the end, so the same seed
 gives the same coordinates on every host.
 
-  uniform_cube   C1: uniform in [0,1]^3
-  plummer        C2: Plummer sphere, scale radius a, truncated at 20a,
+  uniform_cube   uniform in a box (default [0,1]^3)
+  plummer        Plummer sphere, scale radius a, truncated at 20a,
                  centred in the unit box and clipped to it
-  two_clusters   C3: 4:1:1 box, 40%/40% isotropic Gaussians at x = 1 and 3
+  two_clusters   4:1:1 box, 40%/40% isotropic Gaussians at x = 1 and 3
                  (sigma = 0.02 of the long side), 20% uniform, clipped
-  batch_sizes    C4: log-uniform tree sizes
+  batch_sizes    log-uniform tree sizes for batches
   reflect_step   Gaussian displacement with reflecting walls
 """
 import numpy as np
