@@ -351,8 +351,26 @@ __device__ void binProcessBody(const TreeDev &t, unsigned block)
     const int op = int(t.offs[i + 1] - t.offs[i]);
     if (!op) return;
     const std::int64_t dst = t.offs[i];
+    // The new leaf counts follow from the old ones (exact counts of the same
+    // P): a kept leaf keeps its count, a leaf whose right sibling merges into
+    // it (op 0 on i+1) adds it, a split leaf searches its midpoint. Equal to
+    // recounting every leaf (calculateNodeCount) after the pass.
+    const std::uint32_t *nOld = t.N[ab];
+    std::uint32_t *nNew = t.N[ab ^ 1];
     out[dst] = tree[i];
-    if (op == 2) out[dst + 1] = tree[i] + (tree[i + 1] - tree[i]) / 2;
+    if (op == 1)
+    {
+        const bool absorbs = i + 1 < L && t.offs[i + 2] == t.offs[i + 1];
+        nNew[dst] = nOld[i] + (absorbs ? nOld[i + 1] : 0u);
+        return;
+    }
+    const Key mid = tree[i] + (tree[i + 1] - tree[i]) / 2;
+    out[dst + 1] = mid;
+    std::size_t a, b;
+    lowerBoundIndexedPair(t.P, t.n, t.Pidx, tree[i], mid, a, b);
+    const std::uint32_t left = std::uint32_t(b - a);
+    nNew[dst] = left;
+    nNew[dst + 1] = nOld[i] - left;
 }
 
 __global__ __launch_bounds__(kThreads) void binProcessKernel(TreeDev t) { binProcessBody(t, blockIdx.x); }
@@ -1173,8 +1191,7 @@ void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 void binPass(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves)
 {
     binDecideScan(s, t, p, capacity, gridLeaves);
-    binProcessCommit(s, t, gridLeaves);
-    leafCounts(s, t, gridLeaves);
+    binProcessCommit(s, t, gridLeaves); // the process step writes the new leaf counts
 }
 
 void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d)

@@ -480,6 +480,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-tree workloads")
     ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the serving loop")
+    ap.add_argument("--serve-rates", default="50,200,1000,2000,10000", help="arrival rates (requests/s)")
     ap.add_argument("--only-configs", action="store_true", help="run only the other single-tree workloads")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -501,7 +502,8 @@ def main():
         print(json.dumps({"batched": batch_benchmark()}))
         return
     if args.only_serve:
-        print(json.dumps({"serving": serving_benchmark(workers=args.serve_workers)}))
+        rates = tuple(int(r) for r in args.serve_rates.split(","))
+        print(json.dumps({"serving": serving_benchmark(rates=rates, workers=args.serve_workers)}))
         return
     if args.only_configs:
         print(json.dumps({"other_configs": other_configs()}))
