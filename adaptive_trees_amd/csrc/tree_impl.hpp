@@ -338,6 +338,7 @@ private:
             t.gcm[h] = nf_ ? gcm_[h].p : nullptr;
         }
         t.nfDirty = nf_ ? nfDirty_.p : nullptr;
+        t.countDirty = nf_ ? nullptr : countDirty_.p;
         t.groupDirty = nf_ ? groupDirty_.p : nullptr;
         t.offs = offs_.p;
         t.scanStatus = status_.p;
@@ -376,6 +377,7 @@ private:
                 gcm_[h].ensure(leaves, keep, stream_);
             }
         }
+        if (!nf_) countDirty_.ensure(leaves, 0, stream_);
         if (nf_)
         {
             nfDirty_.ensure(leaves, 0, stream_);
@@ -398,6 +400,7 @@ private:
         const int h = activeBuffer_ ^ 1;
         K_[h].ensure(newK, 0, stream_);
         N_[h].ensure(newK, 0, stream_);
+        if (!nf_) countDirty_.ensure(newK, 0, stream_);
         if (nf_)
         {
             NF_[h].ensure(newK, 0, stream_);
@@ -438,6 +441,7 @@ private:
         }
         K_[ab ^ 1].ensure(capacity + 1, 0, stream_);
         N_[ab ^ 1].ensure(capacity, 0, stream_);
+        if (!nf_) countDirty_.ensure(capacity, 0, stream_);
         if (nf_)
         {
             NF_[ab ^ 1].ensure(capacity, 0, stream_);
@@ -499,7 +503,7 @@ private:
     std::uint32_t *permp_ = nullptr;
     DevBuf<Key> K_[2];
     DevBuf<std::uint32_t> N_[2], NF_[2];
-    DevBuf<std::uint8_t> gcm_[2], nfDirty_, groupDirty_;
+    DevBuf<std::uint8_t> gcm_[2], nfDirty_, groupDirty_, countDirty_;
     DevBuf<std::int64_t> offs_;
     DevBuf<std::uint64_t> status_;
     DevBuf<std::uint32_t> ticket_, tasks_, groups_, taskCount_;

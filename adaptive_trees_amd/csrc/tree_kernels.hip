@@ -287,6 +287,15 @@ __device__ void octProcessBody(const TreeDev &t, bool nf, unsigned block)
     if (code == 1)
     {
         out[dst] = node;
+        if (!nf && t.countDirty)
+        {
+            // kept leaf: same count; a merge leader absorbs its 7 siblings (op 0)
+            std::uint32_t c = t.N[ab][i];
+            if (i + 1 < L && ops[i + 2] == ops[i + 1])
+                for (int k = 1; k < 8; ++k) c += t.N[ab][i + k];
+            t.N[ab ^ 1][dst] = c;
+            t.countDirty[dst] = 0;
+        }
         if (nf)
         {
             const bool leader = i + 1 < L && ops[i + 2] - ops[i + 1] == 0;
@@ -326,6 +335,7 @@ __device__ void octProcessBody(const TreeDev &t, bool nf, unsigned block)
     for (int s = 0; s < code; ++s)
     {
         out[dst + s] = node + Key(s) * r;
+        if (!nf && t.countDirty) t.countDirty[dst + s] = 1; // counted by the refresh
         if (nf)
         {
             t.nfDirty[dst + s] = 1;
@@ -468,13 +478,15 @@ __device__ void binInitBody(const TreeDev &t, std::size_t words, unsigned block,
 __global__ void binInitKernel(TreeDev t, std::size_t words) { binInitBody(t, words, blockIdx.x, gridDim.x); }
 
 // N[i] = lb(P, K[i+1]) - lb(P, K[i]) over all slots (calculateNodeCount).
-__device__ void leafCountsBody(const TreeDev &t, unsigned block)
+// Without `full`, leaves whose count the process step derived are skipped.
+__device__ void leafCountsBody(const TreeDev &t, unsigned block, int full = 1)
 {
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
     const int i = int(block * blockDim.x + threadIdx.x);
     if (i >= L) return;
+    if (!full && t.countDirty && !t.countDirty[i]) return;
     const int ab = st->activeBuffer;
     const Key *K = t.K[ab];
     std::size_t a, b;
@@ -482,7 +494,10 @@ __device__ void leafCountsBody(const TreeDev &t, unsigned block)
     const std::size_t c = b - a;
     t.N[ab][i] = c > 0xffffffffu ? 0xffffffffu : unsigned(c);
 }
-__global__ __launch_bounds__(kThreads) void leafCountsKernel(TreeDev t) { leafCountsBody(t, blockIdx.x); }
+__global__ __launch_bounds__(kThreads) void leafCountsKernel(TreeDev t, int full)
+{
+    leafCountsBody(t, blockIdx.x, full);
+}
 
 // ---- near-field work list ------------------------------------------------------
 __device__ inline bool dirtyLeader(const Key *tree, const std::uint8_t *gDirty, int L, int i)
@@ -1027,10 +1042,10 @@ __global__ void multiCommitKernel(TreeSet s, int binary)
     AT_MULTI_PROLOGUE;
     commitBody(t.st, binary, t.scanStatus, t.statusWords, t.scanTicket, t.nfCount, block, blocks);
 }
-__global__ __launch_bounds__(kThreads) void multiLeafCountsKernel(TreeSet s)
+__global__ __launch_bounds__(kThreads) void multiLeafCountsKernel(TreeSet s, int full)
 {
     AT_MULTI_PROLOGUE;
-    leafCountsBody(t, block);
+    leafCountsBody(t, block, full);
 }
 __global__ __launch_bounds__(kThreads) void multiNfCollectKernel(TreeSet s, int full)
 {
@@ -1111,7 +1126,7 @@ void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, co
     if (!p.nf)
     {
         prof::Scope scope("leaf_counts", s);
-        hipLaunchKernelGGL(multiLeafCountsKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves);
+        hipLaunchKernelGGL(multiLeafCountsKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, full ? 1 : 0);
         return;
     }
     prof::Scope scope("near_field", s);
@@ -1155,7 +1170,8 @@ void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, 
     if (!p.nf)
     {
         prof::Scope scope("leaf_counts", s);
-        hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t);
+        hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t,
+                           full ? 1 : 0);
         return;
     }
     prof::Scope scope("near_field", s);
@@ -1186,7 +1202,7 @@ void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 {
     prof::Scope scope("leaf_counts", s);
-    hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t);
+    hipLaunchKernelGGL(leafCountsKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t, 1);
 }
 void binPass(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves)
 {

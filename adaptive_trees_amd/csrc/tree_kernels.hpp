@@ -14,6 +14,7 @@ struct TreeDev
     std::uint32_t *NF[2] = {nullptr, nullptr};   // near-field counts (octree NF)
     std::uint8_t *gcm[2] = {nullptr, nullptr};   // groupCanMerge (octree NF)
     std::uint8_t *nfDirty = nullptr;
+    std::uint8_t *countDirty = nullptr;          // LeafCount octree: leaf count still to be searched
     std::uint8_t *groupDirty = nullptr;
     std::int64_t *offs = nullptr;                // scanned node ops, capacity + 1
     std::uint64_t *scanStatus = nullptr;         // chained-scan status words
@@ -71,6 +72,8 @@ void updateReset(hipStream_t s, const TreeDev &t);
 void octPass(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves);
 void octDecideScan(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves);
 void octProcessCommit(hipStream_t s, const TreeDev &t, const OctParams &p, std::size_t gridLeaves);
+// full: recount every leaf (P changed); otherwise only the leaves the last
+// process step could not derive.
 void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, std::size_t gridLeaves);
 
 // ---- binary ----
