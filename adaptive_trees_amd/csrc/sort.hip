@@ -23,6 +23,7 @@
 #include "sort.hpp"
 
 #include <string>
+#include <type_traits>
 
 #include "profile.hpp"
 
@@ -351,26 +352,33 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
         rLen = std::uint32_t((rBegin + rangeKeys < n ? rBegin + rangeKeys : n) - rBegin);
         valueBase = std::uint32_t(rBegin);
     }
+    if (rLen == 0) return; // block-uniform
     const Key *kr = kin + rBegin;
     const std::uint32_t *vr = vin + rBegin;
+    const std::uint32_t last = rLen - 1;
     const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
     constexpr bool kPreVals = kPre && kVals && !kIota;
     Key preKey[kPre ? ITEMS : 1];
     std::uint32_t preVal[kPreVals ? ITEMS : 1];
-    if constexpr (kPre)
-    {
-        const std::uint32_t first = std::uint32_t(wave) * (ITEMS * 64) + lane;
+    // Loads and stores are kept free of per-lane branches (clamped indices
+    // and selects; the partial last tile is a separate copy of the body) so
+    // that the wait for a tile's keys counts only the loads, not the stores
+    // still in flight from the previous tile.
+    auto prefetch = [&](std::uint32_t first) {
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
         {
             const std::uint32_t idx = first + std::uint32_t(j) * 64;
-            preKey[j] = idx < rLen ? kr[idx] : kInvalidKey;
-            if constexpr (kPreVals) preVal[j] = idx < rLen ? vr[idx] : 0u;
+            const std::uint32_t c = idx < rLen ? idx : last;
+            const Key k = kr[c];
+            preKey[j] = idx < rLen ? k : kInvalidKey;
+            if constexpr (kPreVals) preVal[j] = vr[c];
         }
-    }
+    };
+    if constexpr (kPre) prefetch(std::uint32_t(wave) * (ITEMS * 64) + lane);
 
-    for (std::uint32_t base = 0; base < rLen; base += kTile)
-    {
+    auto tile = [&](std::uint32_t base, auto fullTag) {
+        constexpr bool kFull = decltype(fullTag)::value;
         for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
         // ---- load keys (wave-striped: item j of lane l is element j*64 + l of the wave's chunk)
         Key key[ITEMS];
@@ -384,16 +392,7 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
                 key[j] = preKey[j];
                 if constexpr (kPreVals) curVal[j] = preVal[j];
             }
-            if (base + kTile < rLen)
-            {
-#pragma unroll
-                for (int j = 0; j < ITEMS; ++j)
-                {
-                    const std::uint32_t idx = waveFirst + kTile + std::uint32_t(j) * 64;
-                    preKey[j] = idx < rLen ? kr[idx] : kInvalidKey;
-                    if constexpr (kPreVals) preVal[j] = idx < rLen ? vr[idx] : 0u;
-                }
-            }
+            if (kFull) prefetch(waveFirst + kTile); // the next tile, or clamped reloads of the last key
         }
         else
         {
@@ -401,7 +400,12 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
             for (int j = 0; j < ITEMS; ++j)
             {
                 const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
-                key[j] = idx < rLen ? kr[idx] : kInvalidKey; // padding ranks last with digit 255
+                if constexpr (kFull) key[j] = kr[idx];
+                else
+                {
+                    const Key k = kr[idx < rLen ? idx : last];
+                    key[j] = idx < rLen ? k : kInvalidKey; // padding ranks last with digit 255
+                }
             }
         }
         __syncthreads();
@@ -457,12 +461,12 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
         }
         __syncthreads();
         const std::uint32_t remaining = rLen - base;
-        const int valid = remaining < std::uint32_t(kTile) ? int(remaining) : kTile;
+        const int valid = kFull ? kTile : int(remaining);
 #pragma unroll
         for (int q = 0; q < ITEMS; ++q)
         {
             const int i = q * THREADS + tid;
-            if (i < valid)
+            if (kFull || i < valid)
             {
                 const Key k = stage.keys[i];
                 const unsigned dg = unsigned(k >> shift) & (kRadix - 1);
@@ -482,7 +486,8 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
                 std::uint32_t v;
                 if constexpr (kIota) v = valueBase + idx;
                 else if constexpr (kPreVals) v = curVal[j];
-                else v = idx < rLen ? vr[idx] : 0u;
+                else if constexpr (kFull) v = vr[idx];
+                else v = vr[idx < rLen ? idx : last]; // padding values are staged past `valid`, never written
                 stage.vals[rank[j]] = v;
             }
             __syncthreads();
@@ -490,12 +495,16 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
             for (int q = 0; q < ITEMS; ++q)
             {
                 const int i = q * THREADS + tid;
-                if (i < valid) vout[adjust[digitAt[i]] + std::uint32_t(i)] = stage.vals[i];
+                if (kFull || i < valid) vout[adjust[digitAt[i]] + std::uint32_t(i)] = stage.vals[i];
             }
         }
         if (tid < kRadix) running[tid] += tileCount;
         __syncthreads();
-    }
+    };
+
+    std::uint32_t base = 0;
+    for (; rLen - base >= std::uint32_t(kTile); base += kTile) tile(base, std::true_type{});
+    if (base < rLen) tile(base, std::false_type{});
 }
 
 // ---- segmented sort kernels --------------------------------------------------
