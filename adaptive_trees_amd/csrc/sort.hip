@@ -318,13 +318,10 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
 {
     constexpr int kW = THREADS / 64;
     constexpr int kTile = ITEMS * THREADS;
-    __shared__ union
-    {
-        Key keys[kTile];
-        std::uint32_t vals[kTile];
-    } stage;
-    __shared__ std::uint8_t digitAt[kVals ? kTile : 1]; // digit of each staged position
-    __shared__ std::uint16_t waveCount[kW][kRadix];     // counts and offsets < 2^16
+    __shared__ Key stageKeys[kTile];                         // the tile in output order
+    __shared__ std::uint32_t stageVals[kVals ? kTile : 1];
+    __shared__ std::uint32_t waveCount[kW][kRadix]; // per-wave digit counts, then offsets
+    __shared__ std::uint32_t sink[kW][64];          // ranking adds of non-leader lanes
     __shared__ std::uint32_t running[kRadix];
     __shared__ std::uint32_t adjust[kRadix];
     __shared__ std::uint32_t scratch[kRadix / 64];
@@ -353,6 +350,8 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
         valueBase = std::uint32_t(rBegin);
     }
     if (rLen == 0) return; // block-uniform
+    for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
+    __syncthreads();
     const Key *kr = kin + rBegin;
     const std::uint32_t *vr = vin + rBegin;
     const std::uint32_t last = rLen - 1;
@@ -377,12 +376,14 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
     };
     if constexpr (kPre) prefetch(std::uint32_t(wave) * (ITEMS * 64) + lane);
 
+    // One tile: 5 barriers (ranked, wave totals, offsets, staged, written).
+    // The counters are cleared during the write-out, ready for the next tile.
     auto tile = [&](std::uint32_t base, auto fullTag) {
         constexpr bool kFull = decltype(fullTag)::value;
-        for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
+        constexpr bool kLoadVals = kVals && !kIota;
         // ---- load keys (wave-striped: item j of lane l is element j*64 + l of the wave's chunk)
         Key key[ITEMS];
-        std::uint32_t curVal[kPreVals ? ITEMS : 1];
+        std::uint32_t val[kLoadVals ? ITEMS : 1];
         const std::uint32_t waveFirst = base + std::uint32_t(wave) * (ITEMS * 64) + lane;
         if constexpr (kPre)
         {
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
             for (int j = 0; j < ITEMS; ++j)
             {
                 key[j] = preKey[j];
-                if constexpr (kPreVals) curVal[j] = preVal[j];
+                if constexpr (kPreVals) val[j] = preVal[j];
             }
             if (kFull) prefetch(waveFirst + kTile); // the next tile, or clamped reloads of the last key
         }
@@ -400,18 +401,21 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
             for (int j = 0; j < ITEMS; ++j)
             {
                 const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
-                if constexpr (kFull) key[j] = kr[idx];
-                else
-                {
-                    const Key k = kr[idx < rLen ? idx : last];
-                    key[j] = idx < rLen ? k : kInvalidKey; // padding ranks last with digit 255
-                }
+                const std::uint32_t c = kFull || idx < rLen ? idx : last;
+                const Key k = kr[c];
+                key[j] = kFull || idx < rLen ? k : kInvalidKey; // padding ranks last with digit 255
+                if constexpr (kLoadVals) val[j] = vr[c];        // padding values are never written
             }
         }
-        __syncthreads();
 
-        // ---- rank within the wave: lanes with equal digits via 8 ballots
-        std::uint32_t rank[ITEMS];
+        // ---- rank within the wave: lanes with equal digits via 8 ballots; the
+        // lowest such lane adds the group's size to the wave's digit counter
+        // (the other lanes add 0 to a slot of their own, so every lane issues
+        // the same LDS atomic and the adds of all items are in flight at
+        // once; one wave's LDS operations execute in order), then the old
+        // count is read from that lane.
+        std::uint32_t rank[ITEMS], old[ITEMS];
+        int leader[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
         {
@@ -425,80 +429,85 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
                 match &= bit ? ballot : ~ballot;
             }
             const std::uint32_t lower = __popcll(match & ltMask);
-            const std::uint32_t before = waveCount[wave][dg];
-            rank[j] = before + lower;
-            if (lower == 0) waveCount[wave][dg] = std::uint16_t(before + __popcll(match));
+            const bool lead = lower == 0;
+            std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
+            old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
+            leader[j] = __builtin_ctzll(match);
+            rank[j] = lower;
         }
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) rank[j] += std::uint32_t(__shfl(int(old[j]), leader[j], 64));
         __syncthreads();
 
         // ---- per digit: wave offsets, tile count, tile-local digit start
-        std::uint32_t tileCount = 0;
+        // (threads 0..255, one digit each; wave-inclusive scan, then the
+        // totals of the four digit waves)
+        std::uint32_t tileCount = 0, inc = 0;
         if (tid < kRadix)
         {
 #pragma unroll
             for (int w = 0; w < kW; ++w)
             {
                 const std::uint32_t c = waveCount[w][tid];
-                waveCount[w][tid] = std::uint16_t(tileCount);
+                waveCount[w][tid] = tileCount;
                 tileCount += c;
             }
+            inc = tileCount;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+            {
+                const std::uint32_t t = __shfl_up(inc, o, 64);
+                if (lane >= unsigned(o)) inc += t;
+            }
+            if (lane == 63) scratch[wave] = inc;
         }
-        const std::uint32_t tileStart = digitExclusiveScan(tileCount, scratch);
+        __syncthreads();
         if (tid < kRadix)
         {
+            std::uint32_t before = 0;
 #pragma unroll
-            for (int w = 0; w < kW; ++w) waveCount[w][tid] = std::uint16_t(waveCount[w][tid] + tileStart);
+            for (int w = 0; w < kRadix / 64; ++w)
+                if (w < wave) before += scratch[w];
+            const std::uint32_t tileStart = before + inc - tileCount;
+#pragma unroll
+            for (int w = 0; w < kW; ++w) waveCount[w][tid] += tileStart;
             adjust[tid] = running[tid] - tileStart;
+            running[tid] += tileCount;
         }
         __syncthreads();
 
-        // ---- stage keys in output order, then write runs of equal digits
+        // ---- stage keys (and values) in output order
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j)
         {
-            rank[j] += waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)]; // local position
-            stage.keys[rank[j]] = key[j];
+            const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)];
+            stageKeys[pos] = key[j];
+            if constexpr (kVals)
+            {
+                std::uint32_t v;
+                if constexpr (kIota) v = valueBase + waveFirst + std::uint32_t(j) * 64;
+                else v = val[j];
+                stageVals[pos] = v;
+            }
         }
         __syncthreads();
-        const std::uint32_t remaining = rLen - base;
-        const int valid = kFull ? kTile : int(remaining);
+
+        // ---- write runs of equal digits; clear the counters for the next tile
+        for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
+        const int valid = kFull ? kTile : int(rLen - base);
 #pragma unroll
         for (int q = 0; q < ITEMS; ++q)
         {
             const int i = q * THREADS + tid;
             if (kFull || i < valid)
             {
-                const Key k = stage.keys[i];
-                const unsigned dg = unsigned(k >> shift) & (kRadix - 1);
-                if (kVals) digitAt[i] = std::uint8_t(dg);
-                const std::uint32_t dst = adjust[dg] + std::uint32_t(i);
+                const Key k = stageKeys[i];
+                const std::uint32_t dst = adjust[unsigned(k >> shift) & (kRadix - 1)] + std::uint32_t(i);
                 kout[dst] = k;
                 if (kDigits) digitsOut[dst] = std::uint8_t(k >> nextShift);
+                if (kVals) vout[dst] = stageVals[i];
             }
         }
-        if (kVals)
-        {
-            __syncthreads(); // staged keys are consumed before values overwrite them
-#pragma unroll
-            for (int j = 0; j < ITEMS; ++j)
-            {
-                const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
-                std::uint32_t v;
-                if constexpr (kIota) v = valueBase + idx;
-                else if constexpr (kPreVals) v = curVal[j];
-                else if constexpr (kFull) v = vr[idx];
-                else v = vr[idx < rLen ? idx : last]; // padding values are staged past `valid`, never written
-                stage.vals[rank[j]] = v;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < ITEMS; ++q)
-            {
-                const int i = q * THREADS + tid;
-                if (kFull || i < valid) vout[adjust[digitAt[i]] + std::uint32_t(i)] = stage.vals[i];
-            }
-        }
-        if (tid < kRadix) running[tid] += tileCount;
         __syncthreads();
     };
 
@@ -636,7 +645,9 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         const bool digits = g_digitArray && pass + 1 < kSortPasses;
         prof::Scope scope("sort_pass", s);
         const bool first = iota && pass == 0;
-        switch (pl.threads * 100 + pl.items + (g_prefetch ? 1000000 : 0))
+        // next-tile prefetch is instantiated for tiles of 8 or 12 keys per thread
+        const bool pre = g_prefetch && pl.items <= 12 && pl.threads <= 512;
+        switch (pl.threads * 100 + pl.items + (pre ? 1000000 : 0))
         {
         case 1025608: launchScatter<256, 8, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 1025612: launchScatter<256, 12, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
@@ -649,8 +660,7 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         case 51212: launchScatter<512, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 51216: launchScatter<512, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
         case 102408: launchScatter<1024, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 102412: launchScatter<1024, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        default: launchScatter<1024, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
+        default: throw Error(1, "no scatter kernel for this tile shape");
         }
     }
     AT_HIP(hipGetLastError());
@@ -694,6 +704,9 @@ SortPlan makeSortPlan(std::size_t n)
     p.n = n;
     p.items = g_items;
     p.threads = g_threads;
+    if (p.items * p.threads > kMaxTileKeys)
+        throw Error(1, "sort tile (sort_threads x sort_items) must be at most " + std::to_string(kMaxTileKeys) +
+                           " keys; got " + std::to_string(p.threads) + " x " + std::to_string(p.items));
     p.tileKeys = std::size_t(p.items) * std::size_t(p.threads);
     const std::size_t tiles = (n + p.tileKeys - 1) / p.tileKeys;
     p.rangeTiles = (tiles + std::size_t(g_targetRanges) - 1) / std::size_t(g_targetRanges);
@@ -841,7 +854,7 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
     // coordinate descent: tile shape (with / without next-tile prefetch),
     // then ranges, then the histogram source
     const int shapes[][3] = {{256, 8, 0},  {256, 12, 0}, {256, 16, 0}, {512, 8, 0},  {512, 12, 0}, {512, 16, 0},
-                             {1024, 8, 0}, {1024, 12, 0}, {256, 8, 1}, {256, 12, 1}, {512, 8, 1}, {512, 12, 1}};
+                             {1024, 8, 0}, {256, 8, 1}, {256, 12, 1}, {512, 8, 1}, {512, 12, 1}};
     float best = 1e30f;
     int bt = g_threads, bi = g_items, bp = g_prefetch;
     for (const auto &sh : shapes)

@@ -13,6 +13,7 @@ namespace at
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortPasses = 64 / kRadixBits;
+constexpr int kMaxTileKeys = 8192; // scatter tile (threads x keys per thread) bound by LDS
 constexpr int kSortThreads = 256;
 
 // Tile shape (threads per block, keys per thread) and target number of

@@ -12,8 +12,9 @@ from pairs import Pair, plummer
 
 TUNINGS = [
     (512, 12, 2048, 1, 0), (512, 12, 2048, 0, 0), (256, 8, 64, 1, 0), (256, 16, 4096, 0, 0),
-    (1024, 16, 512, 1, 0), (1024, 8, 65536, 0, 0), (512, 16, 1024, 1, 0),
+    (1024, 8, 512, 1, 0), (1024, 8, 65536, 0, 0), (512, 16, 1024, 1, 0),
     (512, 8, 2048, 1, 1), (256, 12, 64, 0, 1), (512, 12, 4096, 1, 1), (256, 8, 100, 0, 1),
+    (512, 16, 2048, 1, 1), (256, 16, 2048, 0, 1),  # prefetch requested where it is not instantiated
 ]
 KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch")
 
@@ -62,6 +63,15 @@ def test_invalid_tunings_are_rejected():
                  ("sort_prefetch", 3), ("no_such_key", 1)):
         with pytest.raises(_ffi.InvalidArgument):
             lib.call("tune_set", k.encode(), v)
+
+
+@pytest.mark.gpu
+def test_oversized_tile_is_rejected_at_build(tuning):
+    tuning((1024, 16, 2048, 1, 0))
+    x, y, z = plummer(50_000, seed=73)
+    p = Pair(x, y, z, kind=OCTREE, real=np.float32, max_particles_per_leaf=64)
+    with pytest.raises(_ffi.InvalidArgument, match="at most 8192"):
+        p.gpu.build()
 
 
 @pytest.mark.gpu
