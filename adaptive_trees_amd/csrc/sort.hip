@@ -33,12 +33,13 @@ namespace
 {
 constexpr int kWaves = kSortThreads / 64;
 constexpr std::size_t kHistogramBlocks = 4096;
+constexpr std::size_t kXcds = 8; // accelerator complex dies of one MI355X, each with its own L2
 
 int g_items = 8;
 int g_threads = 512;
-int g_targetRanges = 2048;
+int g_targetRanges = 8192; // one 4096-key tile per range at 2^25 keys
 int g_digitArray = 1; // scatter writes the next pass's digit bytes for the histogram
-int g_prefetch = 1;   // scatter loads the next tile while working on the current one
+int g_prefetch = 0;   // scatter loads the next tile while working on the current one
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 
@@ -329,7 +330,16 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
-    const std::size_t r = blockIdx.x;
+    // Workgroups are dispatched round-robin over the 8 XCDs, so block b runs
+    // on XCD b % 8. Each XCD is given a contiguous run of ranges instead:
+    // neighbouring ranges write adjacent pieces of every digit's output, and
+    // running them at the same time on one L2 lets the partial cache lines
+    // at those seams merge before they are written back.
+    std::size_t r = blockIdx.x;
+    {
+        const std::size_t perXcd = ranges / kXcds;
+        if (r < kXcds * perXcd) r = (r % kXcds) * perXcd + r / kXcds;
+    }
     std::size_t rBegin;
     std::uint32_t rLen, valueBase; // range-relative 32-bit offsets (n < 2^32)
     if constexpr (kSeg)
@@ -517,7 +527,7 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
 }
 
 // ---- segmented sort kernels --------------------------------------------------
-constexpr int kSegThreads = 512, kSegItems = 12; // tile shape of the segmented scatter
+constexpr int kSegThreads = 512, kSegItems = 8; // tile shape of the segmented scatter
 
 template <class T, KeyKind K>
 __global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> p, const SegCoords<T> *coords,
@@ -869,7 +879,7 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
     g_items = bi;
     g_prefetch = bp;
     int br = g_targetRanges;
-    for (int r : {512, 1024, 2048, 4096})
+    for (int r : {2048, 4096, 8192, 16384, 32768})
     {
         g_targetRanges = r;
         const float t = timeOne();
@@ -902,9 +912,7 @@ void SegmentedSort::plan(const std::vector<std::size_t> &counts, hipStream_t s)
     total_ = segBegin_[segments_];
     if (total_ >= (std::size_t(1) << 32)) throw Error(3, "segmented sort supports fewer than 2^32 keys in total");
     constexpr std::size_t tile = std::size_t(kSegThreads) * kSegItems;
-    const std::size_t target = (total_ + 2047) / 2048;
-    rangeKeys_ = ((target + tile - 1) / tile) * tile;
-    if (rangeKeys_ < tile) rangeKeys_ = tile;
+    rangeKeys_ = tile; // one tile per range (see the block-to-range mapping of scatterKernel)
     std::vector<SegRange> rs;
     std::vector<std::uint32_t> first(segments_ + 1, 0);
     for (std::size_t g = 0; g < segments_; ++g)

@@ -82,7 +82,7 @@ def test_autotuner_picks_a_valid_setting(tuning):
     r = tune.autotune(1 << 20)
     assert r["sort_threads"] in (256, 512, 1024)
     assert r["sort_items"] in (8, 12, 16)
-    assert r["sort_ranges"] in (512, 1024, 2048, 4096)
+    assert r["sort_ranges"] in (2048, 4096, 8192, 16384, 32768)
     assert r["sort_digit_array"] in (0, 1)
     assert r["sort_ms"] > 0
     assert {k: r[k] for k in tune.KEYS} == tune.get()
