@@ -43,6 +43,18 @@ int g_prefetch = 0;   // scatter loads the next tile while working on the curren
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, so block b runs on
+// XCD b % 8. This maps blocks to rows (key ranges) so that each XCD gets a
+// contiguous run of rows, in order: neighbouring rows write adjacent words
+// of the digit-major histograms and adjacent pieces of every digit's output,
+// and running them at the same time on one L2 lets the partial cache lines
+// at those seams merge there before they are written back.
+__device__ inline std::size_t xcdRow(std::size_t b, std::size_t rows)
+{
+    const std::size_t perXcd = rows / kXcds;
+    return b < kXcds * perXcd ? (b % kXcds) * perXcd + b / kXcds : b;
+}
+
 // Exclusive scan of one value per thread over a 256-thread block.
 __device__ inline std::uint32_t blockExclusiveScan(std::uint32_t v, std::uint32_t *waveTotals)
 {
@@ -112,7 +124,8 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     for (int w = 0; w < kWaves; ++w) hw[w][threadIdx.x] = 0;
     __syncthreads();
     std::uint32_t *h = hw[threadIdx.x >> 6];
-    const std::size_t r = blockIdx.x / unsigned(parts), part = blockIdx.x % unsigned(parts);
+    const std::size_t row = xcdRow(blockIdx.x, gridDim.x);
+    const std::size_t r = row / unsigned(parts), part = row % unsigned(parts);
     const std::size_t partKeys = rangeKeys / std::size_t(parts);
     const std::size_t rEnd = (r + 1) * rangeKeys < n ? (r + 1) * rangeKeys : n;
     const std::size_t begin = r * rangeKeys + part * partKeys;
@@ -142,7 +155,7 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     std::uint32_t t = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) t += hw[w][threadIdx.x];
-    partHist[std::size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    partHist[std::size_t(threadIdx.x) * gridDim.x + row] = t;
 }
 
 // Digit histogram of range blockIdx.x read from the keys: two keys per
@@ -157,7 +170,8 @@ __global__ __launch_bounds__(kSortThreads) void upsweepKeysKernel(const Key *__r
     for (int w = 0; w < kWaves; ++w) h[w][threadIdx.x] = 0;
     __syncthreads();
     std::uint32_t *mine = h[threadIdx.x >> 6];
-    const std::size_t begin = std::size_t(blockIdx.x) * rangeKeys;
+    const std::size_t row = xcdRow(blockIdx.x, gridDim.x);
+    const std::size_t begin = row * rangeKeys;
     const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
     if ((reinterpret_cast<std::uintptr_t>(keys) & 15) == 0)
     {
@@ -189,7 +203,7 @@ __global__ __launch_bounds__(kSortThreads) void upsweepKeysKernel(const Key *__r
     std::uint32_t t = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) t += h[w][threadIdx.x];
-    rangeHist[std::size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    rangeHist[std::size_t(threadIdx.x) * gridDim.x + row] = t;
 }
 
 // Digit histogram of range blockIdx.x from the one-byte digit array: 16
@@ -204,7 +218,8 @@ __global__ __launch_bounds__(kSortThreads) void upsweepDigitsKernel(const std::u
     for (int w = 0; w < kWaves; ++w) h[w][threadIdx.x] = 0;
     __syncthreads();
     std::uint32_t *mine = h[threadIdx.x >> 6];
-    const std::size_t begin = std::size_t(blockIdx.x) * rangeKeys;
+    const std::size_t row = xcdRow(blockIdx.x, gridDim.x);
+    const std::size_t begin = row * rangeKeys;
     const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
     const uint4 *quads = reinterpret_cast<const uint4 *>(digits);
     for (std::size_t i = begin + 16 * threadIdx.x; i < end; i += 16 * kSortThreads)
@@ -225,33 +240,62 @@ __global__ __launch_bounds__(kSortThreads) void upsweepDigitsKernel(const std::u
     std::uint32_t t = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) t += h[w][threadIdx.x];
-    rangeHist[std::size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    rangeHist[std::size_t(threadIdx.x) * gridDim.x + row] = t;
 }
 
 // Block digit blockIdx.x: exclusive prefix of the digit's count over the
 // ranges (a range's count is the sum of its `parts` rows) and the total.
-// Ranges are taken 256 at a time, one per thread, with a running carry.
+// Each thread takes 16 consecutive ranges (4096 per block and round, with a
+// running carry between rounds), so a round costs one block scan.
 __global__ __launch_bounds__(kSortThreads) void scanDigitKernel(const std::uint32_t *__restrict__ hist,
                                                                 std::size_t ranges, int parts,
                                                                 std::uint32_t *rangeOff, std::uint32_t *digitTotal)
 {
+    constexpr int kPer = 16;
     __shared__ std::uint32_t waveTotals[2][kWaves];
     const std::size_t d = blockIdx.x;
     const std::size_t rows = ranges * std::size_t(parts);
     const std::uint32_t *h = hist + d * rows;
+    std::uint32_t *out = rangeOff + d * ranges;
+    // 16-byte vector access when every digit's row and this block's rows are aligned
+    const bool vec = parts == 1 && ranges % 4 == 0 && (reinterpret_cast<std::uintptr_t>(hist) & 15) == 0 &&
+                     (reinterpret_cast<std::uintptr_t>(rangeOff) & 15) == 0;
+    const unsigned lane = laneId();
+    const int wave = threadIdx.x >> 6;
     std::uint32_t carry = 0;
     int buf = 0;
-    for (std::size_t r0 = 0; r0 < ranges; r0 += kSortThreads)
+    for (std::size_t r0 = 0; r0 < ranges; r0 += std::size_t(kPer) * kSortThreads)
     {
-        const std::size_t r = r0 + threadIdx.x;
-        std::uint32_t v = 0;
-        if (r < ranges)
-            for (int q = 0; q < parts; ++q) v += h[r * unsigned(parts) + unsigned(q)];
-        // wave-inclusive scan, then the wave totals (double-buffered so one
-        // barrier per group suffices)
-        const unsigned lane = laneId();
-        const int wave = threadIdx.x >> 6;
-        std::uint32_t inc = v;
+        const std::size_t first = r0 + std::size_t(threadIdx.x) * kPer;
+        std::uint32_t v[kPer];
+        if (vec && first + kPer <= ranges)
+        {
+#pragma unroll
+            for (int q = 0; q < kPer / 4; ++q)
+            {
+                const uint4 c = reinterpret_cast<const uint4 *>(h + first)[q];
+                v[4 * q] = c.x, v[4 * q + 1] = c.y, v[4 * q + 2] = c.z, v[4 * q + 3] = c.w;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+            {
+                v[k] = 0;
+                if (first + k < ranges)
+                    for (int q = 0; q < parts; ++q) v[k] += h[(first + k) * unsigned(parts) + unsigned(q)];
+            }
+        }
+        std::uint32_t sum = 0; // v becomes the thread-exclusive prefix
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            const std::uint32_t t = v[k];
+            v[k] = sum;
+            sum += t;
+        }
+        std::uint32_t inc = sum;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1)
         {
@@ -259,7 +303,7 @@ __global__ __launch_bounds__(kSortThreads) void scanDigitKernel(const std::uint3
             if (lane >= unsigned(o)) inc += t;
         }
         if (lane == 63) waveTotals[buf][wave] = inc;
-        __syncthreads();
+        __syncthreads(); // double-buffered totals: one barrier per round
         std::uint32_t before = 0, total = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w)
@@ -268,7 +312,20 @@ __global__ __launch_bounds__(kSortThreads) void scanDigitKernel(const std::uint3
             before += w < wave ? t : 0u;
             total += t;
         }
-        if (r < ranges) rangeOff[d * ranges + r] = carry + before + inc - v;
+        const std::uint32_t base = carry + before + inc - sum;
+        if (vec && first + kPer <= ranges)
+        {
+#pragma unroll
+            for (int q = 0; q < kPer / 4; ++q)
+                reinterpret_cast<uint4 *>(out + first)[q] =
+                    make_uint4(base + v[4 * q], base + v[4 * q + 1], base + v[4 * q + 2], base + v[4 * q + 3]);
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+                if (first + k < ranges) out[first + k] = base + v[k];
+        }
         carry += total;
         buf ^= 1;
     }
@@ -330,16 +387,7 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
-    // Workgroups are dispatched round-robin over the 8 XCDs, so block b runs
-    // on XCD b % 8. Each XCD is given a contiguous run of ranges instead:
-    // neighbouring ranges write adjacent pieces of every digit's output, and
-    // running them at the same time on one L2 lets the partial cache lines
-    // at those seams merge before they are written back.
-    std::size_t r = blockIdx.x;
-    {
-        const std::size_t perXcd = ranges / kXcds;
-        if (r < kXcds * perXcd) r = (r % kXcds) * perXcd + r / kXcds;
-    }
+    const std::size_t r = xcdRow(blockIdx.x, ranges); // gridDim.x == ranges
     std::size_t rBegin;
     std::uint32_t rLen, valueBase; // range-relative 32-bit offsets (n < 2^32)
     if constexpr (kSeg)
