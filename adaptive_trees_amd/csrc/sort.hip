@@ -84,7 +84,7 @@ __device__ inline void countDigit(std::uint32_t *h, unsigned d)
 {
     const std::uint64_t active = __ballot(1);
     const int leader = __ffsll((long long)active) - 1;
-    const unsigned d0 = __shfl(d, leader, 64);
+    const unsigned d0 = __builtin_amdgcn_readfirstlane(d); // the leader's digit, read by the scalar unit
     if (__ballot(d != d0) == 0)
     {
         if (int(laneId()) == leader) atomicAdd(&h[d0], std::uint32_t(__popcll(active)));
@@ -104,15 +104,36 @@ __device__ inline void flushHistogram(const std::uint32_t *h, std::uint32_t *his
 constexpr int kKeygenUnroll = 4;
 
 template <class T, KeyKind K>
-__device__ inline Key particleKey(const KeygenParams<T> &p, std::size_t i)
+__device__ inline Key coordKey(const KeygenParams<T> &p, T x, T y, T z)
 {
-    const T x = p.x[i], y = p.y[i], z = p.z[i];
     if (K == KeyKind::Binary) return binaryKey<T>(x, y, z, p.geom);
     return isEscaped<T>(x, y, z) ? kInvalidKey : gridKey<T>(x, y, z, p.grid, K == KeyKind::Hilbert);
 }
 
+template <class T, KeyKind K>
+__device__ inline Key particleKey(const KeygenParams<T> &p, std::size_t i)
+{
+    return coordKey<T, K>(p, p.x[i], p.y[i], p.z[i]);
+}
+
+// Four consecutive coordinates from a 16-byte aligned address.
+__device__ inline void load4(const float *a, std::size_t i, float v[4])
+{
+    const float4 q = *reinterpret_cast<const float4 *>(a + i);
+    v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+}
+__device__ inline void load4(const double *a, std::size_t i, double v[4])
+{
+    const double2 q0 = reinterpret_cast<const double2 *>(a + i)[0];
+    const double2 q1 = reinterpret_cast<const double2 *>(a + i)[1];
+    v[0] = q0.x, v[1] = q0.y, v[2] = q1.x, v[3] = q1.y;
+}
+
 // Part `blockIdx.x % parts` of a range: its keys and a partial digit-0
-// histogram; each thread keeps kKeygenUnroll particles in flight.
+// histogram. With 16-byte aligned coordinate arrays each thread takes groups
+// of 4 consecutive particles (16-byte loads and stores; parts start at
+// multiples of 4), two groups in flight; otherwise kKeygenUnroll single
+// particles.
 template <class T, KeyKind K>
 __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, std::size_t n, std::size_t rangeKeys,
                                                              int parts, Key *__restrict__ keys, std::uint32_t *partHist)
@@ -131,18 +152,57 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     const std::size_t begin = r * rangeKeys + part * partKeys;
     const std::size_t end0 = begin + partKeys;
     const std::size_t end = end0 < rEnd ? end0 : rEnd;
-    constexpr std::size_t kStep = std::size_t(kSortThreads) * kKeygenUnroll;
-    std::size_t i = begin + threadIdx.x;
-    for (; i + (kKeygenUnroll - 1) * kSortThreads < end; i += kStep)
+    const bool vec = ((reinterpret_cast<std::uintptr_t>(p.x) | reinterpret_cast<std::uintptr_t>(p.y) |
+                       reinterpret_cast<std::uintptr_t>(p.z) | reinterpret_cast<std::uintptr_t>(keys)) & 15) == 0 &&
+                     begin % 4 == 0;
+    std::size_t i;
+    if (vec)
     {
-        Key k[kKeygenUnroll];
-#pragma unroll
-        for (int u = 0; u < kKeygenUnroll; ++u) k[u] = particleKey<T, K>(p, i + std::size_t(u) * kSortThreads);
-#pragma unroll
-        for (int u = 0; u < kKeygenUnroll; ++u)
+        constexpr int kG = 2;
+        constexpr std::size_t kGroupStep = std::size_t(kSortThreads) * 4;
+        // block-uniform steps of kG * kGroupStep particles, then the tail
+        std::size_t base = begin;
+        for (; base + kG * kGroupStep <= end; base += kG * kGroupStep)
         {
-            keys[i + std::size_t(u) * kSortThreads] = k[u];
-            atomicAdd(&h[unsigned(k[u]) & (kRadix - 1)], 1u);
+            i = base + 4 * std::size_t(threadIdx.x);
+            T x[kG][4], y[kG][4], z[kG][4];
+#pragma unroll
+            for (int g = 0; g < kG; ++g)
+            {
+                load4(p.x, i + g * kGroupStep, x[g]);
+                load4(p.y, i + g * kGroupStep, y[g]);
+                load4(p.z, i + g * kGroupStep, z[g]);
+            }
+#pragma unroll
+            for (int g = 0; g < kG; ++g)
+            {
+                Key k[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) k[u] = coordKey<T, K>(p, x[g][u], y[g][u], z[g][u]);
+                ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(keys + i + g * kGroupStep);
+                dst[0] = make_ulonglong2(k[0], k[1]);
+                dst[1] = make_ulonglong2(k[2], k[3]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) atomicAdd(&h[unsigned(k[u]) & (kRadix - 1)], 1u);
+            }
+        }
+        i = base + threadIdx.x; // the rest of the part, one particle per thread
+    }
+    else
+    {
+        constexpr std::size_t kStep = std::size_t(kSortThreads) * kKeygenUnroll;
+        i = begin + threadIdx.x;
+        for (; i + (kKeygenUnroll - 1) * kSortThreads < end; i += kStep)
+        {
+            Key k[kKeygenUnroll];
+#pragma unroll
+            for (int u = 0; u < kKeygenUnroll; ++u) k[u] = particleKey<T, K>(p, i + std::size_t(u) * kSortThreads);
+#pragma unroll
+            for (int u = 0; u < kKeygenUnroll; ++u)
+            {
+                keys[i + std::size_t(u) * kSortThreads] = k[u];
+                atomicAdd(&h[unsigned(k[u]) & (kRadix - 1)], 1u);
+            }
         }
     }
     for (; i < end; i += kSortThreads)
@@ -206,41 +266,58 @@ __global__ __launch_bounds__(kSortThreads) void upsweepKeysKernel(const Key *__r
     rangeHist[std::size_t(threadIdx.x) * gridDim.x + row] = t;
 }
 
-// Digit histogram of range blockIdx.x from the one-byte digit array: 16
-// digits per load (ranges start at multiples of the tile size), one LDS
-// histogram per wave.
-__global__ __launch_bounds__(kSortThreads) void upsweepDigitsKernel(const std::uint8_t *__restrict__ digits,
+// Digit histograms of kUpRows consecutive ranges from the one-byte digit
+// array: one wave per range, 16 digits per load (ranges start at multiples
+// of the tile size), one LDS histogram per wave. Thread d then writes digit
+// d's counts of the block's ranges, adjacent in the digit-major table, as one
+// 32-byte piece.
+constexpr int kUpRows = 8;
+__global__ __launch_bounds__(kUpRows * 64) void upsweepDigitsKernel(const std::uint8_t *__restrict__ digits,
                                                                     std::size_t n, std::size_t rangeKeys,
-                                                                    std::uint32_t *rangeHist)
+                                                                    std::size_t ranges, std::uint32_t *rangeHist)
 {
-    __shared__ std::uint32_t h[kWaves][kRadix];
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) h[w][threadIdx.x] = 0;
+    __shared__ std::uint32_t h[kUpRows][kRadix];
+    for (int i = threadIdx.x; i < kUpRows * kRadix; i += kUpRows * 64) (&h[0][0])[i] = 0;
     __syncthreads();
-    std::uint32_t *mine = h[threadIdx.x >> 6];
-    const std::size_t row = xcdRow(blockIdx.x, gridDim.x);
-    const std::size_t begin = row * rangeKeys;
-    const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
-    const uint4 *quads = reinterpret_cast<const uint4 *>(digits);
-    for (std::size_t i = begin + 16 * threadIdx.x; i < end; i += 16 * kSortThreads)
+    const int wave = threadIdx.x >> 6;
+    const unsigned lane = laneId();
+    const std::size_t r0 = xcdRow(blockIdx.x, gridDim.x) * kUpRows;
+    const std::size_t r = r0 + std::size_t(wave);
+    std::uint32_t *mine = h[wave];
+    if (r < ranges)
     {
-        if (i + 16 <= end)
+        const std::size_t begin = r * rangeKeys;
+        const std::size_t end = begin + rangeKeys < n ? begin + rangeKeys : n;
+        const uint4 *quads = reinterpret_cast<const uint4 *>(digits);
+        for (std::size_t i = begin + 16 * lane; i < end; i += 16 * 64)
         {
-            const uint4 q = quads[i >> 4];
-            const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+            if (i + 16 <= end)
+            {
+                const uint4 q = quads[i >> 4];
+                const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+                for (int k = 0; k < 4; ++k)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) countDigit(mine, (w[k] >> (8 * b)) & 0xffu);
+                    for (int b = 0; b < 4; ++b) countDigit(mine, (w[k] >> (8 * b)) & 0xffu);
+            }
+            else
+                for (std::size_t j = i; j < end; ++j) atomicAdd(&mine[digits[j]], 1u);
         }
-        else
-            for (std::size_t j = i; j < end; ++j) atomicAdd(&mine[digits[j]], 1u);
     }
     __syncthreads();
-    std::uint32_t t = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) t += h[w][threadIdx.x];
-    rangeHist[std::size_t(threadIdx.x) * gridDim.x + row] = t;
+    if (threadIdx.x < kRadix)
+    {
+        const unsigned d = threadIdx.x;
+        std::uint32_t *out = rangeHist + std::size_t(d) * ranges + r0;
+        if (r0 + kUpRows <= ranges && ranges % 4 == 0)
+        {
+            reinterpret_cast<uint4 *>(out)[0] = make_uint4(h[0][d], h[1][d], h[2][d], h[3][d]);
+            reinterpret_cast<uint4 *>(out)[1] = make_uint4(h[4][d], h[5][d], h[6][d], h[7][d]);
+        }
+        else
+            for (int w = 0; w < kUpRows; ++w)
+                if (r0 + w < ranges) out[w] = h[w][d];
+    }
 }
 
 // Block digit blockIdx.x: exclusive prefix of the digit's count over the
@@ -689,8 +766,8 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         {
             prof::Scope scope("sort_upsweep", s);
             if (g_digitArray)
-                hipLaunchKernelGGL(upsweepDigitsKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s,
-                                   ws.digits, pl.n, pl.rangeKeys, ws.rangeHist);
+                hipLaunchKernelGGL(upsweepDigitsKernel, dim3(unsigned((pl.ranges + kUpRows - 1) / kUpRows)),
+                                   dim3(kUpRows * 64), 0, s, ws.digits, pl.n, pl.rangeKeys, pl.ranges, ws.rangeHist);
             else
                 hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, src,
                                    pl.n, pl.rangeKeys, shift, ws.rangeHist);
