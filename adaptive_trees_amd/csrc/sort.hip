@@ -40,6 +40,8 @@ int g_threads = 512;
 int g_targetRanges = 8192; // one 4096-key tile per range at 2^25 keys
 int g_digitArray = 1; // scatter writes the next pass's digit bytes for the histogram
 int g_prefetch = 0;   // scatter loads the next tile while working on the current one
+int g_msd = 1;        // MSD levels + LDS chunk sort for large (key, value) sorts
+constexpr std::size_t kMsdMinKeys = std::size_t(1) << 20;
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 
@@ -136,7 +138,8 @@ __device__ inline void load4(const double *a, std::size_t i, double v[4])
 // particles.
 template <class T, KeyKind K>
 __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, std::size_t n, std::size_t rangeKeys,
-                                                             int parts, Key *__restrict__ keys, std::uint32_t *partHist)
+                                                             int parts, Key *__restrict__ keys, std::uint32_t *partHist,
+                                                             int histShift)
 {
     // one histogram per wave: the lowest digit is close to uniform, so plain
     // LDS atomics on a wave-private table collide least
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
                 dst[0] = make_ulonglong2(k[0], k[1]);
                 dst[1] = make_ulonglong2(k[2], k[3]);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) atomicAdd(&h[unsigned(k[u]) & (kRadix - 1)], 1u);
+                for (int u = 0; u < 4; ++u) atomicAdd(&h[unsigned(k[u] >> histShift) & (kRadix - 1)], 1u);
             }
         }
         i = base + threadIdx.x; // the rest of the part, one particle per thread
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
             for (int u = 0; u < kKeygenUnroll; ++u)
             {
                 keys[i + std::size_t(u) * kSortThreads] = k[u];
-                atomicAdd(&h[unsigned(k[u]) & (kRadix - 1)], 1u);
+                atomicAdd(&h[unsigned(k[u] >> histShift) & (kRadix - 1)], 1u);
             }
         }
     }
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     {
         const Key k = particleKey<T, K>(p, i);
         keys[i] = k;
-        atomicAdd(&h[unsigned(k) & (kRadix - 1)], 1u);
+        atomicAdd(&h[unsigned(k >> histShift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
     std::uint32_t t = 0;
@@ -435,8 +438,9 @@ __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_
     return before + inc - v;
 }
 
-// Second launch bound = minimum waves per SIMD: two blocks of 512 (or one
-// of 1024) per CU, i.e. at most 128 VGPRs.
+// Second launch bound = minimum waves per SIMD: two blocks of 512 x 8 (or
+// one of 1024) per CU, i.e. at most 128 VGPRs; larger 512-thread tiles fit
+// once per CU (LDS), as do 256-thread blocks' register budgets of 256.
 //
 // kSeg: segmented sort. The range comes from segRanges[blockIdx.x] and
 // rangeOff holds its absolute per-digit start ([range][digit]); values of the
@@ -444,12 +448,25 @@ __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_
 //
 // kPre: the keys (and values) of the next tile are loaded while the current
 // tile is ranked and written (register double buffering).
-template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits, bool kSeg = false, bool kPre = false>
-__global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel(
+//
+// kMsd: a partition level of the MSD sort. The block loops over the
+// *msd.rangeCount ranges of segRanges (grid-stride); a range's digit d starts
+// at msd.segBase[seg][d] + msd.off[d][range].
+struct MsdScatterArgs
+{
+    const std::uint32_t *rangeCount = nullptr;
+    const std::uint32_t *segBase = nullptr; // [segment][digit] absolute start
+    const std::uint32_t *off = nullptr;     // [digit][stride] offset of a range within its segment
+    std::size_t stride = 0;
+};
+
+template <int THREADS, int ITEMS, bool kVals, bool kIota, bool kDigits, bool kSeg = false, bool kPre = false,
+          bool kMsd = false>
+__global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS < 512 ? 2 : 4) void scatterKernel(
     const Key *__restrict__ kin, Key *__restrict__ kout, const std::uint32_t *__restrict__ vin,
     std::uint32_t *__restrict__ vout, std::uint8_t *__restrict__ digitsOut, std::size_t n, std::size_t rangeKeys,
     const std::uint32_t *__restrict__ rangeOff, const std::uint32_t *__restrict__ digitTotal, std::size_t ranges,
-    int shift, int nextShift, const SegRange *__restrict__ segRanges = nullptr)
+    int shift, int nextShift, const SegRange *__restrict__ segRanges = nullptr, MsdScatterArgs msd = {})
 {
     constexpr int kW = THREADS / 64;
     constexpr int kTile = ITEMS * THREADS;
@@ -464,10 +481,18 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
-    const std::size_t r = xcdRow(blockIdx.x, ranges); // gridDim.x == ranges
+    auto body = [&](const std::size_t r) {
     std::size_t rBegin;
     std::uint32_t rLen, valueBase; // range-relative 32-bit offsets (n < 2^32)
-    if constexpr (kSeg)
+    if constexpr (kMsd)
+    {
+        const SegRange d = segRanges[r];
+        rBegin = d.begin;
+        rLen = d.len;
+        valueBase = d.localBase;
+        if (tid < kRadix) running[tid] = msd.segBase[std::size_t(d.seg) * kRadix + tid] + msd.off[tid * msd.stride + r];
+    }
+    else if constexpr (kSeg)
     {
         const SegRange d = segRanges[r];
         rBegin = d.begin;
@@ -649,6 +674,15 @@ __global__ __launch_bounds__(THREADS, THREADS >= 512 ? 4 : 2) void scatterKernel
     std::uint32_t base = 0;
     for (; rLen - base >= std::uint32_t(kTile); base += kTile) tile(base, std::true_type{});
     if (base < rLen) tile(base, std::false_type{});
+    }; // body: every path ends with a barrier, so the next range may reuse the shared state
+
+    if constexpr (kMsd)
+    {
+        const std::uint32_t count = *msd.rangeCount < msd.stride ? *msd.rangeCount : std::uint32_t(msd.stride);
+        for (std::uint32_t w = blockIdx.x; w < count; w += gridDim.x) body(xcdRow(w, count));
+    }
+    else
+        body(xcdRow(blockIdx.x, ranges)); // gridDim.x == ranges
 }
 
 // ---- segmented sort kernels --------------------------------------------------
@@ -717,12 +751,449 @@ __global__ __launch_bounds__(kSortThreads) void segScanKernel(const std::uint32_
     for (std::uint32_t r = r0; r < r1; ++r) off[std::size_t(r) * kRadix + d] += base;
 }
 
+// ---- MSD sort ----------------------------------------------------------------
+// Level 1 partitions all keys by bits 56..63 (the pass machinery above);
+// level L >= 2 partitions only the buckets of more than kMsdChunk keys left
+// by level L-1, by bits 64-8L .. 71-8L. Every bucket of at most kMsdChunk
+// keys is final: runs of them become chunks, sorted in LDS on the bits below
+// their parent's digit. All lists and counts live on the device, so the
+// whole sort is one fixed sequence of launches (graph-capturable); kernels
+// loop over the device-side counts. Levels alternate output buffers: level L
+// writes the odd-level buffer (Y) when L is odd, else X.
+constexpr std::uint32_t kMsdChunk = 4096;             // LDS chunk capacity (keys)
+constexpr int kMsdThreads = 512, kMsdItems = 8;       // scatter tile of levels >= 2
+constexpr std::uint32_t kMsdTile = kMsdThreads * kMsdItems;
+constexpr int kMsdGrid = 1024;                        // grid of the looping level kernels (multiple of 8)
+static_assert(kMsdChunk == kMsdTile, "the local sort holds one scatter tile");
+
+__global__ void msdResetKernel(MsdCounters *c, MsdSeg *root, std::size_t n)
+{
+    c->segs[0] = c->segs[1] = 0;
+    c->ranges[0] = c->ranges[1] = 0;
+    c->chunks = 0;
+    c->overflow = 0;
+    *root = MsdSeg{0, std::uint32_t(n), 0, 0, 0};
+}
+
+// One block per parent segment (grid-stride over *parentCount, or the root):
+// thread d takes sub-bucket d (size totals[parent][d]). Sub-buckets of more
+// than kMsdChunk keys become segments of the next level (with their tile
+// ranges); thread 0 groups the others, in order, into chunks of at most
+// kMsdChunk keys. At the last level every sub-bucket holds equal keys and the
+// parent becomes one chunk that needs no sorting.
+__global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
+    int level, const MsdSeg *__restrict__ parents, const std::uint32_t *parentCount,
+    const std::uint32_t *__restrict__ totals, MsdCounters *cnt, int nextSet, MsdSeg *nextSegs,
+    SegRange *nextRanges, std::uint8_t *nextFirst, std::uint32_t maxSegs, std::uint32_t maxRanges, MsdChunk *chunks,
+    std::uint32_t maxChunks)
+{
+    __shared__ std::uint32_t waveTotals[kWaves];
+    __shared__ std::uint32_t sz[kRadix];
+    __shared__ std::uint64_t bg[kRadix];
+    __shared__ std::uint8_t big[kRadix];
+    const unsigned d = threadIdx.x;
+    const std::uint32_t count = parentCount ? (*parentCount < maxSegs ? *parentCount : maxSegs) : 1u;
+    const bool last = level >= kSortPasses;
+    const std::uint8_t parity = std::uint8_t(level & 1);
+    // a chunk spans sub-buckets of one parent: sort on the bits below the parent's digit
+    const std::uint8_t bits = last ? 0 : std::uint8_t(level == 1 ? 64 : 72 - 8 * level);
+    for (std::uint32_t p = blockIdx.x; p < count; p += gridDim.x)
+    {
+        const MsdSeg par = parents[p];
+        const std::uint32_t s = totals[std::size_t(p) * kRadix + d];
+        const std::uint64_t b = par.begin + blockExclusiveScan(s, waveTotals);
+        const bool isBig = !last && s > kMsdChunk;
+        if (isBig)
+        {
+            const std::uint32_t slot = atomicAdd(&cnt->segs[nextSet], 1u);
+            const std::uint32_t nr = (s + kMsdTile - 1) / kMsdTile;
+            const std::uint32_t r0 = atomicAdd(&cnt->ranges[nextSet], nr);
+            if (slot < maxSegs && r0 <= maxRanges && nr <= maxRanges - r0)
+            {
+                nextSegs[slot] = MsdSeg{b, s, r0, nr, 0};
+                for (std::uint32_t k = 0; k < nr; ++k)
+                {
+                    const std::uint32_t o = k * kMsdTile;
+                    nextRanges[r0 + k] = SegRange{b + o, s - o < kMsdTile ? s - o : kMsdTile, o, slot, 0};
+                    nextFirst[r0 + k] = k == 0;
+                }
+            }
+            else
+                atomicAdd(&cnt->overflow, 1u);
+        }
+        sz[d] = s;
+        bg[d] = b;
+        big[d] = isBig;
+        __syncthreads();
+        if (d == 0)
+        {
+            std::uint64_t cb = 0;
+            std::uint32_t cl = 0;
+            auto flush = [&]() {
+                if (cl == 0) return;
+                const std::uint32_t c = atomicAdd(&cnt->chunks, 1u);
+                if (c < maxChunks) chunks[c] = MsdChunk{cb, cl, bits, parity, 0};
+                else atomicAdd(&cnt->overflow, 1u);
+                cl = 0;
+            };
+            if (last)
+            {
+                cb = par.begin;
+                cl = par.size;
+            }
+            else
+                for (int k = 0; k < kRadix; ++k)
+                {
+                    const std::uint32_t v = sz[k];
+                    if (v == 0) continue;
+                    if (big[k])
+                    {
+                        flush();
+                        continue;
+                    }
+                    if (cl + v > kMsdChunk) flush();
+                    if (cl == 0) cb = bg[k];
+                    cl += v;
+                }
+            flush();
+        }
+        __syncthreads();
+    }
+}
+
+// Digit histogram of the wave's byte range [a, e) into the wave's LDS table
+// (16 digits per load in the aligned middle).
+__device__ inline void countDigitBytes(const std::uint8_t *__restrict__ digits, std::size_t a, std::size_t e,
+                                       std::uint32_t *h, unsigned lane)
+{
+    std::size_t a16 = (a + 15) & ~std::size_t(15);
+    if (a16 > e) a16 = e;
+    if (a + lane < a16) countDigit(h, digits[a + lane]);
+    std::size_t e16 = e & ~std::size_t(15);
+    if (e16 < a16) e16 = a16;
+    const uint4 *quads = reinterpret_cast<const uint4 *>(digits);
+    for (std::size_t i = a16 + 16 * std::size_t(lane); i < e16; i += 16 * 64)
+    {
+        const uint4 q = quads[i >> 4];
+        const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) countDigit(h, (w[k] >> (8 * b)) & 0xffu);
+    }
+    if (e16 + lane < e) countDigit(h, digits[e16 + lane]);
+}
+
+// Digit histograms of the level's ranges, kUpRows per block (one wave each),
+// into the digit-major table hist[d * stride + range].
+__global__ __launch_bounds__(kUpRows * 64) void msdUpsweepKernel(const std::uint8_t *__restrict__ digits,
+                                                                 const SegRange *__restrict__ ranges,
+                                                                 const MsdCounters *cnt, int set,
+                                                                 std::uint32_t *hist, std::size_t stride)
+{
+    __shared__ std::uint32_t h[kUpRows][kRadix];
+    const std::uint32_t count = cnt->ranges[set] < stride ? cnt->ranges[set] : std::uint32_t(stride);
+    const std::uint32_t groups = (count + kUpRows - 1) / kUpRows;
+    const int wave = threadIdx.x >> 6;
+    const unsigned lane = laneId();
+    for (std::uint32_t g = blockIdx.x; g < groups; g += gridDim.x)
+    {
+        const std::size_t r0 = xcdRow(g, groups) * kUpRows;
+        for (int i = threadIdx.x; i < kUpRows * kRadix; i += kUpRows * 64) (&h[0][0])[i] = 0;
+        __syncthreads();
+        const std::size_t r = r0 + std::size_t(wave);
+        if (r < count)
+        {
+            const SegRange rg = ranges[r];
+            countDigitBytes(digits, rg.begin, rg.begin + rg.len, h[wave], lane);
+        }
+        __syncthreads();
+        if (threadIdx.x < kRadix)
+        {
+            const unsigned d = threadIdx.x;
+            std::uint32_t *out = hist + std::size_t(d) * stride + r0;
+            if (r0 + kUpRows <= count)
+            {
+                reinterpret_cast<uint4 *>(out)[0] = make_uint4(h[0][d], h[1][d], h[2][d], h[3][d]);
+                reinterpret_cast<uint4 *>(out)[1] = make_uint4(h[4][d], h[5][d], h[6][d], h[7][d]);
+            }
+            else
+                for (int w = 0; w < kUpRows; ++w)
+                    if (r0 + w < count) out[w] = h[w][d];
+        }
+        __syncthreads();
+    }
+}
+
+// Block digit blockIdx.x: exclusive prefix of the digit's count over the
+// ranges of each segment (a segmented scan; first[r] marks a segment's first
+// range), 16 consecutive ranges per thread and 4096 per round. The count of
+// the digit in a whole segment goes to subTotals[segment][digit]. Block 0
+// also clears the list counters of the next level (its lists are no longer
+// read).
+__global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_t *__restrict__ hist,
+                                                              std::size_t stride,
+                                                              const std::uint8_t *__restrict__ first,
+                                                              const SegRange *__restrict__ ranges, MsdCounters *cnt,
+                                                              int set, std::uint32_t *off, std::uint32_t *subTotals)
+{
+    constexpr int kPer = 16;
+    __shared__ std::uint32_t wS[2][kWaves];
+    __shared__ std::uint32_t wF[2][kWaves];
+    const std::uint32_t count = cnt->ranges[set] < stride ? cnt->ranges[set] : std::uint32_t(stride);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        cnt->segs[set ^ 1] = 0;
+        cnt->ranges[set ^ 1] = 0;
+    }
+    const std::size_t d = blockIdx.x;
+    const std::uint32_t *h = hist + d * stride;
+    std::uint32_t *o = off + d * stride;
+    const unsigned lane = laneId();
+    const int wave = threadIdx.x >> 6;
+    std::uint32_t carry = 0; // open segment's count at the end of the previous round
+    int buf = 0;
+    for (std::uint32_t r0 = 0; r0 < count; r0 += kPer * kSortThreads)
+    {
+        const std::uint32_t f0 = r0 + threadIdx.x * kPer;
+        std::uint32_t v[kPer], ex[kPer];
+        std::uint32_t flags = 0; // bit k: range f0+k starts a segment
+        std::uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            const bool in = f0 + k < count;
+            v[k] = in ? h[f0 + k] : 0u;
+            const bool f = in && first[f0 + k];
+            flags |= std::uint32_t(f) << k;
+            if (f) run = 0;
+            ex[k] = run;
+            run += v[k];
+        }
+        const bool nextFirst = f0 + kPer >= count || first[f0 + kPer];
+        // inclusive segmented scan of the thread aggregates (run, has flag) in the wave
+        std::uint32_t s = run, fl = flags != 0;
+#pragma unroll
+        for (int sh = 1; sh < 64; sh <<= 1)
+        {
+            const std::uint32_t ps = __shfl_up(s, sh, 64), pf = __shfl_up(fl, sh, 64);
+            if (lane >= unsigned(sh))
+            {
+                if (!fl) s += ps;
+                fl |= pf;
+            }
+        }
+        std::uint32_t es = __shfl_up(s, 1, 64), ef = __shfl_up(fl, 1, 64); // exclusive within the wave
+        if (lane == 0) es = 0, ef = 0;
+        if (lane == 63)
+        {
+            wS[buf][wave] = s;
+            wF[buf][wave] = fl;
+        }
+        __syncthreads();
+        // carry into this wave: the round carry, then the earlier waves' totals
+        std::uint32_t cs = carry, all = carry;
+        for (int w = 0; w < kWaves; ++w)
+        {
+            const std::uint32_t ws = wS[buf][w], wf = wF[buf][w];
+            if (w < wave) cs = wf ? ws : cs + ws;
+            all = wf ? ws : all + ws;
+        }
+        const std::uint32_t tin = ef ? es : cs + es;
+        bool seen = false;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            if ((flags >> k) & 1u) seen = true;
+            const std::uint32_t val = seen ? ex[k] : tin + ex[k];
+            const std::uint32_t r = f0 + k;
+            if (r < count)
+            {
+                o[r] = val;
+                const bool end = k + 1 < kPer ? ((flags >> (k + 1)) & 1u) != 0 || r + 1 == count : nextFirst;
+                if (end) subTotals[std::size_t(ranges[r].seg) * kRadix + d] = val + v[k];
+            }
+        }
+        carry = all;
+        buf ^= 1;
+    }
+}
+
+// segBase[segment][d] = segment begin + keys of smaller digits in the segment.
+__global__ __launch_bounds__(kSortThreads) void msdBaseKernel(const MsdSeg *__restrict__ segs, const MsdCounters *cnt,
+                                                              int set, const std::uint32_t *__restrict__ subTotals,
+                                                              std::uint32_t *segBase, std::uint32_t maxSegs)
+{
+    __shared__ std::uint32_t waveTotals[kWaves];
+    const std::uint32_t count = cnt->segs[set] < maxSegs ? cnt->segs[set] : maxSegs;
+    for (std::uint32_t p = blockIdx.x; p < count; p += gridDim.x)
+    {
+        const std::size_t i = std::size_t(p) * kRadix + threadIdx.x;
+        segBase[i] = std::uint32_t(segs[p].begin) + blockExclusiveScan(subTotals[i], waveTotals);
+    }
+}
+
+// Sorts every chunk in LDS: the chunk's keys (and values) in registers, one
+// stable counting pass per 8-bit digit among its low `bits` bits that is not
+// the same for all of its keys (ranking as in scatterKernel), then written to
+// the final buffer. Chunks with bits == 0 are already in order and are only
+// copied when they lie in the other buffer.
+__global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCounters *cnt,
+                                                                     const MsdChunk *__restrict__ chunks, Key *X,
+                                                                     Key *Y, std::uint32_t *vX, std::uint32_t *vY,
+                                                                     int finalIsY, std::uint32_t maxChunks)
+{
+    constexpr int kW = kMsdThreads / 64;
+    __shared__ Key stageKeys[kMsdChunk];
+    __shared__ std::uint32_t stageVals[kMsdChunk];
+    __shared__ std::uint32_t waveCount[kW][kRadix];
+    __shared__ std::uint32_t sink[kW][64];
+    __shared__ std::uint32_t scratch[kRadix / 64];
+    __shared__ Key waveMask[kW];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const unsigned lane = laneId();
+    const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
+    Key *kd = finalIsY ? Y : X;
+    std::uint32_t *vd = finalIsY ? vY : vX;
+    const std::uint32_t count = cnt->chunks < maxChunks ? cnt->chunks : maxChunks;
+    const std::uint32_t first = std::uint32_t(wave) * (kMsdItems * 64) + lane;
+    for (std::uint32_t c = blockIdx.x; c < count; c += gridDim.x)
+    {
+        const MsdChunk ch = chunks[c];
+        const Key *ks = ch.parity ? Y : X;
+        const std::uint32_t *vs = ch.parity ? vY : vX;
+        if (ch.bits == 0 || ch.len > kMsdChunk)
+        {
+            if (ks != kd)
+                for (std::uint32_t i = tid; i < ch.len; i += kMsdThreads)
+                {
+                    kd[ch.begin + i] = ks[ch.begin + i];
+                    vd[ch.begin + i] = vs[ch.begin + i];
+                }
+            continue;
+        }
+        Key key[kMsdItems];
+        std::uint32_t val[kMsdItems];
+        const Key k0 = ks[ch.begin];
+        Key m = 0;
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j)
+        {
+            const std::uint32_t idx = first + std::uint32_t(j) * 64;
+            const bool in = idx < ch.len;
+            const std::uint32_t cidx = in ? idx : 0u;
+            const Key k = ks[ch.begin + cidx];
+            const std::uint32_t v = vs[ch.begin + cidx];
+            key[j] = in ? k : kInvalidKey; // padding ranks last in every digit
+            val[j] = v;
+            m |= in ? k ^ k0 : Key(0);
+        }
+        // bits that differ within the chunk
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m |= Key(__shfl_xor((unsigned long long)m, o, 64));
+        if (lane == 0) waveMask[wave] = m;
+        __syncthreads();
+        m = 0;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) m |= waveMask[w];
+        if (ch.bits < 64) m &= (Key(1) << ch.bits) - 1;
+        for (int shift = 0; shift < 64; shift += kRadixBits)
+        {
+            if (((m >> shift) & (kRadix - 1)) == 0) continue; // block-uniform
+            for (int i = tid; i < kW * kRadix; i += kMsdThreads) (&waveCount[0][0])[i] = 0;
+            __syncthreads();
+            std::uint32_t rank[kMsdItems], old[kMsdItems];
+            int leader[kMsdItems];
+#pragma unroll
+            for (int j = 0; j < kMsdItems; ++j)
+            {
+                const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
+                std::uint64_t match = ~std::uint64_t(0);
+#pragma unroll
+                for (int b = 0; b < kRadixBits; ++b)
+                {
+                    const bool bit = (dg >> b) & 1u;
+                    const std::uint64_t ballot = __ballot(bit);
+                    match &= bit ? ballot : ~ballot;
+                }
+                const std::uint32_t lower = __popcll(match & ltMask);
+                const bool lead = lower == 0;
+                std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
+                old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
+                leader[j] = __builtin_ctzll(match);
+                rank[j] = lower;
+            }
+#pragma unroll
+            for (int j = 0; j < kMsdItems; ++j) rank[j] += std::uint32_t(__shfl(int(old[j]), leader[j], 64));
+            __syncthreads();
+            std::uint32_t tileCount = 0, inc = 0;
+            if (tid < kRadix)
+            {
+#pragma unroll
+                for (int w = 0; w < kW; ++w)
+                {
+                    const std::uint32_t cw = waveCount[w][tid];
+                    waveCount[w][tid] = tileCount;
+                    tileCount += cw;
+                }
+                inc = tileCount;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1)
+                {
+                    const std::uint32_t t = __shfl_up(inc, o, 64);
+                    if (lane >= unsigned(o)) inc += t;
+                }
+                if (lane == 63) scratch[wave] = inc;
+            }
+            __syncthreads();
+            if (tid < kRadix)
+            {
+                std::uint32_t before = 0;
+#pragma unroll
+                for (int w = 0; w < kRadix / 64; ++w)
+                    if (w < wave) before += scratch[w];
+                const std::uint32_t start = before + inc - tileCount;
+#pragma unroll
+                for (int w = 0; w < kW; ++w) waveCount[w][tid] += start;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kMsdItems; ++j)
+            {
+                const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)];
+                stageKeys[pos] = key[j];
+                stageVals[pos] = val[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kMsdItems; ++j)
+            {
+                key[j] = stageKeys[first + std::uint32_t(j) * 64];
+                val[j] = stageVals[first + std::uint32_t(j) * 64];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j)
+        {
+            const std::uint32_t idx = first + std::uint32_t(j) * 64;
+            if (idx < ch.len)
+            {
+                kd[ch.begin + idx] = key[j];
+                vd[ch.begin + idx] = val[j];
+            }
+        }
+        __syncthreads(); // waveMask is rewritten by the next chunk
+    }
+}
+
 template <int THREADS, int ITEMS, bool PRE = false>
 void launchScatter(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, const Key *kin, Key *kout,
-                   const std::uint32_t *vin, std::uint32_t *vout, bool vals, bool iota, bool digits, int shift)
+                   const std::uint32_t *vin, std::uint32_t *vout, bool vals, bool iota, bool digits, int shift,
+                   int next)
 {
     const dim3 grid(unsigned(pl.ranges)), block(THREADS);
-    const int next = shift + kRadixBits;
 #define AT_SCATTER(V, I, D)                                                                                          \
     hipLaunchKernelGGL((scatterKernel<THREADS, ITEMS, V, I, D, false, PRE>), grid, block, 0, s, kin, kout, vin, vout, \
                        ws.digits, pl.n, pl.rangeKeys, ws.rangeOff, ws.digitTotal, pl.ranges, shift, next)
@@ -752,6 +1223,32 @@ void scanOffsets(hipStream_t s, const SortPlan &pl, SortWorkspace &ws)
 
 // Passes 0..7 given pass 0's range histograms in ws.rangeHist. Keys ping-pong
 // between kA (input, and output after the even number of passes) and kB.
+// One scatter pass of the plan's tile shape.
+void dispatchScatter(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, const Key *src, Key *dst,
+                     const std::uint32_t *vsrc, std::uint32_t *vdst, bool vals, bool first, bool digits, int shift,
+                     int next)
+{
+    // next-tile prefetch is instantiated for tiles of 8 or 12 keys per thread
+    const bool pre = g_prefetch && pl.items <= 12 && pl.threads <= 512;
+#define AT_ARGS s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift, next
+    switch (pl.threads * 100 + pl.items + (pre ? 1000000 : 0))
+    {
+    case 1025608: launchScatter<256, 8, true>(AT_ARGS); break;
+    case 1025612: launchScatter<256, 12, true>(AT_ARGS); break;
+    case 1051208: launchScatter<512, 8, true>(AT_ARGS); break;
+    case 1051212: launchScatter<512, 12, true>(AT_ARGS); break;
+    case 25608: launchScatter<256, 8>(AT_ARGS); break;
+    case 25612: launchScatter<256, 12>(AT_ARGS); break;
+    case 25616: launchScatter<256, 16>(AT_ARGS); break;
+    case 51208: launchScatter<512, 8>(AT_ARGS); break;
+    case 51212: launchScatter<512, 12>(AT_ARGS); break;
+    case 51216: launchScatter<512, 16>(AT_ARGS); break;
+    case 102408: launchScatter<1024, 8>(AT_ARGS); break;
+    default: throw Error(1, "no scatter kernel for this tile shape");
+    }
+#undef AT_ARGS
+}
+
 void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Key *kB, std::uint32_t *vA,
                std::uint32_t *vB, bool vals, bool iota, bool firstHistWhole)
 {
@@ -780,23 +1277,70 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         const bool digits = g_digitArray && pass + 1 < kSortPasses;
         prof::Scope scope("sort_pass", s);
         const bool first = iota && pass == 0;
-        // next-tile prefetch is instantiated for tiles of 8 or 12 keys per thread
-        const bool pre = g_prefetch && pl.items <= 12 && pl.threads <= 512;
-        switch (pl.threads * 100 + pl.items + (pre ? 1000000 : 0))
-        {
-        case 1025608: launchScatter<256, 8, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 1025612: launchScatter<256, 12, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 1051208: launchScatter<512, 8, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 1051212: launchScatter<512, 12, true>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 25608: launchScatter<256, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 25612: launchScatter<256, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 25616: launchScatter<256, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 51208: launchScatter<512, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 51212: launchScatter<512, 12>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 51216: launchScatter<512, 16>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        case 102408: launchScatter<1024, 8>(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift); break;
-        default: throw Error(1, "no scatter kernel for this tile shape");
-        }
+        dispatchScatter(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift, shift + kRadixBits);
+    }
+    AT_HIP(hipGetLastError());
+}
+
+// MSD sort of (key, value) pairs given level 1's range histograms of bits
+// 56..63 in ws.rangeHist. Keys start in X (values in vX, or slots when
+// iota); odd levels write Y, even levels X; the result ends in Y when
+// finalIsY, else in X.
+void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *Y, std::uint32_t *vX,
+             std::uint32_t *vY, bool iota, bool firstHistWhole, bool finalIsY)
+{
+    const std::size_t n = pl.n;
+    ws.allocateMsd(n, s);
+    const std::uint32_t maxSegs = std::uint32_t(ws.msdMaxSegs), maxRanges = std::uint32_t(ws.msdStride),
+                        maxChunks = std::uint32_t(ws.msdMaxChunks);
+    hipLaunchKernelGGL(msdResetKernel, dim3(1), dim3(1), 0, s, ws.msdCnt, ws.msdRoot, n);
+    SortPlan scanPlan = pl;
+    if (firstHistWhole) scanPlan.parts = 1;
+    scanOffsets(s, scanPlan, ws);
+    {
+        prof::Scope scope("sort_pass", s);
+        dispatchScatter(s, pl, ws, X, Y, vX, vY, true, iota, true, 56, 48);
+    }
+    hipLaunchKernelGGL(msdPlanKernel, dim3(1), dim3(kSortThreads), 0, s, 1, ws.msdRoot, nullptr, ws.digitTotal,
+                       ws.msdCnt, 0, ws.msdSegs[0], ws.msdRanges[0], ws.msdFirst[0], maxSegs, maxRanges,
+                       ws.msdChunks, maxChunks);
+    const unsigned segGrid = unsigned(maxSegs < 1024 ? maxSegs : 1024);
+    const unsigned upGrid = unsigned((maxRanges / kUpRows) < 1024 ? maxRanges / kUpRows + 1 : 1024);
+    for (int level = 2; level <= kSortPasses; ++level)
+    {
+        prof::Scope scope("msd_level", s);
+        const int set = level & 1;
+        const bool srcY = (level - 1) & 1, dstY = level & 1;
+        const Key *src = srcY ? Y : X;
+        Key *dst = dstY ? Y : X;
+        const std::uint32_t *vsrc = srcY ? vY : vX;
+        std::uint32_t *vdst = dstY ? vY : vX;
+        const int shift = 64 - kRadixBits * level;
+        hipLaunchKernelGGL(msdUpsweepKernel, dim3(upGrid), dim3(kUpRows * 64), 0, s, ws.digits, ws.msdRanges[set],
+                           ws.msdCnt, set, ws.msdHist, ws.msdStride);
+        hipLaunchKernelGGL(msdScanKernel, dim3(kRadix), dim3(kSortThreads), 0, s, ws.msdHist, ws.msdStride,
+                           ws.msdFirst[set], ws.msdRanges[set], ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
+        hipLaunchKernelGGL(msdBaseKernel, dim3(segGrid), dim3(kSortThreads), 0, s, ws.msdSegs[set], ws.msdCnt, set,
+                           ws.msdSubTotals, ws.msdSegBase, maxSegs);
+        const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegBase, ws.msdOff, ws.msdStride};
+        if (level < kSortPasses)
+            hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, true, true, false, true>),
+                               dim3(kMsdGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
+                               std::size_t(0), nullptr, nullptr, std::size_t(0), shift, shift - kRadixBits,
+                               ws.msdRanges[set], a);
+        else
+            hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, false, true, false, true>),
+                               dim3(kMsdGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
+                               std::size_t(0), nullptr, nullptr, std::size_t(0), shift, 0, ws.msdRanges[set], a);
+        hipLaunchKernelGGL(msdPlanKernel, dim3(segGrid), dim3(kSortThreads), 0, s, level, ws.msdSegs[set],
+                           &ws.msdCnt->segs[set], ws.msdSubTotals, ws.msdCnt, set ^ 1, ws.msdSegs[set ^ 1],
+                           ws.msdRanges[set ^ 1], ws.msdFirst[set ^ 1], maxSegs, maxRanges, ws.msdChunks,
+                           maxChunks);
+    }
+    {
+        prof::Scope scope("msd_local", s);
+        hipLaunchKernelGGL(msdLocalSortKernel, dim3(kMsdGrid), dim3(kMsdThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
+                           vX, vY, int(finalIsY), maxChunks);
     }
     AT_HIP(hipGetLastError());
 }
@@ -820,6 +1364,13 @@ void setSortPrefetch(int on)
     if (on != 0 && on != 1) throw Error(1, "sort_prefetch must be 0 or 1");
     g_prefetch = on;
 }
+int sortMsd() { return g_msd; }
+void setSortMsd(int on)
+{
+    if (on != 0 && on != 1) throw Error(1, "sort_msd must be 0 or 1");
+    g_msd = on;
+}
+std::size_t sortMsdMinKeys() { return kMsdMinKeys; }
 int sortDigitArray() { return g_digitArray; }
 void setSortDigitArray(int on)
 {
@@ -876,6 +1427,36 @@ void SortWorkspace::allocate(std::size_t cap, bool withValues, hipStream_t s)
     digitTotal = static_cast<std::uint32_t *>(poolAlloc(kRadix * sizeof(std::uint32_t), s));
 }
 
+void SortWorkspace::allocateMsd(std::size_t cap, hipStream_t s)
+{
+    if (cap <= msdCapacity) return;
+    for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
+                    (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
+                    (void *)msdSubTotals, (void *)msdSegBase, (void *)msdChunks})
+        poolFree(p, s);
+    msdCapacity = cap;
+    // segments hold more than kMsdChunk keys; a level's ranges are its
+    // segments' tiles; chunks: below 2 per kMsdChunk keys of each run of
+    // small buckets, plus one run per parent and per segment over all levels
+    msdMaxSegs = cap / kMsdChunk + 1;
+    msdStride = ((cap / kMsdTile + msdMaxSegs + kUpRows) + 7) & ~std::size_t(7);
+    msdMaxChunks = 16 * (cap / kMsdChunk) + 64;
+    auto alloc = [&](std::size_t bytes) { return poolAlloc(bytes, s); };
+    msdCnt = static_cast<MsdCounters *>(alloc(sizeof(MsdCounters)));
+    msdRoot = static_cast<MsdSeg *>(alloc(sizeof(MsdSeg)));
+    for (int k = 0; k < 2; ++k)
+    {
+        msdSegs[k] = static_cast<MsdSeg *>(alloc(msdMaxSegs * sizeof(MsdSeg)));
+        msdRanges[k] = static_cast<SegRange *>(alloc(msdStride * sizeof(SegRange)));
+        msdFirst[k] = static_cast<std::uint8_t *>(alloc(msdStride));
+    }
+    msdHist = static_cast<std::uint32_t *>(alloc(std::size_t(kRadix) * msdStride * sizeof(std::uint32_t)));
+    msdOff = static_cast<std::uint32_t *>(alloc(std::size_t(kRadix) * msdStride * sizeof(std::uint32_t)));
+    msdSubTotals = static_cast<std::uint32_t *>(alloc(msdMaxSegs * kRadix * sizeof(std::uint32_t)));
+    msdSegBase = static_cast<std::uint32_t *>(alloc(msdMaxSegs * kRadix * sizeof(std::uint32_t)));
+    msdChunks = static_cast<MsdChunk *>(alloc(msdMaxChunks * sizeof(MsdChunk)));
+}
+
 void SortWorkspace::release()
 {
     // owners synchronise their stream before the workspace goes away
@@ -885,6 +1466,13 @@ void SortWorkspace::release()
     keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
     digitTotal = nullptr;
     capacity = 0; maxRanges = 0; values = false;
+    for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
+                    (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
+                    (void *)msdSubTotals, (void *)msdSegBase, (void *)msdChunks})
+        poolFree(p, nullptr);
+    msdCnt = nullptr; msdRoot = nullptr; msdSegs[0] = msdSegs[1] = nullptr; msdRanges[0] = msdRanges[1] = nullptr;
+    msdFirst[0] = msdFirst[1] = nullptr; msdHist = msdOff = msdSubTotals = msdSegBase = nullptr; msdChunks = nullptr;
+    msdCapacity = msdMaxSegs = msdStride = msdMaxChunks = 0;
 }
 
 template <class T>
@@ -895,6 +1483,10 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys per tree");
     ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
+    // MSD: keys into the alternate buffer, histogram of the top digit
+    const bool msd = g_msd && n >= kMsdMinKeys;
+    Key *keys = msd ? ws.keysAlt : keysOut;
+    const int histShift = msd ? 64 - kRadixBits : 0;
     {
         prof::Scope scope("keygen", s);
         const dim3 grid(unsigned(pl.ranges * pl.parts)), block(kSortThreads);
@@ -902,19 +1494,20 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
         {
         case KeyKind::Morton:
             hipLaunchKernelGGL((keygenKernel<T, KeyKind::Morton>), grid, block, 0, s, p, n, pl.rangeKeys, pl.parts,
-                               keysOut, ws.rangeHist);
+                               keys, ws.rangeHist, histShift);
             break;
         case KeyKind::Hilbert:
             hipLaunchKernelGGL((keygenKernel<T, KeyKind::Hilbert>), grid, block, 0, s, p, n, pl.rangeKeys, pl.parts,
-                               keysOut, ws.rangeHist);
+                               keys, ws.rangeHist, histShift);
             break;
         case KeyKind::Binary:
             hipLaunchKernelGGL((keygenKernel<T, KeyKind::Binary>), grid, block, 0, s, p, n, pl.rangeKeys, pl.parts,
-                               keysOut, ws.rangeHist);
+                               keys, ws.rangeHist, histShift);
             break;
         }
     }
-    runPasses(s, pl, ws, keysOut, ws.keysAlt, valsOut, ws.valsAlt, true, true, false);
+    if (msd) msdSort(s, pl, ws, ws.keysAlt, keysOut, ws.valsAlt, valsOut, true, false, true);
+    else runPasses(s, pl, ws, keysOut, ws.keysAlt, valsOut, ws.valsAlt, true, true, false);
 }
 
 void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws, bool iota)
@@ -923,9 +1516,11 @@ void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, Sor
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
     ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
+    const bool msd = g_msd && n >= kMsdMinKeys;
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
-                       0, ws.rangeHist);
-    runPasses(s, pl, ws, keys, ws.keysAlt, vals, ws.valsAlt, true, iota, true);
+                       msd ? 64 - kRadixBits : 0, ws.rangeHist);
+    if (msd) msdSort(s, pl, ws, keys, ws.keysAlt, vals, ws.valsAlt, iota, true, false);
+    else runPasses(s, pl, ws, keys, ws.keysAlt, vals, ws.valsAlt, true, iota, true);
 }
 
 void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws)
@@ -1019,12 +1614,22 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
         if (t < best) best = t, bd = d;
     }
     g_digitArray = bd;
+    int bm = g_msd;
+    if (n >= kMsdMinKeys)
+        for (int m : {0, 1})
+        {
+            g_msd = m;
+            const float t = timeOne();
+            if (t < best) best = t, bm = m;
+        }
+    g_msd = bm;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     AT_HIP(hipStreamSynchronize(s));
     return "{\"sort_threads\": " + std::to_string(bt) + ", \"sort_items\": " + std::to_string(bi) +
            ", \"sort_prefetch\": " + std::to_string(bp) +
            ", \"sort_ranges\": " + std::to_string(br) + ", \"sort_digit_array\": " + std::to_string(bd) +
+           ", \"sort_msd\": " + std::to_string(bm) +
            ", \"keys\": " + std::to_string(n) + ", \"sort_ms\": " + std::to_string(best) + "}";
 }
 

@@ -32,6 +32,12 @@ void setSortDigitArray(int on);
 // 256/512 x 8/12; others ignore it).
 int sortPrefetch();
 void setSortPrefetch(int on);
+// 1: sorts of at least sortMsdMinKeys() (key, value) pairs partition by the
+// high digits first (MSD levels) and finish buckets of up to 4096 keys in
+// LDS; 0: LSD passes over all 8 digits. Results are identical.
+int sortMsd();
+void setSortMsd(int on);
+std::size_t sortMsdMinKeys();
 
 // Times every tile shape, then range count, then histogram source on a
 // stable sort of n random 63-bit (key, index) pairs and keeps the fastest
@@ -53,6 +59,47 @@ struct SortPlan
 };
 SortPlan makeSortPlan(std::size_t n);
 
+// A contiguous range of keys within a segment (segmented sorts and MSD levels).
+struct SegRange
+{
+    std::uint64_t begin;    // first key of the range in the concatenated array
+    std::uint32_t len;      // keys in the range
+    std::uint32_t localBase; // slot of the first key within its segment
+    std::uint32_t seg;
+    std::uint32_t pad;
+};
+
+// ---- MSD sort state (device) ----
+// A level's segment: a bucket of more than kMsdChunk keys that the level
+// partitions by its digit; its ranges are firstRange .. firstRange+ranges-1.
+struct MsdSeg
+{
+    std::uint64_t begin;
+    std::uint32_t size;
+    std::uint32_t firstRange;
+    std::uint32_t ranges;
+    std::uint32_t pad;
+};
+// A run of consecutive finished buckets (at most kMsdChunk keys, or any
+// size when bits == 0) sorted in LDS on its low `bits` bits; parity 1 = it
+// lies in the odd levels' output buffer.
+struct MsdChunk
+{
+    std::uint64_t begin;
+    std::uint32_t len;
+    std::uint8_t bits;
+    std::uint8_t parity;
+    std::uint16_t pad;
+};
+struct MsdCounters
+{
+    std::uint32_t segs[2];   // segments of the levels using list set 0 / 1
+    std::uint32_t ranges[2];
+    std::uint32_t chunks;
+    std::uint32_t overflow;  // list capacity exceeded (never, by the bounds used)
+    std::uint32_t pad[2];
+};
+
 // Device scratch of one sort instance, sized for `capacity` keys.
 struct SortWorkspace
 {
@@ -65,8 +112,21 @@ struct SortWorkspace
     std::size_t capacity = 0;
     std::size_t maxRanges = 0;
     bool values = false;
+    // MSD levels (allocated on first use)
+    MsdCounters *msdCnt = nullptr;
+    MsdSeg *msdRoot = nullptr;
+    MsdSeg *msdSegs[2] = {nullptr, nullptr};
+    SegRange *msdRanges[2] = {nullptr, nullptr};
+    std::uint8_t *msdFirst[2] = {nullptr, nullptr}; // 1 for the first range of a segment
+    std::uint32_t *msdHist = nullptr;      // [kRadix][msdStride]
+    std::uint32_t *msdOff = nullptr;       // [kRadix][msdStride]
+    std::uint32_t *msdSubTotals = nullptr; // [msdMaxSegs][kRadix]
+    std::uint32_t *msdSegBase = nullptr;   // [msdMaxSegs][kRadix]
+    MsdChunk *msdChunks = nullptr;
+    std::size_t msdCapacity = 0, msdMaxSegs = 0, msdStride = 0, msdMaxChunks = 0;
 
     void allocate(std::size_t cap, bool withValues, hipStream_t s);
+    void allocateMsd(std::size_t cap, hipStream_t s);
     void release();
     ~SortWorkspace() { release(); }
 };
@@ -101,14 +161,6 @@ void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws);
 // particle's slot within its segment). Every stage is one launch over all
 // segments: ranges never straddle a segment, the per-segment offset scan
 // runs one block per segment.
-struct SegRange
-{
-    std::uint64_t begin;    // first key of the range in the concatenated array
-    std::uint32_t len;      // keys in the range
-    std::uint32_t localBase; // slot of the first key within its segment
-    std::uint32_t seg;
-    std::uint32_t pad;
-};
 
 template <class T>
 struct SegCoords

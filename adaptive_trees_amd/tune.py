@@ -1,14 +1,15 @@
 """Radix-sort tuning: read, set, or let the native autotuner pick.
 
 Every setting gives bit-identical trees; only the sort's schedule changes
-(tile shape, key ranges per pass, histogram source).
+(tile shape, key ranges per pass, histogram source, MSD levels or LSD
+passes for large sorts).
 """
 import ctypes
 import json
 
 from . import _ffi
 
-KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch")
+KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch", "sort_msd")
 
 
 def get():
