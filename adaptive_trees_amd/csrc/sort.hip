@@ -780,7 +780,8 @@ __global__ void msdResetKernel(MsdCounters *c, MsdSeg *root, std::size_t n)
 // than kMsdChunk keys become segments of the next level (with their tile
 // ranges); thread 0 groups the others, in order, into chunks of at most
 // kMsdChunk keys. At the last level every sub-bucket holds equal keys and the
-// parent becomes one chunk that needs no sorting.
+// parent becomes one chunk that needs no sorting. Each parent reserves its
+// list entries with one atomic per list.
 __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     int level, const MsdSeg *__restrict__ parents, const std::uint32_t *parentCount,
     const std::uint32_t *__restrict__ totals, MsdCounters *cnt, int nextSet, MsdSeg *nextSegs,
@@ -789,8 +790,10 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
 {
     __shared__ std::uint32_t waveTotals[kWaves];
     __shared__ std::uint32_t sz[kRadix];
-    __shared__ std::uint64_t bg[kRadix];
     __shared__ std::uint8_t big[kRadix];
+    __shared__ std::uint32_t chBegin[kRadix], chLen[kRadix]; // chunk starts (relative to the parent) and sizes
+    __shared__ std::uint32_t base[3];                        // reserved segment, range and chunk slots
+    __shared__ std::uint32_t nChunks;
     const unsigned d = threadIdx.x;
     const std::uint32_t count = parentCount ? (*parentCount < maxSegs ? *parentCount : maxSegs) : 1u;
     const bool last = level >= kSortPasses;
@@ -801,62 +804,66 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     {
         const MsdSeg par = parents[p];
         const std::uint32_t s = totals[std::size_t(p) * kRadix + d];
-        const std::uint64_t b = par.begin + blockExclusiveScan(s, waveTotals);
+        const std::uint32_t rel = blockExclusiveScan(s, waveTotals); // sub-bucket start within the parent
         const bool isBig = !last && s > kMsdChunk;
-        if (isBig)
-        {
-            const std::uint32_t slot = atomicAdd(&cnt->segs[nextSet], 1u);
-            const std::uint32_t nr = (s + kMsdTile - 1) / kMsdTile;
-            const std::uint32_t r0 = atomicAdd(&cnt->ranges[nextSet], nr);
-            if (slot < maxSegs && r0 <= maxRanges && nr <= maxRanges - r0)
-            {
-                nextSegs[slot] = MsdSeg{b, s, r0, nr, 0};
-                for (std::uint32_t k = 0; k < nr; ++k)
-                {
-                    const std::uint32_t o = k * kMsdTile;
-                    nextRanges[r0 + k] = SegRange{b + o, s - o < kMsdTile ? s - o : kMsdTile, o, slot, 0};
-                    nextFirst[r0 + k] = k == 0;
-                }
-            }
-            else
-                atomicAdd(&cnt->overflow, 1u);
-        }
+        const std::uint32_t nr = isBig ? (s + kMsdTile - 1) / kMsdTile : 0u;
+        const std::uint32_t segRank = blockExclusiveScan(isBig ? 1u : 0u, waveTotals);
+        const std::uint32_t rangeRank = blockExclusiveScan(nr, waveTotals);
         sz[d] = s;
-        bg[d] = b;
         big[d] = isBig;
+        if (d == kRadix - 1)
+        {
+            const std::uint32_t ns = segRank + (isBig ? 1u : 0u), nrr = rangeRank + nr;
+            base[0] = ns ? atomicAdd(&cnt->segs[nextSet], ns) : 0u;
+            base[1] = nrr ? atomicAdd(&cnt->ranges[nextSet], nrr) : 0u;
+            if ((ns && base[0] + ns > maxSegs) || (nrr && base[1] + nrr > maxRanges)) atomicAdd(&cnt->overflow, 1u);
+        }
         __syncthreads();
+        if (isBig && base[0] + segRank < maxSegs && base[1] + rangeRank + nr <= maxRanges)
+        {
+            const std::uint32_t slot = base[0] + segRank, r0 = base[1] + rangeRank;
+            const std::uint64_t b = par.begin + rel;
+            nextSegs[slot] = MsdSeg{b, s, r0, nr, 0};
+            for (std::uint32_t k = 0; k < nr; ++k)
+            {
+                const std::uint32_t o = k * kMsdTile;
+                nextRanges[r0 + k] = SegRange{b + o, s - o < kMsdTile ? s - o : kMsdTile, o, slot, 0};
+                nextFirst[r0 + k] = k == 0;
+            }
+        }
         if (d == 0)
         {
-            std::uint64_t cb = 0;
-            std::uint32_t cl = 0;
-            auto flush = [&]() {
-                if (cl == 0) return;
-                const std::uint32_t c = atomicAdd(&cnt->chunks, 1u);
-                if (c < maxChunks) chunks[c] = MsdChunk{cb, cl, bits, parity, 0};
-                else atomicAdd(&cnt->overflow, 1u);
-                cl = 0;
-            };
+            std::uint32_t n = 0, cb = 0, cl = 0, at = 0;
             if (last)
             {
-                cb = par.begin;
-                cl = par.size;
+                chBegin[0] = 0;
+                chLen[0] = par.size;
+                n = 1;
             }
             else
                 for (int k = 0; k < kRadix; ++k)
                 {
                     const std::uint32_t v = sz[k];
+                    const std::uint32_t here = at;
+                    at += v;
                     if (v == 0) continue;
-                    if (big[k])
+                    if (big[k] || cl + v > kMsdChunk)
                     {
-                        flush();
-                        continue;
+                        if (cl) chBegin[n] = cb, chLen[n] = cl, ++n;
+                        cl = 0;
+                        if (big[k]) continue;
                     }
-                    if (cl + v > kMsdChunk) flush();
-                    if (cl == 0) cb = bg[k];
+                    if (cl == 0) cb = here;
                     cl += v;
                 }
-            flush();
+            if (cl) chBegin[n] = cb, chLen[n] = cl, ++n;
+            nChunks = n;
+            base[2] = n ? atomicAdd(&cnt->chunks, n) : 0u;
+            if (n && base[2] + n > maxChunks) atomicAdd(&cnt->overflow, 1u);
         }
+        __syncthreads();
+        if (d < nChunks && base[2] + d < maxChunks)
+            chunks[base[2] + d] = MsdChunk{par.begin + chBegin[d], chLen[d], bits, parity, 0};
         __syncthreads();
     }
 }
@@ -1033,11 +1040,17 @@ __global__ __launch_bounds__(kSortThreads) void msdBaseKernel(const MsdSeg *__re
     }
 }
 
-// Sorts every chunk in LDS: the chunk's keys (and values) in registers, one
-// stable counting pass per 8-bit digit among its low `bits` bits that is not
-// the same for all of its keys (ranking as in scatterKernel), then written to
-// the final buffer. Chunks with bits == 0 are already in order and are only
-// copied when they lie in the other buffer.
+// Sorts every chunk in LDS. The chunk's keys (and values) are held in
+// registers; a stable counting pass per 8-bit digit (ranking as in
+// scatterKernel) is run for each digit at or above bit 32 that is not the
+// same for all of the chunk's keys. Runs of keys that then share their high
+// 32 bits are finished by an insertion sort on the low 32 bits, one thread
+// per run, when no run is longer than kFixRun; otherwise the varying low
+// digits and then the high digits get counting passes (plain LSD order).
+// Chunks with bits == 0 are already in order and are only copied when they
+// lie in the other buffer.
+constexpr std::uint32_t kFixRun = 16;
+
 __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCounters *cnt,
                                                                      const MsdChunk *__restrict__ chunks, Key *X,
                                                                      Key *Y, std::uint32_t *vX, std::uint32_t *vY,
@@ -1050,6 +1063,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
     __shared__ std::uint32_t sink[kW][64];
     __shared__ std::uint32_t scratch[kRadix / 64];
     __shared__ Key waveMask[kW];
+    __shared__ std::uint32_t longRun;
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
@@ -1058,6 +1072,86 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
     std::uint32_t *vd = finalIsY ? vY : vX;
     const std::uint32_t count = cnt->chunks < maxChunks ? cnt->chunks : maxChunks;
     const std::uint32_t first = std::uint32_t(wave) * (kMsdItems * 64) + lane;
+    Key key[kMsdItems];
+    std::uint32_t val[kMsdItems];
+
+    // one stable counting pass on the digit at `shift`; the result is in
+    // the registers (striped order) and in stageKeys / stageVals
+    auto pass = [&](int shift) {
+        for (int i = tid; i < kW * kRadix; i += kMsdThreads) (&waveCount[0][0])[i] = 0;
+        __syncthreads();
+        std::uint32_t rank[kMsdItems], old[kMsdItems];
+        int leader[kMsdItems];
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j)
+        {
+            const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
+            std::uint64_t match = ~std::uint64_t(0);
+#pragma unroll
+            for (int b = 0; b < kRadixBits; ++b)
+            {
+                const bool bit = (dg >> b) & 1u;
+                const std::uint64_t ballot = __ballot(bit);
+                match &= bit ? ballot : ~ballot;
+            }
+            const std::uint32_t lower = __popcll(match & ltMask);
+            const bool lead = lower == 0;
+            std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
+            old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
+            leader[j] = __builtin_ctzll(match);
+            rank[j] = lower;
+        }
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j) rank[j] += std::uint32_t(__shfl(int(old[j]), leader[j], 64));
+        __syncthreads();
+        std::uint32_t tileCount = 0, inc = 0;
+        if (tid < kRadix)
+        {
+#pragma unroll
+            for (int w = 0; w < kW; ++w)
+            {
+                const std::uint32_t cw = waveCount[w][tid];
+                waveCount[w][tid] = tileCount;
+                tileCount += cw;
+            }
+            inc = tileCount;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+            {
+                const std::uint32_t t = __shfl_up(inc, o, 64);
+                if (lane >= unsigned(o)) inc += t;
+            }
+            if (lane == 63) scratch[wave] = inc;
+        }
+        __syncthreads();
+        if (tid < kRadix)
+        {
+            std::uint32_t before = 0;
+#pragma unroll
+            for (int w = 0; w < kRadix / 64; ++w)
+                if (w < wave) before += scratch[w];
+            const std::uint32_t start = before + inc - tileCount;
+#pragma unroll
+            for (int w = 0; w < kW; ++w) waveCount[w][tid] += start;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j)
+        {
+            const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)];
+            stageKeys[pos] = key[j];
+            stageVals[pos] = val[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kMsdItems; ++j)
+        {
+            key[j] = stageKeys[first + std::uint32_t(j) * 64];
+            val[j] = stageVals[first + std::uint32_t(j) * 64];
+        }
+        __syncthreads();
+    };
+
     for (std::uint32_t c = blockIdx.x; c < count; c += gridDim.x)
     {
         const MsdChunk ch = chunks[c];
@@ -1073,8 +1167,6 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
                 }
             continue;
         }
-        Key key[kMsdItems];
-        std::uint32_t val[kMsdItems];
         const Key k0 = ks[ch.begin];
         Key m = 0;
 #pragma unroll
@@ -1093,86 +1185,80 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m |= Key(__shfl_xor((unsigned long long)m, o, 64));
         if (lane == 0) waveMask[wave] = m;
+        if (tid == 0) longRun = 0;
         __syncthreads();
         m = 0;
 #pragma unroll
         for (int w = 0; w < kW; ++w) m |= waveMask[w];
         if (ch.bits < 64) m &= (Key(1) << ch.bits) - 1;
-        for (int shift = 0; shift < 64; shift += kRadixBits)
+        const Key hiMask = m & ~Key(0xffffffffu), loMask = m & Key(0xffffffffu);
+        for (int shift = 32; shift < 64; shift += kRadixBits)
+            if ((hiMask >> shift) & (kRadix - 1)) pass(shift);
+        if (loMask)
         {
-            if (((m >> shift) & (kRadix - 1)) == 0) continue; // block-uniform
-            for (int i = tid; i < kW * kRadix; i += kMsdThreads) (&waveCount[0][0])[i] = 0;
-            __syncthreads();
-            std::uint32_t rank[kMsdItems], old[kMsdItems];
-            int leader[kMsdItems];
-#pragma unroll
-            for (int j = 0; j < kMsdItems; ++j)
+            // the chunk (sorted on its high 32 bits) must sit in the stage
+            if (!hiMask)
             {
-                const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
-                std::uint64_t match = ~std::uint64_t(0);
 #pragma unroll
-                for (int b = 0; b < kRadixBits; ++b)
+                for (int j = 0; j < kMsdItems; ++j)
                 {
-                    const bool bit = (dg >> b) & 1u;
-                    const std::uint64_t ballot = __ballot(bit);
-                    match &= bit ? ballot : ~ballot;
+                    stageKeys[first + std::uint32_t(j) * 64] = key[j];
+                    stageVals[first + std::uint32_t(j) * 64] = val[j];
                 }
-                const std::uint32_t lower = __popcll(match & ltMask);
-                const bool lead = lower == 0;
-                std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
-                old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
-                leader[j] = __builtin_ctzll(match);
-                rank[j] = lower;
+                __syncthreads();
             }
-#pragma unroll
-            for (int j = 0; j < kMsdItems; ++j) rank[j] += std::uint32_t(__shfl(int(old[j]), leader[j], 64));
-            __syncthreads();
-            std::uint32_t tileCount = 0, inc = 0;
-            if (tid < kRadix)
+            // runs of equal high halves (the stage holds the chunk in order):
+            // is any longer than kFixRun?
+            for (std::uint32_t p0 = tid; p0 < ch.len; p0 += kMsdThreads)
             {
-#pragma unroll
-                for (int w = 0; w < kW; ++w)
+                const std::uint32_t h = std::uint32_t(stageKeys[p0] >> 32);
+                if (p0 == 0 || std::uint32_t(stageKeys[p0 - 1] >> 32) != h)
                 {
-                    const std::uint32_t cw = waveCount[w][tid];
-                    waveCount[w][tid] = tileCount;
-                    tileCount += cw;
+                    std::uint32_t e = p0 + 1;
+                    while (e < ch.len && e - p0 <= kFixRun && std::uint32_t(stageKeys[e] >> 32) == h) ++e;
+                    if (e - p0 > kFixRun) longRun = 1;
                 }
-                inc = tileCount;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1)
+            }
+            __syncthreads();
+            const bool fix = !longRun;
+            if (fix)
+            {
+                // stable insertion sort of each run on the low 32 bits
+                for (std::uint32_t p0 = tid; p0 < ch.len; p0 += kMsdThreads)
                 {
-                    const std::uint32_t t = __shfl_up(inc, o, 64);
-                    if (lane >= unsigned(o)) inc += t;
+                    const std::uint32_t h = std::uint32_t(stageKeys[p0] >> 32);
+                    if (p0 != 0 && std::uint32_t(stageKeys[p0 - 1] >> 32) == h) continue;
+                    for (std::uint32_t i = p0 + 1; i < ch.len && std::uint32_t(stageKeys[i] >> 32) == h; ++i)
+                    {
+                        const Key kk = stageKeys[i];
+                        const std::uint32_t vv = stageVals[i];
+                        std::uint32_t q = i;
+                        while (q > p0 && std::uint32_t(stageKeys[q - 1]) > std::uint32_t(kk))
+                        {
+                            stageKeys[q] = stageKeys[q - 1];
+                            stageVals[q] = stageVals[q - 1];
+                            --q;
+                        }
+                        stageKeys[q] = kk;
+                        stageVals[q] = vv;
+                    }
                 }
-                if (lane == 63) scratch[wave] = inc;
+                __syncthreads();
             }
-            __syncthreads();
-            if (tid < kRadix)
-            {
-                std::uint32_t before = 0;
-#pragma unroll
-                for (int w = 0; w < kRadix / 64; ++w)
-                    if (w < wave) before += scratch[w];
-                const std::uint32_t start = before + inc - tileCount;
-#pragma unroll
-                for (int w = 0; w < kW; ++w) waveCount[w][tid] += start;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < kMsdItems; ++j)
-            {
-                const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)];
-                stageKeys[pos] = key[j];
-                stageVals[pos] = val[j];
-            }
-            __syncthreads();
 #pragma unroll
             for (int j = 0; j < kMsdItems; ++j)
             {
                 key[j] = stageKeys[first + std::uint32_t(j) * 64];
                 val[j] = stageVals[first + std::uint32_t(j) * 64];
             }
-            __syncthreads();
+            if (!fix)
+            {
+                __syncthreads(); // the passes rewrite the stage
+                for (int shift = 0; shift < 32; shift += kRadixBits)
+                    if ((loMask >> shift) & (kRadix - 1)) pass(shift);
+                for (int shift = 32; shift < 64; shift += kRadixBits)
+                    if ((hiMask >> shift) & (kRadix - 1)) pass(shift);
+            }
         }
 #pragma unroll
         for (int j = 0; j < kMsdItems; ++j)
@@ -1184,7 +1270,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
                 vd[ch.begin + idx] = val[j];
             }
         }
-        __syncthreads(); // waveMask is rewritten by the next chunk
+        __syncthreads(); // waveMask, longRun and the stage are rewritten by the next chunk
     }
 }
 
