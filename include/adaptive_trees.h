@@ -251,7 +251,12 @@ int at_server_get_stats(at_server *server, at_server_stats *out);
  *                     key for the next histogram, 0 = the histogram reads
  *                     the keys;
  * "sort_prefetch":    1 = the scatter loads the next tile while it works on
- *                     the current one (tiles 256/512 x 8/12). */
+ *                     the current one (tiles 256/512 x 8/12);
+ * "sort_msd":         1 = sorts of at least 2^20 (key, value) pairs
+ *                     partition by the high digits first and finish buckets
+ *                     of up to 4096 keys in LDS, 0 = eight LSD passes.
+ * A tile (sort_threads x sort_items) holds at most 8192 keys; a larger one
+ * is rejected when a sort is planned. */
 int at_tune_set(const char *key, int value);
 int at_tune_get(const char *key, int *value);
 /* Autotuner: times the tile shapes, then the range counts, then the
@@ -259,6 +264,15 @@ int at_tune_get(const char *key, int *value);
  * fastest settings and writes them as JSON into report (may be NULL). Every
  * setting gives identical results; only the schedule changes. */
 int at_tune_auto(size_t n, void *stream, char *report, size_t report_len);
+
+/* ---- device radix sort ---------------------------------------------------
+ * Stable sort of n 64-bit keys with 32-bit values, in place, on `stream`
+ * (device pointers; returns after the stream has finished). slot_values != 0:
+ * vals is output only and receives each key's input position 0..n-1 (the
+ * permutation the trees store in Perm). The engine behind every tree's
+ * key sort (no reference counterpart: the reference calls
+ * cub::DeviceRadixSort::SortPairs). n < 2^32. */
+int at_sort_pairs(void *keys, void *vals, size_t n, int slot_values, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,92 @@
+"""The device radix sort on its own (at_sort_pairs) against numpy's stable sort.
+
+Sizes below 2^20 take the LSD passes; larger ones the MSD levels (when
+sort_msd is on) with the LDS chunk sort. Distributions cover what the level
+planning and the chunk sort branch on: uniform 64-bit keys, 63-bit keys with
+the invalid key ~0 mixed in, a handful of distinct values, long duplicate
+runs, clustered high bits, keys that differ only in their low 32 bits
+(runs for the chunk fix-up, short and long), and sizes that are not
+multiples of a tile. Results must be identical (keys and permutation).
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from adaptive_trees_amd import DeviceArray, _ffi, tune
+
+MASK63 = np.uint64((1 << 63) - 1)
+
+
+def keys_of(kind, n, rng):
+    if kind == "uniform":
+        return rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True)
+    if kind == "invalid_mix":
+        k = rng.integers(0, 2**63, n, dtype=np.uint64)
+        k[rng.random(n) < 0.05] = np.uint64(2**64 - 1)
+        k[rng.random(n) < 0.01] = MASK63  # a valid key equal to the invalid key's low 63 bits
+        return k
+    if kind == "few":
+        return rng.choice(np.array([3, 2**40 + 1, 2**62, 2**63 - 1], np.uint64), n)
+    if kind == "dup_runs":
+        vals = rng.integers(0, 2**63, n // 6000 + 2, dtype=np.uint64)
+        return rng.permutation(np.repeat(vals, 6000)[:n])
+    if kind == "clustered":
+        hi = (rng.normal(0, 1, n) * 3 + 100).astype(np.int64).clip(0, 255).astype(np.uint64) << np.uint64(56)
+        return hi | rng.integers(0, 2**56, n, dtype=np.uint64)
+    if kind == "low_half":
+        # the high 32 bits take few values: runs of equal high halves of every length
+        hi = rng.integers(0, n // 3 + 1, n, dtype=np.uint64) << np.uint64(32)
+        return hi | rng.integers(0, 2**32, n, dtype=np.uint64)
+    if kind == "low_half_long":
+        hi = rng.integers(0, 64, n, dtype=np.uint64) << np.uint64(40)
+        return hi | rng.integers(0, 2**32, n, dtype=np.uint64)
+    raise ValueError(kind)
+
+
+def gpu_sort(keys, vals=None):
+    lib = _ffi.hip_lib()
+    n = len(keys)
+    dk = DeviceArray(keys)
+    dv = DeviceArray(vals if vals is not None else np.zeros(n, np.uint32))
+    lib.call("sort_pairs", dk.ptr, dv.ptr, n, 1 if vals is None else 0, None)
+    return dk.download(), dv.download()
+
+
+@pytest.fixture(params=[1, 0], ids=["msd", "lsd"])
+def msd(request):
+    saved = tune.get()
+    tune.set(sort_msd=request.param)
+    yield request.param
+    tune.set(**saved)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["uniform", "invalid_mix", "few", "dup_runs", "clustered", "low_half",
+                                  "low_half_long"])
+@pytest.mark.parametrize("n", [1000, 300_001, (1 << 20) + 777, 3_000_017])
+def test_sort_pairs_matches_stable_argsort(msd, kind, n):
+    print("sort", msd, kind, n, flush=True)
+    rng = np.random.default_rng(zlib.crc32(("%s %d" % (kind, n)).encode()))
+    keys = keys_of(kind, n, rng)
+    k, v = gpu_sort(keys)
+    order = np.argsort(keys, kind="stable").astype(np.uint32)
+    assert np.array_equal(v, order), "%s n=%d: permutation differs at %s" % (kind, n, np.nonzero(v != order)[0][:4])
+    assert np.array_equal(k, keys[order])
+
+
+@pytest.mark.gpu
+def test_sort_pairs_carries_values(msd):
+    rng = np.random.default_rng(91)
+    n = (1 << 20) + 5
+    keys = keys_of("clustered", n, rng)
+    vals = rng.integers(0, 2**32, n, dtype=np.uint32)
+    k, v = gpu_sort(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+
+
+def test_sort_pairs_rejects_null_pointers():
+    lib = _ffi.hip_lib()
+    with pytest.raises(_ffi.InvalidArgument):
+        lib.call("sort_pairs", None, None, 10, 1, None)
