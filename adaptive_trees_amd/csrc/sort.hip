@@ -760,11 +760,11 @@ __global__ __launch_bounds__(kSortThreads) void segScanKernel(const std::uint32_
 // whole sort is one fixed sequence of launches (graph-capturable); kernels
 // loop over the device-side counts. Levels alternate output buffers: level L
 // writes the odd-level buffer (Y) when L is odd, else X.
-constexpr std::uint32_t kMsdChunk = 4096;             // LDS chunk capacity (keys)
+constexpr int kLocThreads = 256, kLocItems = 8;       // LDS chunk sort block shape
+constexpr std::uint32_t kMsdChunk = kLocThreads * kLocItems; // LDS chunk capacity (keys)
 constexpr int kMsdThreads = 512, kMsdItems = 8;       // scatter tile of levels >= 2
 constexpr std::uint32_t kMsdTile = kMsdThreads * kMsdItems;
 constexpr int kMsdGrid = 1024;                        // grid of the looping level kernels (multiple of 8)
-static_assert(kMsdChunk == kMsdTile, "the local sort holds one scatter tile");
 
 __global__ void msdResetKernel(MsdCounters *c, MsdSeg *root, std::size_t n)
 {
@@ -777,11 +777,15 @@ __global__ void msdResetKernel(MsdCounters *c, MsdSeg *root, std::size_t n)
 
 // One block per parent segment (grid-stride over *parentCount, or the root):
 // thread d takes sub-bucket d (size totals[parent][d]). Sub-buckets of more
-// than kMsdChunk keys become segments of the next level (with their tile
-// ranges); thread 0 groups the others, in order, into chunks of at most
-// kMsdChunk keys. At the last level every sub-bucket holds equal keys and the
-// parent becomes one chunk that needs no sorting. Each parent reserves its
-// list entries with one atomic per list.
+// than kMsdChunk keys become segments of the next level; the block writes
+// their tile ranges together (thread t takes ranges t, t+256, ...). The
+// others are grouped, in order, into chunks of at most kMsdChunk keys: a
+// chunk starting at sub-bucket d extends up to the next big sub-bucket or
+// until one more would exceed kMsdChunk keys (the same cuts as a sequential
+// greedy pass); each thread finds that end by a binary search and thread 0
+// follows the chunk starts. At the last level every sub-bucket holds equal
+// keys and the parent becomes one chunk that needs no sorting. Each parent
+// reserves its list entries with one atomic per list.
 __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     int level, const MsdSeg *__restrict__ parents, const std::uint32_t *parentCount,
     const std::uint32_t *__restrict__ totals, MsdCounters *cnt, int nextSet, MsdSeg *nextSegs,
@@ -789,11 +793,17 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     std::uint32_t maxChunks)
 {
     __shared__ std::uint32_t waveTotals[kWaves];
-    __shared__ std::uint32_t sz[kRadix];
-    __shared__ std::uint8_t big[kRadix];
-    __shared__ std::uint32_t chBegin[kRadix], chLen[kRadix]; // chunk starts (relative to the parent) and sizes
-    __shared__ std::uint32_t base[3];                        // reserved segment, range and chunk slots
-    __shared__ std::uint32_t nChunks;
+    __shared__ std::uint32_t relS[kRadix + 1];    // sub-bucket start within the parent
+    __shared__ std::uint32_t inclSmall[kRadix];   // prefix of the small sizes, inclusive
+    __shared__ std::uint16_t nextBig[kRadix + 1]; // first big sub-bucket at or after d
+    __shared__ std::uint16_t nextStart[kRadix + 1]; // first nonempty small sub-bucket at or after d
+    __shared__ std::uint16_t bigAt[kRadix + 1], smallAt[kRadix + 1];
+    __shared__ std::uint16_t chunkEnd[kRadix];
+    __shared__ std::uint32_t rangeRankOf[kRadix + 1]; // ranges before the k-th big sub-bucket
+    __shared__ std::uint16_t bigIndex[kRadix];        // sub-bucket of the k-th big one
+    __shared__ std::uint32_t chBegin[kRadix], chLen[kRadix];
+    __shared__ std::uint32_t base[3];
+    __shared__ std::uint32_t nChunks, nBig, nRanges;
     const unsigned d = threadIdx.x;
     const std::uint32_t count = parentCount ? (*parentCount < maxSegs ? *parentCount : maxSegs) : 1u;
     const bool last = level >= kSortPasses;
@@ -804,36 +814,81 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     {
         const MsdSeg par = parents[p];
         const std::uint32_t s = totals[std::size_t(p) * kRadix + d];
-        const std::uint32_t rel = blockExclusiveScan(s, waveTotals); // sub-bucket start within the parent
+        const std::uint32_t rel = blockExclusiveScan(s, waveTotals);
         const bool isBig = !last && s > kMsdChunk;
+        const bool isSmall = !last && !isBig && s > 0;
         const std::uint32_t nr = isBig ? (s + kMsdTile - 1) / kMsdTile : 0u;
-        const std::uint32_t segRank = blockExclusiveScan(isBig ? 1u : 0u, waveTotals);
+        const std::uint32_t bigRank = blockExclusiveScan(isBig ? 1u : 0u, waveTotals);
         const std::uint32_t rangeRank = blockExclusiveScan(nr, waveTotals);
-        sz[d] = s;
-        big[d] = isBig;
+        const std::uint32_t smallRank = blockExclusiveScan(isSmall ? 1u : 0u, waveTotals);
+        const std::uint32_t smallBefore = blockExclusiveScan(isSmall ? s : 0u, waveTotals);
+        relS[d] = rel;
+        inclSmall[d] = smallBefore + (isSmall ? s : 0u);
+        if (isBig)
+        {
+            bigAt[bigRank] = std::uint16_t(d);
+            bigIndex[bigRank] = std::uint16_t(d);
+            rangeRankOf[bigRank] = rangeRank;
+        }
+        if (isSmall) smallAt[smallRank] = std::uint16_t(d);
         if (d == kRadix - 1)
         {
-            const std::uint32_t ns = segRank + (isBig ? 1u : 0u), nrr = rangeRank + nr;
-            base[0] = ns ? atomicAdd(&cnt->segs[nextSet], ns) : 0u;
+            relS[kRadix] = rel + s;
+            const std::uint32_t nb = bigRank + (isBig ? 1u : 0u), ns = smallRank + (isSmall ? 1u : 0u);
+            const std::uint32_t nrr = rangeRank + nr;
+            bigAt[nb] = kRadix;
+            smallAt[ns] = kRadix;
+            rangeRankOf[nb] = nrr;
+            nBig = nb;
+            nRanges = nrr;
+            base[0] = nb ? atomicAdd(&cnt->segs[nextSet], nb) : 0u;
             base[1] = nrr ? atomicAdd(&cnt->ranges[nextSet], nrr) : 0u;
-            if ((ns && base[0] + ns > maxSegs) || (nrr && base[1] + nrr > maxRanges)) atomicAdd(&cnt->overflow, 1u);
+            if ((nb && base[0] + nb > maxSegs) || (nrr && base[1] + nrr > maxRanges)) atomicAdd(&cnt->overflow, 1u);
         }
         __syncthreads();
-        if (isBig && base[0] + segRank < maxSegs && base[1] + rangeRank + nr <= maxRanges)
-        {
-            const std::uint32_t slot = base[0] + segRank, r0 = base[1] + rangeRank;
-            const std::uint64_t b = par.begin + rel;
-            nextSegs[slot] = MsdSeg{b, s, r0, nr, 0};
-            for (std::uint32_t k = 0; k < nr; ++k)
+        nextBig[d] = bigAt[bigRank];     // d itself when big
+        nextStart[d] = smallAt[smallRank]; // d itself when small
+        if (d == 0) nextBig[kRadix] = kRadix, nextStart[kRadix] = kRadix;
+        const bool fits = base[0] + nBig <= maxSegs && base[1] + nRanges <= maxRanges;
+        if (isBig && fits)
+            nextSegs[base[0] + bigRank] = MsdSeg{par.begin + rel, s, base[1] + rangeRank, nr, 0};
+        __syncthreads();
+        if (fits)
+            for (std::uint32_t t = d; t < nRanges; t += kSortThreads)
             {
-                const std::uint32_t o = k * kMsdTile;
-                nextRanges[r0 + k] = SegRange{b + o, s - o < kMsdTile ? s - o : kMsdTile, o, slot, 0};
-                nextFirst[r0 + k] = k == 0;
+                // the big sub-bucket owning range t: last k with rangeRankOf[k] <= t
+                std::uint32_t lo = 0, hi = nBig; // invariant: rangeRankOf[lo] <= t < rangeRankOf[hi]
+                while (hi - lo > 1)
+                {
+                    const std::uint32_t mid = (lo + hi) >> 1;
+                    if (rangeRankOf[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                const unsigned sb = bigIndex[lo];
+                const std::uint32_t k = t - rangeRankOf[lo], o = k * kMsdTile;
+                const std::uint32_t size = relS[sb + 1] - relS[sb];
+                nextRanges[base[1] + t] = SegRange{par.begin + relS[sb] + o, size - o < kMsdTile ? size - o : kMsdTile, o,
+                                                   base[0] + lo, 0};
+                nextFirst[base[1] + t] = k == 0;
             }
+        // end of a chunk starting at small sub-bucket d
+        if (isSmall)
+        {
+            const std::uint32_t limit = inclSmall[d] - s + kMsdChunk; // small keys before d + capacity
+            const unsigned stop = nextBig[d];
+            unsigned lo = d, hi = stop; // inclSmall[lo] <= limit; find the last such index below stop
+            while (hi - lo > 1)
+            {
+                const unsigned mid = (lo + hi) >> 1;
+                if (inclSmall[mid] <= limit) lo = mid;
+                else hi = mid;
+            }
+            chunkEnd[d] = std::uint16_t(lo + 1);
         }
+        __syncthreads();
         if (d == 0)
         {
-            std::uint32_t n = 0, cb = 0, cl = 0, at = 0;
+            std::uint32_t n = 0;
             if (last)
             {
                 chBegin[0] = 0;
@@ -841,22 +896,12 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
                 n = 1;
             }
             else
-                for (int k = 0; k < kRadix; ++k)
+                for (unsigned c = nextStart[0]; c < kRadix; c = nextStart[chunkEnd[c]])
                 {
-                    const std::uint32_t v = sz[k];
-                    const std::uint32_t here = at;
-                    at += v;
-                    if (v == 0) continue;
-                    if (big[k] || cl + v > kMsdChunk)
-                    {
-                        if (cl) chBegin[n] = cb, chLen[n] = cl, ++n;
-                        cl = 0;
-                        if (big[k]) continue;
-                    }
-                    if (cl == 0) cb = here;
-                    cl += v;
+                    chBegin[n] = relS[c];
+                    chLen[n] = relS[chunkEnd[c]] - relS[c];
+                    ++n;
                 }
-            if (cl) chBegin[n] = cb, chLen[n] = cl, ++n;
             nChunks = n;
             base[2] = n ? atomicAdd(&cnt->chunks, n) : 0u;
             if (n && base[2] + n > maxChunks) atomicAdd(&cnt->overflow, 1u);
@@ -964,14 +1009,37 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
     {
         const std::uint32_t f0 = r0 + threadIdx.x * kPer;
         std::uint32_t v[kPer], ex[kPer];
+        std::uint8_t fb[kPer];
+        if (f0 + kPer <= count) // 16-byte loads (stride and f0 are multiples of 16)
+        {
+            const uint4 q = *reinterpret_cast<const uint4 *>(first + f0);
+            const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) fb[k] = std::uint8_t(w[k >> 2] >> (8 * (k & 3)));
+#pragma unroll
+            for (int k = 0; k < kPer / 4; ++k)
+            {
+                const uint4 c = reinterpret_cast<const uint4 *>(h + f0)[k];
+                v[4 * k] = c.x, v[4 * k + 1] = c.y, v[4 * k + 2] = c.z, v[4 * k + 3] = c.w;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+            {
+                const bool in = f0 + k < count;
+                v[k] = in ? h[f0 + k] : 0u;
+                fb[k] = in ? first[f0 + k] : 0;
+            }
+        }
         std::uint32_t flags = 0; // bit k: range f0+k starts a segment
         std::uint32_t run = 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k)
         {
             const bool in = f0 + k < count;
-            v[k] = in ? h[f0 + k] : 0u;
-            const bool f = in && first[f0 + k];
+            const bool f = in && fb[k];
             flags |= std::uint32_t(f) << k;
             if (f) run = 0;
             ex[k] = run;
@@ -1042,21 +1110,22 @@ __global__ __launch_bounds__(kSortThreads) void msdBaseKernel(const MsdSeg *__re
 
 // Sorts every chunk in LDS. The chunk's keys (and values) are held in
 // registers; a stable counting pass per 8-bit digit (ranking as in
-// scatterKernel) is run for each digit at or above bit 32 that is not the
-// same for all of the chunk's keys. Runs of keys that then share their high
-// 32 bits are finished by an insertion sort on the low 32 bits, one thread
-// per run, when no run is longer than kFixRun; otherwise the varying low
-// digits and then the high digits get counting passes (plain LSD order).
+// scatterKernel) is run for each digit at or above bit kFixShift that is not
+// the same for all of the chunk's keys. Runs of keys that then agree from bit
+// kFixShift up are finished by an insertion sort on the lower bits, one
+// thread per run, when no run is longer than kFixRun; otherwise the varying
+// low digits and then the high digits get counting passes (plain LSD order).
 // Chunks with bits == 0 are already in order and are only copied when they
 // lie in the other buffer.
 constexpr std::uint32_t kFixRun = 16;
+constexpr int kFixShift = 24; // counting passes on the digits above; runs equal above it are fixed up
 
-__global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCounters *cnt,
+__global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCounters *cnt,
                                                                      const MsdChunk *__restrict__ chunks, Key *X,
                                                                      Key *Y, std::uint32_t *vX, std::uint32_t *vY,
                                                                      int finalIsY, std::uint32_t maxChunks)
 {
-    constexpr int kW = kMsdThreads / 64;
+    constexpr int kW = kLocThreads / 64;
     __shared__ Key stageKeys[kMsdChunk];
     __shared__ std::uint32_t stageVals[kMsdChunk];
     __shared__ std::uint32_t waveCount[kW][kRadix];
@@ -1071,19 +1140,19 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
     Key *kd = finalIsY ? Y : X;
     std::uint32_t *vd = finalIsY ? vY : vX;
     const std::uint32_t count = cnt->chunks < maxChunks ? cnt->chunks : maxChunks;
-    const std::uint32_t first = std::uint32_t(wave) * (kMsdItems * 64) + lane;
-    Key key[kMsdItems];
-    std::uint32_t val[kMsdItems];
+    const std::uint32_t first = std::uint32_t(wave) * (kLocItems * 64) + lane;
+    Key key[kLocItems];
+    std::uint32_t val[kLocItems];
 
     // one stable counting pass on the digit at `shift`; the result is in
     // the registers (striped order) and in stageKeys / stageVals
     auto pass = [&](int shift) {
-        for (int i = tid; i < kW * kRadix; i += kMsdThreads) (&waveCount[0][0])[i] = 0;
+        for (int i = tid; i < kW * kRadix; i += kLocThreads) (&waveCount[0][0])[i] = 0;
         __syncthreads();
-        std::uint32_t rank[kMsdItems], old[kMsdItems];
-        int leader[kMsdItems];
+        std::uint32_t rank[kLocItems], old[kLocItems];
+        int leader[kLocItems];
 #pragma unroll
-        for (int j = 0; j < kMsdItems; ++j)
+        for (int j = 0; j < kLocItems; ++j)
         {
             const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
             std::uint64_t match = ~std::uint64_t(0);
@@ -1102,7 +1171,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
             rank[j] = lower;
         }
 #pragma unroll
-        for (int j = 0; j < kMsdItems; ++j) rank[j] += std::uint32_t(__shfl(int(old[j]), leader[j], 64));
+        for (int j = 0; j < kLocItems; ++j) rank[j] += std::uint32_t(__shfl(int(old[j]), leader[j], 64));
         __syncthreads();
         std::uint32_t tileCount = 0, inc = 0;
         if (tid < kRadix)
@@ -1136,7 +1205,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kMsdItems; ++j)
+        for (int j = 0; j < kLocItems; ++j)
         {
             const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)];
             stageKeys[pos] = key[j];
@@ -1144,7 +1213,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kMsdItems; ++j)
+        for (int j = 0; j < kLocItems; ++j)
         {
             key[j] = stageKeys[first + std::uint32_t(j) * 64];
             val[j] = stageVals[first + std::uint32_t(j) * 64];
@@ -1160,7 +1229,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
         if (ch.bits == 0 || ch.len > kMsdChunk)
         {
             if (ks != kd)
-                for (std::uint32_t i = tid; i < ch.len; i += kMsdThreads)
+                for (std::uint32_t i = tid; i < ch.len; i += kLocThreads)
                 {
                     kd[ch.begin + i] = ks[ch.begin + i];
                     vd[ch.begin + i] = vs[ch.begin + i];
@@ -1170,7 +1239,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
         const Key k0 = ks[ch.begin];
         Key m = 0;
 #pragma unroll
-        for (int j = 0; j < kMsdItems; ++j)
+        for (int j = 0; j < kLocItems; ++j)
         {
             const std::uint32_t idx = first + std::uint32_t(j) * 64;
             const bool in = idx < ch.len;
@@ -1191,49 +1260,63 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int w = 0; w < kW; ++w) m |= waveMask[w];
         if (ch.bits < 64) m &= (Key(1) << ch.bits) - 1;
-        const Key hiMask = m & ~Key(0xffffffffu), loMask = m & Key(0xffffffffu);
-        for (int shift = 32; shift < 64; shift += kRadixBits)
+        const Key loMask = m & ((Key(1) << kFixShift) - 1), hiMask = m & ~loMask;
+        for (int shift = kFixShift; shift < 64; shift += kRadixBits)
             if ((hiMask >> shift) & (kRadix - 1)) pass(shift);
         if (loMask)
         {
-            // the chunk (sorted on its high 32 bits) must sit in the stage
+            // the chunk (sorted on its bits from kFixShift up) must sit in the stage
             if (!hiMask)
             {
 #pragma unroll
-                for (int j = 0; j < kMsdItems; ++j)
+                for (int j = 0; j < kLocItems; ++j)
                 {
                     stageKeys[first + std::uint32_t(j) * 64] = key[j];
                     stageVals[first + std::uint32_t(j) * 64] = val[j];
                 }
                 __syncthreads();
             }
-            // runs of equal high halves (the stage holds the chunk in order):
-            // is any longer than kFixRun?
-            for (std::uint32_t p0 = tid; p0 < ch.len; p0 += kMsdThreads)
+            // Runs of keys equal from bit kFixShift up, from the stage: thread t
+            // looks at positions t + k*kLocThreads (independent, conflict-free
+            // reads); only run starts walk their run.
+            auto high = [&](std::uint32_t q) { return stageKeys[q] >> kFixShift; };
+            std::uint32_t starts = 0; // bit k: position t + k*kLocThreads starts a run of two or more
+#pragma unroll 1
+            for (int k = 0; k < kLocItems; ++k)
             {
-                const std::uint32_t h = std::uint32_t(stageKeys[p0] >> 32);
-                if (p0 == 0 || std::uint32_t(stageKeys[p0 - 1] >> 32) != h)
+                const std::uint32_t p0 = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
+                if (p0 + 1 < ch.len)
                 {
-                    std::uint32_t e = p0 + 1;
-                    while (e < ch.len && e - p0 <= kFixRun && std::uint32_t(stageKeys[e] >> 32) == h) ++e;
-                    if (e - p0 > kFixRun) longRun = 1;
+                    const Key h = high(p0);
+                    const bool dupNext = high(p0 + 1) == h;
+                    const bool dupPrev = p0 > 0 && high(p0 - 1) == h;
+                    if (dupNext && !dupPrev)
+                    {
+                        starts |= 1u << k;
+                        std::uint32_t e = p0 + 2;
+                        while (e < ch.len && e - p0 <= kFixRun && high(e) == h) ++e;
+                        if (e - p0 > kFixRun) longRun = 1;
+                    }
                 }
             }
             __syncthreads();
             const bool fix = !longRun;
             if (fix)
             {
-                // stable insertion sort of each run on the low 32 bits
-                for (std::uint32_t p0 = tid; p0 < ch.len; p0 += kMsdThreads)
+                // stable insertion sort of each run on the bits below kFixShift
+#pragma unroll 1
+                for (int k = 0; k < kLocItems; ++k)
                 {
-                    const std::uint32_t h = std::uint32_t(stageKeys[p0] >> 32);
-                    if (p0 != 0 && std::uint32_t(stageKeys[p0 - 1] >> 32) == h) continue;
-                    for (std::uint32_t i = p0 + 1; i < ch.len && std::uint32_t(stageKeys[i] >> 32) == h; ++i)
+                    if (!((starts >> k) & 1u)) continue;
+                    const std::uint32_t p0 = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
+                    const Key h = high(p0);
+                    constexpr Key low = (Key(1) << kFixShift) - 1;
+                    for (std::uint32_t i = p0 + 1; i < ch.len && high(i) == h; ++i)
                     {
                         const Key kk = stageKeys[i];
                         const std::uint32_t vv = stageVals[i];
                         std::uint32_t q = i;
-                        while (q > p0 && std::uint32_t(stageKeys[q - 1]) > std::uint32_t(kk))
+                        while (q > p0 && (stageKeys[q - 1] & low) > (kk & low))
                         {
                             stageKeys[q] = stageKeys[q - 1];
                             stageVals[q] = stageVals[q - 1];
@@ -1246,7 +1329,7 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
                 __syncthreads();
             }
 #pragma unroll
-            for (int j = 0; j < kMsdItems; ++j)
+            for (int j = 0; j < kLocItems; ++j)
             {
                 key[j] = stageKeys[first + std::uint32_t(j) * 64];
                 val[j] = stageVals[first + std::uint32_t(j) * 64];
@@ -1254,14 +1337,14 @@ __global__ __launch_bounds__(kMsdThreads, 4) void msdLocalSortKernel(const MsdCo
             if (!fix)
             {
                 __syncthreads(); // the passes rewrite the stage
-                for (int shift = 0; shift < 32; shift += kRadixBits)
+                for (int shift = 0; shift < kFixShift; shift += kRadixBits)
                     if ((loMask >> shift) & (kRadix - 1)) pass(shift);
-                for (int shift = 32; shift < 64; shift += kRadixBits)
+                for (int shift = kFixShift; shift < 64; shift += kRadixBits)
                     if ((hiMask >> shift) & (kRadix - 1)) pass(shift);
             }
         }
 #pragma unroll
-        for (int j = 0; j < kMsdItems; ++j)
+        for (int j = 0; j < kLocItems; ++j)
         {
             const std::uint32_t idx = first + std::uint32_t(j) * 64;
             if (idx < ch.len)
@@ -1390,10 +1473,15 @@ void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *
     hipLaunchKernelGGL(msdPlanKernel, dim3(1), dim3(kSortThreads), 0, s, 1, ws.msdRoot, nullptr, ws.digitTotal,
                        ws.msdCnt, 0, ws.msdSegs[0], ws.msdRanges[0], ws.msdFirst[0], maxSegs, maxRanges,
                        ws.msdChunks, maxChunks);
-    const unsigned segGrid = unsigned(maxSegs < 1024 ? maxSegs : 1024);
-    const unsigned upGrid = unsigned((maxRanges / kUpRows) < 1024 ? maxRanges / kUpRows + 1 : 1024);
     for (int level = 2; level <= kSortPasses; ++level)
     {
+        // levels past the third see work only for keys that share 24 high
+        // bits in buckets of more than kMsdChunk (heavy duplicates): small
+        // grids keep those launches cheap when they are empty
+        const unsigned cap = level <= 3 ? 1024u : 128u;
+        const unsigned segGrid = unsigned(maxSegs < cap ? maxSegs : cap);
+        const unsigned upGrid = unsigned((maxRanges / kUpRows) < cap ? maxRanges / kUpRows + 1 : cap);
+        const unsigned scatterGrid = level <= 3 ? unsigned(kMsdGrid) : 256u;
         prof::Scope scope("msd_level", s);
         const int set = level & 1;
         const bool srcY = (level - 1) & 1, dstY = level & 1;
@@ -1411,12 +1499,12 @@ void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *
         const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegBase, ws.msdOff, ws.msdStride};
         if (level < kSortPasses)
             hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, true, true, false, true>),
-                               dim3(kMsdGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
+                               dim3(scatterGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
                                std::size_t(0), nullptr, nullptr, std::size_t(0), shift, shift - kRadixBits,
                                ws.msdRanges[set], a);
         else
             hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, false, true, false, true>),
-                               dim3(kMsdGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
+                               dim3(scatterGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
                                std::size_t(0), nullptr, nullptr, std::size_t(0), shift, 0, ws.msdRanges[set], a);
         hipLaunchKernelGGL(msdPlanKernel, dim3(segGrid), dim3(kSortThreads), 0, s, level, ws.msdSegs[set],
                            &ws.msdCnt->segs[set], ws.msdSubTotals, ws.msdCnt, set ^ 1, ws.msdSegs[set ^ 1],
@@ -1425,7 +1513,7 @@ void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *
     }
     {
         prof::Scope scope("msd_local", s);
-        hipLaunchKernelGGL(msdLocalSortKernel, dim3(kMsdGrid), dim3(kMsdThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
+        hipLaunchKernelGGL(msdLocalSortKernel, dim3(kMsdGrid), dim3(kLocThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
                            vX, vY, int(finalIsY), maxChunks);
     }
     AT_HIP(hipGetLastError());

@@ -34,11 +34,12 @@ def keys_of(kind, n, rng):
     if kind == "clustered":
         hi = (rng.normal(0, 1, n) * 3 + 100).astype(np.int64).clip(0, 255).astype(np.uint64) << np.uint64(56)
         return hi | rng.integers(0, 2**56, n, dtype=np.uint64)
-    if kind == "low_half":
-        # the high 32 bits take few values: runs of equal high halves of every length
-        hi = rng.integers(0, n // 3 + 1, n, dtype=np.uint64) << np.uint64(32)
-        return hi | rng.integers(0, 2**32, n, dtype=np.uint64)
-    if kind == "low_half_long":
+    if kind == "short_runs":
+        # bits 24.. take about n/3 values: short runs that agree above bit 24
+        hi = rng.integers(0, n // 3 + 1, n, dtype=np.uint64) << np.uint64(24)
+        return hi | rng.integers(0, 2**24, n, dtype=np.uint64)
+    if kind == "long_runs":
+        # bits 24.. take 2^14 values: runs longer than the chunk fix-up handles
         hi = rng.integers(0, 64, n, dtype=np.uint64) << np.uint64(40)
         return hi | rng.integers(0, 2**32, n, dtype=np.uint64)
     raise ValueError(kind)
@@ -62,8 +63,8 @@ def msd(request):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["uniform", "invalid_mix", "few", "dup_runs", "clustered", "low_half",
-                                  "low_half_long"])
+@pytest.mark.parametrize("kind", ["uniform", "invalid_mix", "few", "dup_runs", "clustered", "short_runs",
+                                  "long_runs"])
 @pytest.mark.parametrize("n", [1000, 300_001, (1 << 20) + 777, 3_000_017])
 def test_sort_pairs_matches_stable_argsort(msd, kind, n):
     print("sort", msd, kind, n, flush=True)
