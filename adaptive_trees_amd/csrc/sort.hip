@@ -412,6 +412,24 @@ __global__ __launch_bounds__(kSortThreads) void scanDigitKernel(const std::uint3
     if (threadIdx.x == 0) digitTotal[d] = carry;
 }
 
+// Lanes of the wave whose digit equals this lane's: for every digit bit, the
+// ballot of lanes with the bit set, complemented where this lane's bit is
+// clear. mask = 0 or ~0 from a sign-extended bit, so each step is one
+// three-input bitwise operation per 32-bit half.
+__device__ inline std::uint64_t laneMatch(unsigned dg)
+{
+    std::uint32_t lo = ~0u, hi = ~0u;
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b)
+    {
+        const std::uint32_t mask = std::uint32_t(__builtin_amdgcn_sbfe(int(dg), b, 1));
+        const std::uint64_t ballot = __ballot(int(mask));
+        lo &= ~(std::uint32_t(ballot) ^ mask);
+        hi &= ~(std::uint32_t(ballot >> 32) ^ mask);
+    }
+    return (std::uint64_t(hi) << 32) | lo;
+}
+
 // Exclusive scan of the 256 per-digit values held by threads 0..255 of a
 // block of any size (all threads must call it).
 __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_t *waveTotals)
@@ -580,14 +598,7 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
         for (int j = 0; j < ITEMS; ++j)
         {
             const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
-            std::uint64_t match = ~std::uint64_t(0);
-#pragma unroll
-            for (int b = 0; b < kRadixBits; ++b)
-            {
-                const bool bit = (dg >> b) & 1u;
-                const std::uint64_t ballot = __ballot(bit);
-                match &= bit ? ballot : ~ballot;
-            }
+            const std::uint64_t match = laneMatch(dg);
             const std::uint32_t lower = __popcll(match & ltMask);
             const bool lead = lower == 0;
             std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
@@ -1155,14 +1166,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         for (int j = 0; j < kLocItems; ++j)
         {
             const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
-            std::uint64_t match = ~std::uint64_t(0);
-#pragma unroll
-            for (int b = 0; b < kRadixBits; ++b)
-            {
-                const bool bit = (dg >> b) & 1u;
-                const std::uint64_t ballot = __ballot(bit);
-                match &= bit ? ballot : ~ballot;
-            }
+            const std::uint64_t match = laneMatch(dg);
             const std::uint32_t lower = __popcll(match & ltMask);
             const bool lead = lower == 0;
             std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
