@@ -469,12 +469,14 @@ __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_
 //
 // kMsd: a partition level of the MSD sort. The block loops over the
 // *msd.rangeCount ranges of segRanges (grid-stride); a range's digit d starts
-// at msd.segBase[seg][d] + msd.off[d][range].
+// at its segment's begin + the segment's keys of smaller digits (an exclusive
+// scan of msd.subTotals[seg]) + msd.off[d][range].
 struct MsdScatterArgs
 {
     const std::uint32_t *rangeCount = nullptr;
-    const std::uint32_t *segBase = nullptr; // [segment][digit] absolute start
-    const std::uint32_t *off = nullptr;     // [digit][stride] offset of a range within its segment
+    const MsdSeg *segs = nullptr;
+    const std::uint32_t *subTotals = nullptr; // [segment][digit] keys of each digit
+    const std::uint32_t *off = nullptr;       // [digit][stride] offset of a range within its segment
     std::size_t stride = 0;
 };
 
@@ -508,7 +510,9 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
         rBegin = d.begin;
         rLen = d.len;
         valueBase = d.localBase;
-        if (tid < kRadix) running[tid] = msd.segBase[std::size_t(d.seg) * kRadix + tid] + msd.off[tid * msd.stride + r];
+        const std::uint32_t total = tid < kRadix ? msd.subTotals[std::size_t(d.seg) * kRadix + tid] : 0u;
+        const std::uint32_t digitBase = std::uint32_t(msd.segs[d.seg].begin) + digitExclusiveScan(total, scratch);
+        if (tid < kRadix) running[tid] = digitBase + msd.off[tid * msd.stride + r];
     }
     else if constexpr (kSeg)
     {
@@ -1105,20 +1109,6 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
     }
 }
 
-// segBase[segment][d] = segment begin + keys of smaller digits in the segment.
-__global__ __launch_bounds__(kSortThreads) void msdBaseKernel(const MsdSeg *__restrict__ segs, const MsdCounters *cnt,
-                                                              int set, const std::uint32_t *__restrict__ subTotals,
-                                                              std::uint32_t *segBase, std::uint32_t maxSegs)
-{
-    __shared__ std::uint32_t waveTotals[kWaves];
-    const std::uint32_t count = cnt->segs[set] < maxSegs ? cnt->segs[set] : maxSegs;
-    for (std::uint32_t p = blockIdx.x; p < count; p += gridDim.x)
-    {
-        const std::size_t i = std::size_t(p) * kRadix + threadIdx.x;
-        segBase[i] = std::uint32_t(segs[p].begin) + blockExclusiveScan(subTotals[i], waveTotals);
-    }
-}
-
 // Sorts every chunk in LDS. The chunk's keys (and values) are held in
 // registers; a stable counting pass per 8-bit digit (ranking as in
 // scatterKernel) is run for each digit at or above bit kFixShift that is not
@@ -1498,9 +1488,7 @@ void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *
                            ws.msdCnt, set, ws.msdHist, ws.msdStride);
         hipLaunchKernelGGL(msdScanKernel, dim3(kRadix), dim3(kSortThreads), 0, s, ws.msdHist, ws.msdStride,
                            ws.msdFirst[set], ws.msdRanges[set], ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
-        hipLaunchKernelGGL(msdBaseKernel, dim3(segGrid), dim3(kSortThreads), 0, s, ws.msdSegs[set], ws.msdCnt, set,
-                           ws.msdSubTotals, ws.msdSegBase, maxSegs);
-        const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegBase, ws.msdOff, ws.msdStride};
+        const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegs[set], ws.msdSubTotals, ws.msdOff, ws.msdStride};
         if (level < kSortPasses)
             hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, true, true, false, true>),
                                dim3(scatterGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
@@ -1610,7 +1598,7 @@ void SortWorkspace::allocateMsd(std::size_t cap, hipStream_t s)
     if (cap <= msdCapacity) return;
     for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
                     (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
-                    (void *)msdSubTotals, (void *)msdSegBase, (void *)msdChunks})
+                    (void *)msdSubTotals, (void *)msdChunks})
         poolFree(p, s);
     msdCapacity = cap;
     // segments hold more than kMsdChunk keys; a level's ranges are its
@@ -1631,7 +1619,6 @@ void SortWorkspace::allocateMsd(std::size_t cap, hipStream_t s)
     msdHist = static_cast<std::uint32_t *>(alloc(std::size_t(kRadix) * msdStride * sizeof(std::uint32_t)));
     msdOff = static_cast<std::uint32_t *>(alloc(std::size_t(kRadix) * msdStride * sizeof(std::uint32_t)));
     msdSubTotals = static_cast<std::uint32_t *>(alloc(msdMaxSegs * kRadix * sizeof(std::uint32_t)));
-    msdSegBase = static_cast<std::uint32_t *>(alloc(msdMaxSegs * kRadix * sizeof(std::uint32_t)));
     msdChunks = static_cast<MsdChunk *>(alloc(msdMaxChunks * sizeof(MsdChunk)));
 }
 
@@ -1646,10 +1633,10 @@ void SortWorkspace::release()
     capacity = 0; maxRanges = 0; values = false;
     for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
                     (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
-                    (void *)msdSubTotals, (void *)msdSegBase, (void *)msdChunks})
+                    (void *)msdSubTotals, (void *)msdChunks})
         poolFree(p, nullptr);
     msdCnt = nullptr; msdRoot = nullptr; msdSegs[0] = msdSegs[1] = nullptr; msdRanges[0] = msdRanges[1] = nullptr;
-    msdFirst[0] = msdFirst[1] = nullptr; msdHist = msdOff = msdSubTotals = msdSegBase = nullptr; msdChunks = nullptr;
+    msdFirst[0] = msdFirst[1] = nullptr; msdHist = msdOff = msdSubTotals = nullptr; msdChunks = nullptr;
     msdCapacity = msdMaxSegs = msdStride = msdMaxChunks = 0;
 }
 

@@ -121,7 +121,6 @@ struct SortWorkspace
     std::uint32_t *msdHist = nullptr;      // [kRadix][msdStride]
     std::uint32_t *msdOff = nullptr;       // [kRadix][msdStride]
     std::uint32_t *msdSubTotals = nullptr; // [msdMaxSegs][kRadix]
-    std::uint32_t *msdSegBase = nullptr;   // [msdMaxSegs][kRadix]
     MsdChunk *msdChunks = nullptr;
     std::size_t msdCapacity = 0, msdMaxSegs = 0, msdStride = 0, msdMaxChunks = 0;
 

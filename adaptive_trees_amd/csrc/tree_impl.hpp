@@ -533,6 +533,7 @@ public:
         perm_.ensure(n ? n : 1, 0, stream_);
         sort_.allocate(n ? n : 1, true, stream_);
         idx_.ensure(indexSize(n) + 1, 0, stream_);
+        tileFirst_.ensure(n / kBoundTile + 2, 0, stream_);
         AT_HIP(hipStreamSynchronize(stream_));
     }
 
@@ -563,7 +564,9 @@ public:
         const TreeDev t = dev();
         const BinParams p{bucket_, maxDepth_};
         updateReset(stream_, t);
-        leafCounts(stream_, t, leafCapacity_);
+        // the particles moved: count every leaf (small leaves: one read of P)
+        if (bucket_ <= kTileCountBucket && n_ > 0) leafCountsByTiles(stream_, t);
+        else leafCounts(stream_, t, leafCapacity_);
         for (unsigned i = 0; i < iterations_; ++i) binPass(stream_, t, p, long(leafCapacity_), leafCapacity_);
         buildTreeView(views_);
         AT_HIP(hipGetLastError());
@@ -648,6 +651,7 @@ private:
         t.st = st_;
         t.P = P_.p;
         t.Pidx = idx_.p;
+        t.tileFirst = tileFirst_.p;
         t.n = n_;
         t.capacity = long(leafCapacity_);
         return t;
@@ -807,6 +811,7 @@ private:
     DevBuf<std::uint64_t> status_;
     DevBuf<std::uint32_t> ticket_;
     DevBuf<Key> idx_;
+    DevBuf<std::uint32_t> tileFirst_; // per tile of P: first leaf boundary it owns (full leaf counts)
     DirectBufs direct_;
     ViewEngine view_;
 };

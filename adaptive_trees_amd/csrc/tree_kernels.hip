@@ -1199,6 +1199,57 @@ void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
     hipLaunchKernelGGL(commitKernel, dim3(clearBlocks(t.statusWords)), dim3(256), 0, s, t.st, 1, t.scanStatus,
                        t.statusWords, t.scanTicket, t.nfCount);
 }
+// Tile b of P is [b T, (b+1) T). Leaf boundary i belongs to the tile that
+// holds its lower bound in P: tile b owns the boundaries with
+// P[bT - 1] < K[i] <= P[(b+1)T - 1] (the last tile also those above every
+// key), i.e. tileFirst[b] <= i < tileFirst[b + 1].
+__global__ __launch_bounds__(kThreads) void tileFirstKernel(TreeDev t, std::size_t tiles)
+{
+    const DevState *st = t.st;
+    if (st->converged) return;
+    const std::uint32_t L = std::uint32_t(st->numLeaves);
+    const Key *K = t.K[st->activeBuffer];
+    for (std::size_t b = std::size_t(blockIdx.x) * kThreads + threadIdx.x; b <= tiles;
+         b += std::size_t(gridDim.x) * kThreads)
+        t.tileFirst[b] = b == 0 ? 0u
+                         : b == tiles ? L + 1
+                                      : std::uint32_t(upperBound(K, std::size_t(L) + 1, t.P[b * kBoundTile - 1]));
+}
+// Block b: the tile in LDS, then the lower bound of each boundary it owns.
+__global__ __launch_bounds__(kThreads) void tileBoundsKernel(TreeDev t)
+{
+    __shared__ Key tp[kBoundTile];
+    const DevState *st = t.st;
+    if (st->converged) return;
+    const Key *K = t.K[st->activeBuffer];
+    const std::size_t s = std::size_t(blockIdx.x) * kBoundTile;
+    const std::size_t len = t.n - s < kBoundTile ? t.n - s : kBoundTile;
+    for (std::size_t i = threadIdx.x; i < len; i += kThreads) tp[i] = t.P[s + i];
+    __syncthreads();
+    std::uint32_t *lb = reinterpret_cast<std::uint32_t *>(t.offs);
+    const std::uint32_t i1 = t.tileFirst[blockIdx.x + 1];
+    for (std::uint32_t i = t.tileFirst[blockIdx.x] + threadIdx.x; i < i1; i += kThreads)
+        lb[i] = std::uint32_t(s + lowerBound(tp, len, K[i]));
+}
+__global__ __launch_bounds__(kThreads) void tileCountsKernel(TreeDev t)
+{
+    const DevState *st = t.st;
+    if (st->converged) return;
+    const int L = st->numLeaves;
+    const std::uint32_t *lb = reinterpret_cast<const std::uint32_t *>(t.offs);
+    std::uint32_t *N = t.N[st->activeBuffer];
+    for (int i = int(blockIdx.x * kThreads + threadIdx.x); i < L; i += int(gridDim.x * kThreads))
+        N[i] = lb[i + 1] - lb[i];
+}
+void leafCountsByTiles(hipStream_t s, const TreeDev &t)
+{
+    prof::Scope scope("leaf_counts", s);
+    const std::size_t tiles = (t.n + kBoundTile - 1) / kBoundTile;
+    hipLaunchKernelGGL(tileFirstKernel, dim3(ceilDiv(tiles + 1, kThreads)), dim3(kThreads), 0, s, t, tiles);
+    hipLaunchKernelGGL(tileBoundsKernel, dim3(unsigned(tiles)), dim3(kThreads), 0, s, t);
+    hipLaunchKernelGGL(tileCountsKernel, dim3(2048), dim3(kThreads), 0, s, t);
+}
+
 void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 {
     prof::Scope scope("leaf_counts", s);

@@ -26,6 +26,7 @@ struct TreeDev
     DevState *st = nullptr;
     const Key *P = nullptr;
     const Key *Pidx = nullptr;                   // sampled search index of P (indexSize(n) keys)
+    std::uint32_t *tileFirst = nullptr;          // full leaf counts by tiles of P: n / kBoundTile + 2 entries
     std::size_t n = 0;                           // particle slots
     long capacity = 0;                           // leaf capacity of a pass (multi-tree launches)
 };
@@ -82,6 +83,12 @@ void binPass(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity,
 void binDecideScan(hipStream_t s, const TreeDev &t, const BinParams &p, long capacity, std::size_t gridLeaves);
 void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves);
 void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves);
+// Full leaf counts read P once, tile by tile (for small leaves, where a
+// search per leaf would touch nearly every line of P anyway); needs
+// t.tileFirst and uses t.offs as scratch.
+constexpr std::size_t kBoundTile = 2048;
+constexpr unsigned kTileCountBucket = 64; // leaf size up to which the tiled count is used
+void leafCountsByTiles(hipStream_t s, const TreeDev &t);
 // One-pass construction of the converged binary tree into K[0]/N[0]:
 // phase 1 returns the number of internal nodes (the host sizes buffers),
 // phase 2 writes the sorted leaf boundaries and their particle counts.
