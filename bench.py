@@ -32,7 +32,11 @@ from adaptive_trees_amd import _ffi, tune, workloads  # noqa: E402
 
 METRIC = "Mparticles/s build+update, % HBM roofline; batched vs plain speedup; p99 latency"
 PEAK_HBM_GBS = 8000.0
-SORT_PASS_BYTES = 24  # read + write of one (8-byte key, 4-byte value) pair per pass
+PAIR_BYTES = 24  # read + write of one (8-byte key, 4-byte value) pair
+# The dominant kernel: the LDS chunk sort of the MSD radix sort (msdLocalSortKernel),
+# one launch per sort over all N pairs; its profile label and its name in rocprofv3 output.
+DOMINANT_LABEL = "msd_local"
+DOMINANT_KERNEL = "msdLocalSortKernel"
 COPY_BYTES = 1 << 30  # calibration copy: reads and writes this many bytes
 
 
@@ -75,11 +79,12 @@ def profile_query(label):
     return (ms.value, n.value) if status == 0 else (0.0, 0)
 
 
-def algorithmic_build_bytes(n, leaves, s=4, passes=8):
-    """Bytes a build must move: keygen N(3s+8) ... per the path's definition
-    (keygen reads 3s and writes 8; each sort pass moves 24 with an 8-byte
-    histogram read; one streaming read of P; K and N written)."""
-    return n * (3 * s + 12) + n * (24 * passes + 8) + 8 * n + 12 * leaves
+def algorithmic_build_bytes(n, leaves, s=4):
+    """Bytes any build must move: key generation reads 3s and writes 8 per
+    particle, a sort reads the keys and writes the sorted (key, slot) pairs
+    (8 + 12), the construction reads P once and writes K and N (8 + 4 per
+    leaf). A floor for the whole build, not a count of what this one moves."""
+    return n * (3 * s + 8) + n * 20 + 8 * n + 12 * leaves
 
 
 def cpu_baseline(x, y, z, x1, y1, z1, box, min_seconds):
@@ -446,21 +451,21 @@ def _counter_rows(path):
 
 
 def traffic_from_counters(fetch_csv, write_csv, copy_bytes):
-    """HBM bytes per launch of the radix scatter from FETCH_SIZE / WRITE_SIZE runs (KiB per
+    """HBM bytes per launch of the dominant kernel from FETCH_SIZE / WRITE_SIZE runs (KiB per
     dispatch), each scaled by the same counter's reading for the copy kernel of copy_bytes."""
-    out = {}
+    out = {"kernel": DOMINANT_KERNEL}
     for label, path in (("fetch", fetch_csv), ("write", write_csv)):
         rows = _counter_rows(path)
         copy = [v for k, v in rows if "copyKernel" in k]
-        scat = [v for k, v in rows if re.search(r"scatterKernel<\d+, \d+, true, false, true", k)]
+        scat = [v for k, v in rows if DOMINANT_KERNEL in k]
         if not copy or not scat:
-            raise SystemExit("no copyKernel / scatterKernel dispatches in %s" % path)
+            raise SystemExit("no copyKernel / %s dispatches in %s" % (DOMINANT_KERNEL, path))
         factor = copy_bytes / (float(np.median(copy)) * 1024.0)
         out[label + "_kib_per_dispatch"] = float(np.median(scat))
         out[label + "_copy_kib"] = float(np.median(copy))
         out[label + "_scale"] = factor
         out[label + "_bytes"] = float(np.median(scat)) * 1024.0 * factor
-    out["sort_pass_bytes_per_launch"] = int(out["fetch_bytes"] + out["write_bytes"])
+    out["bytes_per_launch"] = int(out["fetch_bytes"] + out["write_bytes"])
     return out
 
 
@@ -560,16 +565,24 @@ def main():
     event_ms = e2.ms_since(e0) / args.steps
 
     leaves = int(tree.scalars().num_leaves)
-    sort_ms, sort_n = profile_query("sort_pass")
-    key_ms, key_n = profile_query("keygen")
+    def per_launch(label):
+        ms, cnt = profile_query(label)
+        return ms / max(cnt, 1), cnt
+
+    dom_ms, dom_n = per_launch(DOMINANT_LABEL)
+    key_ms, _ = per_launch("keygen")
+    pass_ms, _ = per_launch("sort_pass")
+    level_ms, level_n = per_launch("msd_level")
+    count_ms, _ = per_launch("leaf_counts")
     dir_ms, dir_n = profile_query("bin_direct")
-    pass_ms = sort_ms / max(sort_n, 1)
-    achieved = SORT_PASS_BYTES * n / (pass_ms * 1e-3) / 1e9 if sort_n else 0.0
+    achieved = PAIR_BYTES * n / (dom_ms * 1e-3) / 1e9 if dom_n else 0.0
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("sort_pass_bytes_per_launch")
+            t = json.load(f)
+            if t.get("kernel") == DOMINANT_KERNEL:
+                traffic = t.get("bytes_per_launch")
 
     # an update that rebalances: reflected Gaussian steps (sigma = 1e-3 of the box)
     x1, y1, z1 = workloads.reflect_step((x, y, z), 1e-3, seed=33, box=box)
@@ -620,22 +633,26 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "radix scatter pass (scatterKernel, 8 per sort)",
+            "kernel": "LDS chunk sort of the MSD radix sort (%s, one launch per sort over all N pairs)"
+                      % DOMINANT_KERNEL,
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic,
-            "bytes_per_launch": SORT_PASS_BYTES * n,
-            "avg_launch_ms": round(pass_ms, 5),
+            "bytes_per_launch": PAIR_BYTES * n,
+            "avg_launch_ms": round(dom_ms, 5),
         },
         "phases_ms": {
             "step_event": round(event_ms, 4),
             "build_wall": round(build_wall_ms, 4),
             "update_static_graph": round(update_ms, 4),
             "update_moved_graph": round(float(np.median(moved_ms)), 4),
-            "keygen": round(key_ms / max(key_n, 1), 4),
-            "sort_pass": round(pass_ms, 4),
+            "keygen": round(key_ms, 4),
+            "sort_level1_pass": round(pass_ms, 4),
+            "sort_levels_2_to_8": round(level_ms * level_n / max(dom_n, 1), 4),
+            "sort_lds_chunks": round(dom_ms, 4),
+            "full_leaf_counts": round(count_ms, 4),
             "binary_construction": round(dir_ms / max(dir_n, 1) * 2, 4),
         },
         "build_roofline": {
