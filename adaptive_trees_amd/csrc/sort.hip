@@ -1,11 +1,12 @@
-// sort.hip -- stable LSD radix sort for gfx950, 8-bit digits, 8 passes.
+// sort.hip -- stable radix sort for gfx950: MSD levels + LDS chunk sort for
+// large sorts, LSD passes (8-bit digits, 8 passes) otherwise.
 //
 // Replaces the reference's cub::DeviceRadixSort::SortPairs calls
 // (src/octree/octree.cu:301-304, src/kdtree3d/kdtree3d.cu:168-170,
 // src/common/tree_view.cu:664-689): ascending by all 64 key bits, equal keys
 // keep their input order.
 //
-// Each pass is range-partitioned, with no communication between blocks:
+// An LSD pass is range-partitioned, with no communication between blocks:
 //   upsweep   digit histogram of each contiguous range of keys. Pass 0's
 //             histograms come out of key generation itself; pass k+1 reads a
 //             one-byte digit array that pass k's scatter wrote, not the keys.
@@ -14,10 +15,12 @@
 //             (hist[digit][row]), so each block reads one contiguous row set;
 //             every scatter block forms the digit bases from the 256 totals.
 //   scatter   one block per range walks its range tile by tile (wave64,
-//             256 threads, ITEMS keys per thread). Keys are ranked per wave
-//             with 8 ballots each (lanes with equal digits, stable in lane
-//             order), staged in LDS in output order, and written in runs of
-//             equal digits at a per-digit running offset kept in LDS.
+//             ITEMS keys per thread). Keys are ranked per wave with 8 ballots
+//             each (lanes with equal digits, stable in lane order), staged in
+//             LDS in output order, and written in runs of equal digits at a
+//             per-digit running offset kept in LDS.
+// The MSD sort (further below) uses the same scatter for its partition
+// levels and finishes small buckets in LDS.
 // A chained (look-back) scan is avoided on purpose: with ~768 tiles resident
 // on the 256 CUs the prefix chain made the look-back cost ~45% of a pass.
 #include "sort.hpp"

@@ -1,4 +1,4 @@
-// sort.hpp -- stable LSD radix sort of 64-bit keys (+ 32-bit values) for gfx950.
+// sort.hpp -- stable radix sort of 64-bit keys (+ 32-bit values) for gfx950.
 #pragma once
 
 #include <string>
@@ -17,7 +17,7 @@ constexpr int kMaxTileKeys = 8192; // scatter tile (threads x keys per thread) b
 constexpr int kSortThreads = 256;
 
 // Tile shape (threads per block, keys per thread) and target number of
-// ranges, chosen by the autotuner; defaults 512, 8 (with next-tile prefetch) and 2048.
+// ranges, chosen by the autotuner; defaults 512, 8 (no prefetch) and 8192.
 int sortItemsPerThread();
 void setSortItemsPerThread(int items);
 int sortTargetRanges();
