@@ -709,7 +709,7 @@ constexpr int kSegThreads = 512, kSegItems = 8; // tile shape of the segmented s
 template <class T, KeyKind K>
 __global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> p, const SegCoords<T> *coords,
                                                                 const SegRange *ranges, Key *keys,
-                                                                std::uint32_t *hist)
+                                                                std::uint32_t *hist, int histShift)
 {
     __shared__ std::uint32_t h[kRadix];
     h[threadIdx.x] = 0;
@@ -723,7 +723,7 @@ __global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> 
     {
         const Key k = particleKey<T, K>(p, std::size_t(d.localBase) + j);
         keys[d.begin + j] = k;
-        countDigit(h, unsigned(k) & (kRadix - 1));
+        countDigit(h, unsigned(k >> histShift) & (kRadix - 1));
     }
     __syncthreads();
     hist[std::size_t(blockIdx.x) * kRadix + threadIdx.x] = h[threadIdx.x];
@@ -753,7 +753,8 @@ __global__ __launch_bounds__(kSortThreads) void segUpsweepDigitsKernel(const std
 // the segment + keys of this digit in earlier ranges of the segment.
 __global__ __launch_bounds__(kSortThreads) void segScanKernel(const std::uint32_t *__restrict__ hist,
                                                               const std::uint32_t *segRange,
-                                                              const std::uint64_t *segBegin, std::uint32_t *off)
+                                                              const std::uint64_t *segBegin, std::uint32_t *off,
+                                                              std::uint32_t *segTotals)
 {
     __shared__ std::uint32_t waveTotals[kWaves];
     const unsigned d = threadIdx.x;
@@ -765,6 +766,7 @@ __global__ __launch_bounds__(kSortThreads) void segScanKernel(const std::uint32_
         off[std::size_t(r) * kRadix + d] = run;
         run += c;
     }
+    if (segTotals) segTotals[std::size_t(blockIdx.x) * kRadix + d] = run; // the segment's count of digit d
     const std::uint32_t base = std::uint32_t(segBegin[blockIdx.x]) + blockExclusiveScan(run, waveTotals);
     for (std::uint32_t r = r0; r < r1; ++r) off[std::size_t(r) * kRadix + d] += base;
 }
@@ -808,7 +810,7 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     int level, const MsdSeg *__restrict__ parents, const std::uint32_t *parentCount,
     const std::uint32_t *__restrict__ totals, MsdCounters *cnt, int nextSet, MsdSeg *nextSegs,
     SegRange *nextRanges, std::uint8_t *nextFirst, std::uint32_t maxSegs, std::uint32_t maxRanges, MsdChunk *chunks,
-    std::uint32_t maxChunks)
+    std::uint32_t maxChunks, std::uint32_t maxParents)
 {
     __shared__ std::uint32_t waveTotals[kWaves];
     __shared__ std::uint32_t relS[kRadix + 1];    // sub-bucket start within the parent
@@ -823,7 +825,7 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     __shared__ std::uint32_t base[3];
     __shared__ std::uint32_t nChunks, nBig, nRanges;
     const unsigned d = threadIdx.x;
-    const std::uint32_t count = parentCount ? (*parentCount < maxSegs ? *parentCount : maxSegs) : 1u;
+    const std::uint32_t count = parentCount ? (*parentCount < maxParents ? *parentCount : maxParents) : 1u;
     const bool last = level >= kSortPasses;
     const std::uint8_t parity = std::uint8_t(level & 1);
     // a chunk spans sub-buckets of one parent: sort on the bits below the parent's digit
@@ -1448,6 +1450,59 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
     AT_HIP(hipGetLastError());
 }
 
+// Levels 2..8 and the LDS chunk sort, after level 1 has partitioned the
+// keys (X -> Y, next-digit bytes in `digits`) and planned level 2's lists
+// (set 0) and level 1's chunks. Odd levels write Y, even levels X; the result
+// ends in Y when finalIsY, else in X.
+void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *digits, Key *X, Key *Y,
+               std::uint32_t *vX, std::uint32_t *vY, bool finalIsY)
+{
+    const std::uint32_t maxSegs = std::uint32_t(ws.msdMaxSegs), maxRanges = std::uint32_t(ws.msdStride),
+                        maxChunks = std::uint32_t(ws.msdMaxChunks);
+    for (int level = 2; level <= kSortPasses; ++level)
+    {
+        prof::Scope scope("msd_level", s);
+        // levels past the third see work only for keys that share 24 high
+        // bits in buckets of more than kMsdChunk (heavy duplicates): small
+        // grids keep those launches cheap when they are empty
+        const unsigned cap = level <= 3 ? 1024u : 128u;
+        const unsigned segGrid = unsigned(maxSegs < cap ? maxSegs : cap);
+        const unsigned upGrid = unsigned((maxRanges / kUpRows) < cap ? maxRanges / kUpRows + 1 : cap);
+        const unsigned scatterGrid = level <= 3 ? unsigned(kMsdGrid) : 256u;
+        const int set = level & 1;
+        const bool srcY = (level - 1) & 1, dstY = level & 1;
+        const Key *src = srcY ? Y : X;
+        Key *dst = dstY ? Y : X;
+        const std::uint32_t *vsrc = srcY ? vY : vX;
+        std::uint32_t *vdst = dstY ? vY : vX;
+        const int shift = 64 - kRadixBits * level;
+        hipLaunchKernelGGL(msdUpsweepKernel, dim3(upGrid), dim3(kUpRows * 64), 0, s, digits, ws.msdRanges[set],
+                           ws.msdCnt, set, ws.msdHist, ws.msdStride);
+        hipLaunchKernelGGL(msdScanKernel, dim3(kRadix), dim3(kSortThreads), 0, s, ws.msdHist, ws.msdStride,
+                           ws.msdFirst[set], ws.msdRanges[set], ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
+        const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegs[set], ws.msdSubTotals, ws.msdOff, ws.msdStride};
+        if (level < kSortPasses)
+            hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, true, true, false, true>),
+                               dim3(scatterGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, digits, n,
+                               std::size_t(0), nullptr, nullptr, std::size_t(0), shift, shift - kRadixBits,
+                               ws.msdRanges[set], a);
+        else
+            hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, false, true, false, true>),
+                               dim3(scatterGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, digits, n,
+                               std::size_t(0), nullptr, nullptr, std::size_t(0), shift, 0, ws.msdRanges[set], a);
+        hipLaunchKernelGGL(msdPlanKernel, dim3(segGrid), dim3(kSortThreads), 0, s, level, ws.msdSegs[set],
+                           &ws.msdCnt->segs[set], ws.msdSubTotals, ws.msdCnt, set ^ 1, ws.msdSegs[set ^ 1],
+                           ws.msdRanges[set ^ 1], ws.msdFirst[set ^ 1], maxSegs, maxRanges, ws.msdChunks,
+                           maxChunks, maxSegs);
+    }
+    {
+        prof::Scope scope("msd_local", s);
+        hipLaunchKernelGGL(msdLocalSortKernel, dim3(kMsdGrid), dim3(kLocThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
+                           vX, vY, int(finalIsY), maxChunks);
+    }
+    AT_HIP(hipGetLastError());
+}
+
 // MSD sort of (key, value) pairs given level 1's range histograms of bits
 // 56..63 in ws.rangeHist. Keys start in X (values in vX, or slots when
 // iota); odd levels write Y, even levels X; the result ends in Y when
@@ -1456,9 +1511,7 @@ void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *
              std::uint32_t *vY, bool iota, bool firstHistWhole, bool finalIsY)
 {
     const std::size_t n = pl.n;
-    ws.allocateMsd(n, s);
-    const std::uint32_t maxSegs = std::uint32_t(ws.msdMaxSegs), maxRanges = std::uint32_t(ws.msdStride),
-                        maxChunks = std::uint32_t(ws.msdMaxChunks);
+    ws.allocateMsd(n, 0, s);
     hipLaunchKernelGGL(msdResetKernel, dim3(1), dim3(1), 0, s, ws.msdCnt, ws.msdRoot, n);
     SortPlan scanPlan = pl;
     if (firstHistWhole) scanPlan.parts = 1;
@@ -1468,50 +1521,9 @@ void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *
         dispatchScatter(s, pl, ws, X, Y, vX, vY, true, iota, true, 56, 48);
     }
     hipLaunchKernelGGL(msdPlanKernel, dim3(1), dim3(kSortThreads), 0, s, 1, ws.msdRoot, nullptr, ws.digitTotal,
-                       ws.msdCnt, 0, ws.msdSegs[0], ws.msdRanges[0], ws.msdFirst[0], maxSegs, maxRanges,
-                       ws.msdChunks, maxChunks);
-    for (int level = 2; level <= kSortPasses; ++level)
-    {
-        // levels past the third see work only for keys that share 24 high
-        // bits in buckets of more than kMsdChunk (heavy duplicates): small
-        // grids keep those launches cheap when they are empty
-        const unsigned cap = level <= 3 ? 1024u : 128u;
-        const unsigned segGrid = unsigned(maxSegs < cap ? maxSegs : cap);
-        const unsigned upGrid = unsigned((maxRanges / kUpRows) < cap ? maxRanges / kUpRows + 1 : cap);
-        const unsigned scatterGrid = level <= 3 ? unsigned(kMsdGrid) : 256u;
-        prof::Scope scope("msd_level", s);
-        const int set = level & 1;
-        const bool srcY = (level - 1) & 1, dstY = level & 1;
-        const Key *src = srcY ? Y : X;
-        Key *dst = dstY ? Y : X;
-        const std::uint32_t *vsrc = srcY ? vY : vX;
-        std::uint32_t *vdst = dstY ? vY : vX;
-        const int shift = 64 - kRadixBits * level;
-        hipLaunchKernelGGL(msdUpsweepKernel, dim3(upGrid), dim3(kUpRows * 64), 0, s, ws.digits, ws.msdRanges[set],
-                           ws.msdCnt, set, ws.msdHist, ws.msdStride);
-        hipLaunchKernelGGL(msdScanKernel, dim3(kRadix), dim3(kSortThreads), 0, s, ws.msdHist, ws.msdStride,
-                           ws.msdFirst[set], ws.msdRanges[set], ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
-        const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegs[set], ws.msdSubTotals, ws.msdOff, ws.msdStride};
-        if (level < kSortPasses)
-            hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, true, true, false, true>),
-                               dim3(scatterGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
-                               std::size_t(0), nullptr, nullptr, std::size_t(0), shift, shift - kRadixBits,
-                               ws.msdRanges[set], a);
-        else
-            hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, false, true, false, true>),
-                               dim3(scatterGrid), dim3(kMsdThreads), 0, s, src, dst, vsrc, vdst, ws.digits, n,
-                               std::size_t(0), nullptr, nullptr, std::size_t(0), shift, 0, ws.msdRanges[set], a);
-        hipLaunchKernelGGL(msdPlanKernel, dim3(segGrid), dim3(kSortThreads), 0, s, level, ws.msdSegs[set],
-                           &ws.msdCnt->segs[set], ws.msdSubTotals, ws.msdCnt, set ^ 1, ws.msdSegs[set ^ 1],
-                           ws.msdRanges[set ^ 1], ws.msdFirst[set ^ 1], maxSegs, maxRanges, ws.msdChunks,
-                           maxChunks);
-    }
-    {
-        prof::Scope scope("msd_local", s);
-        hipLaunchKernelGGL(msdLocalSortKernel, dim3(kMsdGrid), dim3(kLocThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
-                           vX, vY, int(finalIsY), maxChunks);
-    }
-    AT_HIP(hipGetLastError());
+                       ws.msdCnt, 0, ws.msdSegs[0], ws.msdRanges[0], ws.msdFirst[0], std::uint32_t(ws.msdMaxSegs),
+                       std::uint32_t(ws.msdStride), ws.msdChunks, std::uint32_t(ws.msdMaxChunks), 1u);
+    msdLevels(s, ws, n, ws.digits, X, Y, vX, vY, finalIsY);
 }
 } // namespace
 
@@ -1596,20 +1608,22 @@ void SortWorkspace::allocate(std::size_t cap, bool withValues, hipStream_t s)
     digitTotal = static_cast<std::uint32_t *>(poolAlloc(kRadix * sizeof(std::uint32_t), s));
 }
 
-void SortWorkspace::allocateMsd(std::size_t cap, hipStream_t s)
+void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_t s)
 {
-    if (cap <= msdCapacity) return;
+    if (cap <= msdCapacity && parents <= msdParents) return;
     for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
                     (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
                     (void *)msdSubTotals, (void *)msdChunks})
         poolFree(p, s);
     msdCapacity = cap;
+    msdParents = parents;
     // segments hold more than kMsdChunk keys; a level's ranges are its
     // segments' tiles; chunks: below 2 per kMsdChunk keys of each run of
     // small buckets, plus one run per parent and per segment over all levels
+    // (level 1's parents: the whole array, or each segment of a segmented sort)
     msdMaxSegs = cap / kMsdChunk + 1;
     msdStride = ((cap / kMsdTile + msdMaxSegs + kUpRows) + 7) & ~std::size_t(7);
-    msdMaxChunks = 16 * (cap / kMsdChunk) + 64;
+    msdMaxChunks = 16 * (cap / kMsdChunk) + 64 + parents;
     auto alloc = [&](std::size_t bytes) { return poolAlloc(bytes, s); };
     msdCnt = static_cast<MsdCounters *>(alloc(sizeof(MsdCounters)));
     msdRoot = static_cast<MsdSeg *>(alloc(sizeof(MsdSeg)));
@@ -1640,7 +1654,7 @@ void SortWorkspace::release()
         poolFree(p, nullptr);
     msdCnt = nullptr; msdRoot = nullptr; msdSegs[0] = msdSegs[1] = nullptr; msdRanges[0] = msdRanges[1] = nullptr;
     msdFirst[0] = msdFirst[1] = nullptr; msdHist = msdOff = msdSubTotals = nullptr; msdChunks = nullptr;
-    msdCapacity = msdMaxSegs = msdStride = msdMaxChunks = 0;
+    msdCapacity = msdMaxSegs = msdStride = msdMaxChunks = msdParents = 0;
 }
 
 template <class T>
@@ -1834,10 +1848,27 @@ void SegmentedSort::plan(const std::vector<std::size_t> &counts, hipStream_t s)
     keysAlt_.ensure(total_ ? total_ : 1, 0, s);
     valsAlt_.ensure(total_ ? total_ : 1, 0, s);
     digits_.ensure(total_ ? total_ : 1, 0, s);
+    msdReady_ = total_ >= kMsdMinKeys;
+    std::vector<MsdSeg> parents;
+    if (msdReady_)
+    {
+        msd_.allocateMsd(total_, segments_, s);
+        parents_.ensure(segments_, 0, s);
+        parentCount_.ensure(1, 0, s);
+        segTotals_.ensure(segments_ * kRadix, 0, s);
+        for (std::size_t g = 0; g < segments_; ++g)
+            parents.push_back(MsdSeg{std::uint64_t(segBegin_[g]), std::uint32_t(counts[g]), 0, 0, 0});
+    }
     AT_HIP(hipStreamSynchronize(s));
     if (ranges_) AT_HIP(hipMemcpy(range_.p, rs.data(), ranges_ * sizeof(SegRange), hipMemcpyHostToDevice));
     AT_HIP(hipMemcpy(segRange_.p, first.data(), first.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice));
     AT_HIP(hipMemcpy(segBeginD_.p, begins.data(), begins.size() * sizeof(std::uint64_t), hipMemcpyHostToDevice));
+    if (msdReady_)
+    {
+        const std::uint32_t nseg = std::uint32_t(segments_);
+        AT_HIP(hipMemcpy(parents_.p, parents.data(), segments_ * sizeof(MsdSeg), hipMemcpyHostToDevice));
+        AT_HIP(hipMemcpy(parentCount_.p, &nseg, sizeof(nseg), hipMemcpyHostToDevice));
+    }
 }
 
 template <class T>
@@ -1846,23 +1877,48 @@ void SegmentedSort::keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParam
 {
     if (!ranges_) return;
     const dim3 rgrid{unsigned(ranges_)}, block{unsigned(kSortThreads)};
+    // MSD: keys into the alternate buffer, histograms of the top digit
+    const bool msd = g_msd && msdReady_;
+    Key *kgen = msd ? keysAlt_.p : keys;
+    const int histShift = msd ? 64 - kRadixBits : 0;
     {
         prof::Scope scope("keygen", s);
         switch (kind)
         {
         case KeyKind::Morton:
             hipLaunchKernelGGL((segKeygenKernel<T, KeyKind::Morton>), rgrid, block, 0, s, shared, coords, range_.p,
-                               keys, hist_.p);
+                               kgen, hist_.p, histShift);
             break;
         case KeyKind::Hilbert:
             hipLaunchKernelGGL((segKeygenKernel<T, KeyKind::Hilbert>), rgrid, block, 0, s, shared, coords, range_.p,
-                               keys, hist_.p);
+                               kgen, hist_.p, histShift);
             break;
         case KeyKind::Binary:
             hipLaunchKernelGGL((segKeygenKernel<T, KeyKind::Binary>), rgrid, block, 0, s, shared, coords, range_.p,
-                               keys, hist_.p);
+                               kgen, hist_.p, histShift);
             break;
         }
+    }
+    if (msd)
+    {
+        // level 1 per segment (slot values), then the segments are the parents
+        // of the MSD levels; the result ends in keys / vals
+        hipLaunchKernelGGL(msdResetKernel, dim3(1), dim3(1), 0, s, msd_.msdCnt, msd_.msdRoot, total_);
+        hipLaunchKernelGGL(segScanKernel, dim3(unsigned(segments_)), block, 0, s, hist_.p, segRange_.p,
+                           segBeginD_.p, off_.p, segTotals_.p);
+        {
+            prof::Scope scope("sort_pass", s);
+            hipLaunchKernelGGL((scatterKernel<kSegThreads, kSegItems, true, true, true, true>), dim3(unsigned(ranges_)),
+                               dim3(unsigned(kSegThreads)), 0, s, keysAlt_.p, keys, valsAlt_.p, vals, digits_.p, total_,
+                               rangeKeys_, off_.p, nullptr, ranges_, 64 - kRadixBits, 64 - 2 * kRadixBits, range_.p);
+        }
+        hipLaunchKernelGGL(msdPlanKernel, dim3(unsigned(segments_ < 1024 ? segments_ : 1024)), dim3(kSortThreads), 0,
+                           s, 1, parents_.p, parentCount_.p, segTotals_.p, msd_.msdCnt, 0, msd_.msdSegs[0],
+                           msd_.msdRanges[0], msd_.msdFirst[0], std::uint32_t(msd_.msdMaxSegs),
+                           std::uint32_t(msd_.msdStride), msd_.msdChunks, std::uint32_t(msd_.msdMaxChunks),
+                           std::uint32_t(segments_));
+        msdLevels(s, msd_, total_, digits_.p, keysAlt_.p, keys, valsAlt_.p, vals, true);
+        return;
     }
     Key *kA = keys, *kB = keysAlt_.p;
     std::uint32_t *vA = vals, *vB = valsAlt_.p;
@@ -1879,7 +1935,7 @@ void SegmentedSort::keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParam
             hipLaunchKernelGGL(segUpsweepDigitsKernel, rgrid, block, 0, s, digits_.p, range_.p, hist_.p);
         }
         hipLaunchKernelGGL(segScanKernel, dim3(unsigned(segments_)), block, 0, s, hist_.p, segRange_.p,
-                           segBeginD_.p, off_.p);
+                           segBeginD_.p, off_.p, nullptr);
         prof::Scope scope("sort_pass", s);
         const dim3 sgrid{unsigned(ranges_)}, sblock{unsigned(kSegThreads)};
 #define AT_SEG_SCATTER(I, D)                                                                                           \

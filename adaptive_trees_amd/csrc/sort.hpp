@@ -122,10 +122,12 @@ struct SortWorkspace
     std::uint32_t *msdOff = nullptr;       // [kRadix][msdStride]
     std::uint32_t *msdSubTotals = nullptr; // [msdMaxSegs][kRadix]
     MsdChunk *msdChunks = nullptr;
-    std::size_t msdCapacity = 0, msdMaxSegs = 0, msdStride = 0, msdMaxChunks = 0;
+    std::size_t msdCapacity = 0, msdMaxSegs = 0, msdStride = 0, msdMaxChunks = 0, msdParents = 0;
 
     void allocate(std::size_t cap, bool withValues, hipStream_t s);
-    void allocateMsd(std::size_t cap, hipStream_t s);
+    // MSD lists for sorts of up to cap keys whose level 1 has `parents`
+    // parent buckets besides the whole array (segmented sorts)
+    void allocateMsd(std::size_t cap, std::size_t parents, hipStream_t s);
     void release();
     ~SortWorkspace() { release(); }
 };
@@ -191,5 +193,11 @@ private:
     DevBuf<Key> keysAlt_;
     DevBuf<std::uint32_t> valsAlt_;
     DevBuf<std::uint8_t> digits_;
+    // MSD path (totals of at least 2^20 keys): the segments are level 1's parents
+    bool msdReady_ = false;
+    SortWorkspace msd_;
+    DevBuf<MsdSeg> parents_;              // [segments]
+    DevBuf<std::uint32_t> parentCount_;   // [1]
+    DevBuf<std::uint32_t> segTotals_;     // [segments][kRadix]
 };
 } // namespace at

@@ -4,7 +4,10 @@ A batch of uneven trees (0 to 150k particles, uniform or Plummer, some with
 escaped slots) is built with shared launches, then updated through a
 captured graph after the particles move; every member's P, Perm, K, N (and
 NF) and state must equal a plain Tree of the same particles driven by the
-same calls, and a few members are also checked against the oracle.
+same calls, and a few members are also checked against the oracle. Batches
+of more than 2^20 particles in total sort with the MSD levels (every member
+is a parent bucket of level 1); their members, each below 2^20, are
+compared with plain trees that sort with LSD passes.
 """
 import ctypes
 
@@ -18,14 +21,15 @@ from pairs import jitter, plummer, uniform
 
 pytestmark = pytest.mark.gpu
 SIZES = [0, 1, 7, 1000, 65, 20000, 150_000, 3, 50_000, 999, 4096, 2]
+SIZES_MSD = [600_000, 0, 300_001, 5, 250_000, 70_000, 2047, 2049, 100_000]
 NAMES = ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d")
 
 
-def members(seed, real):
+def members(seed, real, sizes=SIZES):
     rng = np.random.default_rng(seed)
     esc = np.finfo(real).max  # the escaped coordinate of the tree's Real type
     out = []
-    for i, n in enumerate(SIZES):
+    for i, n in enumerate(sizes):
         x, y, z = (uniform if i % 2 else plummer)(n, seed=seed + i) if n else (np.zeros(0),) * 3
         x, y, z = (np.array(a, dtype=real) for a in (x, y, z))
         if n > 10 and i % 3 == 0:  # some escaped slots
@@ -54,8 +58,20 @@ def same(member, tree, what, nf):
 @pytest.mark.parametrize("sfc,crit,bucket", [(MORTON, LEAF_COUNT, 64), (HILBERT, LEAF_COUNT, 16),
                                               (MORTON, NF_COUNT, 4096)])
 def test_batch_equals_individual_trees(real, sfc, crit, bucket):
+    run_batch(real, sfc, crit, bucket, SIZES)
+
+
+@pytest.mark.parametrize("real,sfc,crit,bucket", [(np.float32, MORTON, LEAF_COUNT, 64),
+                                                   (np.float64, HILBERT, LEAF_COUNT, 16),
+                                                   (np.float32, HILBERT, NF_COUNT, 64)])
+def test_msd_batch_equals_individual_trees(real, sfc, crit, bucket):
+    assert sum(SIZES_MSD) >= 1 << 20
+    run_batch(real, sfc, crit, bucket, SIZES_MSD)
+
+
+def run_batch(real, sfc, crit, bucket, SIZES):
     s = stream()
-    data = members(100 + bucket, real)
+    data = members(100 + bucket, real, SIZES)
     esc = np.finfo(real).max
     host = [[np.ascontiguousarray(a, dtype=real) for a in m] for m in data]
     dev = [[DeviceArray(a) for a in m] for m in host]
