@@ -432,6 +432,22 @@ __device__ inline std::uint64_t laneMatch(unsigned dg)
     }
     return (std::uint64_t(hi) << 32) | lo;
 }
+// The same, over the digit bits set in `vary` only (bits that are equal on
+// every lane need no ballot).
+__device__ inline std::uint64_t laneMatchVarying(unsigned dg, unsigned vary)
+{
+    std::uint32_t lo = ~0u, hi = ~0u;
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b)
+    {
+        if (!((vary >> b) & 1u)) continue; // uniform
+        const std::uint32_t mask = std::uint32_t(__builtin_amdgcn_sbfe(int(dg), b, 1));
+        const std::uint64_t ballot = __ballot(int(mask));
+        lo &= ~(std::uint32_t(ballot) ^ mask);
+        hi &= ~(std::uint32_t(ballot >> 32) ^ mask);
+    }
+    return (std::uint64_t(hi) << 32) | lo;
+}
 
 // Exclusive scan of the 256 per-digit values held by threads 0..255 of a
 // block of any size (all threads must call it).
@@ -1152,7 +1168,9 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 
     // one stable counting pass on the digit at `shift`; the result is in
     // the registers (striped order) and in stageKeys / stageVals
-    auto pass = [&](int shift) {
+    // `vary`: the digit bits that differ within the chunk (padding keys agree
+    // with the chunk on all other bits)
+    auto pass = [&](int shift, unsigned vary) {
         for (int i = tid; i < kW * kRadix; i += kLocThreads) (&waveCount[0][0])[i] = 0;
         __syncthreads();
         std::uint32_t rank[kLocItems], old[kLocItems];
@@ -1161,7 +1179,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         for (int j = 0; j < kLocItems; ++j)
         {
             const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
-            const std::uint64_t match = laneMatch(dg);
+            const std::uint64_t match = laneMatchVarying(dg, vary);
             const std::uint32_t lower = __popcll(match & ltMask);
             const bool lead = lower == 0;
             std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
@@ -1259,9 +1277,17 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int w = 0; w < kW; ++w) m |= waveMask[w];
         if (ch.bits < 64) m &= (Key(1) << ch.bits) - 1;
+        {
+            // padding: the chunk's constant bits with every varying bit set,
+            // the largest value in every digit (it still ranks last)
+            const Key pad = (k0 & ~m) | m;
+#pragma unroll
+            for (int j = 0; j < kLocItems; ++j)
+                if (first + std::uint32_t(j) * 64 >= ch.len) key[j] = pad;
+        }
         const Key loMask = m & ((Key(1) << kFixShift) - 1), hiMask = m & ~loMask;
         for (int shift = kFixShift; shift < 64; shift += kRadixBits)
-            if ((hiMask >> shift) & (kRadix - 1)) pass(shift);
+            if ((hiMask >> shift) & (kRadix - 1)) pass(shift, unsigned(hiMask >> shift) & (kRadix - 1));
         if (loMask)
         {
             // the chunk (sorted on its bits from kFixShift up) must sit in the stage
@@ -1337,9 +1363,9 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
             {
                 __syncthreads(); // the passes rewrite the stage
                 for (int shift = 0; shift < kFixShift; shift += kRadixBits)
-                    if ((loMask >> shift) & (kRadix - 1)) pass(shift);
+                    if ((loMask >> shift) & (kRadix - 1)) pass(shift, unsigned(loMask >> shift) & (kRadix - 1));
                 for (int shift = kFixShift; shift < 64; shift += kRadixBits)
-                    if ((hiMask >> shift) & (kRadix - 1)) pass(shift);
+                    if ((hiMask >> shift) & (kRadix - 1)) pass(shift, unsigned(hiMask >> shift) & (kRadix - 1));
             }
         }
 #pragma unroll
