@@ -32,37 +32,52 @@ constexpr std::uint64_t kFlagMask = std::uint64_t(3) << 62;
 constexpr int kThreads = 256;
 constexpr int kItems = kPassTile / kThreads;
 
-// Chained scan across tiles (one status word per tile). Called by one thread.
-__device__ void chainedScan(std::uint64_t *status, std::size_t tile, std::uint64_t tileSum, bool tileChg,
-                            std::uint64_t &exclusive, bool &exclChg)
+// Chained scan across tiles (one status word per tile: flag, change bit, sum).
+// A tile publishes its aggregate, looks back for its exclusive prefix and
+// publishes its inclusive prefix. The look-back is done by one whole wave:
+// each step reads the status words of the 64 nearest tiles not yet accounted
+// for (lane k: tile j - k). It stops
+// at the nearest tile with a prefix once every nearer one has published its
+// aggregate; positions before tile 0 read as an empty prefix. Called by all
+// 64 lanes of one wave.
+__device__ void chainedScanWave(std::uint64_t *status, std::size_t tile, std::uint64_t tileSum, bool tileChg,
+                                std::uint64_t &exclusive, bool &exclChg)
 {
+    const unsigned lane = __lane_id();
     std::uint64_t *me = status + tile;
     if (tile == 0)
     {
-        stRel64(me, kPre | (tileChg ? kChg : 0) | tileSum);
+        if (lane == 0) stRel64(me, kPre | (tileChg ? kChg : 0) | tileSum);
         exclusive = 0;
         exclChg = false;
         return;
     }
-    stRel64(me, kAgg | (tileChg ? kChg : 0) | tileSum);
+    if (lane == 0) stRel64(me, kAgg | (tileChg ? kChg : 0) | tileSum);
     std::uint64_t acc = 0;
     bool chg = false;
     std::size_t j = tile - 1;
     for (;;)
     {
-        const std::uint64_t v = ldAcq64(status + j);
+        const std::uint64_t v = j >= lane ? ldAcq64(status + (j - lane)) : kPre;
         const std::uint64_t f = v & kFlagMask;
-        if (!f)
+        const std::uint64_t notReady = __ballot(f == 0);
+        const std::uint64_t pre = __ballot(f == kPre);
+        const int stop = pre ? __ffsll((long long)pre) - 1 : 63; // nearest prefix, or the whole window
+        const std::uint64_t need = stop == 63 ? ~std::uint64_t(0) : (std::uint64_t(1) << (stop + 1)) - 1;
+        if (notReady & need)
         {
             __builtin_amdgcn_s_sleep(1);
             continue;
         }
-        acc += v & kSumMask;
-        chg |= (v & kChg) != 0;
-        if (f == kPre) break;
-        --j;
+        unsigned long long part = int(lane) <= stop ? (v & kSumMask) : 0ull;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+        acc += part;
+        chg |= __ballot(int(lane) <= stop && (v & kChg) != 0) != 0;
+        if (pre) break;
+        j -= 64;
     }
-    stRel64(me, kPre | ((chg || tileChg) ? kChg : 0) | (acc + tileSum));
+    if (lane == 0) stRel64(me, kPre | ((chg || tileChg) ? kChg : 0) | (acc + tileSum));
     exclusive = acc;
     exclChg = chg;
 }
@@ -223,13 +238,16 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
     std::uint64_t tileSum;
     bool tileChg;
     const std::uint64_t local = blockScan64(sum, chg, wsum, wflag, tileSum, tileChg);
-    if (tid == 0)
+    if (tid < 64)
     {
         std::uint64_t ex;
         bool exChg;
-        chainedScan(status, std::size_t(tile), tileSum, tileChg, ex, exChg);
-        prefixShared = ex;
-        prefixChgShared = (exChg || tileChg) ? 1 : 0;
+        chainedScanWave(status, std::size_t(tile), tileSum, tileChg, ex, exChg);
+        if (tid == 0)
+        {
+            prefixShared = ex;
+            prefixChgShared = (exChg || tileChg) ? 1 : 0;
+        }
     }
     __syncthreads();
     std::uint64_t run = prefixShared + local;

@@ -573,7 +573,6 @@ def main():
     key_ms, _ = per_launch("keygen")
     pass_ms, _ = per_launch("sort_pass")
     level_ms, level_n = per_launch("msd_level")
-    count_ms, _ = per_launch("leaf_counts")
     dir_ms, dir_n = profile_query("bin_direct")
     achieved = PAIR_BYTES * n / (dom_ms * 1e-3) / 1e9 if dom_n else 0.0
     traffic = None
@@ -652,7 +651,6 @@ def main():
             "sort_level1_pass": round(pass_ms, 4),
             "sort_levels_2_to_8": round(level_ms * level_n / max(dom_n, 1), 4),
             "sort_lds_chunks": round(dom_ms, 4),
-            "full_leaf_counts": round(count_ms, 4),
             "binary_construction": round(dir_ms / max(dir_n, 1) * 2, 4),
         },
         "build_roofline": {
