@@ -224,6 +224,7 @@ struct BinaryGeom
     int consumed[3];    // bits each axis gave to the prefix
     int order[3];       // axes of one cycle, most significant first
     int triples;        // ceil((63 - prefix) / 3)
+    int prefixAxis;     // the only axis of the prefix levels, or -1 if they mix axes
 };
 
 // Longest-side key of normalised coordinates v (binary_node.cuh:43-59):
@@ -234,6 +235,18 @@ __host__ __device__ inline Key binaryInterleave(Key f0, Key f1, Key f2, const Bi
     if (g.periodic)
     {
         Key hi = 0;
+        if (g.prefixAxis >= 0)
+        {
+            // the prefix is the top `prefix` bits of one axis
+            if (g.prefix > 0)
+            {
+                const Key lowBits = (Key(1) << g.prefix) - 1;
+                if (g.prefixAxis == 0) hi = (f0 >> (63 - g.prefix)) & lowBits, f0 <<= g.prefix;
+                else if (g.prefixAxis == 1) hi = (f1 >> (63 - g.prefix)) & lowBits, f1 <<= g.prefix;
+                else hi = (f2 >> (63 - g.prefix)) & lowBits, f2 <<= g.prefix;
+            }
+        }
+        else
         for (int d = 0; d < g.prefix; ++d)
         {
             const int a = g.axis[d];

@@ -91,6 +91,9 @@ BinaryGeom makeBinaryGeom(const at_box &box, unsigned maxDepth)
     g.triples = (int(kBinLevels) - g.prefix + 2) / 3;
     for (int a = 0; a < 3; ++a) g.consumed[a] = 0;
     for (int d = 0; d < g.prefix; ++d) ++g.consumed[g.axis[d]];
+    g.prefixAxis = g.prefix > 0 ? g.axis[0] : 0;
+    for (int d = 0; d < g.prefix; ++d)
+        if (g.axis[d] != g.axis[0]) g.prefixAxis = -1;
     return g;
 }
 

@@ -40,9 +40,11 @@ def test_octree_nfcount_plummer(sfc):
         p.compare("update %d" % f)
 
 
+# 4:1:1 splits x twice before the x/y/z cycle (one-axis prefix); 4:2:1 splits
+# x, x, y first (a prefix mixing axes)
+@pytest.mark.parametrize("box", [(0, 4, 0, 1, 0, 1), (0, 4, 0, 2, 0, 1)], ids=["4-1-1", "4-2-1"])
 @pytest.mark.parametrize("real", [np.float64, np.float32])
-def test_binary_anisotropic(real):
-    box = (0, 4, 0, 1, 0, 1)
+def test_binary_anisotropic(real, box):
     x, y, z = uniform(40000, 3, box)
     p = Pair(x, y, z, BINARY, real, max_particles_per_leaf=32, box=box, views=ALL_VIEWS)
     p.build()
