@@ -1485,8 +1485,20 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
 {
     const std::uint32_t maxSegs = std::uint32_t(ws.msdMaxSegs), maxRanges = std::uint32_t(ws.msdStride),
                         maxChunks = std::uint32_t(ws.msdMaxChunks);
+    // Outside graph capture the host may look at the number of segments of
+    // level 4 (one small read): when it is zero, levels 4..8 have no work and
+    // are not launched. A captured sort keeps the fixed launch sequence.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    AT_HIP(hipStreamIsCapturing(s, &capture));
     for (int level = 2; level <= kSortPasses; ++level)
     {
+        if (level == 4 && capture == hipStreamCaptureStatusNone)
+        {
+            AT_HIP(hipMemcpyAsync(ws.msdHostCount, &ws.msdCnt->segs[level & 1], sizeof(std::uint32_t),
+                                  hipMemcpyDeviceToHost, s));
+            AT_HIP(hipStreamSynchronize(s));
+            if (*ws.msdHostCount == 0) break;
+        }
         prof::Scope scope("msd_level", s);
         // levels past the third see work only for keys that share 24 high
         // bits in buckets of more than kMsdChunk (heavy duplicates): small
@@ -1652,6 +1664,7 @@ void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_
     msdMaxChunks = 16 * (cap / kMsdChunk) + 64 + parents;
     auto alloc = [&](std::size_t bytes) { return poolAlloc(bytes, s); };
     msdCnt = static_cast<MsdCounters *>(alloc(sizeof(MsdCounters)));
+    if (!msdHostCount) AT_HIP(hipHostMalloc(reinterpret_cast<void **>(&msdHostCount), sizeof(std::uint32_t)));
     msdRoot = static_cast<MsdSeg *>(alloc(sizeof(MsdSeg)));
     for (int k = 0; k < 2; ++k)
     {
@@ -1681,6 +1694,8 @@ void SortWorkspace::release()
     msdCnt = nullptr; msdRoot = nullptr; msdSegs[0] = msdSegs[1] = nullptr; msdRanges[0] = msdRanges[1] = nullptr;
     msdFirst[0] = msdFirst[1] = nullptr; msdHist = msdOff = msdSubTotals = nullptr; msdChunks = nullptr;
     msdCapacity = msdMaxSegs = msdStride = msdMaxChunks = msdParents = 0;
+    if (msdHostCount) (void)hipHostFree(msdHostCount);
+    msdHostCount = nullptr;
 }
 
 template <class T>

@@ -122,6 +122,7 @@ struct SortWorkspace
     std::uint32_t *msdOff = nullptr;       // [kRadix][msdStride]
     std::uint32_t *msdSubTotals = nullptr; // [msdMaxSegs][kRadix]
     MsdChunk *msdChunks = nullptr;
+    std::uint32_t *msdHostCount = nullptr; // pinned: a level's segment count read by the host
     std::size_t msdCapacity = 0, msdMaxSegs = 0, msdStride = 0, msdMaxChunks = 0, msdParents = 0;
 
     void allocate(std::size_t cap, bool withValues, hipStream_t s);
