@@ -164,7 +164,9 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     std::size_t i;
     if (vec)
     {
-        constexpr int kG = 2;
+        // FP32: 16 particles (12 float4 loads in flight) per thread and step,
+        // so a 4096-particle range is one step
+        constexpr int kG = sizeof(T) == 4 ? 4 : 2;
         constexpr std::size_t kGroupStep = std::size_t(kSortThreads) * 4;
         // block-uniform steps of kG * kGroupStep particles, then the tail
         std::size_t base = begin;
@@ -487,7 +489,8 @@ __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_
 // tile is ranked and written (register double buffering).
 //
 // kMsd: a partition level of the MSD sort. The block loops over the
-// *msd.rangeCount ranges of segRanges (grid-stride); a range's digit d starts
+// *msd.rangeCount ranges of segRanges (grid-stride), each of at most one tile
+// (the level plans ranges of THREADS * ITEMS keys); a range's digit d starts
 // at its segment's begin + the segment's keys of smaller digits (an exclusive
 // scan of msd.subTotals[seg]) + msd.off[d][range].
 struct MsdScatterArgs
@@ -507,6 +510,7 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
     const std::uint32_t *__restrict__ rangeOff, const std::uint32_t *__restrict__ digitTotal, std::size_t ranges,
     int shift, int nextShift, const SegRange *__restrict__ segRanges = nullptr, MsdScatterArgs msd = {})
 {
+    static_assert(!(kMsd && kPre), "an MSD level's tile loads its own keys");
     constexpr int kW = THREADS / 64;
     constexpr int kTile = ITEMS * THREADS;
     __shared__ Key stageKeys[kTile];                         // the tile in output order
@@ -523,42 +527,28 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
     auto body = [&](const std::size_t r) {
     std::size_t rBegin;
     std::uint32_t rLen, valueBase; // range-relative 32-bit offsets (n < 2^32)
-    if constexpr (kMsd)
+    std::uint32_t seg = 0;
+    if constexpr (kMsd || kSeg)
     {
         const SegRange d = segRanges[r];
         rBegin = d.begin;
         rLen = d.len;
         valueBase = d.localBase;
-        const std::uint32_t total = tid < kRadix ? msd.subTotals[std::size_t(d.seg) * kRadix + tid] : 0u;
-        const std::uint32_t digitBase = std::uint32_t(msd.segs[d.seg].begin) + digitExclusiveScan(total, scratch);
-        if (tid < kRadix) running[tid] = digitBase + msd.off[tid * msd.stride + r];
-    }
-    else if constexpr (kSeg)
-    {
-        const SegRange d = segRanges[r];
-        rBegin = d.begin;
-        rLen = d.len;
-        valueBase = d.localBase;
-        if (tid < kRadix) running[tid] = rangeOff[r * kRadix + tid];
+        seg = d.seg;
     }
     else
     {
-        const std::uint32_t total = tid < kRadix ? digitTotal[tid] : 0u;
-        const std::uint32_t digitBase = digitExclusiveScan(total, scratch);
-        if (tid < kRadix) running[tid] = digitBase + rangeOff[std::size_t(tid) * ranges + r];
         rBegin = r * rangeKeys;
         rLen = std::uint32_t((rBegin + rangeKeys < n ? rBegin + rangeKeys : n) - rBegin);
         valueBase = std::uint32_t(rBegin);
     }
     if (rLen == 0) return; // block-uniform
-    for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
-    __syncthreads();
     const Key *kr = kin + rBegin;
     const std::uint32_t *vr = vin + rBegin;
     const std::uint32_t last = rLen - 1;
     const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
-    constexpr bool kPreVals = kPre && kVals && !kIota;
-    Key preKey[kPre ? ITEMS : 1];
+    constexpr bool kPreVals = kVals && !kIota;
+    Key preKey[ITEMS];
     std::uint32_t preVal[kPreVals ? ITEMS : 1];
     // Loads and stores are kept free of per-lane branches (clamped indices
     // and selects; the partial last tile is a separate copy of the body) so
@@ -575,12 +565,40 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
             if constexpr (kPreVals) preVal[j] = vr[c];
         }
     };
-    if constexpr (kPre) prefetch(std::uint32_t(wave) * (ITEMS * 64) + lane);
+    // The range's digit totals and offsets are loaded first, then the first
+    // tile's keys; both are in flight together, and the wait for the former
+    // (loads complete in order) leaves the tile's loads outstanding during the
+    // digit scan. (An MSD level's tile loads its own keys after the scan: the
+    // preloaded form of that kernel needs more than 128 VGPRs and spills.)
+    if constexpr (kMsd)
+    {
+        const std::uint32_t total = tid < kRadix ? msd.subTotals[std::size_t(seg) * kRadix + tid] : 0u;
+        const std::uint32_t off = tid < kRadix ? msd.off[tid * msd.stride + r] : 0u;
+        const std::uint32_t digitBase = std::uint32_t(msd.segs[seg].begin) + digitExclusiveScan(total, scratch);
+        if (tid < kRadix) running[tid] = digitBase + off;
+    }
+    else if constexpr (kSeg)
+    {
+        const std::uint32_t off = tid < kRadix ? rangeOff[r * kRadix + tid] : 0u;
+        prefetch(std::uint32_t(wave) * (ITEMS * 64) + lane);
+        if (tid < kRadix) running[tid] = off;
+    }
+    else
+    {
+        const std::uint32_t total = tid < kRadix ? digitTotal[tid] : 0u;
+        const std::uint32_t off = tid < kRadix ? rangeOff[std::size_t(tid) * ranges + r] : 0u;
+        prefetch(std::uint32_t(wave) * (ITEMS * 64) + lane);
+        const std::uint32_t digitBase = digitExclusiveScan(total, scratch);
+        if (tid < kRadix) running[tid] = digitBase + off;
+    }
+    for (int i = tid; i < kW * kRadix; i += THREADS) (&waveCount[0][0])[i] = 0;
+    __syncthreads();
 
     // One tile: 5 barriers (ranked, wave totals, offsets, staged, written).
     // The counters are cleared during the write-out, ready for the next tile.
-    auto tile = [&](std::uint32_t base, auto fullTag) {
+    auto tile = [&](std::uint32_t base, auto fullTag, auto preTag) {
         constexpr bool kFull = decltype(fullTag)::value;
+        constexpr bool kFromPre = decltype(preTag)::value; // keys already in preKey
         constexpr bool kLoadVals = kVals && !kIota;
         // ---- load keys (wave-striped: item j of lane l is element j*64 + l of the wave's chunk)
         Key key[ITEMS];
@@ -595,6 +613,15 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
                 if constexpr (kPreVals) val[j] = preVal[j];
             }
             if (kFull) prefetch(waveFirst + kTile); // the next tile, or clamped reloads of the last key
+        }
+        else if constexpr (kFromPre)
+        {
+#pragma unroll
+            for (int j = 0; j < ITEMS; ++j)
+            {
+                key[j] = preKey[j];
+                if constexpr (kPreVals) val[j] = preVal[j];
+            }
         }
         else
         {
@@ -706,8 +733,33 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
     };
 
     std::uint32_t base = 0;
-    for (; rLen - base >= std::uint32_t(kTile); base += kTile) tile(base, std::true_type{});
-    if (base < rLen) tile(base, std::false_type{});
+    if constexpr (kMsd) // one tile
+    {
+        if (rLen >= std::uint32_t(kTile)) tile(0u, std::true_type{}, std::false_type{});
+        else tile(0u, std::false_type{}, std::false_type{});
+    }
+    else if constexpr (kPre)
+    {
+        for (; rLen - base >= std::uint32_t(kTile); base += kTile) tile(base, std::true_type{}, std::true_type{});
+        if (base < rLen) tile(base, std::false_type{}, std::true_type{});
+    }
+    else
+    {
+        // the first tile from the registers loaded before the prologue (a
+        // separate copy, so that they are dead in the loop)
+        if (rLen >= std::uint32_t(kTile))
+        {
+            tile(0u, std::true_type{}, std::true_type{});
+            base = kTile;
+        }
+        else
+        {
+            tile(0u, std::false_type{}, std::true_type{});
+            base = rLen;
+        }
+        for (; rLen - base >= std::uint32_t(kTile); base += kTile) tile(base, std::true_type{}, std::false_type{});
+        if (base < rLen) tile(base, std::false_type{}, std::false_type{});
+    }
     }; // body: every path ends with a barrier, so the next range may reuse the shared state
 
     if constexpr (kMsd)
@@ -960,14 +1012,26 @@ __device__ inline void countDigitBytes(const std::uint8_t *__restrict__ digits, 
     std::size_t e16 = e & ~std::size_t(15);
     if (e16 < a16) e16 = a16;
     const uint4 *quads = reinterpret_cast<const uint4 *>(digits);
-    for (std::size_t i = a16 + 16 * std::size_t(lane); i < e16; i += 16 * 64)
+    constexpr int kQuads = 4; // 16-byte loads in flight per lane (a 4096-key range in one step)
+    for (std::size_t i0 = a16 + 16 * std::size_t(lane); i0 < e16; i0 += kQuads * 16 * 64)
     {
-        const uint4 q = quads[i >> 4];
-        const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        uint4 q[kQuads];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int u = 0; u < kQuads; ++u)
+        {
+            const std::size_t i = i0 + std::size_t(u) * 16 * 64;
+            q[u] = i < e16 ? quads[i >> 4] : make_uint4(0, 0, 0, 0);
+        }
 #pragma unroll
-            for (int b = 0; b < 4; ++b) countDigit(h, (w[k] >> (8 * b)) & 0xffu);
+        for (int u = 0; u < kQuads; ++u)
+        {
+            if (i0 + std::size_t(u) * 16 * 64 >= e16) break; // a lane stops at its first quad past the end
+            const std::uint32_t w[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) countDigit(h, (w[k] >> (8 * b)) & 0xffu);
+        }
     }
     if (e16 + lane < e) countDigit(h, digits[e16 + lane]);
 }
@@ -1134,13 +1198,19 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
 // registers; a stable counting pass per 8-bit digit (ranking as in
 // scatterKernel) is run for each digit at or above bit kFixShift that is not
 // the same for all of the chunk's keys. Runs of keys that then agree from bit
-// kFixShift up are finished by an insertion sort on the lower bits, one
-// thread per run, when no run is longer than kFixRun; otherwise the varying
-// low digits and then the high digits get counting passes (plain LSD order).
+// kFixShift up are finished on the lower bits: runs of up to kFixRun keys by
+// an insertion sort (one thread per run), runs of up to kWaveRun keys by a
+// rank sort of one wave per run. If a run is longer, the chunk takes a
+// counting pass on the varying digit just below kFixShift, the high digits
+// again (LSD order) and the same fix-up from bit kFixShift2; if that fails
+// too, every varying digit gets a counting pass (plain LSD order).
 // Chunks with bits == 0 are already in order and are only copied when they
 // lie in the other buffer.
 constexpr std::uint32_t kFixRun = 16;
-constexpr int kFixShift = 24; // counting passes on the digits above; runs equal above it are fixed up
+constexpr std::uint32_t kWaveRun = 64;
+constexpr int kFixShift = 24;  // counting passes on the digits above; runs equal above it are fixed up
+constexpr int kFixShift2 = 16; // the same after one more pass
+constexpr std::uint32_t kMaxWaveRuns = kMsdChunk / (kFixRun + 1) + 1;
 
 __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCounters *cnt,
                                                                      const MsdChunk *__restrict__ chunks, Key *X,
@@ -1154,7 +1224,9 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
     __shared__ std::uint32_t sink[kW][64];
     __shared__ std::uint32_t scratch[kRadix / 64];
     __shared__ Key waveMask[kW];
-    __shared__ std::uint32_t longRun;
+    __shared__ std::uint32_t longRun;  // a run longer than kWaveRun was seen
+    __shared__ std::uint32_t waveRuns; // runs of kFixRun+1 .. kWaveRun keys: (start << 8) | length
+    __shared__ std::uint32_t waveRun[kMaxWaveRuns];
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
@@ -1170,7 +1242,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
     // the registers (striped order) and in stageKeys / stageVals
     // `vary`: the digit bits that differ within the chunk (padding keys agree
     // with the chunk on all other bits)
-    auto pass = [&](int shift, unsigned vary) {
+    auto pass = [&](int shift, unsigned vary) __attribute__((always_inline)) {
         for (int i = tid; i < kW * kRadix; i += kLocThreads) (&waveCount[0][0])[i] = 0;
         __syncthreads();
         std::uint32_t rank[kLocItems], old[kLocItems];
@@ -1271,7 +1343,6 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) m |= Key(__shfl_xor((unsigned long long)m, o, 64));
         if (lane == 0) waveMask[wave] = m;
-        if (tid == 0) longRun = 0;
         __syncthreads();
         m = 0;
 #pragma unroll
@@ -1285,27 +1356,16 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
             for (int j = 0; j < kLocItems; ++j)
                 if (first + std::uint32_t(j) * 64 >= ch.len) key[j] = pad;
         }
-        const Key loMask = m & ((Key(1) << kFixShift) - 1), hiMask = m & ~loMask;
-        for (int shift = kFixShift; shift < 64; shift += kRadixBits)
-            if ((hiMask >> shift) & (kRadix - 1)) pass(shift, unsigned(hiMask >> shift) & (kRadix - 1));
-        if (loMask)
-        {
-            // the chunk (sorted on its bits from kFixShift up) must sit in the stage
-            if (!hiMask)
-            {
-#pragma unroll
-                for (int j = 0; j < kLocItems; ++j)
-                {
-                    stageKeys[first + std::uint32_t(j) * 64] = key[j];
-                    stageVals[first + std::uint32_t(j) * 64] = val[j];
-                }
-                __syncthreads();
-            }
-            // Runs of keys equal from bit kFixShift up, from the stage: thread t
-            // looks at positions t + k*kLocThreads (independent, conflict-free
-            // reads); only run starts walk their run.
-            auto high = [&](std::uint32_t q) { return stageKeys[q] >> kFixShift; };
-            std::uint32_t starts = 0; // bit k: position t + k*kLocThreads starts a run of two or more
+        // Fix-up of the runs of keys equal from bit `shift` up, with the chunk
+        // sorted on those bits in the stage. Returns false, changing nothing,
+        // when a run is longer than kWaveRun.
+        auto fixRuns = [&](int shift) __attribute__((always_inline)) -> bool {
+            if (tid == 0) longRun = 0, waveRuns = 0;
+            __syncthreads();
+            auto high = [&](std::uint32_t q) { return stageKeys[q] >> shift; };
+            // thread t looks at positions t + k*kLocThreads (independent,
+            // conflict-free reads); only run starts walk their run
+            std::uint32_t starts = 0; // bit k: position t + k*kLocThreads starts a run of 2..kFixRun
 #pragma unroll 1
             for (int k = 0; k < kLocItems; ++k)
             {
@@ -1317,56 +1377,117 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                     const bool dupPrev = p0 > 0 && high(p0 - 1) == h;
                     if (dupNext && !dupPrev)
                     {
-                        starts |= 1u << k;
                         std::uint32_t e = p0 + 2;
-                        while (e < ch.len && e - p0 <= kFixRun && high(e) == h) ++e;
-                        if (e - p0 > kFixRun) longRun = 1;
+                        while (e < ch.len && e - p0 <= kWaveRun && high(e) == h) ++e;
+                        const std::uint32_t len = e - p0;
+                        if (len > kWaveRun) longRun = 1;
+                        else if (len > kFixRun) waveRun[atomicAdd(&waveRuns, 1u)] = (p0 << 8) | len;
+                        else starts |= 1u << k;
                     }
                 }
             }
             __syncthreads();
-            const bool fix = !longRun;
-            if (fix)
-            {
-                // stable insertion sort of each run on the bits below kFixShift
+            if (longRun) return false; // block-uniform
+            // stable insertion sort of each short run (keys of a run agree above
+            // `shift`, so whole keys compare as their low bits)
 #pragma unroll 1
-                for (int k = 0; k < kLocItems; ++k)
+            for (int k = 0; k < kLocItems; ++k)
+            {
+                if (!((starts >> k) & 1u)) continue;
+                const std::uint32_t p0 = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
+                const Key h = high(p0);
+                for (std::uint32_t i = p0 + 1; i < ch.len && high(i) == h; ++i)
                 {
-                    if (!((starts >> k) & 1u)) continue;
-                    const std::uint32_t p0 = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
-                    const Key h = high(p0);
-                    constexpr Key low = (Key(1) << kFixShift) - 1;
-                    for (std::uint32_t i = p0 + 1; i < ch.len && high(i) == h; ++i)
+                    const Key kk = stageKeys[i];
+                    const std::uint32_t vv = stageVals[i];
+                    std::uint32_t q = i;
+                    while (q > p0 && stageKeys[q - 1] > kk)
                     {
-                        const Key kk = stageKeys[i];
-                        const std::uint32_t vv = stageVals[i];
-                        std::uint32_t q = i;
-                        while (q > p0 && (stageKeys[q - 1] & low) > (kk & low))
-                        {
-                            stageKeys[q] = stageKeys[q - 1];
-                            stageVals[q] = stageVals[q - 1];
-                            --q;
-                        }
-                        stageKeys[q] = kk;
-                        stageVals[q] = vv;
+                        stageKeys[q] = stageKeys[q - 1];
+                        stageVals[q] = stageVals[q - 1];
+                        --q;
                     }
+                    stageKeys[q] = kk;
+                    stageVals[q] = vv;
                 }
-                __syncthreads();
             }
+            // longer runs: one wave each; lane i ranks the run's key i against
+            // all keys of the run (ties by position: stable)
+            const std::uint32_t nw = waveRuns;
+#pragma unroll 1
+            for (std::uint32_t r = std::uint32_t(wave); r < nw; r += kW)
+            {
+                const std::uint32_t e = waveRun[r], p0 = e >> 8, len = e & 0xffu;
+                const bool in = lane < len;
+                const Key kk = in ? stageKeys[p0 + lane] : Key(0);
+                const std::uint32_t vv = in ? stageVals[p0 + lane] : 0u;
+                std::uint32_t rank = 0;
+#pragma unroll 1
+                for (std::uint32_t j = 0; j < len; ++j)
+                {
+                    // readlane returns int: widen through unsigned (no sign extension)
+                    const Key kj = Key(unsigned(__builtin_amdgcn_readlane(int(unsigned(kk)), int(j)))) |
+                                   (Key(unsigned(__builtin_amdgcn_readlane(int(unsigned(kk >> 32)), int(j)))) << 32);
+                    rank += (kj < kk || (kj == kk && j < lane)) ? 1u : 0u;
+                }
+                if (in)
+                {
+                    stageKeys[p0 + rank] = kk;
+                    stageVals[p0 + rank] = vv;
+                }
+            }
+            __syncthreads();
+            return true;
+        };
+        auto toStage = [&]() __attribute__((always_inline)) {
+#pragma unroll
+            for (int j = 0; j < kLocItems; ++j)
+            {
+                stageKeys[first + std::uint32_t(j) * 64] = key[j];
+                stageVals[first + std::uint32_t(j) * 64] = val[j];
+            }
+            __syncthreads();
+        };
+        auto fromStage = [&]() __attribute__((always_inline)) {
 #pragma unroll
             for (int j = 0; j < kLocItems; ++j)
             {
                 key[j] = stageKeys[first + std::uint32_t(j) * 64];
                 val[j] = stageVals[first + std::uint32_t(j) * 64];
             }
-            if (!fix)
+        };
+        // counting passes on the varying digits in [lo, 64), lowest first
+        auto passesFrom = [&](int lo, Key mask) __attribute__((always_inline)) {
+            bool any = false;
+            for (int shift = lo; shift < 64; shift += kRadixBits)
+                if ((mask >> shift) & (kRadix - 1))
+                {
+                    pass(shift, unsigned(mask >> shift) & (kRadix - 1));
+                    any = true;
+                }
+            return any;
+        };
+        const Key hiMask = m & ~((Key(1) << kFixShift) - 1);
+        // passes leave the result in the registers and the stage
+        if (!passesFrom(kFixShift, hiMask)) toStage();
+        if (m & ((Key(1) << kFixShift) - 1))
+        {
+            bool done = fixRuns(kFixShift);
+            if (!done)
             {
+                // the digit below kFixShift, then the high digits again
+                const Key hiMask2 = m & ~((Key(1) << kFixShift2) - 1);
                 __syncthreads(); // the passes rewrite the stage
-                for (int shift = 0; shift < kFixShift; shift += kRadixBits)
-                    if ((loMask >> shift) & (kRadix - 1)) pass(shift, unsigned(loMask >> shift) & (kRadix - 1));
-                for (int shift = kFixShift; shift < 64; shift += kRadixBits)
-                    if ((hiMask >> shift) & (kRadix - 1)) pass(shift, unsigned(hiMask >> shift) & (kRadix - 1));
+                if (!passesFrom(kFixShift2, hiMask2)) toStage();
+                if (m & ((Key(1) << kFixShift2) - 1)) done = fixRuns(kFixShift2);
+                else done = true;
+                if (!done)
+                {
+                    __syncthreads();
+                    passesFrom(0, m);
+                }
             }
+            if (done) fromStage();
         }
 #pragma unroll
         for (int j = 0; j < kLocItems; ++j)

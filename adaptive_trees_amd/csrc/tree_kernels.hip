@@ -35,11 +35,14 @@ constexpr int kItems = kPassTile / kThreads;
 // Chained scan across tiles (one status word per tile: flag, change bit, sum).
 // A tile publishes its aggregate, looks back for its exclusive prefix and
 // publishes its inclusive prefix. The look-back is done by one whole wave:
-// each step reads the status words of the 64 nearest tiles not yet accounted
-// for (lane k: tile j - k). It stops
-// at the nearest tile with a prefix once every nearer one has published its
-// aggregate; positions before tile 0 read as an empty prefix. Called by all
-// 64 lanes of one wave.
+// each step loads the status words of the kLookWindows * 64 nearest tiles not
+// yet accounted for (window w, lane k: tile j - 64 w - k), all loads in
+// flight at once, then consumes the windows nearest first. It stops at the
+// nearest tile with a prefix once every nearer one has published its
+// aggregate; a window with a tile not yet published ends the step (the
+// windows before it are kept). Positions before tile 0 read as an empty
+// prefix. Called by all 64 lanes of one wave.
+constexpr int kLookWindows = 4;
 __device__ void chainedScanWave(std::uint64_t *status, std::size_t tile, std::uint64_t tileSum, bool tileChg,
                                 std::uint64_t &exclusive, bool &exclChg)
 {
@@ -58,24 +61,41 @@ __device__ void chainedScanWave(std::uint64_t *status, std::size_t tile, std::ui
     std::size_t j = tile - 1;
     for (;;)
     {
-        const std::uint64_t v = j >= lane ? ldAcq64(status + (j - lane)) : kPre;
-        const std::uint64_t f = v & kFlagMask;
-        const std::uint64_t notReady = __ballot(f == 0);
-        const std::uint64_t pre = __ballot(f == kPre);
-        const int stop = pre ? __ffsll((long long)pre) - 1 : 63; // nearest prefix, or the whole window
-        const std::uint64_t need = stop == 63 ? ~std::uint64_t(0) : (std::uint64_t(1) << (stop + 1)) - 1;
-        if (notReady & need)
-        {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        unsigned long long part = int(lane) <= stop ? (v & kSumMask) : 0ull;
+        std::uint64_t v[kLookWindows];
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
-        acc += part;
-        chg |= __ballot(int(lane) <= stop && (v & kChg) != 0) != 0;
-        if (pre) break;
-        j -= 64;
+        for (int w = 0; w < kLookWindows; ++w)
+        {
+            const std::size_t back = std::size_t(w) * 64 + lane;
+            v[w] = j >= back ? ldAcq64(status + (j - back)) : kPre;
+        }
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int w = 0; w < kLookWindows; ++w)
+        {
+            const std::uint64_t f = v[w] & kFlagMask;
+            const std::uint64_t notReady = __ballot(f == 0);
+            const std::uint64_t pre = __ballot(f == kPre);
+            const int stop = pre ? __ffsll((long long)pre) - 1 : 63; // nearest prefix, or the whole window
+            const std::uint64_t need = stop == 63 ? ~std::uint64_t(0) : (std::uint64_t(1) << (stop + 1)) - 1;
+            if (notReady & need) break; // wave-uniform
+            unsigned long long part = int(lane) <= stop ? (v[w] & kSumMask) : 0ull;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+            acc += part;
+            chg |= __ballot(int(lane) <= stop && (v[w] & kChg) != 0) != 0;
+            ++used;
+            if (pre)
+            {
+                done = true;
+                break;
+            }
+        }
+        if (done) break;
+        // a consumed window without a prefix lies wholly above tile 0 (tile 0
+        // always holds one), so j stays >= 0
+        j -= std::size_t(used) * 64;
+        if (used < kLookWindows) __builtin_amdgcn_s_sleep(1);
     }
     if (lane == 0) stRel64(me, kPre | ((chg || tileChg) ? kChg : 0) | (acc + tileSum));
     exclusive = acc;
@@ -210,6 +230,12 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
     __shared__ std::uint64_t prefixShared;
     __shared__ int prefixChgShared;
     const int tid = threadIdx.x;
+    // A pass after convergence decides nothing and its offsets are never
+    // read (process and refresh return on the flag; the commit clears the
+    // status words and ticket). Only update passes reach this: a build stops
+    // at the pass that converges. A block that reads the flag set by the last
+    // tile of this same launch would have drawn a ticket past that tile.
+    if (st->converged) return;
     if (tid == 0) tileShared = atomicAdd(ticket, 1u);
     __syncthreads();
     const long tile = tileShared;
@@ -765,10 +791,23 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
     std::int8_t *tab[2] = {reinterpret_cast<std::int8_t *>(pw + spanP),
                            reinterpret_cast<std::int8_t *>(pw + spanP) + spanD};
     std::int8_t *dl = tab[1] + spanD;
-    for (int s = tid; s < spanP; s += kThreads)
+    // eight loads in flight per thread before the first LDS store
+    for (int s0 = 0; s0 < spanP; s0 += 8 * kThreads)
     {
-        const long j = origin + s;
-        pw[s] = (j >= 0 && j < nact) ? P[j] : Key(0);
+        Key v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            const int s = s0 + k * kThreads + tid;
+            const long j = origin + s;
+            v[k] = (s < spanP && j >= 0 && j < nact) ? P[j] : Key(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            const int s = s0 + k * kThreads + tid;
+            if (s < spanP) pw[s] = v[k];
+        }
     }
     __syncthreads();
     for (int s = tid; s < spanD; s += kThreads)
@@ -1242,7 +1281,22 @@ __global__ __launch_bounds__(kThreads) void tileBoundsKernel(TreeDev t)
     const Key *K = t.K[st->activeBuffer];
     const std::size_t s = std::size_t(blockIdx.x) * kBoundTile;
     const std::size_t len = t.n - s < kBoundTile ? t.n - s : kBoundTile;
-    for (std::size_t i = threadIdx.x; i < len; i += kThreads) tp[i] = t.P[s + i];
+    {
+        constexpr int kPer = kBoundTile / kThreads;
+        Key v[kPer]; // all loads in flight before the first LDS store
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            const std::size_t i = threadIdx.x + std::size_t(k) * kThreads;
+            v[k] = i < len ? t.P[s + i] : Key(0);
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            const std::size_t i = threadIdx.x + std::size_t(k) * kThreads;
+            if (i < len) tp[i] = v[k];
+        }
+    }
     __syncthreads();
     std::uint32_t *lb = reinterpret_cast<std::uint32_t *>(t.offs);
     const std::uint32_t i1 = t.tileFirst[blockIdx.x + 1];

@@ -4,8 +4,9 @@ Sizes below 2^20 take the LSD passes; larger ones the MSD levels (when
 sort_msd is on) with the LDS chunk sort. Distributions cover what the level
 planning and the chunk sort branch on: uniform 64-bit keys, 63-bit keys with
 the invalid key ~0 mixed in, a handful of distinct values, long duplicate
-runs, clustered high bits, keys that differ only in their low 32 bits
-(runs for the chunk fix-up, short and long), and sizes that are not
+runs, clustered high bits, keys whose high bits repeat in runs for the chunk
+fix-up (short runs for the insertion sort, runs of 17..64 for the wave rank
+sort, and longer ones for each fallback), and sizes that are not
 multiples of a tile. Results must be identical (keys and permutation).
 """
 import zlib
@@ -38,10 +39,25 @@ def keys_of(kind, n, rng):
         # bits 24.. take about n/3 values: short runs that agree above bit 24
         hi = rng.integers(0, n // 3 + 1, n, dtype=np.uint64) << np.uint64(24)
         return hi | rng.integers(0, 2**24, n, dtype=np.uint64)
+    if kind == "wave_runs":
+        # bits 24.. take about n/30 values: runs of 17..64 keys that agree above
+        # bit 24 (the chunk sort ranks them one wave per run)
+        hi = rng.integers(0, n // 30 + 1, n, dtype=np.uint64) << np.uint64(24)
+        return hi | rng.integers(0, 2**24, n, dtype=np.uint64)
+    if kind == "mid_runs":
+        # runs of ~200 that agree above bit 24 but are short above bit 16 (the
+        # chunk sort's first fallback)
+        hi = rng.integers(0, n // 200 + 1, n, dtype=np.uint64) << np.uint64(24)
+        return hi | rng.integers(0, 2**24, n, dtype=np.uint64)
     if kind == "long_runs":
         # bits 24.. take 2^14 values: runs longer than the chunk fix-up handles
         hi = rng.integers(0, 64, n, dtype=np.uint64) << np.uint64(40)
         return hi | rng.integers(0, 2**32, n, dtype=np.uint64)
+    if kind == "low_bits":
+        # chunks whose keys differ only below bit 16: every fix-up fails and
+        # the chunk takes plain LSD passes
+        hi = rng.integers(0, 64, n, dtype=np.uint64) << np.uint64(40)
+        return hi | rng.integers(0, 2**16, n, dtype=np.uint64)
     raise ValueError(kind)
 
 
@@ -64,7 +80,7 @@ def msd(request):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["uniform", "invalid_mix", "few", "dup_runs", "clustered", "short_runs",
-                                  "long_runs"])
+                                  "wave_runs", "mid_runs", "long_runs", "low_bits"])
 @pytest.mark.parametrize("n", [1000, 300_001, (1 << 20) + 777, 3_000_017])
 def test_sort_pairs_matches_stable_argsort(msd, kind, n):
     print("sort", msd, kind, n, flush=True)
