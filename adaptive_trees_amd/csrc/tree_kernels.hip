@@ -768,6 +768,7 @@ __device__ inline Key bitsBetween(int lo, int hi) // bits lo..hi inclusive, 0 <=
 }
 
 constexpr int kQ = kPassTile; // particles per tile of the binary construction
+constexpr int kWindowBucket = 64; // buckets up to this take the register windows, larger a sparse table
 
 // Per tile of kQ particles: delta(q) via a sparse table over the D window in
 // LDS, the number of boundaries of each interval q, and the tile's total.
@@ -787,10 +788,11 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
     const long origin = q0 - 1 - B; // particle index of pw[0] and tab[.][0]
     const int spanP = kQ + 2 * B + 2;
     const int spanD = kQ + B + 1;
+    const int rowD = (spanD + 7) & ~7; // 8-byte aligned rows
     Key *pw = reinterpret_cast<Key *>(lds);
     std::int8_t *tab[2] = {reinterpret_cast<std::int8_t *>(pw + spanP),
-                           reinterpret_cast<std::int8_t *>(pw + spanP) + spanD};
-    std::int8_t *dl = tab[1] + spanD;
+                           reinterpret_cast<std::int8_t *>(pw + spanP) + rowD};
+    std::int8_t *dl = tab[1] + rowD;
     // eight loads in flight per thread before the first LDS store
     for (int s0 = 0; s0 < spanP; s0 += 8 * kThreads)
     {
@@ -822,37 +824,88 @@ __global__ __launch_bounds__(kThreads) void binCountKernel(const Key *P, const D
         tab[0][s] = std::int8_t(D);
     }
     __syncthreads();
-    int lg = 0;
-    while ((2 << lg) <= B + 1) ++lg; // floor(log2(bucket + 1))
-    int cur = 0;
-    for (int t = 0; t < lg; ++t)
-    {
-        const int step = 1 << t;
-        for (int s = tid; s < spanD; s += kThreads)
-        {
-            std::int8_t v = tab[cur][s];
-            if (s + step < spanD)
-            {
-                const std::int8_t w = tab[cur][s + step];
-                v = w > v ? w : v;
-            }
-            tab[cur ^ 1][s] = v;
-        }
-        __syncthreads();
-        cur ^= 1;
-    }
-    const int pw2 = 1 << lg;
     // dl[u] = delta(q0 - 1 + u): windows [q-B, q] are tab indices [u, u+B]
-    for (int u = tid; u <= kQ; u += kThreads)
+    if (B <= kWindowBucket)
     {
-        const long q = q0 - 1 + u;
-        int d = -1;
-        if (q >= 0 && q < nact)
+        // Thread t: the nine windows u = 8t .. 8t+8 in registers. They share
+        // tab[8t+8 .. 8t+B]; window i adds a suffix of tab[8t .. 8t+7] on the
+        // left and a prefix of tab[8t+B+1 .. 8t+B+8] on the right.
+        static_assert(kItems == 8, "eight windows per thread");
+        const int u0 = tid * kItems;
+        const std::int8_t *t0 = tab[0];
+        int win[kItems + 1];
+        if (B >= kItems)
         {
-            const int a = tab[cur][u], b = tab[cur][u + B - pw2 + 1];
-            d = a > b ? a : b;
+            int common = -128;
+            for (int o = kItems; o <= B; ++o) common = max(common, int(t0[u0 + o]));
+            int sl[kItems + 1];
+            sl[kItems] = -128;
+#pragma unroll
+            for (int i = kItems - 1; i >= 0; --i) sl[i] = max(sl[i + 1], int(t0[u0 + i]));
+            int pr = -128;
+            win[0] = max(sl[0], common);
+#pragma unroll
+            for (int i = 1; i <= kItems; ++i)
+            {
+                pr = max(pr, int(t0[u0 + B + i]));
+                win[i] = max(max(sl[i], common), pr);
+            }
         }
-        dl[u] = std::int8_t(d);
+        else
+        {
+#pragma unroll
+            for (int i = 0; i <= kItems; ++i)
+            {
+                int w = -128;
+                for (int o = 0; o <= B; ++o) w = max(w, int(t0[u0 + i + o]));
+                win[i] = w;
+            }
+        }
+        std::uint64_t packed = 0;
+#pragma unroll
+        for (int i = 0; i <= kItems; ++i)
+        {
+            const long q = q0 - 1 + u0 + i;
+            const int d = (q >= 0 && q < nact) ? win[i] : -1;
+            if (i < kItems) packed |= std::uint64_t(std::uint8_t(d)) << (8 * i);
+            else if (tid == kThreads - 1) dl[kQ] = std::int8_t(d);
+        }
+        *reinterpret_cast<std::uint64_t *>(dl + u0) = packed;
+    }
+    else
+    {
+        // sparse table: after round t, tab[cur][s] = max of tab[0][s .. s + 2^t - 1]
+        int lg = 0;
+        while ((2 << lg) <= B + 1) ++lg; // floor(log2(bucket + 1))
+        int cur = 0;
+        for (int t = 0; t < lg; ++t)
+        {
+            const int step = 1 << t;
+            for (int s = tid; s < spanD; s += kThreads)
+            {
+                std::int8_t v = tab[cur][s];
+                if (s + step < spanD)
+                {
+                    const std::int8_t w = tab[cur][s + step];
+                    v = w > v ? w : v;
+                }
+                tab[cur ^ 1][s] = v;
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        const int pw2 = 1 << lg;
+        for (int u = tid; u <= kQ; u += kThreads)
+        {
+            const long q = q0 - 1 + u;
+            int d = -1;
+            if (q >= 0 && q < nact)
+            {
+                const int a = tab[cur][u], b = tab[cur][u + B - pw2 + 1];
+                d = a > b ? a : b;
+            }
+            dl[u] = std::int8_t(d);
+        }
     }
     __syncthreads();
     std::uint32_t sum = 0;
@@ -1340,7 +1393,8 @@ void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDire
     prof::Scope scope("bin_direct", s);
     const unsigned tiles = unsigned(t.n / kQ + 1);
     const int B = int(p.bucket);
-    const std::size_t lds = std::size_t(kQ + 2 * B + 2) * sizeof(Key) + 2 * std::size_t(kQ + B + 1) + kQ + 1;
+    const std::size_t rowD = (std::size_t(kQ + B + 1) + 7) & ~std::size_t(7);
+    const std::size_t lds = std::size_t(kQ + 2 * B + 2) * sizeof(Key) + 2 * rowD + kQ + 8;
     hipLaunchKernelGGL(binCountKernel, dim3(tiles), dim3(kThreads), lds, s, t.P, t.st, p.bucket, p.maxDepth, d.delta,
                        d.cnt, d.tileSum);
     hipLaunchKernelGGL(binScanTilesKernel, dim3(1), dim3(1024), 0, s, t.st, d.tileSum, d.total);

@@ -290,3 +290,22 @@ def test_binary_large_bucket_pass_build(bucket):
         t.computeViews()
         t.doBuffersNeedResize()
     same(g, o, "bucket %d" % bucket, BINARY)
+
+
+@pytest.mark.parametrize("bucket", [2, 7, 8, 9, 63, 64, 65, 300, 1024])
+def test_binary_direct_build_window_edges(bucket):
+    """The direct construction's window maxima: register windows for buckets
+    up to 64 (with a separate form below 8), a sparse table above, up to the
+    direct limit of 1024; a clustered set so that windows mix depths."""
+    rng = np.random.default_rng(300 + bucket)
+    n = 150_000 + bucket
+    pts = np.concatenate([0.3 + 0.002 * rng.standard_normal((n // 3, 3)), rng.random((n - n // 3, 3))]).clip(0, 1)
+    host = [np.ascontiguousarray(pts[:, a], dtype=np.float32) for a in range(3)]
+    dev = [DeviceArray(a) for a in host]
+    cfg = make_config(max_particles_per_leaf=bucket, box=(0.0, 2.0, 0.0, 1.0, 0.0, 1.0))
+    g = Tree(*dev, n, cfg, real=np.float32, kind=BINARY)
+    o = OracleTree(*host, n, cfg, real=np.float32, kind=BINARY)
+    for t in (g, o):
+        t.build()
+        t.doBuffersNeedResize()
+    same(g, o, "direct bucket %d" % bucket, BINARY)
