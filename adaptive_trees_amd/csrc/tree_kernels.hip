@@ -947,8 +947,16 @@ __global__ __launch_bounds__(1024) void binScanTilesKernel(const DevState *st, s
     const int tid = threadIdx.x;
     const long per = (tiles + 1023) / 1024;
     const long b = long(tid) * per, e = b + per < tiles ? b + per : tiles;
+    constexpr int kBatch = 8; // loads in flight per thread
     unsigned long long local = 0;
-    for (long i = b; i < e; ++i) local += tileSum[i];
+    for (long i0 = b; i0 < e; i0 += kBatch)
+    {
+        std::uint32_t v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) v[k] = i0 + k < e ? tileSum[i0 + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) local += v[k];
+    }
     const unsigned lane = __lane_id();
     const int wave = tid >> 6;
     unsigned long long inc = local;
@@ -962,27 +970,40 @@ __global__ __launch_bounds__(1024) void binScanTilesKernel(const DevState *st, s
     __syncthreads();
     unsigned long long run = inc - local;
     for (int w = 0; w < wave; ++w) run += wsum[w];
-    for (long i = b; i < e; ++i)
+    for (long i0 = b; i0 < e; i0 += kBatch)
     {
-        const std::uint32_t v = tileSum[i];
-        tileSum[i] = std::uint32_t(run);
-        run += v;
+        std::uint32_t v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) v[k] = i0 + k < e ? tileSum[i0 + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k)
+            if (i0 + k < e)
+            {
+                tileSum[i0 + k] = std::uint32_t(run);
+                run += v[k];
+            }
     }
     if (tid == 1023) *total = run;
 }
 
 // Writes the boundaries of each interval q at 1 + (boundaries before q).
+// Few intervals hold boundaries, so each wave first lists its non-empty
+// intervals (with their output positions) in LDS and then emits them one
+// per lane: the loads of P and delta for a wave's intervals go out together.
 __global__ __launch_bounds__(kThreads) void binEmitKernel(const Key *P, const DevState *st, const std::int8_t *delta,
                                                           const std::uint8_t *cnt, const std::uint32_t *tileOff,
                                                           Key *K, std::uint32_t *bound, long leaves)
 {
-    __shared__ std::uint32_t wsum[kThreads / 64];
+    constexpr int kW = kThreads / 64;
+    __shared__ std::uint32_t wsum[kW];
+    __shared__ uint2 list[kW][64 * kItems]; // (interval, first output position)
     const int tid = threadIdx.x;
     const long nact = long(st->numActiveParticles);
     const long q0 = long(blockIdx.x) * kQ;
     if (q0 > nact) return;
     static_assert(kItems == 8, "one 8-byte load of interval counts per thread");
     const long qFirst = q0 + long(tid) * kItems;
+    const std::uint32_t tileBase = tileOff[blockIdx.x];
     // cnt is allocated to whole tiles; entries past nact are masked
     const uint2 raw = *reinterpret_cast<const uint2 *>(cnt + qFirst);
     std::uint32_t c[kItems];
@@ -1008,43 +1029,18 @@ __global__ __launch_bounds__(kThreads) void binEmitKernel(const Key *P, const De
     __syncthreads();
     std::uint32_t before = 0;
     for (int w = 0; w < wave; ++w) before += wsum[w];
-    long pos = 1 + long(tileOff[blockIdx.x]) + long(before + inc - sum);
+    long pos = 1 + long(tileBase) + long(before + inc - sum);
+    const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
+    std::uint32_t listed = 0; // wave-uniform
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
     {
         const long q = qFirst + k;
-        if (q > nact) break;
-        if (c[k])
-        {
-            const bool hp = q > 0, hc = q < nact;
-            const Key prev = hp ? P[q - 1] : 0, curk = hc ? P[q] : 0;
-            const int dp = hp ? delta[q - 1] : -1;
-            const int dc = hc ? delta[q] : -1;
-            const int cp = (hp && hc) ? cpl63(prev, curk) : -1;
-            Key m = (dp > cp ? bitsBetween(62 - dp, 61 - cp) : 0) & ~prev;
-            while (m)
-            {
-                const int bit = __ffsll((long long)m) - 1;
-                const Key mid = (prev & ~((Key(2) << bit) - 1)) | (Key(1) << bit);
-                if (pos < leaves) { K[pos] = mid; bound[pos] = std::uint32_t(q); }
-                ++pos;
-                m &= m - 1;
-            }
-            if (hp && hc && cp <= dc)
-            {
-                const int bit = 62 - cp;
-                if (pos < leaves) { K[pos] = curk & ~((Key(1) << bit) - 1); bound[pos] = std::uint32_t(q); }
-                ++pos;
-            }
-            m = (dc > cp ? bitsBetween(62 - dc, 61 - cp) : 0) & curk;
-            while (m)
-            {
-                const int bit = 63 - clz64(m);
-                if (pos < leaves) { K[pos] = curk & ~((Key(1) << bit) - 1); bound[pos] = std::uint32_t(q); }
-                ++pos;
-                m &= ~(Key(1) << bit);
-            }
-        }
+        const bool has = c[k] != 0;
+        const std::uint64_t b = __ballot(has);
+        if (has) list[wave][listed + __popcll(b & ltMask)] = make_uint2(std::uint32_t(q), std::uint32_t(pos));
+        listed += std::uint32_t(__popcll(b));
+        pos += long(c[k]);
         if (q == nact)
         {
             K[0] = 0;
@@ -1054,6 +1050,41 @@ __global__ __launch_bounds__(kThreads) void binEmitKernel(const Key *P, const De
                 K[leaves] = kMaxKey;
                 bound[leaves] = std::uint32_t(nact);
             }
+        }
+    }
+    __syncthreads();
+    for (std::uint32_t e = lane; e < listed; e += 64)
+    {
+        const uint2 it = list[wave][e];
+        const long q = long(it.x);
+        long at = long(it.y);
+        const bool hp = q > 0, hc = q < nact;
+        const Key prev = hp ? P[q - 1] : 0, curk = hc ? P[q] : 0;
+        const int dp = hp ? delta[q - 1] : -1;
+        const int dc = hc ? delta[q] : -1;
+        const int cp = (hp && hc) ? cpl63(prev, curk) : -1;
+        Key m = (dp > cp ? bitsBetween(62 - dp, 61 - cp) : 0) & ~prev;
+        while (m)
+        {
+            const int bit = __ffsll((long long)m) - 1;
+            const Key mid = (prev & ~((Key(2) << bit) - 1)) | (Key(1) << bit);
+            if (at < leaves) { K[at] = mid; bound[at] = std::uint32_t(q); }
+            ++at;
+            m &= m - 1;
+        }
+        if (hp && hc && cp <= dc)
+        {
+            const int bit = 62 - cp;
+            if (at < leaves) { K[at] = curk & ~((Key(1) << bit) - 1); bound[at] = std::uint32_t(q); }
+            ++at;
+        }
+        m = (dc > cp ? bitsBetween(62 - dc, 61 - cp) : 0) & curk;
+        while (m)
+        {
+            const int bit = 63 - clz64(m);
+            if (at < leaves) { K[at] = curk & ~((Key(1) << bit) - 1); bound[at] = std::uint32_t(q); }
+            ++at;
+            m &= ~(Key(1) << bit);
         }
     }
 }
