@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIP_LIB = os.path.join(HERE, "libadaptive_trees_hip.so")
 
-AT_OK, AT_EINVAL, AT_ELOGIC, AT_ELENGTH, AT_EHIP, AT_ENOMEM = range(6)
+AT_OK, AT_EINVAL, AT_ELOGIC, AT_ELENGTH, AT_EHIP, AT_ENOMEM, AT_EIO = range(7)
 OCTREE, BINARY = 0, 1
 LEAF_COUNT, NF_COUNT = 0, 1
 MORTON, HILBERT = 0, 1
@@ -85,6 +85,10 @@ class LengthError(TreeError):
     pass
 
 
+class InputError(TreeError):
+    """A particle or manifest file that cannot be read (std::runtime_error)."""
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("workers", ctypes.c_int32), ("max_batch", ctypes.c_int32),
                 ("update_graphs", ctypes.c_int32), ("reserved0", ctypes.c_int32)]
@@ -96,7 +100,7 @@ class ServerStats(ctypes.Structure):
                                                 "update_batches")]
 
 
-_ERRORS = {AT_EINVAL: InvalidArgument, AT_ELOGIC: LogicError, AT_ELENGTH: LengthError}
+_ERRORS = {AT_EINVAL: InvalidArgument, AT_ELOGIC: LogicError, AT_ELENGTH: LengthError, AT_EIO: InputError}
 
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
@@ -206,9 +210,22 @@ def hip_lib():
                                    ctypes.POINTER(_SZ)],
             "server_tree_scalars": [_P, ctypes.c_uint64, ctypes.POINTER(Scalars)],
             "server_get_stats": [_P, ctypes.POINTER(ServerStats)],
+            "io_read_columns": [ctypes.c_char_p, _I, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+                                ctypes.POINTER(_SZ)],
+            "io_read_frame": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+                              ctypes.POINTER(_SZ)],
+            "io_manifest_open": [ctypes.c_char_p, ctypes.POINTER(_P)],
+            "io_manifest_info": [_P, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(_SZ)],
+            "io_manifest_frame": [_P, _SZ, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_long), ctypes.c_char_p, _SZ,
+                                  ctypes.c_char_p, _SZ],
         }.items():
             f = getattr(d, "at_" + name)
             f.argtypes = args
             f.restype = _I
+        for name, args in {"io_free": [_P], "io_manifest_close": [_P]}.items():
+            f = getattr(d, "at_" + name)
+            f.argtypes = args
+            f.restype = None
         _hip = lib
     return _hip

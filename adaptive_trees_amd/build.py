@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libadaptive_trees_hip.so")
 ARCH = os.environ.get("AT_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["sort.hip", "tree_kernels.hip", "views.hip", "engine.hip", "batch.hip", "server.hip", "profile.hip"]
+SOURCES = ["sort.hip", "tree_kernels.hip", "views.hip", "engine.hip", "batch.hip", "server.hip", "profile.hip",
+           "io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -204,11 +205,23 @@ __host__ __device__ inline Key gridKey(T x, T y, T z, const GridBox<T> &b, bool 
 
 // ---- binary longest-side keys (binary_node.cuh:43-59) ------------------------
 
+// min(floor(clamp(v, 0, 1) * 2^63), 2^63 - 1). The product is exact, so the
+// integer part is taken from the bits: c = m 2^(e - 1075) with the 53-bit
+// significand m gives m << (e - 1012) or m >> (1012 - e). Zero, subnormals
+// and NaN give 0 (NaN as the device's double -> integer conversion does).
 __host__ __device__ inline Key fixed63(double v)
 {
     const double c1 = (1.0 < v) ? 1.0 : v;
     const double c0 = (0.0 > c1) ? 0.0 : c1;
-    const Key q = Key(c0 * 9223372036854775808.0);
+    std::uint64_t bits;
+    __builtin_memcpy(&bits, &c0, sizeof(bits));
+    const int e = int((bits >> 52) & 0x7ff);
+    const Key m = (bits & ((Key(1) << 52) - 1)) | (Key(1) << 52);
+    const int sh = e - 1012;
+    Key q;
+    if (e == 0 || e == 0x7ff) q = 0;
+    else if (sh >= 0) q = m << sh; // sh <= 11 (c0 <= 1)
+    else q = sh <= -64 ? Key(0) : m >> -sh;
     return q < kMaxKey - 1 ? q : kMaxKey - 1;
 }
 
@@ -225,7 +238,29 @@ struct BinaryGeom
     int order[3];       // axes of one cycle, most significant first
     int triples;        // ceil((63 - prefix) / 3)
     int prefixAxis;     // the only axis of the prefix levels, or -1 if they mix axes
+    double inv[3];      // correctly rounded 1 / len
+    int fastDiv;        // 1: every len and inv lies in [2^-1000, 2^1000] (binaryUnit's fast form)
 };
+
+// (x - lo) / len, as far as fixed63 can tell: the correctly rounded quotient
+// on (0, len), and 0 / 1 below / above it. The fast form is Markstein's
+// division (IBM J. Res. Develop. 34(1), 1990): from q = a * inv a residual
+// r = fma(-len, q, a) is exact, and q + r * inv rounded once is the
+// correctly rounded quotient when inv = RN(1 / len) and q is within one ulp
+// of a / len; the first correction brings q within one ulp, the second
+// rounds it. The operands stay far from overflow and underflow by fastDiv
+// and 0 < a < len.
+__host__ __device__ inline double binaryUnit(double a, double len, double inv, int fast)
+{
+    if (!fast) return a / len;
+    if (a <= 0.0) return 0.0;
+    if (a >= len) return 1.0;
+    double q = a * inv;
+    double r = fma(-len, q, a);
+    q = fma(r, inv, q);
+    r = fma(-len, q, a);
+    return fma(r, inv, q);
+}
 
 // Longest-side key of normalised coordinates v (binary_node.cuh:43-59):
 // depth d takes the next most significant bit of axis[d]'s 63-bit fixed point.
@@ -281,9 +316,9 @@ template <class T>
 __host__ __device__ inline Key binaryKey(T x, T y, T z, const BinaryGeom &g)
 {
     if (isEscaped<T>(x, y, z)) return kInvalidKey;
-    const double v0 = (double(x) - g.lo[0]) / g.len[0];
-    const double v1 = (double(y) - g.lo[1]) / g.len[1];
-    const double v2 = (double(z) - g.lo[2]) / g.len[2];
+    const double v0 = binaryUnit(double(x) - g.lo[0], g.len[0], g.inv[0], g.fastDiv);
+    const double v1 = binaryUnit(double(y) - g.lo[1], g.len[1], g.inv[1], g.fastDiv);
+    const double v2 = binaryUnit(double(z) - g.lo[2], g.len[2], g.inv[2], g.fastDiv);
     return binaryInterleave(fixed63(v0), fixed63(v1), fixed63(v2), g);
 }
 

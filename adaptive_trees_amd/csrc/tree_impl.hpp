@@ -94,6 +94,13 @@ BinaryGeom makeBinaryGeom(const at_box &box, unsigned maxDepth)
     g.prefixAxis = g.prefix > 0 ? g.axis[0] : 0;
     for (int d = 0; d < g.prefix; ++d)
         if (g.axis[d] != g.axis[0]) g.prefixAxis = -1;
+    g.fastDiv = 1;
+    for (int a = 0; a < 3; ++a)
+    {
+        g.inv[a] = 1.0 / g.len[a]; // IEEE division: correctly rounded
+        const double lo = std::ldexp(1.0, -1000), hi = std::ldexp(1.0, 1000);
+        if (!(g.len[a] >= lo && g.len[a] <= hi && g.inv[a] >= lo && g.inv[a] <= hi)) g.fastDiv = 0;
+    }
     return g;
 }
 

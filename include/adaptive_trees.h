@@ -38,7 +38,8 @@ enum {
     AT_ELOGIC = 2,  /* std::logic_error */
     AT_ELENGTH = 3, /* std::length_error */
     AT_EHIP = 4,    /* HIP runtime failure */
-    AT_ENOMEM = 5   /* device or host allocation failure */
+    AT_ENOMEM = 5,  /* device or host allocation failure */
+    AT_EIO = 6      /* std::runtime_error of the particle-file readers */
 };
 
 /* ---- enums (include/adaptive_octree/config.hpp:48-50) ------------------ */
@@ -273,6 +274,37 @@ int at_tune_auto(size_t n, void *stream, char *report, size_t report_len);
  * key sort (no reference counterpart: the reference calls
  * cub::DeviceRadixSort::SortPairs). n < 2^32. */
 int at_sort_pairs(void *keys, void *vals, size_t n, int slot_values, void *stream);
+
+/* ---- particle input formats (host memory) ----------------------------------
+ * Text particle files of the reference's tests (tests/particle_io.hpp:35-139,
+ * particle_io::readColumns): `columns` (1..16) numbers per row as strtod
+ * reads them, each followed by whitespace, '#' or the end of the row; blank
+ * rows and rows starting with '#' are skipped; a row "ESC" keeps its slot
+ * with every column set to DBL_MAX (the escaped-particle marker,
+ * include/adaptive_octree/config.hpp:103-113). On success *data is a
+ * library-allocated array [columns][*rows] (column-major; release with
+ * at_io_free). A malformed row fails with AT_EIO and names path:line. The
+ * x, y, z of particle_io::readParticles are columns 1..3 ("charge x y z")
+ * or 0..2 ("x y z charge") of a four-column read. */
+int at_io_read_columns(const char *path, int columns, double **data, size_t *rows);
+/* One frame of the benchmark harness (benchmarks/drivers/common.hpp:124-170,
+ * bench::readFrame): format "cxyz" / "xyzc" = text rows of four numbers
+ * (from_chars, one leading '+' allowed) or rows starting with "ESC"; format
+ * "bin" = x y z doubles per slot, NaN x = escaped. *xyz = [3][*n] doubles
+ * (release with at_io_free). */
+int at_io_read_frame(const char *path, const char *format, double **xyz, size_t *n);
+void at_io_free(void *p);
+/* Harness manifest (benchmarks/drivers/common.hpp:62-104, bench::readManifest):
+ * "dataset NAME", "slots N", "box xmin xmax ymin ymax zmin zmax" and
+ * "frame INDEX ID FORMAT PATH" lines (PATH relative to the manifest's
+ * directory). Fails (AT_EIO) without frames or with slots == 0. Strings are
+ * copied NUL-terminated and truncated to the buffer lengths given. */
+int at_io_manifest_open(const char *path, void **manifest);
+int at_io_manifest_info(const void *manifest, char *dataset, size_t dataset_len, size_t *slots, double *box,
+                        size_t *frames);
+int at_io_manifest_frame(const void *manifest, size_t i, int *index, long *id, char *format, size_t format_len,
+                         char *path, size_t path_len);
+void at_io_manifest_close(void *manifest);
 
 #ifdef __cplusplus
 }

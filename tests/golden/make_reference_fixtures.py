@@ -7,7 +7,10 @@ Source (MIT licence, nrottstegge/Adaptive_Trees): tests/verification_data/
       maxParticlesPerLeaf = 64*64*27, Morton, box [0,1]^3, all views
       (tests/tree_test.cu:155-209).
 
-Output: tests/golden/reference_fixture.npz. Coordinates are parsed with
+Output: tests/golden/reference_fixture.npz, and tests/golden/
+snapshot_0000_head.dat = the first HEAD_ROWS particle rows of
+snapshot_0000.dat with its two '#' header lines, byte for byte (a text
+input for the particle-file readers). Coordinates are parsed with
 Python's correctly rounded float(), the same value strtod() gives the
 reference reader (tests/particle_io.hpp:98-110). leafCounts_d.csv is not
 converted: the reference test does not compare it and its content is stale.
@@ -22,6 +25,8 @@ import numpy as np
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 SRC = os.path.join(REF, "tests", "verification_data")
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_fixture.npz")
+HEAD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "snapshot_0000_head.dat")
+HEAD_ROWS = 1500
 
 ARRAYS = {
     "particleKeys_d": np.uint64, "perm_d": np.uint32, "cornerstone_d": np.uint64,
@@ -72,6 +77,13 @@ def main():
             out["scalar_" + k] = np.array(int(v), dtype=np.int64)
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    with open(os.path.join(SRC, "snapshot_0000.dat"), "rb") as f:
+        lines = f.read().split(b"\n")
+    header = [l for l in lines[:2] if l.startswith(b"#")]
+    assert len(header) == 2, "expected two '#' header lines"
+    with open(HEAD, "wb") as f:
+        f.write(b"\n".join(lines[:2 + HEAD_ROWS]) + b"\n")
+    print("wrote", HEAD, os.path.getsize(HEAD), "bytes")
 
 
 if __name__ == "__main__":
