@@ -271,6 +271,30 @@ int at_tree_max_achieved_depth(at_tree *t, uint32_t *d)
         *d = treeOf(t).maxAchievedDepth();
     });
 }
+int at_tree_leaf_count_violations(at_tree *t, uint32_t bucket, uint64_t *overfull, uint64_t *mergeable)
+{
+    return guarded([&] {
+        needOutputs(overfull, mergeable);
+        at::TreeBase &tree = treeOf(t);
+        const void *K = nullptr, *P = nullptr;
+        std::size_t nk = 0, np = 0, cap = 0;
+        tree.buffer(AT_BUF_CORNERSTONE, &K, &nk, &cap);
+        tree.buffer(AT_BUF_PARTICLE_KEYS, &P, &np, &cap);
+        at_scalars sc{};
+        tree.scalars(&sc);
+        const hipStream_t s = tree.stream();
+        at::DevBuf<unsigned long long> counts;
+        counts.ensure(2, 0, s);
+        AT_HIP(hipMemsetAsync(counts.p, 0, 2 * sizeof(unsigned long long), s));
+        at::leafCountViolations(s, static_cast<const at::Key *>(K), nk ? nk - 1 : 0, static_cast<const at::Key *>(P), np, bucket,
+                            sc.tree_type == AT_TREE_BINARY ? 1 : 3, counts.p);
+        unsigned long long h[2] = {0, 0};
+        AT_HIP(hipMemcpyAsync(h, counts.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        AT_HIP(hipStreamSynchronize(s));
+        *overfull = h[0];
+        *mergeable = h[1];
+    });
+}
 const char *at_last_error(void) { return g_error.c_str(); }
 
 int at_device_count(int *count) { return hipStatus(hipGetDeviceCount(count)); }

@@ -1406,6 +1406,48 @@ void leafCountsByTiles(hipStream_t s, const TreeDev &t)
     hipLaunchKernelGGL(tileCountsKernel, dim3(2048), dim3(kThreads), 0, s, t);
 }
 
+__global__ __launch_bounds__(kThreads) void leafViolationsKernel(const Key *K, std::size_t L, const Key *P,
+                                                                 std::size_t n, unsigned bucket, int bits,
+                                                                 unsigned long long *counts)
+{
+    const std::size_t group = std::size_t(1) << bits;
+    unsigned long long over = 0, merge = 0;
+    for (std::size_t i = std::size_t(blockIdx.x) * kThreads + threadIdx.x; i < L;
+         i += std::size_t(gridDim.x) * kThreads)
+    {
+        const Key range = K[i + 1] - K[i];
+        const std::size_t a = lowerBound(P, n, K[i]);
+        if (lowerBound(P, n, K[i + 1]) - a > bucket && range > 1) ++over;
+        // 2^bits leaves spanning one aligned parent are exactly its children
+        if (range < kMaxKey && i + group <= L)
+        {
+            const Key parent = range << bits;
+            if (K[i] % parent == 0 && K[i + group] - K[i] == parent && lowerBound(P, n, K[i + group]) - a <= bucket)
+                ++merge;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+    {
+        over += __shfl_xor(over, o, 64);
+        merge += __shfl_xor(merge, o, 64);
+    }
+    if (__lane_id() == 0)
+    {
+        if (over) atomicAdd(&counts[0], over);
+        if (merge) atomicAdd(&counts[1], merge);
+    }
+}
+
+void leafCountViolations(hipStream_t s, const Key *K, std::size_t leaves, const Key *P, std::size_t n,
+                         unsigned bucket, int bits, unsigned long long *counts)
+{
+    if (!leaves) return;
+    const std::size_t want = (leaves + kThreads - 1) / kThreads;
+    hipLaunchKernelGGL(leafViolationsKernel, dim3(unsigned(want < 4096 ? want : 4096)), dim3(kThreads), 0, s, K,
+                       leaves, P, n, bucket, bits, counts);
+}
+
 void leafCounts(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 {
     prof::Scope scope("leaf_counts", s);

@@ -107,6 +107,15 @@ void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDire
 void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves);
 void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves);
 
+// Leaf-count convergence check (benchmarks/drivers/cstone_check.cu:17-43)
+// over leaf boundaries K[0..leaves] and sorted keys P[0..n), populations
+// recounted from P: counts[0] = leaves with more than `bucket` particles
+// and a key range above 1; counts[1] = aligned groups of 2^bits sibling
+// leaves (8 octree, 2 binary) that exactly tile their parent and hold at
+// most `bucket` particles together. Adds to counts (device, two u64).
+void leafCountViolations(hipStream_t s, const Key *K, std::size_t leaves, const Key *P, std::size_t n,
+                         unsigned bucket, int bits, unsigned long long *counts);
+
 // Builds t.Pidx from t.P (after every sort).
 void buildSearchIndex(hipStream_t s, const TreeDev &t);
 unsigned indexBlocksFor(std::size_t n);

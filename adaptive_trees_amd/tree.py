@@ -178,6 +178,17 @@ class Tree:
         self.lib.call("tree_max_achieved_depth", self.handle, ctypes.byref(d))
         return d.value
 
+    def leafCountViolations(self, bucket=None):
+        """(overfull leaves, mergeable sibling groups) of the leaf-count
+        convergence check (benchmarks/drivers/cstone_check.cu:17-43), for
+        ``bucket`` particles per leaf (default: the tree's
+        maxParticlesPerLeaf). HIP trees only."""
+        if bucket is None:
+            bucket = self.scalars().max_particles_per_leaf
+        over, merge = ctypes.c_uint64(), ctypes.c_uint64()
+        self.lib.call("tree_leaf_count_violations", self.handle, int(bucket), ctypes.byref(over), ctypes.byref(merge))
+        return over.value, merge.value
+
     def device_counters(self):
         """(numLeaves, numNodes, numActiveParticles) read from the device."""
         pl, pn, pa = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()

@@ -137,6 +137,16 @@ int at_tree_device_counters(at_tree *tree, const int32_t **num_leaves,
                             const int32_t **num_nodes, const uint32_t **num_active);
 /* maxAchievedDepth() (octree.cu:666-670, kdtree3d.cu:404-415); synchronises. */
 int at_tree_max_achieved_depth(at_tree *tree, uint32_t *depth);
+/* Leaf-count convergence check of the reference's harness
+ * (benchmarks/drivers/cstone_check.cu:17-43) on the tree as its getters see
+ * it (call doBuffersNeedResize after update), populations recounted from P:
+ * *overfull = leaves with more than `bucket` particles and a key range
+ * above 1; *mergeable = aligned groups of sibling leaves (8 for an octree;
+ * for a binary tree the analogous 2) that exactly tile their parent and
+ * hold at most `bucket` particles together. A converged LeafCount tree has
+ * neither (unless leaves reach the finest level); a fixed-pass update may
+ * leave some. Synchronises. */
+int at_tree_leaf_count_violations(at_tree *tree, uint32_t bucket, uint64_t *overfull, uint64_t *mergeable);
 /* Last error message of the calling thread ("" if none). */
 const char *at_last_error(void);
 
