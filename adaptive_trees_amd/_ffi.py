@@ -210,6 +210,7 @@ def hip_lib():
                                    ctypes.POINTER(_SZ)],
             "server_tree_scalars": [_P, ctypes.c_uint64, ctypes.POINTER(Scalars)],
             "server_get_stats": [_P, ctypes.POINTER(ServerStats)],
+            "tree_print": [_P, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ)],
             "tree_leaf_count_violations": [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.POINTER(ctypes.c_uint64)],
             "io_read_columns": [ctypes.c_char_p, _I, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),

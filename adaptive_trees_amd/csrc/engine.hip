@@ -295,6 +295,19 @@ int at_tree_leaf_count_violations(at_tree *t, uint32_t bucket, uint64_t *overful
         *mergeable = h[1];
     });
 }
+int at_tree_print(at_tree *t, char *buf, size_t len, size_t *needed)
+{
+    return guarded([&] {
+        const std::string text = at::printTree(treeOf(t));
+        if (needed) *needed = text.size() + 1;
+        if (buf && len)
+        {
+            const std::size_t k = std::min(len - 1, text.size());
+            std::memcpy(buf, text.data(), k);
+            buf[k] = '\0';
+        }
+    });
+}
 const char *at_last_error(void) { return g_error.c_str(); }
 
 int at_device_count(int *count) { return hipStatus(hipGetDeviceCount(count)); }

@@ -59,6 +59,9 @@ struct MemberSlot
 // Message of the last failing C-ABI call on this thread.
 std::string &lastError();
 
+// The tree's print() text (print.cpp).
+std::string printTree(TreeBase &t);
+
 TreeBase *createTree(int realBytes, int type, const void *x, const void *y, const void *z, std::size_t n,
                      const at_config &c);
 } // namespace at

@@ -178,6 +178,21 @@ class Tree:
         self.lib.call("tree_max_achieved_depth", self.handle, ctypes.byref(d))
         return d.value
 
+    def print(self, file=None):
+        """print(std::ostream&): the reference's diagnostic dump
+        (src/octree/octree_print.cu, src/kdtree3d/kdtree3d_print.cu), written
+        to ``file`` (default sys.stdout). HIP trees only."""
+        import sys
+        (file or sys.stdout).write(self.printString())
+
+    def printString(self):
+        """The text of print()."""
+        need = ctypes.c_size_t()
+        self.lib.call("tree_print", self.handle, None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(max(need.value, 1))
+        self.lib.call("tree_print", self.handle, buf, len(buf), ctypes.byref(need))
+        return buf.value.decode()
+
     def leafCountViolations(self, bucket=None):
         """(overfull leaves, mergeable sibling groups) of the leaf-count
         convergence check (benchmarks/drivers/cstone_check.cu:17-43), for
@@ -221,8 +236,12 @@ class Tree:
         ptr, size, cap = self.buffer_info(name)
         n = size if count is None else count
         dt = np.dtype(_ffi.BUFFER_DTYPES[_ffi.BUFFERS[name]])
+        if not ptr:
+            # a size without storage (NF of a LeafCount octree, whose contents
+            # the reference leaves unset): zeros
+            return np.zeros(n, dtype=dt)
         out = np.empty(n, dtype=dt)
-        if n and ptr:
+        if n:
             self._copy_out(out.ctypes.data, ptr, out.nbytes)
         return out
 

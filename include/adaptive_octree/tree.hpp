@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <climits>
 #include <ostream>
+#include <string>
 #include <type_traits>
 #include <utility>
 
@@ -88,12 +89,15 @@ public:
     }
     // Summary of the host-visible state (the reference's print() layout is
     // not reproduced).
+    // the reference's diagnostic dump (octree_print.cu, kdtree3d_print.cu)
     void print(std::ostream &out) const
     {
-        const at_scalars s = scalars();
-        out << (Kind == TreeType::Octree ? "Octree" : "KDTree3D") << "<" << (sizeof(Real) == 4 ? "float" : "double")
-            << ">: particles " << s.num_particles << ", active " << s.num_active_particles << ", leaves "
-            << s.num_leaves << ", nodes " << s.num_nodes << ", leaf capacity " << s.leaf_capacity << "\n";
+        std::size_t need = 0;
+        detail::check(at_tree_print(t_, nullptr, 0, &need));
+        std::string text(need, '\0');
+        detail::check(at_tree_print(t_, text.data(), text.size(), &need));
+        text.resize(need ? need - 1 : 0);
+        out << text;
     }
 
     bool doBuffersNeedResize() const

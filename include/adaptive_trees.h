@@ -147,6 +147,12 @@ int at_tree_max_achieved_depth(at_tree *tree, uint32_t *depth);
  * neither (unless leaves reach the finest level); a fixed-pass update may
  * leave some. Synchronises. */
 int at_tree_leaf_count_violations(at_tree *tree, uint32_t bucket, uint64_t *overfull, uint64_t *mergeable);
+/* print(std::ostream&) (src/octree/octree_print.cu:57-382,
+ * src/kdtree3d/kdtree3d_print.cu:11-43): the same text, formatted as on a
+ * stream in its default state. Copies at most len-1 characters and a NUL
+ * into buf (may be NULL with len 0); *needed (may be NULL) receives the
+ * full length + 1. Synchronises. */
+int at_tree_print(at_tree *tree, char *buf, size_t len, size_t *needed);
 /* Last error message of the calling thread ("" if none). */
 const char *at_last_error(void);
 
