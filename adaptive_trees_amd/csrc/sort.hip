@@ -853,6 +853,7 @@ constexpr std::uint32_t kMsdChunk = kLocThreads * kLocItems; // LDS chunk capaci
 constexpr int kMsdThreads = 512, kMsdItems = 8;       // scatter tile of levels >= 2
 constexpr std::uint32_t kMsdTile = kMsdThreads * kMsdItems;
 constexpr int kMsdGrid = 1024;                        // grid of the looping level kernels (multiple of 8)
+constexpr unsigned kMsdScanRows = 16;                 // chunk rows of the level scan's grid (x 256 digits)
 
 __global__ void msdResetKernel(MsdCounters *c, MsdSeg *root, std::size_t n)
 {
@@ -1077,23 +1078,29 @@ __global__ __launch_bounds__(kUpRows * 64) void msdUpsweepKernel(const std::uint
     }
 }
 
-// Block digit blockIdx.x: exclusive prefix of the digit's count over the
-// ranges of each segment (a segmented scan; first[r] marks a segment's first
-// range), 16 consecutive ranges per thread and 4096 per round. The count of
-// the digit in a whole segment goes to subTotals[segment][digit]. Block 0
-// also clears the list counters of the next level (its lists are no longer
-// read).
+// Exclusive prefix of digit blockIdx.x's count over the ranges of each
+// segment (a segmented scan; first[r] marks a segment's first range), in
+// chunks of kScanChunk ranges: block (d, y) scans chunks y, y + gridDim.y,
+// ... with 4 consecutive ranges per thread. A chunk that starts inside a
+// segment first sums the digit's counts of that segment's earlier ranges
+// (from segs[seg].firstRange) as its carry. The count of the digit in a
+// whole segment goes to subTotals[segment][digit]. Block (0, 0) also clears
+// the list counters of the next level (its lists are no longer read).
+constexpr int kScanPer = 4;
+constexpr std::uint32_t kScanChunk = kScanPer * kSortThreads;
 __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_t *__restrict__ hist,
                                                               std::size_t stride,
                                                               const std::uint8_t *__restrict__ first,
-                                                              const SegRange *__restrict__ ranges, MsdCounters *cnt,
+                                                              const SegRange *__restrict__ ranges,
+                                                              const MsdSeg *__restrict__ segs, MsdCounters *cnt,
                                                               int set, std::uint32_t *off, std::uint32_t *subTotals)
 {
-    constexpr int kPer = 16;
-    __shared__ std::uint32_t wS[2][kWaves];
-    __shared__ std::uint32_t wF[2][kWaves];
+    constexpr int kPer = kScanPer;
+    __shared__ std::uint32_t wS[kWaves];
+    __shared__ std::uint32_t wF[kWaves];
+    __shared__ std::uint32_t carryShared;
     const std::uint32_t count = cnt->ranges[set] < stride ? cnt->ranges[set] : std::uint32_t(stride);
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
     {
         cnt->segs[set ^ 1] = 0;
         cnt->ranges[set ^ 1] = 0;
@@ -1103,25 +1110,20 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
     std::uint32_t *o = off + d * stride;
     const unsigned lane = laneId();
     const int wave = threadIdx.x >> 6;
-    std::uint32_t carry = 0; // open segment's count at the end of the previous round
-    int buf = 0;
-    for (std::uint32_t r0 = 0; r0 < count; r0 += kPer * kSortThreads)
+    for (std::uint32_t r0 = blockIdx.y * kScanChunk; r0 < count; r0 += gridDim.y * kScanChunk)
     {
+        // the chunk's own ranges (4-byte flag and 16-byte count loads: stride
+        // and f0 are multiples of 4), loaded with the carry's inputs
         const std::uint32_t f0 = r0 + threadIdx.x * kPer;
         std::uint32_t v[kPer], ex[kPer];
         std::uint8_t fb[kPer];
-        if (f0 + kPer <= count) // 16-byte loads (stride and f0 are multiples of 16)
+        if (f0 + kPer <= count)
         {
-            const uint4 q = *reinterpret_cast<const uint4 *>(first + f0);
-            const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+            const std::uint32_t q = *reinterpret_cast<const std::uint32_t *>(first + f0);
+            const uint4 c = *reinterpret_cast<const uint4 *>(h + f0);
+            v[0] = c.x, v[1] = c.y, v[2] = c.z, v[3] = c.w;
 #pragma unroll
-            for (int k = 0; k < kPer; ++k) fb[k] = std::uint8_t(w[k >> 2] >> (8 * (k & 3)));
-#pragma unroll
-            for (int k = 0; k < kPer / 4; ++k)
-            {
-                const uint4 c = reinterpret_cast<const uint4 *>(h + f0)[k];
-                v[4 * k] = c.x, v[4 * k + 1] = c.y, v[4 * k + 2] = c.z, v[4 * k + 3] = c.w;
-            }
+            for (int k = 0; k < kPer; ++k) fb[k] = std::uint8_t(q >> (8 * k));
         }
         else
         {
@@ -1133,6 +1135,17 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
                 fb[k] = in ? first[f0 + k] : 0;
             }
         }
+        const bool nextFirst = f0 + kPer >= count || first[f0 + kPer];
+        // carry: the digit's count in the ranges of the segment open at r0
+        // that precede the chunk (block-uniform branch)
+        std::uint32_t part = 0;
+        if (!first[r0])
+        {
+            const std::uint32_t f = segs[ranges[r0].seg].firstRange;
+            for (std::uint32_t r = f + threadIdx.x; r < r0; r += kSortThreads) part += h[r];
+        }
+#pragma unroll
+        for (int o2 = 32; o2 >= 1; o2 >>= 1) part += __shfl_xor(part, o2, 64);
         std::uint32_t flags = 0; // bit k: range f0+k starts a segment
         std::uint32_t run = 0;
 #pragma unroll
@@ -1145,7 +1158,6 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
             ex[k] = run;
             run += v[k];
         }
-        const bool nextFirst = f0 + kPer >= count || first[f0 + kPer];
         // inclusive segmented scan of the thread aggregates (run, has flag) in the wave
         std::uint32_t s = run, fl = flags != 0;
 #pragma unroll
@@ -1162,17 +1174,20 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
         if (lane == 0) es = 0, ef = 0;
         if (lane == 63)
         {
-            wS[buf][wave] = s;
-            wF[buf][wave] = fl;
+            wS[wave] = s;
+            wF[wave] = fl;
         }
+        if (threadIdx.x == 0) carryShared = 0;
         __syncthreads();
-        // carry into this wave: the round carry, then the earlier waves' totals
-        std::uint32_t cs = carry, all = carry;
-        for (int w = 0; w < kWaves; ++w)
+        if (lane == 0 && part) atomicAdd(&carryShared, part);
+        __syncthreads();
+        const std::uint32_t carry = carryShared;
+        // carry into this wave: the chunk's carry, then the earlier waves' totals
+        std::uint32_t cs = carry;
+        for (int w = 0; w < wave; ++w)
         {
-            const std::uint32_t ws = wS[buf][w], wf = wF[buf][w];
-            if (w < wave) cs = wf ? ws : cs + ws;
-            all = wf ? ws : all + ws;
+            const std::uint32_t ws = wS[w], wf = wF[w];
+            cs = wf ? ws : cs + ws;
         }
         const std::uint32_t tin = ef ? es : cs + es;
         bool seen = false;
@@ -1189,8 +1204,7 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
                 if (end) subTotals[std::size_t(ranges[r].seg) * kRadix + d] = val + v[k];
             }
         }
-        carry = all;
-        buf ^= 1;
+        __syncthreads(); // wS, wF and carryShared are rewritten by the next chunk
     }
 }
 
@@ -1637,8 +1651,10 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
         const int shift = 64 - kRadixBits * level;
         hipLaunchKernelGGL(msdUpsweepKernel, dim3(upGrid), dim3(kUpRows * 64), 0, s, digits, ws.msdRanges[set],
                            ws.msdCnt, set, ws.msdHist, ws.msdStride);
-        hipLaunchKernelGGL(msdScanKernel, dim3(kRadix), dim3(kSortThreads), 0, s, ws.msdHist, ws.msdStride,
-                           ws.msdFirst[set], ws.msdRanges[set], ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
+        const unsigned scanChunks = unsigned((ws.msdStride + kScanChunk - 1) / kScanChunk);
+        hipLaunchKernelGGL(msdScanKernel, dim3(kRadix, scanChunks < kMsdScanRows ? scanChunks : kMsdScanRows),
+                           dim3(kSortThreads), 0, s, ws.msdHist, ws.msdStride, ws.msdFirst[set], ws.msdRanges[set],
+                           ws.msdSegs[set], ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
         const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegs[set], ws.msdSubTotals, ws.msdOff, ws.msdStride};
         if (level < kSortPasses)
             hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, true, true, false, true>),
