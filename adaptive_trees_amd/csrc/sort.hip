@@ -854,6 +854,7 @@ constexpr int kMsdThreads = 512, kMsdItems = 8;       // scatter tile of levels 
 constexpr std::uint32_t kMsdTile = kMsdThreads * kMsdItems;
 constexpr int kMsdGrid = 1024;                        // grid of the looping level kernels (multiple of 8)
 constexpr unsigned kMsdScanRows = 16;                 // chunk rows of the level scan's grid (x 256 digits)
+constexpr unsigned kRootPlanBlocks = 32;              // blocks sharing the range writes of the root plan
 
 __global__ void msdResetKernel(MsdCounters *c, MsdSeg *root, std::size_t n)
 {
@@ -864,17 +865,20 @@ __global__ void msdResetKernel(MsdCounters *c, MsdSeg *root, std::size_t n)
     *root = MsdSeg{0, std::uint32_t(n), 0, 0, 0};
 }
 
-// One block per parent segment (grid-stride over *parentCount, or the root):
-// thread d takes sub-bucket d (size totals[parent][d]). Sub-buckets of more
-// than kMsdChunk keys become segments of the next level; the block writes
-// their tile ranges together (thread t takes ranges t, t+256, ...). The
-// others are grouped, in order, into chunks of at most kMsdChunk keys: a
-// chunk starting at sub-bucket d extends up to the next big sub-bucket or
-// until one more would exceed kMsdChunk keys (the same cuts as a sequential
-// greedy pass); each thread finds that end by a binary search and thread 0
-// follows the chunk starts. At the last level every sub-bucket holds equal
-// keys and the parent becomes one chunk that needs no sorting. Each parent
-// reserves its list entries with one atomic per list.
+// One block per parent segment (grid-stride over *parentCount): thread d
+// takes sub-bucket d (size totals[parent][d]). Sub-buckets of more than
+// kMsdChunk keys become segments of the next level; the block writes their
+// tile ranges together (thread t takes ranges t, t+256, ...). The others are
+// grouped, in order, into chunks of at most kMsdChunk keys: a chunk starting
+// at sub-bucket d extends up to the next big sub-bucket or until one more
+// would exceed kMsdChunk keys (the same cuts as a sequential greedy pass);
+// each thread finds that end by a binary search and thread 0 follows the
+// chunk starts. At the last level every sub-bucket holds equal keys and the
+// parent becomes one chunk that needs no sorting. A parent reserves its
+// entries of the three lists in one round trip (three atomics issued
+// together). Root plan (parentCount == nullptr: the whole array, lists
+// empty): every block plans the root, block 0 sets the list counts and
+// writes the segments and chunks, and the blocks share the range writes.
 __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     int level, const MsdSeg *__restrict__ parents, const std::uint32_t *parentCount,
     const std::uint32_t *__restrict__ totals, MsdCounters *cnt, int nextSet, MsdSeg *nextSegs,
@@ -894,12 +898,14 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
     __shared__ std::uint32_t base[3];
     __shared__ std::uint32_t nChunks, nBig, nRanges;
     const unsigned d = threadIdx.x;
-    const std::uint32_t count = parentCount ? (*parentCount < maxParents ? *parentCount : maxParents) : 1u;
+    const bool root = parentCount == nullptr;
+    const std::uint32_t count = root ? 1u : (*parentCount < maxParents ? *parentCount : maxParents);
     const bool last = level >= kSortPasses;
     const std::uint8_t parity = std::uint8_t(level & 1);
     // a chunk spans sub-buckets of one parent: sort on the bits below the parent's digit
     const std::uint8_t bits = last ? 0 : std::uint8_t(level == 1 ? 64 : 72 - 8 * level);
-    for (std::uint32_t p = blockIdx.x; p < count; p += gridDim.x)
+    const bool lead = !root || blockIdx.x == 0; // writes counts, segments and chunks
+    for (std::uint32_t p = root ? 0u : blockIdx.x; p < count; p += root ? count : gridDim.x)
     {
         const MsdSeg par = parents[p];
         const std::uint32_t s = totals[std::size_t(p) * kRadix + d];
@@ -930,36 +936,12 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
             rangeRankOf[nb] = nrr;
             nBig = nb;
             nRanges = nrr;
-            base[0] = nb ? atomicAdd(&cnt->segs[nextSet], nb) : 0u;
-            base[1] = nrr ? atomicAdd(&cnt->ranges[nextSet], nrr) : 0u;
-            if ((nb && base[0] + nb > maxSegs) || (nrr && base[1] + nrr > maxRanges)) atomicAdd(&cnt->overflow, 1u);
         }
         __syncthreads();
         nextBig[d] = bigAt[bigRank];     // d itself when big
         nextStart[d] = smallAt[smallRank]; // d itself when small
         if (d == 0) nextBig[kRadix] = kRadix, nextStart[kRadix] = kRadix;
-        const bool fits = base[0] + nBig <= maxSegs && base[1] + nRanges <= maxRanges;
-        if (isBig && fits)
-            nextSegs[base[0] + bigRank] = MsdSeg{par.begin + rel, s, base[1] + rangeRank, nr, 0};
         __syncthreads();
-        if (fits)
-            for (std::uint32_t t = d; t < nRanges; t += kSortThreads)
-            {
-                // the big sub-bucket owning range t: last k with rangeRankOf[k] <= t
-                std::uint32_t lo = 0, hi = nBig; // invariant: rangeRankOf[lo] <= t < rangeRankOf[hi]
-                while (hi - lo > 1)
-                {
-                    const std::uint32_t mid = (lo + hi) >> 1;
-                    if (rangeRankOf[mid] <= t) lo = mid;
-                    else hi = mid;
-                }
-                const unsigned sb = bigIndex[lo];
-                const std::uint32_t k = t - rangeRankOf[lo], o = k * kMsdTile;
-                const std::uint32_t size = relS[sb + 1] - relS[sb];
-                nextRanges[base[1] + t] = SegRange{par.begin + relS[sb] + o, size - o < kMsdTile ? size - o : kMsdTile, o,
-                                                   base[0] + lo, 0};
-                nextFirst[base[1] + t] = k == 0;
-            }
         // end of a chunk starting at small sub-bucket d
         if (isSmall)
         {
@@ -992,11 +974,59 @@ __global__ __launch_bounds__(kSortThreads) void msdPlanKernel(
                     ++n;
                 }
             nChunks = n;
-            base[2] = n ? atomicAdd(&cnt->chunks, n) : 0u;
-            if (n && base[2] + n > maxChunks) atomicAdd(&cnt->overflow, 1u);
+            const std::uint32_t nb = nBig, nrr = nRanges;
+            if (root)
+            {
+                // the lists are empty: this parent's entries start at 0
+                base[0] = base[1] = base[2] = 0;
+                if (lead)
+                {
+                    cnt->segs[nextSet] = nb;
+                    cnt->ranges[nextSet] = nrr;
+                    cnt->chunks = n;
+                }
+            }
+            else
+            {
+                // three independent atomics: one round trip
+                const std::uint32_t b0 = nb ? atomicAdd(&cnt->segs[nextSet], nb) : 0u;
+                const std::uint32_t b1 = nrr ? atomicAdd(&cnt->ranges[nextSet], nrr) : 0u;
+                const std::uint32_t b2 = n ? atomicAdd(&cnt->chunks, n) : 0u;
+                base[0] = b0;
+                base[1] = b1;
+                base[2] = b2;
+            }
+            if (lead && ((nb && base[0] + nb > maxSegs) || (nrr && base[1] + nrr > maxRanges) ||
+                         (n && base[2] + n > maxChunks)))
+                atomicAdd(&cnt->overflow, 1u);
         }
         __syncthreads();
-        if (d < nChunks && base[2] + d < maxChunks)
+        const bool fits = base[0] + nBig <= maxSegs && base[1] + nRanges <= maxRanges;
+        if (lead && isBig && fits)
+            nextSegs[base[0] + bigRank] = MsdSeg{par.begin + rel, s, base[1] + rangeRank, nr, 0};
+        if (fits)
+        {
+            const std::uint32_t t0 = root ? blockIdx.x * kSortThreads : 0u;
+            const std::uint32_t step = root ? gridDim.x * kSortThreads : kSortThreads;
+            for (std::uint32_t t = t0 + d; t < nRanges; t += step)
+            {
+                // the big sub-bucket owning range t: last k with rangeRankOf[k] <= t
+                std::uint32_t lo = 0, hi = nBig; // invariant: rangeRankOf[lo] <= t < rangeRankOf[hi]
+                while (hi - lo > 1)
+                {
+                    const std::uint32_t mid = (lo + hi) >> 1;
+                    if (rangeRankOf[mid] <= t) lo = mid;
+                    else hi = mid;
+                }
+                const unsigned sb = bigIndex[lo];
+                const std::uint32_t k = t - rangeRankOf[lo], o = k * kMsdTile;
+                const std::uint32_t size = relS[sb + 1] - relS[sb];
+                nextRanges[base[1] + t] = SegRange{par.begin + relS[sb] + o, size - o < kMsdTile ? size - o : kMsdTile, o,
+                                                   base[0] + lo, 0};
+                nextFirst[base[1] + t] = k == 0;
+            }
+        }
+        if (lead && d < nChunks && base[2] + d < maxChunks)
             chunks[base[2] + d] = MsdChunk{par.begin + chBegin[d], chLen[d], bits, parity, 0};
         __syncthreads();
     }
@@ -1083,7 +1113,7 @@ __global__ __launch_bounds__(kUpRows * 64) void msdUpsweepKernel(const std::uint
 // chunks of kScanChunk ranges: block (d, y) scans chunks y, y + gridDim.y,
 // ... with 4 consecutive ranges per thread. A chunk that starts inside a
 // segment first sums the digit's counts of that segment's earlier ranges
-// (from segs[seg].firstRange) as its carry. The count of the digit in a
+// as its carry. The count of the digit in a
 // whole segment goes to subTotals[segment][digit]. Block (0, 0) also clears
 // the list counters of the next level (its lists are no longer read).
 constexpr int kScanPer = 4;
@@ -1091,8 +1121,7 @@ constexpr std::uint32_t kScanChunk = kScanPer * kSortThreads;
 __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_t *__restrict__ hist,
                                                               std::size_t stride,
                                                               const std::uint8_t *__restrict__ first,
-                                                              const SegRange *__restrict__ ranges,
-                                                              const MsdSeg *__restrict__ segs, MsdCounters *cnt,
+                                                              const SegRange *__restrict__ ranges, MsdCounters *cnt,
                                                               int set, std::uint32_t *off, std::uint32_t *subTotals)
 {
     constexpr int kPer = kScanPer;
@@ -1137,13 +1166,11 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
         }
         const bool nextFirst = f0 + kPer >= count || first[f0 + kPer];
         // carry: the digit's count in the ranges of the segment open at r0
-        // that precede the chunk (block-uniform branch)
+        // that precede the chunk (block-uniform branch); range k of a
+        // segment starts kMsdTile * k keys into it
         std::uint32_t part = 0;
-        if (!first[r0])
-        {
-            const std::uint32_t f = segs[ranges[r0].seg].firstRange;
-            for (std::uint32_t r = f + threadIdx.x; r < r0; r += kSortThreads) part += h[r];
-        }
+        const std::uint32_t into = ranges[r0].localBase / kMsdTile;
+        for (std::uint32_t r = r0 - into + threadIdx.x; r < r0; r += kSortThreads) part += h[r];
 #pragma unroll
         for (int o2 = 32; o2 >= 1; o2 >>= 1) part += __shfl_xor(part, o2, 64);
         std::uint32_t flags = 0; // bit k: range f0+k starts a segment
@@ -1639,7 +1666,8 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
         // bits in buckets of more than kMsdChunk (heavy duplicates): small
         // grids keep those launches cheap when they are empty
         const unsigned cap = level <= 3 ? 1024u : 128u;
-        const unsigned segGrid = unsigned(maxSegs < cap ? maxSegs : cap);
+        const unsigned planCap = level <= 3 ? 2048u : 128u; // one parent per block (8 blocks per CU fit)
+        const unsigned segGrid = unsigned(maxSegs < planCap ? maxSegs : planCap);
         const unsigned upGrid = unsigned((maxRanges / kUpRows) < cap ? maxRanges / kUpRows + 1 : cap);
         const unsigned scatterGrid = level <= 3 ? unsigned(kMsdGrid) : 256u;
         const int set = level & 1;
@@ -1654,7 +1682,7 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
         const unsigned scanChunks = unsigned((ws.msdStride + kScanChunk - 1) / kScanChunk);
         hipLaunchKernelGGL(msdScanKernel, dim3(kRadix, scanChunks < kMsdScanRows ? scanChunks : kMsdScanRows),
                            dim3(kSortThreads), 0, s, ws.msdHist, ws.msdStride, ws.msdFirst[set], ws.msdRanges[set],
-                           ws.msdSegs[set], ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
+                           ws.msdCnt, set, ws.msdOff, ws.msdSubTotals);
         const MsdScatterArgs a{&ws.msdCnt->ranges[set], ws.msdSegs[set], ws.msdSubTotals, ws.msdOff, ws.msdStride};
         if (level < kSortPasses)
             hipLaunchKernelGGL((scatterKernel<kMsdThreads, kMsdItems, true, false, true, true, false, true>),
@@ -1695,7 +1723,8 @@ void msdSort(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *X, Key *
         prof::Scope scope("sort_pass", s);
         dispatchScatter(s, pl, ws, X, Y, vX, vY, true, iota, true, 56, 48);
     }
-    hipLaunchKernelGGL(msdPlanKernel, dim3(1), dim3(kSortThreads), 0, s, 1, ws.msdRoot, nullptr, ws.digitTotal,
+    hipLaunchKernelGGL(msdPlanKernel, dim3(kRootPlanBlocks), dim3(kSortThreads), 0, s, 1, ws.msdRoot, nullptr,
+                       ws.digitTotal,
                        ws.msdCnt, 0, ws.msdSegs[0], ws.msdRanges[0], ws.msdFirst[0], std::uint32_t(ws.msdMaxSegs),
                        std::uint32_t(ws.msdStride), ws.msdChunks, std::uint32_t(ws.msdMaxChunks), 1u);
     msdLevels(s, ws, n, ws.digits, X, Y, vX, vY, finalIsY);
