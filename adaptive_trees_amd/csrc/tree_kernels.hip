@@ -31,6 +31,10 @@ constexpr std::uint64_t kSumMask = kChg - 1;
 constexpr std::uint64_t kFlagMask = std::uint64_t(3) << 62;
 constexpr int kThreads = 256;
 constexpr int kItems = kPassTile / kThreads;
+// single-tree decision scans: 1024 x kItems leaves per tile, so a pass draws
+// a quarter of the tile tickets and look-back steps of kPassTile tiles
+constexpr int kDecideThreads = 1024;
+inline std::size_t decideTiles(std::size_t cap) { return cap / (std::size_t(kDecideThreads) * kItems) + 1; }
 
 // Chained scan across tiles (one status word per tile: flag, change bit, sum).
 // A tile publishes its aggregate, looks back for its exclusive prefix and
@@ -103,6 +107,7 @@ __device__ void chainedScanWave(std::uint64_t *status, std::size_t tile, std::ui
 }
 
 // Block-wide exclusive scan of a 64-bit value and OR of a flag.
+template <int THREADS>
 __device__ std::uint64_t blockScan64(std::uint64_t v, bool flag, std::uint64_t *wsum, int *wflag,
                                      std::uint64_t &total, bool &anyFlag)
 {
@@ -126,7 +131,7 @@ __device__ std::uint64_t blockScan64(std::uint64_t v, bool flag, std::uint64_t *
     total = 0;
     anyFlag = false;
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w)
+    for (int w = 0; w < THREADS / 64; ++w)
     {
         if (w < wave) before += wsum[w];
         total += wsum[w];
@@ -220,13 +225,16 @@ struct BinDecider
     }
 };
 
-template <class Dec>
+// THREADS x kItems leaves per tile (kDecideThreads for single trees, kThreads
+// in the batch kernels)
+template <int THREADS, class Dec>
 __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std::int64_t *offs,
                                std::uint64_t *status, std::uint32_t *ticket)
 {
+    constexpr long kTile = long(THREADS) * kItems;
     __shared__ std::uint32_t tileShared;
-    __shared__ std::uint64_t wsum[kThreads / 64];
-    __shared__ int wflag[kThreads / 64];
+    __shared__ std::uint64_t wsum[THREADS / 64];
+    __shared__ int wflag[THREADS / 64];
     __shared__ std::uint64_t prefixShared;
     __shared__ int prefixChgShared;
     const int tid = threadIdx.x;
@@ -240,7 +248,7 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
     __syncthreads();
     const long tile = tileShared;
     const long L = st->numLeaves;
-    const long base = tile * kPassTile;
+    const long base = tile * kTile;
     if (base > L) return; // tiles past the sentinel index hold no operations
     const bool skip = st->converged || st->needsResize;
     const int ab = st->activeBuffer;
@@ -263,7 +271,7 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
     }
     std::uint64_t tileSum;
     bool tileChg;
-    const std::uint64_t local = blockScan64(sum, chg, wsum, wflag, tileSum, tileChg);
+    const std::uint64_t local = blockScan64<THREADS>(sum, chg, wsum, wflag, tileSum, tileChg);
     if (tid < 64)
     {
         std::uint64_t ex;
@@ -304,11 +312,11 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
 }
 
 template <class Dec>
-__global__ __launch_bounds__(kThreads) void decideScanKernel(Dec dec, DevState *st, long capacity,
+__global__ __launch_bounds__(kDecideThreads) void decideScanKernel(Dec dec, DevState *st, long capacity,
                                                              std::int64_t *offs, std::uint64_t *status,
                                                              std::uint32_t *ticket)
 {
-    decideScanBody(dec, st, capacity, offs, status, ticket);
+    decideScanBody<kDecideThreads>(dec, st, capacity, offs, status, ticket);
 }
 
 // ---- octree scatter (processNode) -------------------------------------------
@@ -1171,7 +1179,7 @@ __global__ __launch_bounds__(kThreads) void multiOctDecideScanKernel(TreeSet s, 
 {
     AT_MULTI_PROLOGUE;
     const OctDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, {t.NF[0], t.NF[1]}, {t.gcm[0], t.gcm[1]}, bucket, nf != 0};
-    decideScanBody(d, t.st, t.capacity, t.offs, t.scanStatus, t.scanTicket);
+    decideScanBody<kThreads>(d, t.st, t.capacity, t.offs, t.scanStatus, t.scanTicket);
 }
 __global__ __launch_bounds__(kThreads) void multiOctProcessKernel(TreeSet s, int nf)
 {
@@ -1297,7 +1305,8 @@ void octDecideScan(hipStream_t s, const TreeDev &t, const OctParams &p, long cap
 {
     OctDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, {t.NF[0], t.NF[1]}, {t.gcm[0], t.gcm[1]}, p.bucket, p.nf};
     prof::Scope scope("decide_scan", s);
-    hipLaunchKernelGGL((decideScanKernel<OctDecider>), dim3(unsigned(passTiles(gridLeaves))), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((decideScanKernel<OctDecider>), dim3(unsigned(decideTiles(gridLeaves))), dim3(kDecideThreads),
+                       0, s,
                        d, t.st, capacity, t.offs, t.scanStatus, t.scanTicket);
 }
 void octProcessCommit(hipStream_t s, const TreeDev &t, const OctParams &p, std::size_t gridLeaves)
@@ -1331,7 +1340,8 @@ void binDecideScan(hipStream_t s, const TreeDev &t, const BinParams &p, long cap
 {
     BinDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, p.bucket, p.maxDepth};
     prof::Scope scope("decide_scan", s);
-    hipLaunchKernelGGL((decideScanKernel<BinDecider>), dim3(unsigned(passTiles(gridLeaves))), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((decideScanKernel<BinDecider>), dim3(unsigned(decideTiles(gridLeaves))), dim3(kDecideThreads),
+                       0, s,
                        d, t.st, capacity, t.offs, t.scanStatus, t.scanTicket);
 }
 void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
