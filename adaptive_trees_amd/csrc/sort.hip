@@ -1235,23 +1235,34 @@ __global__ __launch_bounds__(kSortThreads) void msdScanKernel(const std::uint32_
     }
 }
 
+// A value that is the same on every lane, kept in scalar registers.
+__device__ inline Key uniformKey(Key v)
+{
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v))));
+    const unsigned hi = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v >> 32))));
+    return Key(lo) | (Key(hi) << 32);
+}
+
 // Sorts every chunk in LDS. The chunk's keys (and values) are held in
 // registers; a stable counting pass per 8-bit digit (ranking as in
-// scatterKernel) is run for each digit at or above bit kFixShift that is not
-// the same for all of the chunk's keys. Runs of keys that then agree from bit
-// kFixShift up are finished on the lower bits: runs of up to kFixRun keys by
-// an insertion sort (one thread per run), runs of up to kWaveRun keys by a
-// rank sort of one wave per run. If a run is longer, the chunk takes a
-// counting pass on the varying digit just below kFixShift, the high digits
-// again (LSD order) and the same fix-up from bit kFixShift2; if that fails
-// too, every varying digit gets a counting pass (plain LSD order).
-// Chunks with bits == 0 are already in order and are only copied when they
-// lie in the other buffer.
+// scatterKernel) works on d = key - base, base the chunk's smallest key (d
+// orders the chunk's keys as the keys do), and covers the kFixBits highest
+// bits of the chunk's span: from the span's highest bit `top` down to
+// fix = top + 1 - kFixBits (two windows of 8 bits, which need not be byte
+// aligned). A chunk of 2048 keys spread evenly over its span then has runs
+// of 2048 / 2^kFixBits keys on average that agree above `fix`, whatever its
+// depth in the key space; the runs are finished on the lower bits: pairs by
+// one compare-and-swap, runs of up to kFixRun keys by an insertion sort (one
+// thread per run), runs of up to kWaveRun keys by a rank sort of one wave
+// per run. If a run is longer, the chunk takes a counting pass on the 8 bits
+// below `fix`, the high windows again (LSD order) and the same fix-up from
+// bit fix - 8; if that fails too, every bit of the span gets a counting pass
+// (plain LSD order). Chunks with bits == 0 are already in order and are only
+// copied when they lie in the other buffer.
 constexpr std::uint32_t kFixRun = 16;
 constexpr std::uint32_t kWaveRun = 64;
-constexpr int kFixShift = 24;  // counting passes on the digits above; runs equal above it are fixed up
-constexpr int kFixShift2 = 16; // the same after one more pass
-constexpr std::uint32_t kMaxWaveRuns = kMsdChunk / (kFixRun + 1) + 1;
+constexpr int kFixBits = 2 * kRadixBits; // bits sorted by counting passes before the fix-up
+constexpr std::uint32_t kRunCap = kMsdChunk / 3 + 1; // runs of 3 or more keys in a chunk
 
 __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCounters *cnt,
                                                                      const MsdChunk *__restrict__ chunks, Key *X,
@@ -1264,10 +1275,11 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
     __shared__ std::uint32_t waveCount[kW][kRadix];
     __shared__ std::uint32_t sink[kW][64];
     __shared__ std::uint32_t scratch[kRadix / 64];
-    __shared__ Key waveMask[kW];
-    __shared__ std::uint32_t longRun;  // a run longer than kWaveRun was seen
-    __shared__ std::uint32_t waveRuns; // runs of kFixRun+1 .. kWaveRun keys: (start << 8) | length
-    __shared__ std::uint32_t waveRun[kMaxWaveRuns];
+    __shared__ Key waveLo[kW], waveHi[kW];
+    __shared__ std::uint32_t longRun;   // a run longer than kWaveRun was seen
+    __shared__ std::uint32_t shortRuns; // runs of 3 .. kFixRun keys, from the front of runList
+    __shared__ std::uint32_t waveRuns;  // runs of kFixRun+1 .. kWaveRun keys, from the back
+    __shared__ std::uint32_t runList[kRunCap]; // (start << 8) | length
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
@@ -1278,11 +1290,12 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
     const std::uint32_t first = std::uint32_t(wave) * (kLocItems * 64) + lane;
     Key key[kLocItems];
     std::uint32_t val[kLocItems];
+    Key base = 0; // the chunk's smallest key; digits are taken from key - base
 
-    // one stable counting pass on the digit at `shift`; the result is in
-    // the registers (striped order) and in stageKeys / stageVals
-    // `vary`: the digit bits that differ within the chunk (padding keys agree
-    // with the chunk on all other bits)
+    // one stable counting pass on the digit of key - base at `shift`; the
+    // result is in the registers (striped order) and in stageKeys / stageVals
+    // `vary`: the digit bits that may differ within the chunk (padding keys
+    // agree with the chunk on all other bits)
     auto pass = [&](int shift, unsigned vary) __attribute__((always_inline)) {
         for (int i = tid; i < kW * kRadix; i += kLocThreads) (&waveCount[0][0])[i] = 0;
         __syncthreads();
@@ -1291,7 +1304,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int j = 0; j < kLocItems; ++j)
         {
-            const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
+            const unsigned dg = unsigned((key[j] - base) >> shift) & (kRadix - 1);
             const std::uint64_t match = laneMatchVarying(dg, vary);
             const std::uint32_t lower = __popcll(match & ltMask);
             const bool lead = lower == 0;
@@ -1337,7 +1350,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int j = 0; j < kLocItems; ++j)
         {
-            const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)];
+            const std::uint32_t pos = rank[j] + waveCount[wave][unsigned((key[j] - base) >> shift) & (kRadix - 1)];
             stageKeys[pos] = key[j];
             stageVals[pos] = val[j];
         }
@@ -1356,7 +1369,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         const MsdChunk ch = chunks[c];
         const Key *ks = ch.parity ? Y : X;
         const std::uint32_t *vs = ch.parity ? vY : vX;
-        if (ch.bits == 0 || ch.len > kMsdChunk)
+        if (ch.bits == 0 || ch.len > kMsdChunk || ch.len < 2)
         {
             if (ks != kd)
                 for (std::uint32_t i = tid; i < ch.len; i += kLocThreads)
@@ -1366,8 +1379,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                 }
             continue;
         }
-        const Key k0 = ks[ch.begin];
-        Key m = 0;
+        Key lo = ~Key(0), hi = 0;
 #pragma unroll
         for (int j = 0; j < kLocItems; ++j)
         {
@@ -1375,69 +1387,114 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
             const bool in = idx < ch.len;
             const std::uint32_t cidx = in ? idx : 0u;
             const Key k = ks[ch.begin + cidx];
-            const std::uint32_t v = vs[ch.begin + cidx];
-            key[j] = in ? k : kInvalidKey; // padding ranks last in every digit
-            val[j] = v;
-            m |= in ? k ^ k0 : Key(0);
+            key[j] = k;
+            val[j] = vs[ch.begin + cidx];
+            if (in)
+            {
+                lo = k < lo ? k : lo;
+                hi = k > hi ? k : hi;
+            }
         }
-        // bits that differ within the chunk
+        // the chunk's smallest and largest keys
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) m |= Key(__shfl_xor((unsigned long long)m, o, 64));
-        if (lane == 0) waveMask[wave] = m;
-        __syncthreads();
-        m = 0;
-#pragma unroll
-        for (int w = 0; w < kW; ++w) m |= waveMask[w];
-        if (ch.bits < 64) m &= (Key(1) << ch.bits) - 1;
+        for (int o = 32; o > 0; o >>= 1)
         {
-            // padding: the chunk's constant bits with every varying bit set,
-            // the largest value in every digit (it still ranks last)
-            const Key pad = (k0 & ~m) | m;
+            const Key a = Key(__shfl_xor((unsigned long long)lo, o, 64));
+            const Key b = Key(__shfl_xor((unsigned long long)hi, o, 64));
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+        }
+        if (lane == 0) waveLo[wave] = lo, waveHi[wave] = hi;
+        __syncthreads();
+        lo = waveLo[0], hi = waveHi[0];
+#pragma unroll
+        for (int w = 1; w < kW; ++w)
+        {
+            lo = waveLo[w] < lo ? waveLo[w] : lo;
+            hi = waveHi[w] > hi ? waveHi[w] : hi;
+        }
+        base = lo;
+        const Key span = hi - lo; // block-uniform
+        const int top = span ? 63 - __builtin_clzll(span) : 0;
+        const Key dmask = top == 63 ? ~Key(0) : (Key(2) << top) - 1; // the bits key - base may use
+        {
+            // padding: key - base = dmask, the largest value in every digit
+            // (it still ranks last)
+            const Key pad = base + dmask;
 #pragma unroll
             for (int j = 0; j < kLocItems; ++j)
                 if (first + std::uint32_t(j) * 64 >= ch.len) key[j] = pad;
         }
-        // Fix-up of the runs of keys equal from bit `shift` up, with the chunk
-        // sorted on those bits in the stage. Returns false, changing nothing,
-        // when a run is longer than kWaveRun.
+        // Fix-up of the runs of keys whose key - base agree from bit `shift`
+        // up, with the chunk sorted on those bits in the stage. Returns false,
+        // changing nothing, when a run is longer than kWaveRun.
         auto fixRuns = [&](int shift) __attribute__((always_inline)) -> bool {
-            if (tid == 0) longRun = 0, waveRuns = 0;
+            if (tid == 0) longRun = 0, shortRuns = 0, waveRuns = 0;
             __syncthreads();
-            auto high = [&](std::uint32_t q) { return stageKeys[q] >> shift; };
-            // thread t looks at positions t + k*kLocThreads (independent,
-            // conflict-free reads); only run starts walk their run
-            std::uint32_t starts = 0; // bit k: position t + k*kLocThreads starts a run of 2..kFixRun
-#pragma unroll 1
+            auto high = [&](std::uint32_t q) { return (stageKeys[q] - base) >> shift; };
+            // position p = t + k*kLocThreads of thread t: a wave's lanes hold
+            // 64 consecutive positions, so a key's successor is the next
+            // lane's (the row's last lane reads the next row's first key), and
+            // a run's length inside the row follows from one ballot; only a
+            // run that leaves the row walks on in LDS. Nothing is written
+            // before the barrier: every key's high bits stay as read.
+            std::uint32_t pairs = 0; // bit k: position t + k*kLocThreads starts a run of 2
+#pragma unroll 2
             for (int k = 0; k < kLocItems; ++k)
             {
-                const std::uint32_t p0 = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
-                if (p0 + 1 < ch.len)
+                const std::uint32_t p = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
+                const std::uint32_t row = p - lane; // the row's first position (wave-uniform)
+                const Key kp = p < ch.len ? stageKeys[p] : Key(0);
+                const Key nextRow = uniformKey(row + 64 < ch.len ? stageKeys[row + 64] : Key(0));
+                const Key prevRow = uniformKey(row > 0 ? stageKeys[row - 1] : Key(0));
+                Key kn = Key(__shfl_down((unsigned long long)kp, 1, 64));
+                if (lane == 63) kn = nextRow;
+                const Key h = (kp - base) >> shift;
+                const bool eqNext = p + 1 < ch.len && ((kn - base) >> shift) == h;
+                const std::uint64_t nextMask = __ballot(eqNext);
+                const bool eqPrev = lane > 0 ? ((nextMask >> (lane - 1)) & 1u) != 0
+                                             : p > 0 && p < ch.len && ((prevRow - base) >> shift) == h;
+                if (eqNext && !eqPrev)
                 {
-                    const Key h = high(p0);
-                    const bool dupNext = high(p0 + 1) == h;
-                    const bool dupPrev = p0 > 0 && high(p0 - 1) == h;
-                    if (dupNext && !dupPrev)
-                    {
-                        std::uint32_t e = p0 + 2;
-                        while (e < ch.len && e - p0 <= kWaveRun && high(e) == h) ++e;
-                        const std::uint32_t len = e - p0;
-                        if (len > kWaveRun) longRun = 1;
-                        else if (len > kFixRun) waveRun[atomicAdd(&waveRuns, 1u)] = (p0 << 8) | len;
-                        else starts |= 1u << k;
-                    }
+                    // bits lane.. of nextMask: the run's positions that equal their successor
+                    const std::uint64_t rest = ~(nextMask >> lane);
+                    const std::uint32_t ones = rest ? std::uint32_t(__builtin_ctzll(rest)) : 64u;
+                    std::uint32_t e = p + ones + 1;
+                    if (ones == 64u - lane)
+                        while (e < ch.len && e - p <= kWaveRun && high(e) == h) ++e;
+                    const std::uint32_t len = e - p;
+                    if (len == 2) pairs |= 1u << k;
+                    else if (len > kWaveRun) longRun = 1;
+                    else if (len > kFixRun) runList[kRunCap - 1 - atomicAdd(&waveRuns, 1u)] = (p << 8) | len;
+                    else runList[atomicAdd(&shortRuns, 1u)] = (p << 8) | len;
                 }
             }
             __syncthreads();
             if (longRun) return false; // block-uniform
-            // stable insertion sort of each short run (keys of a run agree above
-            // `shift`, so whole keys compare as their low bits)
+            // pairs: one compare-and-swap (stable: equal keys stay)
 #pragma unroll 1
-            for (int k = 0; k < kLocItems; ++k)
+            for (std::uint32_t bitsLeft = pairs; bitsLeft; bitsLeft &= bitsLeft - 1)
+                {
+                    const std::uint32_t p = std::uint32_t(tid) + std::uint32_t(__builtin_ctz(bitsLeft)) * kLocThreads;
+                    const Key a = stageKeys[p], b = stageKeys[p + 1];
+                    if (b < a)
+                    {
+                        const std::uint32_t va = stageVals[p], vb = stageVals[p + 1];
+                        stageKeys[p] = b;
+                        stageKeys[p + 1] = a;
+                        stageVals[p] = vb;
+                        stageVals[p + 1] = va;
+                    }
+                }
+            // runs of 3..kFixRun keys: a stable insertion sort, one thread per
+            // run (keys of a run agree above `shift`, so whole keys compare
+            // as their low bits)
+            const std::uint32_t ns = shortRuns;
+#pragma unroll 1
+            for (std::uint32_t r = std::uint32_t(tid); r < ns; r += kLocThreads)
             {
-                if (!((starts >> k) & 1u)) continue;
-                const std::uint32_t p0 = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
-                const Key h = high(p0);
-                for (std::uint32_t i = p0 + 1; i < ch.len && high(i) == h; ++i)
+                const std::uint32_t e = runList[r], p0 = e >> 8, end = p0 + (e & 0xffu);
+                for (std::uint32_t i = p0 + 1; i < end; ++i)
                 {
                     const Key kk = stageKeys[i];
                     const std::uint32_t vv = stageVals[i];
@@ -1458,7 +1515,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll 1
             for (std::uint32_t r = std::uint32_t(wave); r < nw; r += kW)
             {
-                const std::uint32_t e = waveRun[r], p0 = e >> 8, len = e & 0xffu;
+                const std::uint32_t e = runList[kRunCap - 1 - r], p0 = e >> 8, len = e & 0xffu;
                 const bool in = lane < len;
                 const Key kk = in ? stageKeys[p0 + lane] : Key(0);
                 const std::uint32_t vv = in ? stageVals[p0 + lane] : 0u;
@@ -1480,15 +1537,6 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
             __syncthreads();
             return true;
         };
-        auto toStage = [&]() __attribute__((always_inline)) {
-#pragma unroll
-            for (int j = 0; j < kLocItems; ++j)
-            {
-                stageKeys[first + std::uint32_t(j) * 64] = key[j];
-                stageVals[first + std::uint32_t(j) * 64] = val[j];
-            }
-            __syncthreads();
-        };
         auto fromStage = [&]() __attribute__((always_inline)) {
 #pragma unroll
             for (int j = 0; j < kLocItems; ++j)
@@ -1497,38 +1545,36 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                 val[j] = stageVals[first + std::uint32_t(j) * 64];
             }
         };
-        // counting passes on the varying digits in [lo, 64), lowest first
+        // counting passes on the digits of key - base in [lo, 64) that mask
+        // covers, lowest first
         auto passesFrom = [&](int lo, Key mask) __attribute__((always_inline)) {
-            bool any = false;
             for (int shift = lo; shift < 64; shift += kRadixBits)
-                if ((mask >> shift) & (kRadix - 1))
-                {
-                    pass(shift, unsigned(mask >> shift) & (kRadix - 1));
-                    any = true;
-                }
-            return any;
+                if ((mask >> shift) & (kRadix - 1)) pass(shift, unsigned(mask >> shift) & (kRadix - 1));
         };
-        const Key hiMask = m & ~((Key(1) << kFixShift) - 1);
-        // passes leave the result in the registers and the stage
-        if (!passesFrom(kFixShift, hiMask)) toStage();
-        if (m & ((Key(1) << kFixShift) - 1))
+        if (span)
         {
-            bool done = fixRuns(kFixShift);
-            if (!done)
+            // passes on [fix, top], the kFixBits highest bits of the span;
+            // they leave the result in the registers and the stage
+            const int fix = top >= kFixBits ? top + 1 - kFixBits : 0;
+            passesFrom(fix, dmask & ~((Key(1) << fix) - 1));
+            if (fix > 0)
             {
-                // the digit below kFixShift, then the high digits again
-                const Key hiMask2 = m & ~((Key(1) << kFixShift2) - 1);
-                __syncthreads(); // the passes rewrite the stage
-                if (!passesFrom(kFixShift2, hiMask2)) toStage();
-                if (m & ((Key(1) << kFixShift2) - 1)) done = fixRuns(kFixShift2);
-                else done = true;
+                bool done = fixRuns(fix);
                 if (!done)
                 {
-                    __syncthreads();
-                    passesFrom(0, m);
+                    // the 8 bits below fix, then the high windows again
+                    const int fix2 = fix > kRadixBits ? fix - kRadixBits : 0;
+                    __syncthreads(); // the passes rewrite the stage
+                    passesFrom(fix2, dmask & ~((Key(1) << fix2) - 1));
+                    done = fix2 == 0 || fixRuns(fix2);
+                    if (!done)
+                    {
+                        __syncthreads();
+                        passesFrom(0, dmask);
+                    }
                 }
+                if (done) fromStage();
             }
-            if (done) fromStage();
         }
 #pragma unroll
         for (int j = 0; j < kLocItems; ++j)
@@ -1540,7 +1586,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                 vd[ch.begin + idx] = val[j];
             }
         }
-        __syncthreads(); // waveMask, longRun and the stage are rewritten by the next chunk
+        __syncthreads(); // waveLo / waveHi, the run lists and the stage are rewritten by the next chunk
     }
 }
 

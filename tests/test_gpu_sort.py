@@ -19,6 +19,19 @@ from adaptive_trees_amd import DeviceArray, _ffi, tune
 MASK63 = np.uint64((1 << 63) - 1)
 
 
+def chunk_groups(n, rng, per_group, low_bits):
+    """Keys that form LDS chunks of ~1500 keys agreeing above bit 48 (the
+    MSD levels stop at the second digit), whose highest varying bit is 47: a
+    group id g with ~1500 / per_group values ends at bit 47, random bits fill
+    [0, low_bits), and the bits between are zero. The chunk's passes cover
+    bits 32..47, so keys agree above bit 32 in runs of ~per_group (one run
+    per g), and above bit 24 in runs of ~per_group / 2^(32 - low_bits)."""
+    c = rng.integers(0, max(n // 1500, 1), n, dtype=np.uint64) << np.uint64(48)
+    groups = max(1500 // per_group, 2)
+    g = rng.integers(0, groups, n, dtype=np.uint64) << np.uint64(48 - groups.bit_length())
+    return c | g | rng.integers(0, 2**low_bits, n, dtype=np.uint64)
+
+
 def keys_of(kind, n, rng):
     if kind == "uniform":
         return rng.integers(0, 2**64 - 1, n, dtype=np.uint64, endpoint=True)
@@ -36,23 +49,19 @@ def keys_of(kind, n, rng):
         hi = (rng.normal(0, 1, n) * 3 + 100).astype(np.int64).clip(0, 255).astype(np.uint64) << np.uint64(56)
         return hi | rng.integers(0, 2**56, n, dtype=np.uint64)
     if kind == "short_runs":
-        # bits 24.. take about n/3 values: short runs that agree above bit 24
-        hi = rng.integers(0, n // 3 + 1, n, dtype=np.uint64) << np.uint64(24)
-        return hi | rng.integers(0, 2**24, n, dtype=np.uint64)
+        # runs of ~5 keys that agree above bit 32 (the chunk's passes cover
+        # bits 32..47): the insertion fix-up
+        return chunk_groups(n, rng, 5, 32)
     if kind == "wave_runs":
-        # bits 24.. take about n/30 values: runs of 17..64 keys that agree above
-        # bit 24 (the chunk sort ranks them one wave per run)
-        hi = rng.integers(0, n // 30 + 1, n, dtype=np.uint64) << np.uint64(24)
-        return hi | rng.integers(0, 2**24, n, dtype=np.uint64)
+        # runs of 17..64 keys that agree above bit 32: one wave per run
+        return chunk_groups(n, rng, 30, 32)
     if kind == "mid_runs":
-        # runs of ~200 that agree above bit 24 but are short above bit 16 (the
+        # runs of ~200 that agree above bit 32 but are short above bit 24 (the
         # chunk sort's first fallback)
-        hi = rng.integers(0, n // 200 + 1, n, dtype=np.uint64) << np.uint64(24)
-        return hi | rng.integers(0, 2**24, n, dtype=np.uint64)
+        return chunk_groups(n, rng, 190, 32)
     if kind == "long_runs":
-        # bits 24.. take 2^14 values: runs longer than the chunk fix-up handles
-        hi = rng.integers(0, 64, n, dtype=np.uint64) << np.uint64(40)
-        return hi | rng.integers(0, 2**32, n, dtype=np.uint64)
+        # runs of ~200 that agree above bit 24: both fix-ups fail, plain LSD
+        return chunk_groups(n, rng, 190, 24)
     if kind == "low_bits":
         # chunks whose keys differ only below bit 16: every fix-up fails and
         # the chunk takes plain LSD passes
