@@ -240,6 +240,15 @@ struct BinaryGeom
     int prefixAxis;     // the only axis of the prefix levels, or -1 if they mix axes
     double inv[3];      // correctly rounded 1 / len
     int fastDiv;        // 1: every len and inv lies in [2^-1000, 2^1000] (binaryUnit's fast form)
+    // Branch-free form (binaryKeyFast): fastDiv, a periodic schedule whose
+    // prefix levels all split one axis, and at most 32 key bits per axis.
+    // Slot i of the cycle splits axis order[i] and contributes the top n_i
+    // bits of that axis's 63-bit fixed point: n_i = triples, plus prefix for
+    // the prefix axis.
+    int fast;
+    double slotLo[3], slotLen[3], slotInv[3]; // lo, len, inv of axis order[i]
+    double slotScale[3];      // 2^n_i
+    std::uint32_t slotMax[3]; // 2^n_i - 1
 };
 
 // (x - lo) / len, as far as fixed63 can tell: the correctly rounded quotient
@@ -310,6 +319,50 @@ __host__ __device__ inline Key binaryInterleave(Key f0, Key f1, Key f2, const Bi
         c <<= 1;
     }
     return key;
+}
+
+// binaryKey for geometries with g.fast, without branches. Per slot: the
+// quotient q as binaryUnit computes it (selects for its two early returns),
+// then the top n bits of fixed63(q) directly as floor(q * 2^n) -- exact, as
+// the scaling is by a power of two -- with 2^n - 1 at q = 1 (fixed63 clamps
+// to 2^63 - 2 there) and 0 for NaN. The prefix bits are the bits above the
+// low `triples` of the prefix slot (the other slots have none), and the
+// three slots are interleaved a byte at a time through `spread`, the
+// 256-entry table of spread3.
+template <class T>
+__host__ __device__ inline Key binaryKeyFast(T x, T y, T z, const BinaryGeom &g, const std::uint32_t *spread)
+{
+    std::uint32_t t[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const int a = g.order[i];
+        const double av = double(a == 0 ? x : (a == 1 ? y : z)) - g.slotLo[i];
+        const double len = g.slotLen[i], inv = g.slotInv[i];
+        double q = av * inv;
+        double r = fma(-len, q, av);
+        q = fma(r, inv, q);
+        r = fma(-len, q, av);
+        q = fma(r, inv, q);
+        q = av <= 0.0 ? 0.0 : q;
+        q = av >= len ? 1.0 : q;
+        const double s = q * g.slotScale[i];
+        const bool one = s >= g.slotScale[i];
+        const double sc = s >= 0.0 && !one ? s : 0.0; // NaN -> 0
+        t[i] = one ? g.slotMax[i] : std::uint32_t(sc);
+    }
+    const int m = g.triples;
+    const std::uint32_t hi = (t[0] | t[1] | t[2]) >> m;
+    const std::uint32_t mask = (std::uint32_t(1) << m) - 1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t[i] &= mask;
+    auto bytes = [&](int sh) {
+        return (spread[(t[0] >> sh) & 255u] << 2) | (spread[(t[1] >> sh) & 255u] << 1) | spread[(t[2] >> sh) & 255u];
+    };
+    const Key full = Key(bytes(0)) | (Key(bytes(8)) << 24) | (Key(bytes(16)) << 48);
+    const int R = int(kBinLevels) - g.prefix;
+    const Key key = (Key(hi) << R) | (full >> (3 * m - R));
+    return isEscaped<T>(x, y, z) ? kInvalidKey : key;
 }
 
 template <class T>

@@ -134,30 +134,21 @@ __device__ inline void load4(const double *a, std::size_t i, double v[4])
     v[0] = q0.x, v[1] = q0.y, v[2] = q1.x, v[3] = q1.y;
 }
 
-// Part `blockIdx.x % parts` of a range: its keys and a partial digit-0
-// histogram. With 16-byte aligned coordinate arrays each thread takes groups
-// of 4 consecutive particles (16-byte loads and stores; parts start at
-// multiples of 4), two groups in flight; otherwise kKeygenUnroll single
-// particles.
-template <class T, KeyKind K>
-__global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, std::size_t n, std::size_t rangeKeys,
-                                                             int parts, Key *__restrict__ keys, std::uint32_t *partHist,
-                                                             int histShift)
+template <class T, KeyKind K, bool FAST>
+__device__ inline Key keyOf(const KeygenParams<T> &p, const std::uint32_t *spread, T x, T y, T z)
 {
-    // one histogram per wave: the lowest digit is close to uniform, so plain
-    // LDS atomics on a wave-private table collide least
-    __shared__ std::uint32_t hw[kWaves][kRadix];
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) hw[w][threadIdx.x] = 0;
-    __syncthreads();
-    std::uint32_t *h = hw[threadIdx.x >> 6];
-    const std::size_t row = xcdRow(blockIdx.x, gridDim.x);
-    const std::size_t r = row / unsigned(parts), part = row % unsigned(parts);
-    const std::size_t partKeys = rangeKeys / std::size_t(parts);
-    const std::size_t rEnd = (r + 1) * rangeKeys < n ? (r + 1) * rangeKeys : n;
-    const std::size_t begin = r * rangeKeys + part * partKeys;
-    const std::size_t end0 = begin + partKeys;
-    const std::size_t end = end0 < rEnd ? end0 : rEnd;
+    if constexpr (FAST) return binaryKeyFast<T>(x, y, z, p.geom, spread);
+    else return coordKey<T, K>(p, x, y, z);
+}
+
+// Keys of particles [begin, end) into keys, their digits at histShift
+// counted in h. With 16-byte aligned coordinate arrays each thread takes
+// groups of 4 consecutive particles (16-byte loads and stores), several
+// groups in flight; otherwise kKeygenUnroll single particles.
+template <class T, KeyKind K, bool FAST>
+__device__ __forceinline__ void keygenPart(const KeygenParams<T> &p, const std::uint32_t *spread, std::uint32_t *h,
+                                           Key *__restrict__ keys, std::size_t begin, std::size_t end, int histShift)
+{
     const bool vec = ((reinterpret_cast<std::uintptr_t>(p.x) | reinterpret_cast<std::uintptr_t>(p.y) |
                        reinterpret_cast<std::uintptr_t>(p.z) | reinterpret_cast<std::uintptr_t>(keys)) & 15) == 0 &&
                      begin % 4 == 0;
@@ -186,7 +177,7 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
             {
                 Key k[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) k[u] = coordKey<T, K>(p, x[g][u], y[g][u], z[g][u]);
+                for (int u = 0; u < 4; ++u) k[u] = keyOf<T, K, FAST>(p, spread, x[g][u], y[g][u], z[g][u]);
                 ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(keys + i + g * kGroupStep);
                 dst[0] = make_ulonglong2(k[0], k[1]);
                 dst[1] = make_ulonglong2(k[2], k[3]);
@@ -204,7 +195,11 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
         {
             Key k[kKeygenUnroll];
 #pragma unroll
-            for (int u = 0; u < kKeygenUnroll; ++u) k[u] = particleKey<T, K>(p, i + std::size_t(u) * kSortThreads);
+            for (int u = 0; u < kKeygenUnroll; ++u)
+            {
+                const std::size_t j = i + std::size_t(u) * kSortThreads;
+                k[u] = keyOf<T, K, FAST>(p, spread, p.x[j], p.y[j], p.z[j]);
+            }
 #pragma unroll
             for (int u = 0; u < kKeygenUnroll; ++u)
             {
@@ -215,10 +210,44 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     }
     for (; i < end; i += kSortThreads)
     {
-        const Key k = particleKey<T, K>(p, i);
+        const Key k = keyOf<T, K, FAST>(p, spread, p.x[i], p.y[i], p.z[i]);
         keys[i] = k;
         atomicAdd(&h[unsigned(k >> histShift) & (kRadix - 1)], 1u);
     }
+}
+
+// Part `blockIdx.x % parts` of a range: its keys and a partial digit-0
+// histogram (parts start at multiples of 4). Binary keys take the
+// branch-free form when the geometry allows it (a block-uniform choice).
+template <class T, KeyKind K>
+__global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, std::size_t n, std::size_t rangeKeys,
+                                                             int parts, Key *__restrict__ keys, std::uint32_t *partHist,
+                                                             int histShift)
+{
+    static_assert(kSortThreads == kRadix, "one table entry and one digit per thread");
+    // one histogram per wave: the lowest digit is close to uniform, so plain
+    // LDS atomics on a wave-private table collide least
+    __shared__ std::uint32_t hw[kWaves][kRadix];
+    __shared__ std::uint32_t spread[K == KeyKind::Binary ? kRadix : 1]; // spread3 of every byte
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) hw[w][threadIdx.x] = 0;
+    if constexpr (K == KeyKind::Binary) spread[threadIdx.x] = std::uint32_t(spread3(threadIdx.x));
+    __syncthreads();
+    std::uint32_t *h = hw[threadIdx.x >> 6];
+    const std::size_t row = xcdRow(blockIdx.x, gridDim.x);
+    const std::size_t r = row / unsigned(parts), part = row % unsigned(parts);
+    const std::size_t partKeys = rangeKeys / std::size_t(parts);
+    const std::size_t rEnd = (r + 1) * rangeKeys < n ? (r + 1) * rangeKeys : n;
+    const std::size_t begin = r * rangeKeys + part * partKeys;
+    const std::size_t end0 = begin + partKeys;
+    const std::size_t end = end0 < rEnd ? end0 : rEnd;
+    if constexpr (K == KeyKind::Binary)
+    {
+        if (p.geom.fast) keygenPart<T, K, true>(p, spread, h, keys, begin, end, histShift);
+        else keygenPart<T, K, false>(p, spread, h, keys, begin, end, histShift);
+    }
+    else
+        keygenPart<T, K, false>(p, spread, h, keys, begin, end, histShift);
     __syncthreads();
     std::uint32_t t = 0;
 #pragma unroll

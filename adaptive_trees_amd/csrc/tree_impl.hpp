@@ -101,6 +101,19 @@ BinaryGeom makeBinaryGeom(const at_box &box, unsigned maxDepth)
         const double lo = std::ldexp(1.0, -1000), hi = std::ldexp(1.0, 1000);
         if (!(g.len[a] >= lo && g.len[a] <= hi && g.inv[a] >= lo && g.inv[a] <= hi)) g.fastDiv = 0;
     }
+    // the branch-free form (binaryKeyFast)
+    g.fast = g.fastDiv && g.prefix < int(kBinLevels) && (g.prefix == 0 || g.prefixAxis >= 0) ? 1 : 0;
+    for (int i = 0; i < 3 && g.fast; ++i)
+    {
+        const int a = g.order[i];
+        const int bits = g.triples + (g.prefix > 0 && a == g.prefixAxis ? g.prefix : 0);
+        if (bits > 32) g.fast = 0;
+        g.slotLo[i] = g.lo[a];
+        g.slotLen[i] = g.len[a];
+        g.slotInv[i] = g.inv[a];
+        g.slotScale[i] = std::ldexp(1.0, bits);
+        g.slotMax[i] = bits == 32 ? 0xffffffffu : (std::uint32_t(1) << bits) - 1;
+    }
     return g;
 }
 
