@@ -1591,15 +1591,36 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                 bool done = fixRuns(fix);
                 if (!done)
                 {
+                    // A run longer than kWaveRun (duplicate-heavy keys). The
+                    // fallback passes take only the bits of key - base that
+                    // vary within the chunk -- their OR, as the smallest key's
+                    // is 0 -- skipping windows in which no key differs (a
+                    // chunk of two distinct values takes one pass), with the
+                    // padding set to that OR (the largest value in every
+                    // digit, equal to the keys on the other bits).
+                    Key vary = 0;
+#pragma unroll
+                    for (int j = 0; j < kLocItems; ++j)
+                        if (first + std::uint32_t(j) * 64 < ch.len) vary |= key[j] - base;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) vary |= Key(__shfl_xor((unsigned long long)vary, o, 64));
+                    if (lane == 0) waveLo[wave] = vary;
+                    __syncthreads(); // also: the passes below rewrite the stage
+                    vary = 0;
+#pragma unroll
+                    for (int w = 0; w < kW; ++w) vary |= waveLo[w];
+#pragma unroll
+                    for (int j = 0; j < kLocItems; ++j)
+                        if (first + std::uint32_t(j) * 64 >= ch.len) key[j] = base + vary;
                     // the 8 bits below fix, then the high windows again
                     const int fix2 = fix > kRadixBits ? fix - kRadixBits : 0;
-                    __syncthreads(); // the passes rewrite the stage
-                    passesFrom(fix2, dmask & ~((Key(1) << fix2) - 1));
-                    done = fix2 == 0 || fixRuns(fix2);
+                    const Key below2 = (Key(1) << fix2) - 1;
+                    passesFrom(fix2, vary & ~below2);
+                    done = (vary & below2) == 0 || fixRuns(fix2);
                     if (!done)
                     {
                         __syncthreads();
-                        passesFrom(0, dmask);
+                        passesFrom(0, vary);
                     }
                 }
                 if (done) fromStage();

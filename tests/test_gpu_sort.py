@@ -62,6 +62,10 @@ def keys_of(kind, n, rng):
     if kind == "long_runs":
         # runs of ~200 that agree above bit 24: both fix-ups fail, plain LSD
         return chunk_groups(n, rng, 190, 24)
+    if kind == "two_values":
+        # chunks of two distinct keys (bit 46 apart) in runs of ~750: the
+        # fallback passes take only the window holding the varying bit
+        return chunk_groups(n, rng, 750, 0)
     if kind == "low_bits":
         # chunks whose keys differ only below bit 16: every fix-up fails and
         # the chunk takes plain LSD passes
@@ -89,7 +93,7 @@ def msd(request):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["uniform", "invalid_mix", "few", "dup_runs", "clustered", "short_runs",
-                                  "wave_runs", "mid_runs", "long_runs", "low_bits"])
+                                  "wave_runs", "mid_runs", "long_runs", "two_values", "low_bits"])
 @pytest.mark.parametrize("n", [1000, 300_001, (1 << 20) + 777, 3_000_017])
 def test_sort_pairs_matches_stable_argsort(msd, kind, n):
     print("sort", msd, kind, n, flush=True)
