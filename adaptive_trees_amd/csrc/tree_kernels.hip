@@ -564,62 +564,93 @@ __device__ inline bool dirtyLeader(const Key *tree, const std::uint8_t *gDirty, 
 // their own NF becomes one group item (its 8 stencils share a 4x4x4 block of
 // cells); other dirty leaves are single items, dirty group leaders parent
 // items. Items are appended with one atomic per list per block.
+// Block `block` classifies ROWS rows of kThreads leaves, then takes its
+// places in both lists with one atomic each (every block's count on the same
+// two counters: with one row per block, 4.7M leaves made ~18K serialised
+// atomics per counter and pass).
+template <int ROWS>
 __device__ void nfCollectBody(const TreeDev &t, int full, unsigned block)
 {
-    __shared__ std::uint32_t waveCnt[kThreads / 64][2];
+    __shared__ std::uint32_t waveCnt[ROWS][kThreads / 64][2];
     __shared__ std::uint32_t blockBase[2];
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
-    const int i = int(block * blockDim.x + threadIdx.x);
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
-    bool own = false, lead = false, grp = false;
-    if (i < L)
-    {
-        own = full || t.nfDirty[i];
-        lead = dirtyLeader(tree, full ? nullptr : t.groupDirty, L, i);
-        unsigned level = 0;
-        const int s = own ? siblingIndex(tree, L, i, level) : -1;
-        if (s >= 0)
-        {
-            const int g = i - s;
-            bool all = true;
-            if (!full)
-                for (int k = 0; k < 8; ++k) all = all && t.nfDirty[g + k] != 0;
-            if (all)
-            {
-                own = false;
-                grp = s == 0;
-            }
-        }
-    }
-    const std::uint64_t bo = __ballot(own), bl = __ballot(lead), bg = __ballot(grp);
     const unsigned lane = __lane_id();
     const int wave = threadIdx.x >> 6;
-    const std::uint32_t no = __popcll(bo), nl = __popcll(bl), ng = __popcll(bg);
-    if (lane == 0)
+    std::uint32_t flags = 0; // bits 3r, 3r+1, 3r+2: own, lead, grp of row r
+#pragma unroll 1
+    for (int r = 0; r < ROWS; ++r)
     {
-        waveCnt[wave][0] = no + nl;
-        waveCnt[wave][1] = ng;
+        const int i = int((block * unsigned(ROWS) + unsigned(r)) * blockDim.x + threadIdx.x);
+        bool own = false, lead = false, grp = false;
+        if (i < L)
+        {
+            own = full || t.nfDirty[i];
+            lead = dirtyLeader(tree, full ? nullptr : t.groupDirty, L, i);
+            unsigned level = 0;
+            const int s = own ? siblingIndex(tree, L, i, level) : -1;
+            if (s >= 0)
+            {
+                const int g = i - s;
+                bool all = true;
+                if (!full)
+                    for (int k = 0; k < 8; ++k) all = all && t.nfDirty[g + k] != 0;
+                if (all)
+                {
+                    own = false;
+                    grp = s == 0;
+                }
+            }
+        }
+        flags |= (std::uint32_t(own) | std::uint32_t(lead) << 1 | std::uint32_t(grp) << 2) << (3 * r);
+        const std::uint64_t bo = __ballot(own), bl = __ballot(lead), bg = __ballot(grp);
+        if (lane == 0)
+        {
+            waveCnt[r][wave][0] = std::uint32_t(__popcll(bo) + __popcll(bl));
+            waveCnt[r][wave][1] = std::uint32_t(__popcll(bg));
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0)
     {
+        // exclusive prefix over (row, wave), in place
         std::uint32_t a = 0, b = 0;
-        for (int w = 0; w < kThreads / 64; ++w) a += waveCnt[w][0], b += waveCnt[w][1];
+        for (int r = 0; r < ROWS; ++r)
+            for (int w = 0; w < kThreads / 64; ++w)
+            {
+                const std::uint32_t ca = waveCnt[r][w][0], cb = waveCnt[r][w][1];
+                waveCnt[r][w][0] = a;
+                waveCnt[r][w][1] = b;
+                a += ca;
+                b += cb;
+            }
         blockBase[0] = a ? atomicAdd(&t.nfCount[0], a) : 0u;
         blockBase[1] = b ? atomicAdd(&t.nfCount[1], b) : 0u;
     }
     __syncthreads();
-    std::uint32_t baseIdx = blockBase[0], baseGrp = blockBase[1];
-    for (int w = 0; w < wave; ++w) baseIdx += waveCnt[w][0], baseGrp += waveCnt[w][1];
     const std::uint64_t lt = (std::uint64_t(1) << lane) - 1;
-    if (own) t.nfTasks[baseIdx + __popcll(bo & lt)] = std::uint32_t(i) << 1;
-    if (lead) t.nfTasks[baseIdx + no + __popcll(bl & lt)] = (std::uint32_t(i) << 1) | 1u;
-    if (grp) t.nfGroups[baseGrp + __popcll(bg & lt)] = std::uint32_t(i);
+#pragma unroll 1
+    for (int r = 0; r < ROWS; ++r)
+    {
+        const int i = int((block * unsigned(ROWS) + unsigned(r)) * blockDim.x + threadIdx.x);
+        const bool own = (flags >> (3 * r)) & 1u, lead = (flags >> (3 * r + 1)) & 1u,
+                   grp = (flags >> (3 * r + 2)) & 1u;
+        const std::uint64_t bo = __ballot(own), bl = __ballot(lead), bg = __ballot(grp);
+        const std::uint32_t baseIdx = blockBase[0] + waveCnt[r][wave][0];
+        const std::uint32_t baseGrp = blockBase[1] + waveCnt[r][wave][1];
+        if (own) t.nfTasks[baseIdx + __popcll(bo & lt)] = std::uint32_t(i) << 1;
+        if (lead) t.nfTasks[baseIdx + __popcll(bo) + __popcll(bl & lt)] = (std::uint32_t(i) << 1) | 1u;
+        if (grp) t.nfGroups[baseGrp + __popcll(bg & lt)] = std::uint32_t(i);
+    }
 }
-__global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full) { nfCollectBody(t, full, blockIdx.x); }
+constexpr int kCollectRows = 8; // rows of kThreads leaves per block (single trees)
+__global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full)
+{
+    nfCollectBody<kCollectRows>(t, full, blockIdx.x);
+}
 
 // Near-field count of the level-`level` node at `start`: its own population
 // times the population of the 27 same-level cells, periodic at 2^21 per axis
@@ -1199,7 +1230,7 @@ __global__ __launch_bounds__(kThreads) void multiLeafCountsKernel(TreeSet s, int
 __global__ __launch_bounds__(kThreads) void multiNfCollectKernel(TreeSet s, int full)
 {
     AT_MULTI_PROLOGUE;
-    nfCollectBody(t, full, block);
+    nfCollectBody<1>(t, full, block);
 }
 __global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsigned bucket, int hilbert)
 {
@@ -1325,7 +1356,8 @@ void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, 
         return;
     }
     prof::Scope scope("near_field", s);
-    hipLaunchKernelGGL(nfCollectKernel, dim3(ceilDiv(gridLeaves, kThreads)), dim3(kThreads), 0, s, t, full ? 1 : 0);
+    hipLaunchKernelGGL(nfCollectKernel, dim3(ceilDiv(gridLeaves, kThreads * kCollectRows)), dim3(kThreads), 0, s, t,
+                       full ? 1 : 0);
     hipLaunchKernelGGL(nfComputeKernel, dim3(2048), dim3(kThreads), 0, s, t, p.bucket, p.hilbert ? 1 : 0);
     hipLaunchKernelGGL(nfGroupKernel, dim3(2048), dim3(kThreads), 0, s, t, p.hilbert ? 1 : 0);
 }
