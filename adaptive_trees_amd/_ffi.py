@@ -186,6 +186,7 @@ def hip_lib():
             "tune_get": [ctypes.c_char_p, ctypes.POINTER(_I)],
             "tune_auto": [_SZ, _P, ctypes.c_char_p, _SZ],
             "sort_pairs": [_P, _P, _SZ, _I, _P],
+            "sort_pairs_noalloc": [_P, _P, _SZ, _I, _P],
             "batch_create": [_I, _I, _SZ, _P, _P, _P, _P, ctypes.POINTER(Config), ctypes.POINTER(_P)],
             "batch_destroy": [_P],
             "batch_build": [_P],

@@ -405,6 +405,15 @@ int at_sort_pairs(void *keys, void *vals, size_t n, int slot_values, void *strea
         AT_HIP(hipStreamSynchronize(hipStream_t(stream))); // before the workspace is released
     });
 }
+int at_sort_pairs_noalloc(void *keys, void *vals, size_t n, int slot_values, void *stream)
+{
+    return guarded([&] {
+        if (n && (!keys || !vals)) throw at::Error(AT_EINVAL, "keys and values must be device pointers");
+        at::sortPairsNoAlloc(hipStream_t(stream), static_cast<at::Key *>(keys), static_cast<std::uint32_t *>(vals), n,
+                             slot_values != 0);
+        AT_HIP(hipGetLastError());
+    });
+}
 int at_tune_auto(size_t n, void *stream, char *report, size_t report_len)
 {
     return guarded([&] {

@@ -79,9 +79,9 @@ bool query(const char *label, double *ms, unsigned long long *launches)
     return false;
 }
 
-Scope::Scope(const char *label, hipStream_t s) : s_(s)
+Scope::Scope(const char *label, hipStream_t s, bool active) : s_(s)
 {
-    if (!g_on) return;
+    if (!g_on || !active) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     hipEvent_t start = nullptr;

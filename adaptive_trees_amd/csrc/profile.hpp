@@ -14,11 +14,12 @@ void reset();
 bool query(const char *label, double *ms, unsigned long long *launches);
 
 // Records a start/stop event pair around the launches in its lifetime when
-// timing is enabled and the stream is not being captured into a graph.
+// timing is enabled, `active` is set and the stream is not being captured
+// into a graph. An inactive scope makes no HIP call and no allocation.
 class Scope
 {
 public:
-    Scope(const char *label, hipStream_t s);
+    Scope(const char *label, hipStream_t s, bool active = true);
     ~Scope();
     Scope(const Scope &) = delete;
     Scope &operator=(const Scope &) = delete;

@@ -25,6 +25,7 @@
 // on the 256 CUs the prefix chain made the look-back cost ~45% of a pass.
 #include "sort.hpp"
 
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -1702,7 +1703,7 @@ void dispatchScatter(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, const
 }
 
 void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Key *kB, std::uint32_t *vA,
-               std::uint32_t *vB, bool vals, bool iota, bool firstHistWhole)
+               std::uint32_t *vB, bool vals, bool iota, bool firstHistWhole, bool timed = true)
 {
     for (int pass = 0; pass < kSortPasses; ++pass)
     {
@@ -1713,7 +1714,7 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         const int shift = pass * kRadixBits;
         if (pass > 0)
         {
-            prof::Scope scope("sort_upsweep", s);
+            prof::Scope scope("sort_upsweep", s, timed);
             if (g_digitArray)
                 hipLaunchKernelGGL(upsweepDigitsKernel, dim3(unsigned((pl.ranges + kUpRows - 1) / kUpRows)),
                                    dim3(kUpRows * 64), 0, s, ws.digits, pl.n, pl.rangeKeys, pl.ranges, ws.rangeHist);
@@ -1727,7 +1728,7 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
         if (pass > 0 || firstHistWhole) scanPlan.parts = 1;
         scanOffsets(s, scanPlan, ws);
         const bool digits = g_digitArray && pass + 1 < kSortPasses;
-        prof::Scope scope("sort_pass", s);
+        prof::Scope scope("sort_pass", s, timed);
         const bool first = iota && pass == 0;
         dispatchScatter(s, pl, ws, src, dst, vsrc, vdst, vals, first, digits, shift, shift + kRadixBits);
     }
@@ -1942,6 +1943,13 @@ void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_
 
 void SortWorkspace::release()
 {
+    if (borrowed)
+    {
+        keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
+        digitTotal = nullptr;
+        capacity = 0; maxRanges = 0; values = false; borrowed = false;
+        return;
+    }
     // owners synchronise their stream before the workspace goes away
     for (void *p : {(void *)keysAlt, (void *)valsAlt, (void *)digits, (void *)rangeHist, (void *)rangeOff,
                     (void *)digitTotal})
@@ -2017,6 +2025,81 @@ void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws)
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
                        0, ws.rangeHist);
     runPasses(s, pl, ws, keys, ws.keysAlt, nullptr, nullptr, false, false, true);
+}
+
+// ---- sort without allocation ------------------------------------------------------
+// The LSD path of sortPairs over a workspace carved from `scratch`. Its
+// largest plan: kNoAllocMaxKeys keys in tiles of the smallest shape (256 x 8
+// keys) gives the most ranges, each with one histogram row of 256 digits.
+namespace
+{
+constexpr std::size_t naAlign(std::size_t b) { return (b + 255) & ~std::size_t(255); }
+constexpr std::size_t kNaRanges = (kNoAllocMaxKeys + 8 * kSortThreads - 1) / (8 * kSortThreads);
+constexpr std::size_t kNaKeys = 0;
+constexpr std::size_t kNaVals = kNaKeys + naAlign(kNoAllocMaxKeys * sizeof(Key));
+constexpr std::size_t kNaDigits = kNaVals + naAlign(kNoAllocMaxKeys * sizeof(std::uint32_t));
+constexpr std::size_t kNaHist = kNaDigits + naAlign(kNoAllocMaxKeys + 16);
+constexpr std::size_t kNaOff = kNaHist + naAlign(kNaRanges * kRadix * sizeof(std::uint32_t));
+constexpr std::size_t kNaTotal = kNaOff + naAlign(kNaRanges * kRadix * sizeof(std::uint32_t));
+constexpr std::size_t kNaBytes = kNaTotal + naAlign(kRadix * sizeof(std::uint32_t));
+} // namespace
+
+// 14.1 MiB per GPU. Host code never uses this name as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace
+{
+constexpr int kNaMaxDevices = 64;
+std::mutex g_naMu;
+unsigned char *g_naBase[kNaMaxDevices] = {};
+
+// Device address of the current GPU's scratch array, looked up once per
+// device with that device current.
+unsigned char *noAllocBase()
+{
+    int dev = 0;
+    AT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) throw Error(1, "device index out of range for the no-alloc sort");
+    std::lock_guard<std::mutex> lk(g_naMu);
+    if (!g_naBase[dev])
+    {
+        void *ptr = nullptr;
+        AT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw Error(4, "hipGetSymbolAddress returned no address for the sort scratch array");
+        g_naBase[dev] = static_cast<unsigned char *>(ptr);
+    }
+    return g_naBase[dev];
+}
+} // namespace
+
+std::size_t noAllocScratchBytes() { return kNaBytes; }
+
+void sortPairsNoAlloc(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, bool iota)
+{
+    if (n > kNoAllocMaxKeys)
+        throw Error(3, "sort_pairs_noalloc sorts at most " + std::to_string(kNoAllocMaxKeys) + " pairs; got " +
+                           std::to_string(n));
+    if (n == 0) return;
+    unsigned char *base = noAllocBase();
+    const SortPlan pl = makeSortPlan(n);
+    if (pl.ranges > kNaRanges) throw Error(1, "sort plan exceeds the no-alloc workspace");
+    SortWorkspace ws;
+    ws.borrowed = true;
+    ws.keysAlt = reinterpret_cast<Key *>(base + kNaKeys);
+    ws.valsAlt = reinterpret_cast<std::uint32_t *>(base + kNaVals);
+    ws.digits = base + kNaDigits;
+    ws.rangeHist = reinterpret_cast<std::uint32_t *>(base + kNaHist);
+    ws.rangeOff = reinterpret_cast<std::uint32_t *>(base + kNaOff);
+    ws.digitTotal = reinterpret_cast<std::uint32_t *>(base + kNaTotal);
+    ws.capacity = kNoAllocMaxKeys;
+    ws.maxRanges = kNaRanges;
+    ws.values = true;
+    // every histogram row and offset the passes read is written first in the
+    // same call, so nothing needs clearing
+    hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
+                       0, ws.rangeHist);
+    runPasses(s, pl, ws, keys, ws.keysAlt, vals, ws.valsAlt, true, iota, true, false);
 }
 
 // ---- autotuner ------------------------------------------------------------------

@@ -124,6 +124,7 @@ struct SortWorkspace
     MsdChunk *msdChunks = nullptr;
     std::uint32_t *msdHostCount = nullptr; // pinned: a level's segment count read by the host
     std::size_t msdCapacity = 0, msdMaxSegs = 0, msdStride = 0, msdMaxChunks = 0, msdParents = 0;
+    bool borrowed = false; // pointers into memory the workspace does not own (release() only forgets them)
 
     void allocate(std::size_t cap, bool withValues, hipStream_t s);
     // MSD lists for sorts of up to cap keys whose level 1 has `parents`
@@ -156,6 +157,14 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
 void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws,
                bool valuesAreIndices);
 void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws);
+
+// Stable sort of at most kNoAllocMaxKeys pairs, like sortPairs, whose
+// workspace is a fixed device array of the library (one per GPU, held until
+// the process ends): no allocation of any kind. Asynchronous on `s`; every
+// call on one GPU shares the array, so calls that overlap on one GPU race.
+constexpr std::size_t kNoAllocMaxKeys = std::size_t(1) << 20;
+void sortPairsNoAlloc(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, bool valuesAreIndices);
+std::size_t noAllocScratchBytes();
 
 // ---- segmented sort: many independent key arrays in one set of launches ----
 // Segment s holds counts[s] particles at [segBegin[s], segBegin[s+1]) of one

@@ -290,6 +290,17 @@ int at_tune_auto(size_t n, void *stream, char *report, size_t report_len);
  * key sort (no reference counterpart: the reference calls
  * cub::DeviceRadixSort::SortPairs). n < 2^32. */
 int at_sort_pairs(void *keys, void *vals, size_t n, int slot_values, void *stream);
+/* The same sort with nothing allocated: no counterpart in the reference (nor
+ * an interface it replaces); for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Its workspace is a fixed device array of the library, 14.1 MiB
+ * on each GPU it runs on, held until the process ends, so n is limited to
+ * 2^20 pairs (larger n fails with AT_ELENGTH before any launch). Results
+ * equal at_sort_pairs. Asynchronous: returns once the work is queued on
+ * `stream`; synchronise before reading keys/vals. Every call on one GPU
+ * shares the array, so calls that overlap on one GPU (other streams or
+ * threads) race; calls on one stream are ordered. */
+int at_sort_pairs_noalloc(void *keys, void *vals, size_t n, int slot_values, void *stream);
 
 /* ---- particle input formats (host memory) ----------------------------------
  * Text particle files of the reference's tests (tests/particle_io.hpp:35-139,
