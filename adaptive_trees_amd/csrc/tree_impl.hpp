@@ -201,7 +201,11 @@ public:
     }
     std::size_t buildLeaves() const { return buildL_; }
     std::size_t buildNewLeaves() const { return buildNewL_; }
-    void buildPrePass() { ensureOps(buildL_ + 1); }
+    // Physical buffers of the build passes follow the capacities of the
+    // mirrored vectors (which grow with 25% headroom), not the exact leaf
+    // counts: a pass that adds a few leaves then reuses the buffers instead
+    // of reallocating every one of them (a new pool block each time).
+    void buildPrePass() { ensureOps(std::max(buildL_ + 1, vK_.cap)); }
     // After the decision pass: false when the tree did not change (done).
     bool buildAfterDecide(const DevState &s)
     {
@@ -209,7 +213,7 @@ public:
         buildNewL_ = std::size_t(s.newNumLeaves);
         // the reference grows the target buffers with 25% headroom
         vTmp_.resizeHeadroom(buildNewL_ + 1);
-        ensureInactive(buildNewL_ + 1);
+        ensureInactive(std::max(buildNewL_ + 1, vTmp_.cap));
         return true;
     }
     void buildAfterProcess()

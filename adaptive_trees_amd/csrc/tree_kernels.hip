@@ -320,22 +320,24 @@ __global__ __launch_bounds__(kDecideThreads) void decideScanKernel(Dec dec, DevS
 }
 
 // ---- octree scatter (processNode) -------------------------------------------
+// One thread per old leaf writes a kept leaf and the children of splits into
+// 8; the children of the level-jumping splits (64, 512 or 4096) are written
+// by the whole wave, one such leaf after the other, so a pass that splits a
+// few leaves into thousands of children each does not run on a few lanes.
 __device__ void octProcessBody(const TreeDev &t, bool nf, unsigned block)
 {
     DevState *st = t.st;
     if (st->converged) return;
     const int L = st->numLeaves;
     const int i = int(block * blockDim.x + threadIdx.x);
-    if (i >= L) return;
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
     Key *out = t.K[ab ^ 1];
     const std::int64_t *ops = t.offs;
     if (i == 0) out[st->newNumLeaves] = nodeRange(0);
-    const int code = int(ops[i + 1] - ops[i]);
-    if (code == 0) return;
-    const Key node = tree[i];
-    const std::int64_t dst = ops[i];
+    const int code = i < L ? int(ops[i + 1] - ops[i]) : 0;
+    const Key node = i < L && code != 0 ? tree[i] : Key(0);
+    const std::int64_t dst = i < L ? ops[i] : 0;
     if (code == 1)
     {
         out[dst] = node;
@@ -379,19 +381,44 @@ __device__ void octProcessBody(const TreeDev &t, bool nf, unsigned block)
                     t.groupDirty[dst] = 1;
             }
         }
-        return;
     }
-    const unsigned level = treeLevel(tree[i + 1] - node);
-    const unsigned diff = code == 8 ? 1u : (code == 64 ? 2u : (code == 512 ? 3u : 4u));
-    const Key r = nodeRange(level + diff);
-    for (int s = 0; s < code; ++s)
+    Key r = 0;
+    if (code > 1)
     {
-        out[dst + s] = node + Key(s) * r;
-        if (!nf && t.countDirty) t.countDirty[dst + s] = 1; // counted by the refresh
-        if (nf)
+        const unsigned level = treeLevel(tree[i + 1] - node);
+        const unsigned diff = code == 8 ? 1u : (code == 64 ? 2u : (code == 512 ? 3u : 4u));
+        r = nodeRange(level + diff);
+    }
+    if (code == 8)
+        for (int s = 0; s < 8; ++s)
         {
-            t.nfDirty[dst + s] = 1;
-            t.groupDirty[dst + s] = 1;
+            out[dst + s] = node + Key(s) * r;
+            if (!nf && t.countDirty) t.countDirty[dst + s] = 1; // counted by the refresh
+            if (nf)
+            {
+                t.nfDirty[dst + s] = 1;
+                t.groupDirty[dst + s] = 1;
+            }
+        }
+    // level jumps: every lane of the wave takes part (the lanes past L too)
+    std::uint64_t big = __ballot(code > 8);
+    const unsigned lane = __lane_id();
+    while (big)
+    {
+        const int src = __ffsll(static_cast<unsigned long long>(big)) - 1;
+        big &= big - 1;
+        const int n = __shfl(code, src, 64);
+        const Key first = __shfl(node, src, 64), step = __shfl(r, src, 64);
+        const std::int64_t base = __shfl(dst, src, 64);
+        for (int s = int(lane); s < n; s += 64)
+        {
+            out[base + s] = first + Key(s) * step;
+            if (!nf && t.countDirty) t.countDirty[base + s] = 1;
+            if (nf)
+            {
+                t.nfDirty[base + s] = 1;
+                t.groupDirty[base + s] = 1;
+            }
         }
     }
 }
