@@ -1,466 +1,358 @@
-// bench_driver.cpp -- the reference's benchmark driver protocol and CSV
-// schema (benchmarks/drivers/common.hpp:182-510 with the v3 adapter of
-// benchmarks/drivers/driver_v3.cu) over this library's C++ facade:
+// bench_driver.cpp -- runs one dataset of the reference's benchmark harness
+// through one tree of this library and writes the harness's CSV rows, so
+// the reference's analysis scripts read the results like their own
+// (protocol and columns: benchmarks/drivers/common.hpp:182-489 with the v3
+// adapter of benchmarks/drivers/driver_v3.cu).
 //
 //   bench_driver --manifest M --config octree_leafcount|octree_nfcount|kdtree3d_leafcount
 //                --mode steps|cold --version V --commit H --run R --out FILE [--max-snapshots N]
 //
-// Frame 0 is built from the root (plus views); "cold" writes one row with
-// its times. "steps" writes a row for frame 0 and then, for every further
-// frame, uploads the coordinates and runs one update + views -- replayed
-// from a captured HIP graph, or enqueued directly when BENCH_V3_PATH=direct
-// -- repeating it after a capacity growth (timed as maintenance) while the
-// tree reports needsResize. Structure columns are recounted from P and K.
-// Exit status 3 = out of memory, 1 = any other failure.
-#include <hip/hip_runtime_api.h>
-
-#include <algorithm>
-#include <bit>
-#include <chrono>
-#include <cmath>
-#include <cstdint>
-#include <cstdlib>
+// Frame 0 is built from the root and given all views ("cold" writes one row
+// with those times and stops). "steps" then loads each further frame and
+// runs one update + views: replayed from a captured HIP graph (with event
+// nodes around the two phases), or enqueued directly when
+// BENCH_V3_PATH=direct. A step whose tree reports needsResize grows the
+// buffers (timed as maintenance), is captured again and is repeated, at
+// most 64 attempts. Structure columns are recounted from P and K.
+// Exit status: 0 done, 3 out of device/host memory, 1 anything else.
+#include <cstdio>
 #include <cstring>
-#include <fstream>
-#include <iomanip>
 #include <iostream>
-#include <limits>
-#include <memory>
-#include <new>
-#include <sstream>
-#include <stdexcept>
 #include <string>
 #include <vector>
 
-#include "adaptive_octree/tree.hpp"
-
-namespace ao = adaptive_octree;
+#include "harness.hpp"
 
 namespace
 {
-using Key = ao::KeyType;
-using Clock = std::chrono::steady_clock;
-constexpr Key kRootEnd = Key(1) << 63;
-constexpr int kOomExit = 3;
+using namespace harness;
 
-void hipOk(hipError_t e, const char *what)
+// The three configurations of the harness (common.hpp:196-202): tree type,
+// criterion and bucket (LeafCount 64, NFCount 64*64*27).
+struct Setup
 {
-    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
-}
-#define BENCH_HIP(x) hipOk((x), #x)
-
-void ioOk(int status)
-{
-    if (status == AT_ENOMEM) throw std::bad_alloc();
-    if (status != AT_OK) throw std::runtime_error(at_last_error());
-}
-
-// ---- input: manifest and frames through the library's readers ----
-struct Frame
-{
-    int index;
-    long id;
-    std::string format, path;
+    const char *name;
+    int tree, criterion;
+    unsigned bucket;
 };
-struct Manifest
-{
-    std::string dataset;
-    std::size_t slots = 0;
-    double box[6]{};
-    std::vector<Frame> frames;
+constexpr Setup kSetups[] = {
+    {"octree_leafcount", AT_TREE_OCTREE, AT_LEAF_COUNT, 64},
+    {"octree_nfcount", AT_TREE_OCTREE, AT_NF_COUNT, 64u * 64u * 27u},
+    {"kdtree3d_leafcount", AT_TREE_BINARY, AT_LEAF_COUNT, 64},
 };
 
-Manifest readManifest(const std::string &path)
+const Setup &setupNamed(const std::string &name)
 {
-    void *h = nullptr;
-    ioOk(at_io_manifest_open(path.c_str(), &h));
-    std::unique_ptr<void, void (*)(void *)> guard(h, at_io_manifest_close);
-    Manifest m;
-    std::vector<char> name(4096), fmt(64), fpath(8192);
-    std::size_t frames = 0;
-    ioOk(at_io_manifest_info(h, name.data(), name.size(), &m.slots, m.box, &frames));
-    m.dataset = name.data();
-    for (std::size_t i = 0; i < frames; ++i)
-    {
-        Frame f{};
-        ioOk(at_io_manifest_frame(h, i, &f.index, &f.id, fmt.data(), fmt.size(), fpath.data(), fpath.size()));
-        f.format = fmt.data();
-        f.path = fpath.data();
-        m.frames.push_back(f);
-    }
-    return m;
-}
-
-void readFrame(const Frame &f, std::vector<double> &x, std::vector<double> &y, std::vector<double> &z)
-{
-    double *xyz = nullptr;
-    std::size_t n = 0;
-    ioOk(at_io_read_frame(f.path.c_str(), f.format.c_str(), &xyz, &n));
-    std::unique_ptr<double, void (*)(void *)> guard(xyz, at_io_free);
-    x.assign(xyz, xyz + n);
-    y.assign(xyz + n, xyz + 2 * n);
-    z.assign(xyz + 2 * n, xyz + 3 * n);
-}
-
-// ---- configuration and arguments (common.hpp:184-231) ----
-enum class Kind { Octree, KDTree3D };
-struct RunConfig
-{
-    std::string name;
-    Kind tree;
-    bool nf;
-    unsigned limit;
-    unsigned levelBits() const { return tree == Kind::Octree ? 3u : 1u; }
-};
-
-RunConfig parseConfig(const std::string &name)
-{
-    if (name == "octree_leafcount") return {name, Kind::Octree, false, 64};
-    if (name == "octree_nfcount") return {name, Kind::Octree, true, 64u * 64u * 27u};
-    if (name == "kdtree3d_leafcount") return {name, Kind::KDTree3D, false, 64};
+    for (const Setup &s : kSetups)
+        if (name == s.name) return s;
     throw std::invalid_argument("unknown config " + name);
 }
 
-struct Args
+struct FrameRef
 {
-    std::string manifest, config, mode = "steps", version, commit, out;
-    int run = 0;
-    int maxSnapshots = -1;
+    int index = 0;
+    long id = 0;
+    std::string format, path;
+};
+struct Dataset
+{
+    std::string name;
+    std::size_t slots = 0;
+    double box[6]{};
+    std::vector<FrameRef> frames;
 };
 
-Args parseArgs(int argc, char **argv)
+Dataset openManifest(const std::string &path)
 {
-    Args a;
-    for (int i = 1; i + 1 < argc; i += 2)
+    void *m = nullptr;
+    HARNESS_LIB(at_io_manifest_open(path.c_str(), &m));
+    std::unique_ptr<void, void (*)(void *)> hold(m, at_io_manifest_close);
+    Dataset d;
+    std::vector<char> name(4096), fmt(64), file(8192);
+    std::size_t frames = 0;
+    HARNESS_LIB(at_io_manifest_info(m, name.data(), name.size(), &d.slots, d.box, &frames));
+    d.name = name.data();
+    d.frames.resize(frames);
+    for (std::size_t i = 0; i < frames; ++i)
     {
-        const std::string k = argv[i], v = argv[i + 1];
-        if (k == "--manifest") a.manifest = v;
-        else if (k == "--config") a.config = v;
-        else if (k == "--mode") a.mode = v;
-        else if (k == "--version") a.version = v;
-        else if (k == "--commit") a.commit = v;
-        else if (k == "--run") a.run = std::stoi(v);
-        else if (k == "--out") a.out = v;
-        else if (k == "--max-snapshots") a.maxSnapshots = std::stoi(v);
-        else throw std::invalid_argument("unknown option " + k);
+        FrameRef &f = d.frames[i];
+        HARNESS_LIB(at_io_manifest_frame(m, i, &f.index, &f.id, fmt.data(), fmt.size(), file.data(), file.size()));
+        f.format = fmt.data();
+        f.path = file.data();
     }
-    if (a.manifest.empty() || a.config.empty() || a.out.empty() || (a.mode != "steps" && a.mode != "cold"))
-        throw std::invalid_argument("usage: --manifest M --config C --mode steps|cold --version V --commit H "
-                                    "--run R --out FILE [--max-snapshots N]");
-    return a;
+    return d;
 }
 
-// ---- structure, recounted from P and K (common.hpp:233-264) ----
-struct Structure
+void loadFrame(const FrameRef &f, std::vector<double> (&xyz)[3])
 {
-    std::size_t leaves = 0, nodes = 0, emptyLeaves = 0, active = 0, slots = 0;
-    unsigned maxDepth = 0;
-};
-
-Structure structure(const std::vector<Key> &P, const std::vector<Key> &K, unsigned levelBits)
-{
-    Structure s;
-    s.slots = P.size();
-    s.leaves = K.size() - 1;
-    if (K.front() != 0 || K.back() != kRootEnd) throw std::runtime_error("K does not span the root");
-    std::vector<std::size_t> pos(K.size());
-    for (std::size_t i = 0; i < K.size(); ++i)
-        pos[i] = std::size_t(std::lower_bound(P.begin(), P.end(), K[i]) - P.begin());
-    s.active = pos.back() - pos.front();
-    for (std::size_t i = 0; i + 1 < K.size(); ++i)
-    {
-        s.emptyLeaves += pos[i + 1] == pos[i];
-        const unsigned rangeBits = unsigned(std::bit_width(K[i + 1] - K[i]) - 1);
-        s.maxDepth = std::max(s.maxDepth, (63u - rangeBits) / levelBits);
-    }
-    s.nodes = s.leaves + (s.leaves - 1) / ((1u << levelBits) - 1);
-    return s;
+    double *cols = nullptr;
+    std::size_t n = 0;
+    HARNESS_LIB(at_io_read_frame(f.path.c_str(), f.format.c_str(), &cols, &n));
+    std::unique_ptr<double, void (*)(void *)> hold(cols, at_io_free);
+    for (int a = 0; a < 3; ++a) xyz[a].assign(cols + std::size_t(a) * n, cols + std::size_t(a + 1) * n);
 }
 
-// ---- timing (common.hpp:268-298) ----
-struct Events
-{
-    // begin/treeEnd/end bracket the phases (inside the graph on the graph
-    // path); outerBegin/outerEnd bracket the whole submission
-    hipEvent_t outerBegin{}, begin{}, treeEnd{}, end{}, outerEnd{};
-    Events()
-    {
-        for (auto *e : {&outerBegin, &begin, &treeEnd, &end, &outerEnd}) BENCH_HIP(hipEventCreate(e));
-    }
-    ~Events()
-    {
-        for (auto e : {outerBegin, begin, treeEnd, end, outerEnd}) (void)hipEventDestroy(e);
-    }
-};
-
-double elapsed(hipEvent_t a, hipEvent_t b)
-{
-    float ms = 0;
-    BENCH_HIP(hipEventElapsedTime(&ms, a, b));
-    return ms;
-}
-double msSince(Clock::time_point t0) { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
-
-// ---- CSV (common.hpp:300-336) ----
-std::string num(double v)
-{
-    if (std::isnan(v)) return "nan";
-    std::ostringstream s;
-    s << std::setprecision(10) << v;
-    return s.str();
-}
-
-constexpr const char *kStepHeader =
-    "version,commit,path,config,dataset,run,snapshot,frame_id,tree_update_ms,view_ms,total_ms,wall_ms,"
-    "attempts,maintenance_ms,iterations,num_nodes,num_leaves,max_depth,max_depth_octree_equiv,"
-    "num_empty_leaves,empty_leaf_pct,avg_particles_per_leaf,particles_active,particles_escaped,particle_slots,"
-    "lib_num_nodes";
-constexpr const char *kColdHeader =
-    "version,commit,path,config,dataset,process_rep,initial_build_ms,initial_tree_gpu_ms,initial_view_gpu_ms,"
-    "num_leaves,num_nodes,max_depth";
-
-struct StepTiming
-{
-    double tree = NAN, view = NAN, total = NAN, wall = NAN, maintenance = 0;
-    int attempts = 0, iterations = -1;
-};
-
-void writeStep(std::ostream &o, const Args &a, const char *path, const RunConfig &c, const Manifest &m,
-               const Frame &f, const StepTiming &t, const Structure &s, double libNodes)
-{
-    const double maxDepthEquiv = c.tree == Kind::KDTree3D ? s.maxDepth / 3.0 : double(s.maxDepth);
-    o << a.version << ',' << a.commit << ',' << path << ',' << c.name << ',' << m.dataset << ',' << a.run << ','
-      << f.index << ',' << f.id << ',' << num(t.tree) << ',' << num(t.view) << ',' << num(t.total) << ','
-      << num(t.wall) << ',' << t.attempts << ',' << num(t.maintenance) << ',' << t.iterations << ',' << s.nodes
-      << ',' << s.leaves << ',' << s.maxDepth << ',' << num(maxDepthEquiv) << ',' << s.emptyLeaves << ','
-      << num(100.0 * double(s.emptyLeaves) / double(s.leaves)) << ',' << num(double(s.active) / double(s.leaves))
-      << ',' << s.active << ',' << s.slots - s.active << ',' << s.slots << ',' << num(libNodes) << '\n';
-    o.flush();
-}
-
-// ---- the v3 adapter (driver_v3.cu:14-121) on Tree<double, Octree|Binary> ----
-template <ao::TreeType TK>
-class V3
+// One step of the protocol: update() then all views, bracketed by the
+// "begin" / "tree" / "views" events -- captured once into a graph (event
+// nodes record when the graph runs) or enqueued directly on every step.
+class Stepper
 {
 public:
-    using Tree = ao::Tree<double, TK>;
-    static bool useGraph()
+    Stepper(TreeHandle &tree, hipStream_t s, bool graph, EventSet &ev) : tree_(tree), s_(s), graph_(graph), ev_(ev) {}
+    ~Stepper() { drop(); }
+    void capture()
     {
-        static const bool g = !(std::getenv("BENCH_V3_PATH") && std::string(std::getenv("BENCH_V3_PATH")) == "direct");
-        return g;
-    }
-    static const char *pathName() { return useGraph() ? "graph" : "direct"; }
-    static bool supports(const RunConfig &c) { return (c.tree == Kind::KDTree3D) == (TK == ao::TreeType::Binary); }
-    static hipStream_t stream()
-    {
-        static hipStream_t s = [] {
-            hipStream_t t;
-            BENCH_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
-            return t;
-        }();
-        return s;
-    }
-
-    V3(const RunConfig &c, const Manifest &m, double *x, double *y, double *z, std::size_t n, hipStream_t s) : s_(s)
-    {
-        typename Tree::Config cfg;
-        cfg.maxParticlesPerLeaf = c.limit;
-        cfg.splitCriterion = c.nf ? ao::SplitCriterion::NFCount : ao::SplitCriterion::LeafCount;
-        cfg.sfcKind = ao::SfcKind::Morton;
-        cfg.box = {m.box[0], m.box[1], m.box[2], m.box[3], m.box[4], m.box[5]};
-        cfg.stream = s;
-        tree_ = std::make_unique<Tree>(x, y, z, n, cfg);
-    }
-    ~V3() { discard(); }
-
-    void build() { tree_->build(); }
-    void views()
-    {
-        tree_->computeViews(kAllViews);
-        while (tree_->doBuffersNeedResize())
+        if (!graph_) return;
+        drop();
+        HARNESS_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeGlobal));
+        try
         {
-            tree_->resizeBuffers();
-            tree_->computeViews(kAllViews);
+            enqueue(hipEventRecordExternal);
         }
+        catch (...)
+        {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        HARNESS_HIP(hipStreamEndCapture(s_, &g));
+        const hipError_t e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HARNESS_HIP(e);
+        HARNESS_HIP(hipGraphUpload(exec_, s_));
+        HARNESS_HIP(hipStreamSynchronize(s_));
     }
-    void prepare(Events &ev)
+    void step()
     {
-        if (!useGraph()) return;
-        BENCH_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeGlobal));
-        enqueue(ev, hipEventRecordExternal);
-        BENCH_HIP(hipStreamEndCapture(s_, &graph_));
-        BENCH_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
-        BENCH_HIP(hipGraphUpload(exec_, s_));
-        BENCH_HIP(hipStreamSynchronize(s_));
+        if (graph_) HARNESS_HIP(hipGraphLaunch(exec_, s_));
+        else enqueue(hipEventRecordDefault);
     }
-    void step(Events &ev)
+    void grow()
     {
-        if (useGraph()) BENCH_HIP(hipGraphLaunch(exec_, s_));
-        else enqueue(ev, hipEventRecordDefault);
+        drop();
+        tree_.resizeBuffers();
+        capture();
     }
-    bool needsResize() { return tree_->doBuffersNeedResize(); }
-    void resize(Events &ev)
-    {
-        discard();
-        tree_->resizeBuffers();
-        prepare(ev);
-    }
-    std::vector<Key> P() const { return tree_->particleKeys_d().to_host(); }
-    std::vector<Key> K() const { return tree_->cornerstone_d().to_host(); }
-    double libNodes() const { return double(tree_->numNodes()); }
-    int iterations() const { return -1; }
 
 private:
-    static constexpr ao::ViewConfig kAllViews{true, true, true};
-    // the same event placement on both paths (inside the graph on the graph path)
-    void enqueue(Events &ev, unsigned flags)
+    void enqueue(unsigned flags)
     {
-        BENCH_HIP(hipEventRecordWithFlags(ev.begin, s_, flags));
-        tree_->update();
-        BENCH_HIP(hipEventRecordWithFlags(ev.treeEnd, s_, flags));
-        tree_->computeViews(kAllViews);
-        BENCH_HIP(hipEventRecordWithFlags(ev.end, s_, flags));
+        HARNESS_HIP(hipEventRecordWithFlags(ev_["begin"], s_, flags));
+        tree_.update();
+        HARNESS_HIP(hipEventRecordWithFlags(ev_["tree"], s_, flags));
+        tree_.allViews();
+        HARNESS_HIP(hipEventRecordWithFlags(ev_["views"], s_, flags));
     }
-    void discard()
+    void drop()
     {
         if (exec_) (void)hipGraphExecDestroy(exec_);
-        if (graph_) (void)hipGraphDestroy(graph_);
         exec_ = nullptr;
-        graph_ = nullptr;
     }
+    TreeHandle &tree_;
     hipStream_t s_;
-    std::unique_ptr<Tree> tree_;
-    hipGraph_t graph_{};
-    hipGraphExec_t exec_{};
+    bool graph_;
+    EventSet &ev_;
+    hipGraphExec_t exec_ = nullptr;
 };
 
-// common.hpp:349-489
-template <class Adapter>
-int runDriver(int argc, char **argv)
+const std::vector<std::string> kStepColumns = {
+    "version", "commit", "path", "config", "dataset", "run", "snapshot", "frame_id", "tree_update_ms", "view_ms",
+    "total_ms", "wall_ms", "attempts", "maintenance_ms", "iterations", "num_nodes", "num_leaves", "max_depth",
+    "max_depth_octree_equiv", "num_empty_leaves", "empty_leaf_pct", "avg_particles_per_leaf", "particles_active",
+    "particles_escaped", "particle_slots", "lib_num_nodes"};
+const std::vector<std::string> kColdColumns = {
+    "version", "commit", "path", "config", "dataset", "process_rep", "initial_build_ms", "initial_tree_gpu_ms",
+    "initial_view_gpu_ms", "num_leaves", "num_nodes", "max_depth"};
+
+struct Options
 {
-    try
+    std::string manifest, config, mode = "steps", version, commit, out;
+    int run = 0, maxSnapshots = -1;
+};
+
+struct StepTimes
+{
+    double tree = NAN, view = NAN, total = NAN, wall = NAN, maintenance = 0;
+    int attempts = 0;
+};
+
+int drive(int argc, char **argv)
+{
+    Options o;
+    Flags flags;
+    flags.text("--manifest", &o.manifest);
+    flags.text("--config", &o.config);
+    flags.text("--mode", &o.mode);
+    flags.text("--version", &o.version);
+    flags.text("--commit", &o.commit);
+    flags.text("--out", &o.out);
+    flags.integer("--run", &o.run);
+    flags.integer("--max-snapshots", &o.maxSnapshots);
+    const std::string usage = "usage: --manifest M --config C --mode steps|cold --version V --commit H --run R "
+                              "--out FILE [--max-snapshots N]";
+    flags.parse(argc, argv, usage);
+    if (o.manifest.empty() || o.config.empty() || o.out.empty() || (o.mode != "steps" && o.mode != "cold"))
+        throw std::invalid_argument(usage);
+    const Setup &setup = setupNamed(o.config);
+    const bool binary = setup.tree == AT_TREE_BINARY;
+    const char *env = std::getenv("BENCH_V3_PATH");
+    const bool graph = !(env && std::string(env) == "direct");
+    const char *pathName = graph ? "graph" : "direct";
+
+    HARNESS_HIP(hipFree(nullptr)); // runtime start-up, outside the measurements
+    const Dataset d = openManifest(o.manifest);
+    std::size_t freeB = 0, totalB = 0;
+    HARNESS_HIP(hipMemGetInfo(&freeB, &totalB));
+    std::cerr << "GPU memory free/total [MiB]: " << (freeB >> 20) << '/' << (totalB >> 20) << '\n';
+
+    std::vector<double> xyz[3];
+    loadFrame(d.frames.front(), xyz);
+    if (xyz[0].size() != d.slots) throw std::runtime_error("frame 0 has a slot count other than the manifest's");
+    DeviceXYZ coords(d.slots);
+    coords.upload(xyz);
+
+    hipStream_t s = nullptr;
+    HARNESS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::unique_ptr<std::FILE, int (*)(std::FILE *)> out(std::fopen(o.out.c_str(), "w"), std::fclose);
+    if (!out) throw std::runtime_error("cannot open " + o.out);
+    EventSet ev{"outer_begin", "begin", "tree", "views", "outer_end"};
+
+    at_config cfg{};
+    cfg.max_depth = -1;
+    cfg.update_iterations = -1;
+    cfg.max_particles_per_leaf = setup.bucket;
+    cfg.split_criterion = setup.criterion;
+    cfg.sfc_kind = AT_MORTON;
+    cfg.box = at_box{d.box[0], d.box[1], d.box[2], d.box[3], d.box[4], d.box[5]};
+    cfg.stream = s;
+
+    // frame 0: construction + build from the root + first views
+    HARNESS_HIP(hipDeviceSynchronize());
+    const auto t0 = Clock::now();
+    HARNESS_HIP(hipEventRecord(ev["begin"], s));
+    TreeHandle tree(setup.tree, coords, cfg);
+    tree.build();
+    HARNESS_HIP(hipEventRecord(ev["tree"], s));
+    tree.viewsUntilSized();
+    HARNESS_HIP(hipEventRecord(ev["views"], s));
+    HARNESS_HIP(hipDeviceSynchronize());
+    const double coldWall = msSince(t0);
+
+    auto measure = [&] { return census(tree.fetch<Key>(AT_BUF_PARTICLE_KEYS), tree.fetch<Key>(AT_BUF_CORNERSTONE), binary); };
+    const LeafCensus first = measure();
+    if (first.slots != d.slots) throw std::runtime_error("P does not hold one key per slot");
+
+    if (o.mode == "cold")
     {
-        const Args args = parseArgs(argc, argv);
-        const RunConfig config = parseConfig(args.config);
-        if (!Adapter::supports(config)) throw std::invalid_argument("config not supported by this version");
-        BENCH_HIP(hipFree(nullptr)); // context creation, outside every measurement
-
-        const Manifest m = readManifest(args.manifest);
-        std::size_t freeB = 0, totalB = 0;
-        BENCH_HIP(hipMemGetInfo(&freeB, &totalB));
-        std::cerr << "GPU memory free/total [MiB]: " << (freeB >> 20) << '/' << (totalB >> 20) << '\n';
-
-        std::vector<double> x, y, z;
-        readFrame(m.frames.front(), x, y, z);
-        if (x.size() != m.slots) throw std::runtime_error("slot count differs from manifest");
-        const std::size_t n = x.size();
-        double *x_d = nullptr, *y_d = nullptr, *z_d = nullptr;
-        BENCH_HIP(hipMalloc(&x_d, std::max<std::size_t>(1, n) * sizeof(double)));
-        BENCH_HIP(hipMalloc(&y_d, std::max<std::size_t>(1, n) * sizeof(double)));
-        BENCH_HIP(hipMalloc(&z_d, std::max<std::size_t>(1, n) * sizeof(double)));
-        auto upload = [&] {
-            BENCH_HIP(hipMemcpy(x_d, x.data(), n * sizeof(double), hipMemcpyHostToDevice));
-            BENCH_HIP(hipMemcpy(y_d, y.data(), n * sizeof(double), hipMemcpyHostToDevice));
-            BENCH_HIP(hipMemcpy(z_d, z.data(), n * sizeof(double), hipMemcpyHostToDevice));
-        };
-        upload();
-
-        const hipStream_t s = Adapter::stream();
-        Events ev;
-        std::ofstream out(args.out);
-        if (!out) throw std::runtime_error("cannot open " + args.out);
-
-        // initial build: allocation + build from the root + first views
-        BENCH_HIP(hipDeviceSynchronize());
-        const auto t0 = Clock::now();
-        BENCH_HIP(hipEventRecord(ev.begin, s));
-        Adapter tree(config, m, x_d, y_d, z_d, n, s);
-        tree.build();
-        BENCH_HIP(hipEventRecord(ev.treeEnd, s));
-        tree.views();
-        BENCH_HIP(hipEventRecord(ev.end, s));
-        BENCH_HIP(hipDeviceSynchronize());
-        const double buildWall = msSince(t0);
-        const Structure s0 = structure(tree.P(), tree.K(), config.levelBits());
-        if (s0.slots != n) throw std::runtime_error("P size differs from slot count");
-
-        if (args.mode == "cold")
-        {
-            out << kColdHeader << '\n'
-                << args.version << ',' << args.commit << ',' << Adapter::pathName() << ',' << config.name << ','
-                << m.dataset << ',' << args.run << ',' << num(buildWall) << ',' << num(elapsed(ev.begin, ev.treeEnd))
-                << ',' << num(elapsed(ev.treeEnd, ev.end)) << ',' << s0.leaves << ',' << s0.nodes << ','
-                << s0.maxDepth << '\n';
-            return 0;
-        }
-
-        out << kStepHeader << '\n';
-        StepTiming none;
-        none.iterations = tree.iterations();
-        writeStep(out, args, Adapter::pathName(), config, m, m.frames.front(), none, s0, tree.libNodes());
-        tree.prepare(ev);
-
-        const std::size_t last = args.maxSnapshots > 0 ? std::min<std::size_t>(args.maxSnapshots, m.frames.size())
-                                                       : m.frames.size();
-        for (std::size_t k = 1; k < last; ++k)
-        {
-            const Frame &f = m.frames[k];
-            readFrame(f, x, y, z);
-            if (x.size() != n) throw std::runtime_error("slot count changed in " + f.path);
-            upload();
-
-            StepTiming t{0, 0, 0, 0, 0, 0, -1};
-            bool resize = false;
-            do
-            {
-                BENCH_HIP(hipDeviceSynchronize());
-                const auto w0 = Clock::now();
-                BENCH_HIP(hipEventRecord(ev.outerBegin, s));
-                tree.step(ev);
-                BENCH_HIP(hipEventRecord(ev.outerEnd, s));
-                resize = tree.needsResize();
-                BENCH_HIP(hipDeviceSynchronize());
-                t.wall += msSince(w0);
-                t.tree += elapsed(ev.begin, ev.treeEnd);
-                t.view += elapsed(ev.treeEnd, ev.end);
-                t.total += elapsed(ev.outerBegin, ev.outerEnd);
-                ++t.attempts;
-                if (resize)
-                {
-                    const auto r0 = Clock::now();
-                    tree.resize(ev);
-                    BENCH_HIP(hipDeviceSynchronize());
-                    t.maintenance += msSince(r0);
-                }
-            } while (resize && t.attempts < 64);
-            if (resize) throw std::runtime_error("buffers still need resizing after 64 attempts");
-            t.iterations = tree.iterations();
-            const Structure st = structure(tree.P(), tree.K(), config.levelBits());
-            writeStep(out, args, Adapter::pathName(), config, m, f, t, st, tree.libNodes());
-        }
-        BENCH_HIP(hipFree(x_d));
-        BENCH_HIP(hipFree(y_d));
-        BENCH_HIP(hipFree(z_d));
+        CsvTable cold(out.get(), kColdColumns);
+        cold.row()
+            .put("version", o.version)
+            .put("commit", o.commit)
+            .put("path", pathName)
+            .put("config", setup.name)
+            .put("dataset", d.name)
+            .put("process_rep", o.run)
+            .put("initial_build_ms", gfmt(coldWall, 10))
+            .put("initial_tree_gpu_ms", gfmt(ev.elapsed("begin", "tree"), 10))
+            .put("initial_view_gpu_ms", gfmt(ev.elapsed("tree", "views"), 10))
+            .put("num_leaves", (long long)first.leaves)
+            .put("num_nodes", (long long)first.nodes)
+            .put("max_depth", (long long)first.maxDepth)
+            .write();
         return 0;
     }
-    catch (const std::bad_alloc &e)
+
+    CsvTable steps(out.get(), kStepColumns);
+    auto emit = [&](const FrameRef &f, const StepTimes &t, const LeafCensus &c) {
+        const double lib = double(tree.scalars().num_nodes);
+        steps.row()
+            .put("version", o.version)
+            .put("commit", o.commit)
+            .put("path", pathName)
+            .put("config", setup.name)
+            .put("dataset", d.name)
+            .put("run", o.run)
+            .put("snapshot", f.index)
+            .put("frame_id", f.id)
+            .put("tree_update_ms", gfmt(t.tree, 10))
+            .put("view_ms", gfmt(t.view, 10))
+            .put("total_ms", gfmt(t.total, 10))
+            .put("wall_ms", gfmt(t.wall, 10))
+            .put("attempts", t.attempts)
+            .put("maintenance_ms", gfmt(t.maintenance, 10))
+            .put("iterations", -1) // fixed pass counts: the library reports no iteration count
+            .put("num_nodes", (long long)c.nodes)
+            .put("num_leaves", (long long)c.leaves)
+            .put("max_depth", (long long)c.maxDepth)
+            .put("max_depth_octree_equiv", gfmt(binary ? c.maxDepth / 3.0 : double(c.maxDepth), 10))
+            .put("num_empty_leaves", (long long)c.empty)
+            .put("empty_leaf_pct", gfmt(100.0 * double(c.empty) / double(c.leaves), 10))
+            .put("avg_particles_per_leaf", gfmt(double(c.active) / double(c.leaves), 10))
+            .put("particles_active", (long long)c.active)
+            .put("particles_escaped", (long long)(c.slots - c.active))
+            .put("particle_slots", (long long)c.slots)
+            .put("lib_num_nodes", gfmt(lib, 10))
+            .write();
+    };
+    emit(d.frames.front(), StepTimes{}, first);
+
+    Stepper stepper(tree, s, graph, ev);
+    stepper.capture();
+    const std::size_t frames = o.maxSnapshots > 0 ? std::min<std::size_t>(std::size_t(o.maxSnapshots), d.frames.size())
+                                                  : d.frames.size();
+    for (std::size_t k = 1; k < frames; ++k)
     {
-        std::cerr << "bench_driver: out of memory: " << e.what() << '\n';
-        return kOomExit;
+        const FrameRef &f = d.frames[k];
+        loadFrame(f, xyz);
+        if (xyz[0].size() != d.slots) throw std::runtime_error("frame " + f.path + " changes the slot count");
+        coords.upload(xyz);
+        StepTimes t{0, 0, 0, 0, 0, 0};
+        for (;;)
+        {
+            HARNESS_HIP(hipDeviceSynchronize());
+            const auto w0 = Clock::now();
+            HARNESS_HIP(hipEventRecord(ev["outer_begin"], s));
+            stepper.step();
+            HARNESS_HIP(hipEventRecord(ev["outer_end"], s));
+            const bool grow = tree.needsResize();
+            HARNESS_HIP(hipDeviceSynchronize());
+            t.wall += msSince(w0);
+            t.tree += ev.elapsed("begin", "tree");
+            t.view += ev.elapsed("tree", "views");
+            t.total += ev.elapsed("outer_begin", "outer_end");
+            ++t.attempts;
+            if (!grow) break;
+            if (t.attempts >= 64) throw std::runtime_error("frame " + f.path + ": capacity still short after 64 attempts");
+            t.maintenance += wallMs([&] { stepper.grow(); });
+        }
+        emit(f, t, measure());
     }
-    catch (const std::exception &e)
-    {
-        const std::string what = e.what();
-        std::cerr << "bench_driver: " << what << '\n';
-        return what.find("out of memory") != std::string::npos ? kOomExit : 1;
-    }
+    HARNESS_HIP(hipStreamDestroy(s));
+    return 0;
 }
 } // namespace
 
 int main(int argc, char **argv)
 {
-    for (int i = 1; i + 1 < argc; ++i)
-        if (std::string(argv[i]) == "--config" && std::string(argv[i + 1]) == "kdtree3d_leafcount")
-            return runDriver<V3<ao::TreeType::Binary>>(argc, argv);
-    return runDriver<V3<ao::TreeType::Octree>>(argc, argv);
+    try
+    {
+        return drive(argc, argv);
+    }
+    catch (const harness::OutOfMemory &e)
+    {
+        std::cerr << "bench_driver: " << e.what() << '\n';
+        return 3;
+    }
+    catch (const std::bad_alloc &)
+    {
+        std::cerr << "bench_driver: out of host memory\n";
+        return 3;
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "bench_driver: " << e.what() << '\n';
+        return 1;
+    }
 }

@@ -309,3 +309,33 @@ def test_binary_direct_build_window_edges(bucket):
         t.build()
         t.doBuffersNeedResize()
     same(g, o, "direct bucket %d" % bucket, BINARY)
+
+
+@pytest.mark.parametrize("sfc", [MORTON, HILBERT])
+@pytest.mark.parametrize("crit,bucket", [(LEAF_COUNT, 8), (NF_COUNT, 300)])
+def test_octree_update_rejected_for_capacity(sfc, crit, bucket):
+    """Octree updates whose passes overflow the leaf capacity
+    (octree_rebalance.cuh:1177-1273: scan[cap] > cap rejects the whole pass
+    and sets needsResize until resizeBuffers): the tree is built with every
+    slot escaped (one leaf, a capacity of a few), then the particles arrive
+    uniformly, so the split passes of the following updates outgrow the
+    capacity again and again as it grows by a quarter per resize; then they
+    contract into a small cluster and spread out again. Direct, captured and
+    oracle trees agree after every call."""
+    rng = np.random.default_rng(77 + bucket + sfc)
+    base = rng.random((4000, 3))
+    t = Trio(np.full((4000, 3), FMAX), OCTREE, sfc_kind=sfc, split_criterion=crit, max_particles_per_leaf=bucket)
+    t.compare("build")
+    resized = 0
+    for k, f in enumerate((1.0, 0.2, 0.01, 0.003, 1.0)):
+        print("contraction", f, flush=True)
+        t.upload(0.3 + (base - 0.3) * f)
+        for p in range(12):
+            grow = t.step()
+            t.compare("set %d pass %d" % (k, p))
+            if grow:
+                resized += 1
+                t.resize()
+    print("resizes", resized, flush=True)
+    assert resized > 0
+    t.graph.close()

@@ -151,3 +151,31 @@ def test_manifest(tmp_path):
             io.read_manifest(write(d, "bad.txt", bad))
     with pytest.raises(_ffi.InputError):
         io.read_manifest(str(d / "none.txt"))
+
+
+def test_large_files_parse_in_chunks(tmp_path):
+    """Files of many MiB are parsed by several threads in row-aligned chunks:
+    the rows come out in file order and a bad row far into the file is
+    reported with its line number in the whole file."""
+    rng = np.random.default_rng(3)
+    n = 260_000
+    v = rng.standard_normal((n, 4)) * 10.0 ** rng.integers(-30, 30, (n, 1))
+    esc = rng.random(n) < 0.01
+    rows = ["ESC" if e else "%.17g %.17g %.17g %.17g" % tuple(r) for r, e in zip(v, esc)]
+    rows.insert(1000, "# a comment")
+    rows.insert(200_000, "")
+    p = write(tmp_path, "big.dat", "\n".join(rows) + "\n")
+    assert os.path.getsize(p) > (16 << 20)
+    c = io.read_columns(p, 4)
+    want = np.where(esc[:, None], E, v).T
+    assert c.shape == (4, n) and np.array_equal(c, want)
+    x, y, z = io.read_frame(p, "cxyz")
+    assert np.array_equal(x, want[1]) and np.array_equal(z, want[3])
+    rows[250_000] = "1 2 3 oops"
+    q = write(tmp_path, "bad_big.dat", "\n".join(rows) + "\n")
+    with pytest.raises(_ffi.InputError) as e:
+        io.read_columns(q, 4)
+    assert "bad_big.dat:250001" in str(e.value)
+    with pytest.raises(_ffi.InputError) as e:
+        io.read_frame(q, "xyzc")
+    assert "bad_big.dat:250001" in str(e.value)
