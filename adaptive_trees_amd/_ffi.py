@@ -121,6 +121,7 @@ TREE_PROTOS = {
 }
 
 
+
 class Lib:
     """A loaded library plus its function prefix."""
 
@@ -185,6 +186,7 @@ def hip_lib():
             "tune_set": [ctypes.c_char_p, _I],
             "tune_get": [ctypes.c_char_p, ctypes.POINTER(_I)],
             "tune_auto": [_SZ, _P, ctypes.c_char_p, _SZ],
+            "tree_update_passes": [_P, ctypes.POINTER(ctypes.c_uint32)],
             "sort_pairs": [_P, _P, _SZ, _I, _P],
             "sort_pairs_noalloc": [_P, _P, _SZ, _I, _P],
             "batch_create": [_I, _I, _SZ, _P, _P, _P, _P, ctypes.POINTER(Config), ctypes.POINTER(_P)],

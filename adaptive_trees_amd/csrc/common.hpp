@@ -40,6 +40,9 @@ struct DevState
     int numNodes;
     int activeBuffer;
     unsigned numActiveParticles;
+    // not part of RebalanceState: passes committed since the last reset
+    // (an update's passes that changed the tree, before it converged)
+    unsigned passes;
 };
 
 __host__ __device__ inline int clz64(Key x)

@@ -264,6 +264,17 @@ int at_tree_device_counters(at_tree *t, const int32_t **l, const int32_t **nn, c
         *a = &s->numActiveParticles;
     });
 }
+int at_tree_update_passes(at_tree *t, uint32_t *passes)
+{
+    return guarded([&] {
+        needOutputs(passes);
+        at::TreeBase &tree = treeOf(t);
+        at::DevState h{};
+        AT_HIP(hipMemcpyAsync(&h, tree.deviceState(), sizeof(h), hipMemcpyDeviceToHost, tree.stream()));
+        AT_HIP(hipStreamSynchronize(tree.stream()));
+        *passes = h.passes;
+    });
+}
 int at_tree_max_achieved_depth(at_tree *t, uint32_t *d)
 {
     return guarded([&] {

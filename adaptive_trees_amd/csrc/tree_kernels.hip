@@ -475,6 +475,7 @@ __device__ void commitBody(DevState *st, int binary, std::uint64_t *status, std:
             st->numLeaves = st->newNumLeaves;
             if (binary) st->numNodes = (st->numLeaves - 1) * 2 + 1;
             st->activeBuffer ^= 1;
+            st->passes += 1;
         }
         *ticket = 0;
         if (nfCount) nfCount[0] = nfCount[1] = 0;
@@ -496,6 +497,7 @@ __device__ void resetBody(DevState *st, const Key *P, std::size_t n, std::uint64
     {
         st->changed = 0;
         st->converged = 0;
+        st->passes = 0;
         st->numActiveParticles = n ? unsigned(lowerBound(P, n, kMaxKey)) : 0u;
         *ticket = 0;
         if (nfCount) nfCount[0] = nfCount[1] = 0;

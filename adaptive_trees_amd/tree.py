@@ -178,6 +178,13 @@ class Tree:
         self.lib.call("tree_max_achieved_depth", self.handle, ctypes.byref(d))
         return d.value
 
+    def updatePasses(self):
+        """Passes of the last update() that changed the tree (HIP trees only;
+        synchronises)."""
+        d = ctypes.c_uint32()
+        self.lib.call("tree_update_passes", self.handle, ctypes.byref(d))
+        return d.value
+
     def print(self, file=None):
         """print(std::ostream&): the reference's diagnostic dump
         (src/octree/octree_print.cu, src/kdtree3d/kdtree3d_print.cu), written

@@ -1,23 +1,31 @@
 """Benchmark of the adaptive-tree build/update path on one MI355X.
 
-Workload (the 32M-particle build the headline roofline target is quoted on):
-binary longest-side tree, N = 2^25 FP32 particles in a 4:1:1 box, 40% + 40%
-isotropic Gaussian clusters (sigma = 0.02 of the long side, centres at 1/4
-and 3/4 of x) + 20% uniform background, LeafCount 32.
+Headline workload (the 32M-particle binary tree the 60% roofline target is
+quoted on): N = 2^25 FP32 particles in a 4:1:1 box, 40% + 40% isotropic
+Gaussian clusters (sigma = 0.02 of the long side, centres at 1/4 and 3/4 of
+x) + 20% uniform background, longest-side bisection, LeafCount 32. A second
+snapshot B is the first (A) moved by reflected Gaussian steps of sigma = 1e-3
+of each side.
 
-One step = build() followed by one hipGraph-replayed update() of the same
-resident particles; value = 2 N / step time (particles through the path per
-second). Coordinates are generated on the host, uploaded once, and stay in
-HBM for the whole timed region.
+One step = build() on the resident snapshot, a device copy of the other
+snapshot into the tree's borrowed coordinates (the particles move), and one
+hipGraph-replayed update() that rebalances for the moved particles; the
+snapshots alternate between steps. value = 2 N / step time (every particle
+once through build and once through update). Coordinates are generated on
+the host and uploaded once; the timed region touches HBM only.
 
-Output: one JSON line (metric, value, roofline of the dominant kernel, CPU
-baseline of the oracle on a bounded sample, per-phase timings).
+Output: one JSON line -- the metric, the roofline of the dominant kernel,
+the build's graded and compulsory fractions of the HBM roofline, phase
+times, parity flags against the oracle, the other configurations (65,536
+FP64 octree with views, 2^24 FP32 Plummer NFCount octree), the 512-tree
+batch, the serving loop, and the CPU baseline of the oracle.
 """
 import argparse
 import ctypes
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 
@@ -38,6 +46,7 @@ PAIR_BYTES = 24  # read + write of one (8-byte key, 4-byte value) pair
 DOMINANT_LABEL = "msd_local"
 DOMINANT_KERNEL = "msdLocalSortKernel"
 COPY_BYTES = 1 << 30  # calibration copy: reads and writes this many bytes
+MSD_CHUNK = 2048      # buckets above this size are partitioned again by the next MSD level
 
 
 def _lib():
@@ -79,44 +88,406 @@ def profile_query(label):
     return (ms.value, n.value) if status == 0 else (0.0, 0)
 
 
-def algorithmic_build_bytes(n, leaves, s=4):
-    """Bytes any build must move: key generation reads 3s and writes 8 per
-    particle, a sort reads the keys and writes the sorted (key, slot) pairs
-    (8 + 12), the construction reads P once and writes K and N (8 + 4 per
-    leaf). A floor for the whole build, not a count of what this one moves."""
-    return n * (3 * s + 8) + n * 20 + 8 * n + 12 * leaves
+def copy_into(dst, src, stream):
+    for d, s in zip(dst, src):
+        _lib().call("memcpy_async", d.ptr, s.ptr, s.nbytes, 3, stream.p)
 
 
-def cpu_baseline(x, y, z, x1, y1, z1, box, min_seconds):
-    """The oracle (oracle/, OpenMP C++) on the same workload in host memory: build() then one
-    update() on the moved positions, repeated until min_seconds of CPU work; the median repeat
-    is reported."""
-    from oracle.binding import OracleTree, num_threads
-    n = len(x)
-    hx, hy, hz = x.copy(), y.copy(), z.copy()
-    cfg = make_config(max_particles_per_leaf=32, box=box)
-    t = OracleTree(hx, hy, hz, n, cfg, real=np.float32, kind=BINARY)
-    times = []
-    start = time.perf_counter()
-    while len(times) < 3 or (time.perf_counter() - start < min_seconds and len(times) < 50):
-        hx[:], hy[:], hz[:] = x, y, z
-        t0 = time.perf_counter()
-        t.build()
-        hx[:], hy[:], hz[:] = x1, y1, z1
-        t.update()
-        t.doBuffersNeedResize()
-        times.append(time.perf_counter() - t0)
-    dt = float(np.median(times))
-    return {"value": round(2 * n / dt / 1e6, 3), "unit": "Mparticles/s", "cores": num_threads(),
-            "kind": "port",
-            "sample": "oracle build() + one update() of the full %d-particle workload in host memory, "
-                      "median of %d repeats (%.2f s each, %.1f s total)" % (n, len(times), dt, sum(times))}
+# ---- the roofline's algorithmic bytes (per build / update / views) ----------------------
+def build_bytes(n, s, p, leaves, nf=False):
+    """Key generation N(3s+12), a sort of p pair passes plus one key read for
+    the digit histograms N(24p+8), one streaming read of P and the writes of
+    K and N (8N + 12L); NFCount adds one merged read of P and the NF writes."""
+    b = n * (3 * s + 12) + n * (24 * p + 8) + 8 * n + 12 * leaves
+    return b + (8 * n + 4 * leaves if nf else 0)
 
 
+def update_bytes(n, s, p, leaves, passes, nf=False):
+    """Key generation and sort as in a build, one read of P, and 40 bytes per
+    leaf for each pass that changed the tree (NFCount: 8N + 8 per leaf and pass)."""
+    b = n * (3 * s + 12) + n * (24 * p + 8) + 8 * n + 40 * passes * leaves
+    return b + (8 * n + 8 * passes * leaves if nf else 0)
+
+
+def view_bytes(n, nodes, leaves):
+    return 16 * n + 26 * nodes + 12 * leaves
+
+
+def frac(nbytes, ms):
+    return round(nbytes / (ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4) if ms > 0 else None
+
+
+def msd_pair_passes(P):
+    """Pair passes of the MSD sort over these (sorted) keys: level 1 moves
+    every key, level l >= 2 the keys of the buckets of more than 2048 keys
+    that level l-1 left (equal 8(l-1)-bit prefixes), and the LDS chunk sort
+    moves every key once more."""
+    P = np.asarray(P, dtype=np.uint64)
+    n = len(P)
+    if n == 0:
+        return 0.0
+    moved = n
+    for level in range(2, 9):
+        prefix = P >> np.uint64(64 - 8 * (level - 1))
+        cuts = np.flatnonzero(prefix[1:] != prefix[:-1]) + 1
+        sizes = np.diff(np.concatenate([[0], cuts, [n]]))
+        big = int(sizes[sizes > MSD_CHUNK].sum())
+        if big == 0:
+            break
+        moved += big
+    return round((moved + n) / n, 4)
+
+
+def host_cores():
+    """(threads of this process's CPU share, physical cores per lscpu)."""
+    share = len(os.sched_getaffinity(0))
+    phys = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        cps = re.search(r"Core\(s\) per socket:\s*(\d+)", out)
+        sk = re.search(r"Socket\(s\):\s*(\d+)", out)
+        if cps and sk:
+            phys = int(cps.group(1)) * int(sk.group(1))
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return share, phys
+
+
+def get_tree(t, names=("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d")):
+    return {k: t.get(k) for k in names}
+
+
+def same_arrays(a, b):
+    return all(np.array_equal(a[k], b[k]) for k in a)
+
+
+# ---- headline: 32M binary tree ---------------------------------------------------------------
+def headline(args):
+    n = args.n
+    box = workloads.ANISO_BOX
+    tuning = tune.get() if args.no_autotune else tune.autotune(n)
+    progress("sort tuning: %s" % json.dumps(tuning))
+    A = workloads.two_clusters(n, seed=3, dtype=np.float32)
+    B = workloads.reflect_step(A, 1e-3, seed=33, box=box)
+    snaps = [[DeviceArray(a) for a in A], [DeviceArray(a) for a in B]]
+    coords = [DeviceArray(a) for a in A]
+    stream = Stream()
+    cfg = make_config(max_particles_per_leaf=32, box=box, stream=stream.p)
+    tree = Tree(*coords, n, cfg, real=np.float32, kind=BINARY)
+    tree.build()
+    graph = ctypes.c_void_p()
+    _lib().call("graph_capture_update", tree.handle, stream.p, ctypes.byref(graph))
+    addr = tree.buffer_info("cornerstone_d")[0]
+    state = {"at": 0}  # which snapshot the coordinates hold
+
+    def step():
+        tree.build()
+        state["at"] ^= 1
+        copy_into(coords, snaps[state["at"]], stream)
+        _lib().call("graph_launch", graph.value, stream.p)
+
+    for _ in range(args.warmup):
+        step()
+    stream.sync()
+    assert tree.buffer_info("cornerstone_d")[0] == addr, "build moved the captured buffers"
+    resizes = 0
+    while tree.doBuffersNeedResize():  # (never at this size: capacity after build covers the moves)
+        resizes += 1
+        tree.resizeBuffers()
+        _lib().call("graph_destroy", graph.value)
+        _lib().call("graph_capture_update", tree.handle, stream.p, ctypes.byref(graph))
+
+    # the timed steps
+    _lib().call("profile_reset")
+    _lib().call("profile_enable", 1)
+    e0, e1 = Event(), Event()
+    stream.sync()
+    e0.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    stream.sync()
+    wall = time.perf_counter() - t0
+    _lib().call("profile_enable", 0)
+    ms_per_step = wall * 1e3 / args.steps
+    event_ms = e1.ms_since(e0) / args.steps
+    while tree.doBuffersNeedResize():
+        resizes += 1
+        tree.resizeBuffers()
+
+    def per_launch(label):
+        ms, cnt = profile_query(label)
+        return ms / max(cnt, 1), cnt
+
+    dom_ms, dom_n = per_launch(DOMINANT_LABEL)
+    key_ms, _ = per_launch("keygen")
+    pass_ms, _ = per_launch("sort_pass")
+    level_ms, level_n = per_launch("msd_level")
+    dir_ms, dir_n = per_launch("bin_direct")
+    achieved = PAIR_BYTES * n / (dom_ms * 1e-3) / 1e9 if dom_n else 0.0
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f)
+            if t.get("kernel") == DOMINANT_KERNEL:
+                traffic = t.get("bytes_per_launch")
+
+    # phases, event-timed one at a time outside the timed region (medians of 5)
+    ea, eb = Event(), Event()
+
+    def timed(fn):
+        stream.sync()
+        ea.record(stream)
+        fn()
+        eb.record(stream)
+        stream.sync()
+        return eb.ms_since(ea)
+
+    def to(k):
+        copy_into(coords, snaps[k], stream)
+        state["at"] = k
+
+    build_ms, moved_ms, static_ms, copy_ms, passes = [], [], [], [], []
+    for _ in range(5):
+        to(0)
+        build_ms.append(timed(tree.build))
+        copy_ms.append(timed(lambda: copy_into(coords, snaps[1], stream)))
+        state["at"] = 1
+        moved_ms.append(timed(lambda: _lib().call("graph_launch", graph.value, stream.p)))
+        passes.append(tree.updatePasses())
+        static_ms.append(timed(lambda: _lib().call("graph_launch", graph.value, stream.p)))
+    build_med, moved_med = float(np.median(build_ms)), float(np.median(moved_ms))
+    to(0)
+    stream.sync()
+    t0 = time.perf_counter()
+    tree.build()
+    stream.sync()
+    build_wall_ms = (time.perf_counter() - t0) * 1e3
+
+    # results of the parity protocol: build(A), then the graph update on B
+    to(0)
+    tree.build()
+    stream.sync()
+    after_build = get_tree(tree)
+    leaves = len(after_build["leafCounts_d"])
+    p_build = msd_pair_passes(after_build["particleKeys_d"])
+    to(1)
+    _lib().call("graph_launch", graph.value, stream.p)
+    stream.sync()
+    while tree.doBuffersNeedResize():
+        resizes += 1
+        tree.resizeBuffers()
+        tree.update()
+    after_update = get_tree(tree)
+    moved_passes = int(np.median(passes))
+    p_update = msd_pair_passes(after_update["particleKeys_d"])
+    leaves_update = len(after_update["leafCounts_d"])
+
+    # views of the same tree (reported separately; not part of the step)
+    view_ms = []
+    for _ in range(5):
+        view_ms.append(timed(lambda: tree.computeViews((1, 1, 1))))
+        while tree.doBuffersNeedResize():
+            tree.resizeBuffers()
+    view_med = float(np.median(view_ms[1:]))
+    nodes = int(tree.scalars().num_nodes)
+
+    # stream-copy ceiling of this GPU (event-timed copy kernel, 1 GiB read + 1 GiB written)
+    src, dst = DeviceArray(COPY_BYTES), DeviceArray(COPY_BYTES)
+    cps = []
+    for _ in range(4):
+        cps.append(timed(lambda: _lib().call("bench_copy", dst.ptr, src.ptr, COPY_BYTES, stream.p)))
+    copy_ceiling = 2 * COPY_BYTES / (float(np.median(cps[1:])) * 1e-3) / 1e9
+    del src, dst
+
+    b_graded = build_bytes(n, 4, p_build, leaves)
+    b_comp = build_bytes(n, 4, 1, leaves)
+    u_graded = update_bytes(n, 4, p_update, leaves_update, moved_passes)
+    out = {
+        "metric": METRIC,
+        "value": round(2 * n / (ms_per_step * 1e-3) / 1e6, 2),
+        "unit": "Mparticles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "binary longest-side tree, N=2^25 FP32, 4:1:1 box, two Gaussian clusters (sigma 0.02 of "
+                        "the long side) + 20% uniform, LeafCount 32; step = build() + device copy of the moved "
+                        "snapshot (reflected steps, sigma 1e-3 of each side) + graph-replayed update(), snapshots "
+                        "alternating",
+            "particles": n,
+            "leaves": leaves,
+            "tree": "binary",
+            "bucket": 32,
+            "sort_tuning": tuning,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "LDS chunk sort of the MSD radix sort (%s, one launch per sort over all N pairs)"
+                      % DOMINANT_KERNEL,
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": PAIR_BYTES * n,
+            "avg_launch_ms": round(dom_ms, 5),
+        },
+        "phases_ms": {
+            "step_event": round(event_ms, 4),
+            "build": round(build_med, 4),
+            "build_wall": round(build_wall_ms, 4),
+            "snapshot_copy": round(float(np.median(copy_ms)), 4),
+            "update_moved_graph": round(moved_med, 4),
+            "update_static_graph": round(float(np.median(static_ms)), 4),
+            "keygen": round(key_ms, 4),
+            "sort_level1_pass": round(pass_ms, 4),
+            "sort_levels_2_to_8": round(level_ms * level_n / max(dom_n, 1), 4),
+            "sort_lds_chunks": round(dom_ms, 4),
+            "binary_construction": round(2 * dir_ms, 4),  # two launches (count, emit) per build
+        },
+        "rates_Mparticles_per_s": {
+            "build": round(n / (build_med * 1e-3) / 1e6, 1),
+            "update_moved": round(n / (moved_med * 1e-3) / 1e6, 1),
+            "step_2N": round(2 * n / (ms_per_step * 1e-3) / 1e6, 1),
+        },
+        "build_roofline": {
+            "sort_pair_passes_p": p_build,
+            "bytes_graded": int(b_graded),
+            "frac_graded": frac(b_graded, build_med),
+            "bytes_compulsory_p1": int(b_comp),
+            "frac_compulsory": frac(b_comp, build_med),
+            "copy_ceiling_GBs": round(copy_ceiling, 1),
+        },
+        "update_roofline": {
+            "sort_pair_passes_p": p_update,
+            "passes_changed_I": moved_passes,
+            "leaves": leaves_update,
+            "bytes_graded": int(u_graded),
+            "frac_graded": frac(u_graded, moved_med),
+            "frac_compulsory": frac(update_bytes(n, 4, 1, leaves_update, moved_passes), moved_med),
+        },
+        "views": {
+            "ms": round(view_med, 4),
+            "nodes": nodes,
+            "bytes": int(view_bytes(n, nodes, leaves_update)),
+            "frac": frac(view_bytes(n, nodes, leaves_update), view_med),
+        },
+        "resizes": resizes,
+    }
+    _lib().call("graph_destroy", graph.value)
+    tree.close()
+    return out, (A, B, box, after_build, after_update)
+
+
+# ---- other single-tree configurations ------------------------------------------------------------
+def other_configs(check):
+    """65,536 FP64 uniform octree (Morton, LeafCount 64): build median and
+    views, checked against the oracle; 2^24 FP32 Plummer octree (a = 0.05,
+    Hilbert, NFCount 4096): build median of 10 after warm-up, then 10
+    updates of reflected Gaussian steps (sigma 1e-3), the build checked
+    against the oracle."""
+    stream = Stream()
+    ea, eb = Event(), Event()
+
+    def timed(fn):
+        stream.sync()
+        ea.record(stream)
+        fn()
+        eb.record(stream)
+        stream.sync()
+        return eb.ms_since(ea)
+
+    out = {}
+    n = 65536
+    x, y, z = workloads.uniform_cube(n, seed=1, dtype=np.float64)
+    dev = [DeviceArray(a) for a in (x, y, z)]
+    t = Tree(*dev, n, make_config(max_particles_per_leaf=64, stream=stream.p), real=np.float64, kind=OCTREE)
+    times = [timed(t.build) for _ in range(25)]
+    views = []
+    for _ in range(10):
+        views.append(timed(lambda: t.computeViews((1, 1, 1))))
+        while t.doBuffersNeedResize():
+            t.resizeBuffers()
+    leaves, nodes = int(t.scalars().num_leaves), int(t.scalars().num_nodes)
+    exact = None
+    if check:
+        from oracle.binding import OracleTree
+        o = OracleTree(x.copy(), y.copy(), z.copy(), n, make_config(max_particles_per_leaf=64, views=(1, 1, 1)),
+                       real=np.float64, kind=OCTREE)
+        o.build()
+        exact = all(np.array_equal(t.get(k), o.get(k)) for k in
+                    ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d", "sfcBoxIndex_d", "parentIndex_d",
+                     "childIndex_d", "particleCounts_d", "sfcParticleIndex_d"))
+    out["octree_65k_fp64_uniform"] = {
+        "build_ms_median": round(float(np.median(times[5:])), 4), "leaves": leaves, "nodes": nodes,
+        "views_ms_median": round(float(np.median(views[2:])), 4),
+        "views_frac": frac(view_bytes(n, nodes, leaves), float(np.median(views[2:]))),
+        "bit_exact_vs_oracle": exact, "note": "launch-bound (about 16 MB of traffic): reported, not graded"}
+    t.close()
+    progress("65k octree done")
+
+    n = 1 << 24
+    pts = workloads.plummer(n, seed=2, dtype=np.float32)
+    dev = [DeviceArray(a) for a in pts]
+    cfg = dict(max_particles_per_leaf=4096, split_criterion=NF_COUNT, sfc_kind=HILBERT)
+    t = Tree(*dev, n, make_config(stream=stream.p, **cfg), real=np.float32, kind=OCTREE)
+    builds = [timed(t.build) for _ in range(13)]
+    build_med = float(np.median(builds[3:]))
+    leaves_b = int(t.scalars().num_leaves)
+    p_b = msd_pair_passes(t.get("particleKeys_d"))
+    exact = None
+    if check:
+        from oracle.binding import OracleTree
+        o = OracleTree(*[a.copy() for a in pts], n, make_config(**cfg), real=np.float32, kind=OCTREE)
+        o.build()
+        exact = all(np.array_equal(t.get(k), o.get(k)) for k in
+                    ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d", "nearFieldCounts_d"))
+        del o
+    upd, passes, resizes, cur = [], [], 0, pts
+    for step in range(10):
+        cur = workloads.reflect_step(cur, 1e-3, seed=200 + step)
+        for d, a in zip(dev, cur):
+            d.upload(a)
+        upd.append(timed(t.update))
+        passes.append(t.updatePasses())
+        while t.doBuffersNeedResize():
+            resizes += 1
+            t.resizeBuffers()
+            t.update()
+    leaves_u = int(t.scalars().num_leaves)
+    p_u = msd_pair_passes(t.get("particleKeys_d"))
+    upd_med = float(np.median(upd))
+    I = int(np.median(passes))
+    out["octree_16M_fp32_plummer_hilbert_nf4096"] = {
+        "build_ms_median_of_10": round(build_med, 3), "build_ms_first": round(builds[0], 3),
+        "update_ms_median": round(upd_med, 3), "update_ms_max": round(float(np.max(upd)), 3),
+        "resizes": resizes, "leaves_after_build": leaves_b, "leaves": leaves_u,
+        "sort_pair_passes_p": p_b, "update_passes_changed_I": I,
+        "build_frac_graded": frac(build_bytes(n, 4, p_b, leaves_b, nf=True), build_med),
+        "build_frac_compulsory": frac(build_bytes(n, 4, 1, leaves_b, nf=True), build_med),
+        "update_frac_graded": frac(update_bytes(n, 4, p_u, leaves_u, I, nf=True), upd_med),
+        "update_frac_compulsory": frac(update_bytes(n, 4, 1, leaves_u, I, nf=True), upd_med),
+        "Mparticles_per_s_build": round(n / (build_med * 1e-3) / 1e6, 1),
+        "Mparticles_per_s_update": round(n / (upd_med * 1e-3) / 1e6, 1),
+        "bit_exact_build_vs_oracle": exact}
+    t.close()
+    progress("16M NF octree done")
+    return out
+
+
+# ---- batches -----------------------------------------------------------------------------------------
 def batch_workload(trees, seed):
-    """512 octree inputs of the batched workload: sizes log-uniform in [1e3, 2e6], uniform or
-    Plummer per tree by a seed bit, FP32, plus one reflected-Gaussian moved snapshot
-    (sigma = 1e-3 of the unit box). Returns sizes and two concatenated (x, y, z) snapshots."""
+    """512 octree inputs: sizes log-uniform in [1e3, 2e6], uniform or Plummer per tree by a seed
+    bit, FP32, plus one reflected-Gaussian moved snapshot (sigma = 1e-3 of the unit box)."""
     sizes = workloads.batch_sizes(trees, seed=seed)
     rng = np.random.default_rng(seed)
     kinds = rng.integers(0, 2, size=trees)
@@ -136,57 +507,56 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
     the plain path (one Tree per input on one stream: build() + rounds x (update() +
     doBuffersNeedResize())) on the same device-resident particles. Each round first copies the
     next snapshot into the borrowed coordinate arrays (one device copy per axis for all trees,
-    the same in both paths)."""
+    the same in both paths). Every member is compared with its plain tree afterwards."""
     lib = _lib()
     sizes, first, moved = batch_workload(trees, seed)
     progress("batched workload generated: %d trees, %d particles" % (trees, int(sizes.sum())))
     total = int(sizes.sum())
     off = np.concatenate([[0], np.cumsum(sizes)])
-    cur = [DeviceArray(a) for a in first]
+    cur_b = [DeviceArray(a) for a in first]
+    cur_p = [DeviceArray(a) for a in first]
     snaps = [[DeviceArray(a) for a in first], [DeviceArray(a) for a in moved]]
-    nbytes = total * 4
     stream = Stream()
     cfg = dict(max_particles_per_leaf=64)
 
-    def ptrs(a):
+    def ptrs(cur, a):
         return [cur[a].ptr + 4 * int(off[k]) for k in range(trees)]
 
-    def load(r):
-        for a in range(3):
-            lib.call("memcpy_async", cur[a].ptr, snaps[r % 2][a].ptr, nbytes, 3, stream.p)
+    def load(cur, r):
+        copy_into(cur, snaps[r % 2], stream)
 
-    def resize_plain(ts):
-        for t in ts:
-            while t.doBuffersNeedResize():
-                t.resizeBuffers()
-                t.update()
-
-    # plain: 512 independent trees on one stream
-    plain = [Tree(*(ptrs(a)[k] for a in range(3)), int(sizes[k]), make_config(stream=stream.p, **cfg),
+    resizes = {"plain": 0, "batch": 0}
+    plain = [Tree(*(ptrs(cur_p, a)[k] for a in range(3)), int(sizes[k]), make_config(stream=stream.p, **cfg),
                   real=np.float32, kind=OCTREE) for k in range(trees)]
 
     def run_plain():
-        load(0)
+        load(cur_p, 0)
         for t in plain:
             t.build()
         for r in range(1, rounds + 1):
-            load(r)
+            load(cur_p, r)
             for t in plain:
                 t.update()
-            resize_plain(plain)
+            for t in plain:
+                while t.doBuffersNeedResize():
+                    resizes["plain"] += 1
+                    t.resizeBuffers()
+                    t.update()
         stream.sync()
 
-    batch = Batch(ptrs(0), ptrs(1), ptrs(2), sizes, make_config(stream=stream.p, **cfg), real=np.float32)
+    batch = Batch(ptrs(cur_b, 0), ptrs(cur_b, 1), ptrs(cur_b, 2), sizes, make_config(stream=stream.p, **cfg),
+                  real=np.float32)
     graph = [None]
 
     def run_batch():
-        load(0)
+        load(cur_b, 0)
         batch.build()
         graph[0] = graph[0] or batch.capture_update(stream.p)
         for r in range(1, rounds + 1):
-            load(r)
+            load(cur_b, r)
             lib.call("graph_launch", graph[0], stream.p)
             while batch.needsResize().any():
+                resizes["batch"] += 1
                 batch.resizeBuffers()
                 lib.call("graph_destroy", graph[0])
                 graph[0] = batch.capture_update(stream.p)
@@ -196,15 +566,15 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
     run_plain()  # warm-up: allocations reach their steady state
     run_batch()
     progress("batched/plain warm-up done")
+    resizes = {"plain": 0, "batch": 0}
     t0 = time.perf_counter()
     run_plain()
     plain_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     run_batch()
     batch_s = time.perf_counter() - t0
-    sample = list(range(0, trees, max(1, trees // 32)))
     exact = all(np.array_equal(batch.tree(k).get(n), plain[k].get(n))
-                for k in sample for n in ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d"))
+                for k in range(trees) for n in ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d"))
     work = total * (rounds + 1)
     return {
         "workload": "%d octrees, sizes log-uniform [1e3, 2e6] (total %d), FP32 uniform/Plummer, Morton, "
@@ -216,76 +586,13 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
         "speedup": round(plain_s / batch_s, 3),
         "batched_Mparticles_per_s": round(work / batch_s / 1e6, 1),
         "plain_Mparticles_per_s": round(work / plain_s / 1e6, 1),
-        "bit_exact_sample_trees": len(sample),
-        "bit_exact": bool(exact),
+        "resizes": resizes,
+        "bit_exact_all_members_vs_plain": bool(exact),
+        "oracle_parity": "tests/test_gpu_fullsize.py::test_c4_batch_512_trees (every 16th member)",
     }
 
 
-def other_configs():
-    """The other single-tree workloads: 65,536 FP64 uniform octree (Morton, LeafCount 64) build,
-    checked bit for bit against the oracle; and a 2^24-particle FP32 Plummer octree (a = 0.05,
-    Hilbert, NFCount 4096) build followed by 10 updates of reflected Gaussian steps (sigma 1e-3)."""
-    from oracle.binding import OracleTree
-    lib = _lib()
-    stream = Stream()
-    e0, e1 = Event(), Event()
-    out = {}
-    # 65k FP64 uniform octree
-    n = 65536
-    x, y, z = workloads.uniform_cube(n, seed=1, dtype=np.float64)
-    dev = [DeviceArray(a) for a in (x, y, z)]
-    t = Tree(*dev, n, make_config(max_particles_per_leaf=64, stream=stream.p), real=np.float64, kind=OCTREE)
-    times = []
-    for _ in range(25):
-        e0.record(stream)
-        t.build()
-        e1.record(stream)
-        stream.sync()
-        times.append(e1.ms_since(e0))
-    o = OracleTree(x.copy(), y.copy(), z.copy(), n, make_config(max_particles_per_leaf=64), real=np.float64,
-                   kind=OCTREE)
-    o.build()
-    exact = all(np.array_equal(t.get(k), o.get(k)) for k in ("particleKeys_d", "perm_d", "cornerstone_d",
-                                                              "leafCounts_d"))
-    out["octree_65k_fp64_uniform_build"] = {"build_ms_median": round(float(np.median(times[5:])), 4),
-                                            "leaves": int(t.scalars().num_leaves), "bit_exact_vs_oracle": exact}
-    t.close()
-    progress("65k octree done")
-    # 16M FP32 Plummer Hilbert NFCount octree, build + 10 updates
-    n = 1 << 24
-    pts = workloads.plummer(n, seed=2, dtype=np.float32)
-    dev = [DeviceArray(a) for a in pts]
-    t = Tree(*dev, n, make_config(max_particles_per_leaf=4096, split_criterion=NF_COUNT, sfc_kind=HILBERT,
-                                  stream=stream.p), real=np.float32, kind=OCTREE)
-    e0.record(stream)
-    t.build()
-    e1.record(stream)
-    stream.sync()
-    build_ms = e1.ms_since(e0)
-    upd, resizes, cur = [], 0, pts
-    for step in range(10):
-        cur = workloads.reflect_step(cur, 1e-3, seed=200 + step)
-        for d, a in zip(dev, cur):
-            d.upload(a)
-        e0.record(stream)
-        t.update()
-        e1.record(stream)
-        stream.sync()
-        upd.append(e1.ms_since(e0))
-        while t.doBuffersNeedResize():
-            resizes += 1
-            t.resizeBuffers()
-            t.update()
-    out["octree_16M_fp32_plummer_hilbert_nf4096"] = {
-        "build_ms": round(build_ms, 3), "update_ms_mean": round(float(np.mean(upd)), 3),
-        "update_ms_max": round(float(np.max(upd)), 3), "resizes": resizes,
-        "leaves": int(t.scalars().num_leaves),
-        "Mparticles_per_s": round(11 * n / ((build_ms + sum(upd)) * 1e-3) / 1e6, 1)}
-    t.close()
-    progress("16M NF octree done")
-    return out
-
-
+# ---- serving -------------------------------------------------------------------------------------------
 def serving_trace(rate, requests, pool, seed):
     """Poisson arrivals at `rate`/s. The first 8 requests build resident trees; afterwards 30%
     build a new tree (the oldest resident tree is released beyond 16) and 70% update a random
@@ -314,9 +621,9 @@ def serving_trace(rate, requests, pool, seed):
     return trace
 
 
-def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5, workers=4):
-    """The serving loop (4 worker streams, 200 GB budget) against serial plain calls on the
-    same trace; latency = result ready - arrival, per request."""
+def serving_benchmark(rates, requests, seed=5, workers=4):
+    """The serving loop (worker streams, 200 GB budget) against serial plain calls on the same
+    trace; latency = result ready - arrival, per request."""
     rng = np.random.default_rng(seed)
     sizes = np.rint(np.exp(rng.uniform(np.log(1e4), np.log(8e6), size=20))).astype(np.int64)
     pool = []  # per entry: two snapshots, host float64
@@ -335,7 +642,6 @@ def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5, 
     results = []
     for rate in rates:
         trace = serving_trace(rate, requests, pool, seed + rate)
-        # inputs of every request, prepared before the clock starts
         prep = []
         for r in trace:
             if r[0] == "build":
@@ -350,11 +656,7 @@ def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5, 
                     snapshot(p, real, w)
             else:
                 prep.append(None)
-        info = {}
-        for r in trace:
-            if r[0] == "build":
-                info[r[2]] = r
-        # served
+        info = {r[2]: r for r in trace if r[0] == "build"}
         server = Server(hbm_budget_bytes=200 << 30, workers=workers, max_batch=16)
         ids, turn, reqs = {}, {}, []
         start = time.perf_counter()
@@ -387,7 +689,6 @@ def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5, 
         stats = server.stats()
         server.close()
         served_s = (last - first) / 1e3
-        # serial plain calls: one Tree per build, requests in arrival order on one stream
         plain, turn2, slat = {}, {}, []
         start = time.perf_counter()
         for r, bufs in zip(trace, prep):
@@ -426,14 +727,87 @@ def serving_benchmark(rates=(50, 200, 1000, 2000, 10000), requests=160, seed=5, 
             "requests_per_s": round(len(reqs) / served_s, 1),
             "Mparticles_per_s": round(work / served_s / 1e6, 1),
             "serial_requests_per_s": round(len(reqs) / serial_s, 1),
+            "served_over_serial": round(serial_s / served_s, 3),
             "batched_updates": int(stats["batched_updates"]), "peak_GB": round(stats["bytes_peak"] / 1e9, 3),
         })
         progress("serving rate %d/s done" % rate)
     return {"workload": "Poisson trace, %d requests per rate, sizes log-uniform [1e4, 8e6], 70%% octree / 30%% "
-                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; %d worker streams with "
-                        "per-tree update graphs, 200 GB budget; serial = plain Tree calls in arrival order; "
-                        "10000/s is an overload probe of capacity" % (requests, workers),
+                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; %d worker streams, "
+                        "200 GB budget; serial = plain Tree calls in arrival order on one thread"
+                        % (requests, workers),
             "rates": results}
+
+
+# ---- CPU baseline ----------------------------------------------------------------------------------------
+def cpu_baseline(A, B, box, gpu_build, gpu_update):
+    """The oracle (oracle/, OpenMP C++) on the headline workload in host memory: build() on
+    snapshot A then update() on B, 1 warm-up + 5 timed repeats, build and update timed
+    separately (medians); the last repeat's trees are compared with the GPU's. It runs in a
+    child process (this script with --cpu-child) so that its thread placement
+    (OMP_PLACES=cores, OMP_PROC_BIND=close, which binds the starting thread too) leaves the
+    GPU measurements' host threads alone; inputs and results pass through files."""
+    import tempfile
+    share, phys = host_cores()
+    with tempfile.TemporaryDirectory(dir="/dev/shm" if os.path.isdir("/dev/shm") else None) as d:
+        np.savez(os.path.join(d, "in.npz"), ax=A[0], ay=A[1], az=A[2], bx=B[0], by=B[1], bz=B[2],
+                 box=np.array(box, dtype=np.float64))
+        env = dict(os.environ, OMP_PLACES="cores", OMP_PROC_BIND="close")
+        env.setdefault("OMP_NUM_THREADS", str(share))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", d], env=env,
+                           capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError("CPU baseline failed:\n" + r.stderr[-2000:])
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        got = np.load(os.path.join(d, "out.npz"))
+        exact_build = all(np.array_equal(gpu_build[k], got["build_" + k]) for k in gpu_build)
+        exact_update = all(np.array_equal(gpu_update[k], got["update_" + k]) for k in gpu_update)
+    n = len(A[0])
+    b, u = res["build_s"], res["update_s"]
+    return {
+        "value": round(2 * n / (b + u) / 1e6, 3), "unit": "Mparticles/s", "cores": res["threads"], "kind": "port",
+        "physical_cores_lscpu": phys, "cpu_share_threads": share,
+        "omp": {"OMP_NUM_THREADS": env.get("OMP_NUM_THREADS"), "OMP_PLACES": "cores", "OMP_PROC_BIND": "close"},
+        "build_ms_median": round(b * 1e3, 2), "update_ms_median": round(u * 1e3, 2),
+        "build_Mparticles_per_s": round(n / b / 1e6, 2), "update_Mparticles_per_s": round(n / u / 1e6, 2),
+        "sample": "oracle build() on snapshot A + update() on the moved snapshot B of the full %d-particle "
+                  "workload in host memory, 1 warm-up + 5 timed repeats (%.1f s of CPU work)"
+                  % (n, res["total_s"]),
+    }, exact_build, exact_update
+
+
+def cpu_child(d):
+    """The CPU baseline's child process (see cpu_baseline)."""
+    from oracle.binding import OracleTree, num_threads
+    z = np.load(os.path.join(d, "in.npz"))
+    A, B = (z["ax"], z["ay"], z["az"]), (z["bx"], z["by"], z["bz"])
+    box = tuple(float(v) for v in z["box"])
+    n = len(A[0])
+    h = [a.copy() for a in A]
+    t = OracleTree(*h, n, make_config(max_particles_per_leaf=32, box=box), real=np.float32, kind=BINARY)
+    tb, tu = [], []
+    for rep in range(6):
+        for dst, a in zip(h, A):
+            dst[:] = a
+        t0 = time.perf_counter()
+        t.build()
+        t1 = time.perf_counter()
+        if rep == 5:
+            cpu_build = get_tree(t)
+        for dst, a in zip(h, B):
+            dst[:] = a
+        t2 = time.perf_counter()
+        t.update()
+        t.doBuffersNeedResize()
+        t3 = time.perf_counter()
+        if rep:
+            tb.append(t1 - t0)
+            tu.append(t3 - t2)
+    cpu_update = get_tree(t)
+    out = {"build_" + k: v for k, v in cpu_build.items()}
+    out.update({"update_" + k: v for k, v in cpu_update.items()})
+    np.savez(os.path.join(d, "out.npz"), **out)
+    print(json.dumps({"build_s": float(np.median(tb)), "update_s": float(np.median(tu)), "threads": num_threads(),
+                      "total_s": float(sum(tb) + sum(tu))}))
 
 
 def _counter_rows(path):
@@ -475,8 +849,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 25)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline work to time")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and the oracle checks)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the known-size copy kernel (for hardware counter scaling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
@@ -485,12 +858,17 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-tree workloads")
     ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the serving loop")
-    ap.add_argument("--serve-rates", default="50,200,1000,2000,10000", help="arrival rates (requests/s)")
+    ap.add_argument("--serve-rates", default="50,100,200,500,1000,2000", help="arrival rates (requests/s)")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="requests per arrival rate")
     ap.add_argument("--only-configs", action="store_true", help="run only the other single-tree workloads")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--cpu-child", metavar="DIR", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args.cpu_child)
+        return
     if args.traffic:
         t = traffic_from_counters(args.traffic[0], args.traffic[1], COPY_BYTES)
         with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
@@ -502,173 +880,49 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (requested %d)" % max(args.gpus, world),
               file=sys.stderr)
         sys.exit(2)
-
+    rates = tuple(int(r) for r in args.serve_rates.split(","))
     if args.only_batch:
         print(json.dumps({"batched": batch_benchmark()}))
         return
     if args.only_serve:
-        rates = tuple(int(r) for r in args.serve_rates.split(","))
-        print(json.dumps({"serving": serving_benchmark(rates=rates, workers=args.serve_workers)}))
+        print(json.dumps({"serving": serving_benchmark(rates, args.serve_requests, workers=args.serve_workers)}))
         return
     if args.only_configs:
-        print(json.dumps({"other_configs": other_configs()}))
+        print(json.dumps({"other_configs": other_configs(not args.no_cpu)}))
         return
 
-    n = args.n
-    box = workloads.ANISO_BOX
-    tuning = tune.get() if args.no_autotune else tune.autotune(n)
-    progress("sort tuning: %s" % json.dumps(tuning))
-    x, y, z = workloads.two_clusters(n, seed=3, dtype=np.float32)
-    dx, dy, dz = DeviceArray(x), DeviceArray(y), DeviceArray(z)
-    stream = Stream()
-    cfg = make_config(max_particles_per_leaf=32, box=box, stream=stream.p)
-    tree = Tree(dx, dy, dz, n, cfg, real=np.float32, kind=BINARY)
-    tree.build()
-    graph = ctypes.c_void_p()
-    _lib().call("graph_capture_update", tree.handle, stream.p, ctypes.byref(graph))
-    addr = tree.buffer_info("cornerstone_d")[0]
-
-    def step():
-        tree.build()
-        _lib().call("graph_launch", graph.value, stream.p)
-
-    for _ in range(args.warmup):
-        step()
-    stream.sync()
-    assert tree.buffer_info("cornerstone_d")[0] == addr, "build moved the captured buffers"
-
-    # phase timings (outside the timed region)
-    e0, e1, e2 = Event(), Event(), Event()
-    tree.build()
-    e0.record(stream)
-    _lib().call("graph_launch", graph.value, stream.p)
-    e1.record(stream)
-    stream.sync()
-    update_ms = e1.ms_since(e0)
-    t0 = time.perf_counter()
-    tree.build()
-    stream.sync()
-    build_wall_ms = (time.perf_counter() - t0) * 1e3
-
-    _lib().call("profile_reset")
-    _lib().call("profile_enable", 1)
-    stream.sync()
-    e0.record(stream)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    e2.record(stream)
-    stream.sync()
-    wall = time.perf_counter() - t0
-    _lib().call("profile_enable", 0)
-    ms_per_step = wall * 1e3 / args.steps
-    event_ms = e2.ms_since(e0) / args.steps
-
-    leaves = int(tree.scalars().num_leaves)
-    def per_launch(label):
-        ms, cnt = profile_query(label)
-        return ms / max(cnt, 1), cnt
-
-    dom_ms, dom_n = per_launch(DOMINANT_LABEL)
-    key_ms, _ = per_launch("keygen")
-    pass_ms, _ = per_launch("sort_pass")
-    level_ms, level_n = per_launch("msd_level")
-    dir_ms, dir_n = profile_query("bin_direct")
-    achieved = PAIR_BYTES * n / (dom_ms * 1e-3) / 1e9 if dom_n else 0.0
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            t = json.load(f)
-            if t.get("kernel") == DOMINANT_KERNEL:
-                traffic = t.get("bytes_per_launch")
-
-    # an update that rebalances: reflected Gaussian steps (sigma = 1e-3 of the box)
-    x1, y1, z1 = workloads.reflect_step((x, y, z), 1e-3, seed=33, box=box)
-    moved_ms = []
-    for r in range(3):
-        tree.build()
-        for d, a in ((dx, x1), (dy, y1), (dz, z1)):
-            d.upload(a)
-        stream.sync()
-        e0.record(stream)
-        _lib().call("graph_launch", graph.value, stream.p)
-        e1.record(stream)
-        stream.sync()
-        moved_ms.append(e1.ms_since(e0))
-        for d, a in ((dx, x), (dy, y), (dz, z)):
-            d.upload(a)
-    tree.doBuffersNeedResize()
-
+    out, (A, B, box, after_build, after_update) = headline(args)
+    progress("headline done")
     if args.copy_calibration:
-        nbytes = COPY_BYTES
-        src, dst = DeviceArray(nbytes), DeviceArray(nbytes)
+        stream = Stream()
+        src, dst = DeviceArray(COPY_BYTES), DeviceArray(COPY_BYTES)
         for _ in range(3):
-            _lib().call("bench_copy", dst.ptr, src.ptr, nbytes, stream.p)
+            _lib().call("bench_copy", dst.ptr, src.ptr, COPY_BYTES, stream.p)
         stream.sync()
-
-    build_bytes = algorithmic_build_bytes(n, leaves)
-    out = {
-        "metric": METRIC,
-        "value": round(2 * n / (ms_per_step * 1e-3) / 1e6, 2),
-        "unit": "Mparticles/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic",
-        "config": {
-            "workload": "binary longest-side tree build() + graph-replayed update(), N=2^25 FP32, "
-                        "4:1:1 box, two Gaussian clusters + 20% uniform, LeafCount 32",
-            "particles": n,
-            "leaves": leaves,
-            "tree": "binary",
-            "bucket": 32,
-            "sort_tuning": tuning,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "LDS chunk sort of the MSD radix sort (%s, one launch per sort over all N pairs)"
-                      % DOMINANT_KERNEL,
-            "achieved": round(achieved, 1),
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": traffic,
-            "bytes_per_launch": PAIR_BYTES * n,
-            "avg_launch_ms": round(dom_ms, 5),
-        },
-        "phases_ms": {
-            "step_event": round(event_ms, 4),
-            "build_wall": round(build_wall_ms, 4),
-            "update_static_graph": round(update_ms, 4),
-            "update_moved_graph": round(float(np.median(moved_ms)), 4),
-            "keygen": round(key_ms, 4),
-            "sort_level1_pass": round(pass_ms, 4),
-            "sort_levels_2_to_8": round(level_ms * level_n / max(dom_n, 1), 4),
-            "sort_lds_chunks": round(dom_ms, 4),
-            "binary_construction": round(dir_ms / max(dir_n, 1) * 2, 4),
-        },
-        "build_roofline": {
-            "algorithmic_bytes": int(build_bytes),
-            "frac_of_peak": round(build_bytes / (build_wall_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4),
-        },
-    }
-    tree.close()
     if not args.no_configs:
-        out["other_configs"] = other_configs()
+        out["other_configs"] = other_configs(not args.no_cpu)
     if not args.no_batch:
         out["batched"] = batch_benchmark()
+        progress("batch done")
     if not args.no_serve:
-        out["serving"] = serving_benchmark(workers=args.serve_workers)
+        out["serving"] = serving_benchmark(rates, args.serve_requests, workers=args.serve_workers)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(x, y, z, x1, y1, z1, box, args.cpu_seconds)
+        base, exact_build, exact_update = cpu_baseline(A, B, box, after_build, after_update)
+        out["cpu_baseline"] = base
     else:
         out["cpu_baseline"] = None
+        exact_build = exact_update = None
+    oc = out.get("other_configs", {})
+    out["parity"] = {
+        "C3_32M_binary_build_vs_oracle": exact_build,
+        "C3_32M_binary_moved_graph_update_vs_oracle": exact_update,
+        "C1_65k_octree_build_and_views_vs_oracle": oc.get("octree_65k_fp64_uniform", {}).get("bit_exact_vs_oracle"),
+        "C2_16M_nfcount_build_vs_oracle": oc.get("octree_16M_fp32_plummer_hilbert_nf4096", {}).get(
+            "bit_exact_build_vs_oracle"),
+        "C4_512_members_vs_plain_trees": out.get("batched", {}).get("bit_exact_all_members_vs_plain"),
+        "full_size_tests": "tests/test_gpu_fullsize.py: C1 build + 3 updates, C2 build + 10 updates + views, "
+                           "C3 graph updates, C4 all members vs plain + 32 vs oracle, C5 served mix to 8M",
+    }
     print(json.dumps(out))
 
 

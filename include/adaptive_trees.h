@@ -135,6 +135,10 @@ int at_tree_scalars(at_tree *tree, at_scalars *out);
 /* numLeaves_d(), numNodes_d(), numActiveParticles_d() (octree.hpp:62-66). */
 int at_tree_device_counters(at_tree *tree, const int32_t **num_leaves,
                             const int32_t **num_nodes, const uint32_t **num_active);
+/* Measurement (no reference counterpart): the passes of the last update()
+ * that changed the tree, i.e. ran before it converged (a pass rejected for
+ * capacity does not count). Synchronises the tree's stream. */
+int at_tree_update_passes(at_tree *tree, uint32_t *passes);
 /* maxAchievedDepth() (octree.cu:666-670, kdtree3d.cu:404-415); synchronises. */
 int at_tree_max_achieved_depth(at_tree *tree, uint32_t *depth);
 /* Leaf-count convergence check of the reference's harness
