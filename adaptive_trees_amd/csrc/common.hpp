@@ -149,6 +149,118 @@ __host__ __device__ inline void hilbertDecode(Key key, unsigned &ox, unsigned &o
     }
     ox = px; oy = py; oz = pz;
 }
+// ---- Hilbert by table: the same curve as a 48-state automaton ---------------
+// Every level of hilbertKey maps the remaining coordinate words through a
+// signed permutation (which word becomes x, y, z, and which are
+// complemented), so the map accumulated over the levels above is one of 48
+// states: state = 8 * (index of the axis permutation) + flip mask. One table
+// row per (state, octant) gives the level's digit and the next state:
+//   kHilbertEnc[8 s + raw octant of the input bits]     = digit | next << 3
+//   kHilbertDec[8 s + digit]                            = raw octant | next << 3
+// (raw octant = x << 2 | y << 1 | z of the untransformed coordinates). The
+// tables are derived at compile time from the bitwise levels above;
+// tests/cpp/hilbert_tables.cpp checks them against hilbertKey/hilbertDecode.
+namespace hilbert_table
+{
+constexpr int kPerms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+constexpr int permIndex(int a, int b, int c)
+{
+    for (int i = 0; i < 6; ++i)
+        if (kPerms[i][0] == a && kPerms[i][1] == b && kPerms[i][2] == c) return i;
+    return -1;
+}
+// The transition of state s for transformed bits (xi, yi, zi) = oct.
+constexpr int next(int s, unsigned oct)
+{
+    const int *p = kPerms[s >> 3];
+    int f[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1}; // flip of output word 0, 1, 2
+    const unsigned xi = oct >> 2, yi = (oct >> 1) & 1u, zi = oct & 1u;
+    const unsigned nyi = yi ^ 1u, nzi = zi ^ 1u;
+    f[0] ^= int(xi & (nyi | zi));
+    f[1] ^= int((xi & (yi | zi)) | (yi & nzi));
+    f[2] ^= int((xi & nyi & nzi) | (yi & nzi));
+    int q[3] = {p[0], p[1], p[2]}, g[3] = {f[0], f[1], f[2]};
+    if (zi)
+    {
+        q[0] = p[1], q[1] = p[2], q[2] = p[0];
+        g[0] = f[1], g[1] = f[2], g[2] = f[0];
+    }
+    else if (!yi)
+    {
+        q[0] = p[2], q[2] = p[0];
+        g[0] = f[2], g[2] = f[0];
+    }
+    return permIndex(q[0], q[1], q[2]) * 8 + (g[0] << 2 | g[1] << 1 | g[2]);
+}
+// Transformed octant of state s for the raw input bits r (x << 2 | y << 1 | z).
+constexpr unsigned transformed(int s, unsigned r)
+{
+    const int *p = kPerms[s >> 3];
+    unsigned o = 0;
+    for (int i = 0; i < 3; ++i)
+    {
+        const unsigned bit = ((r >> (2 - p[i])) & 1u) ^ unsigned((s >> (2 - i)) & 1);
+        o |= bit << (2 - i);
+    }
+    return o;
+}
+constexpr unsigned kVisit[8] = {0, 1, 3, 2, 7, 6, 4, 5};
+struct Tables
+{
+    std::uint16_t enc[48 * 8];
+    std::uint16_t dec[48 * 8];
+};
+constexpr Tables make()
+{
+    Tables t{};
+    for (int s = 0; s < 48; ++s)
+        for (unsigned r = 0; r < 8; ++r)
+        {
+            const unsigned oct = transformed(s, r);
+            const unsigned digit = kVisit[oct];
+            const unsigned nx = unsigned(next(s, oct));
+            t.enc[8 * s + int(r)] = std::uint16_t(digit | nx << 3);
+            t.dec[8 * s + int(digit)] = std::uint16_t(r | nx << 3);
+        }
+    return t;
+}
+inline constexpr Tables kTables = make();
+constexpr int kStart = 0; // identity permutation, nothing complemented
+} // namespace hilbert_table
+
+// Encode / decode through the tables (enc / dec: kTables.enc / .dec, e.g.
+// staged in LDS by the calling kernel).
+__host__ __device__ inline Key hilbertKeyTable(const std::uint16_t *enc, unsigned px, unsigned py, unsigned pz)
+{
+    Key key = 0;
+    unsigned s = hilbert_table::kStart;
+#pragma unroll
+    for (int level = int(kOctLevels) - 1; level >= 0; --level)
+    {
+        const unsigned r = ((px >> level) & 1u) << 2 | ((py >> level) & 1u) << 1 | ((pz >> level) & 1u);
+        const unsigned e = enc[8 * s + r];
+        key = (key << 3) | Key(e & 7u);
+        s = e >> 3;
+    }
+    return key;
+}
+__host__ __device__ inline void hilbertDecodeTable(const std::uint16_t *dec, Key key, unsigned &ox, unsigned &oy,
+                                                   unsigned &oz)
+{
+    unsigned x = 0, y = 0, z = 0, s = hilbert_table::kStart;
+#pragma unroll
+    for (int level = int(kOctLevels) - 1; level >= 0; --level)
+    {
+        const unsigned d = unsigned(key >> (3 * level)) & 7u;
+        const unsigned e = dec[8 * s + d];
+        x = (x << 1) | ((e >> 2) & 1u);
+        y = (y << 1) | ((e >> 1) & 1u);
+        z = (z << 1) | (e & 1u);
+        s = e >> 3;
+    }
+    ox = x, oy = y, oz = z;
+}
+
 __host__ __device__ inline Key sfcEncode(unsigned x, unsigned y, unsigned z, bool hilbert)
 {
     return hilbert ? hilbertKey(x, y, z) : mortonKey(x, y, z);
@@ -156,6 +268,17 @@ __host__ __device__ inline Key sfcEncode(unsigned x, unsigned y, unsigned z, boo
 __host__ __device__ inline void sfcDecode(Key k, bool hilbert, unsigned &x, unsigned &y, unsigned &z)
 {
     if (hilbert) hilbertDecode(k, x, y, z);
+    else { x = gather3(k >> 2); y = gather3(k >> 1); z = gather3(k); }
+}
+// The same through the Hilbert tables (enc / dec as in hilbertKeyTable).
+__host__ __device__ inline Key sfcEncodeT(unsigned x, unsigned y, unsigned z, bool hilbert, const std::uint16_t *enc)
+{
+    return hilbert ? hilbertKeyTable(enc, x, y, z) : mortonKey(x, y, z);
+}
+__host__ __device__ inline void sfcDecodeT(Key k, bool hilbert, const std::uint16_t *dec, unsigned &x, unsigned &y,
+                                           unsigned &z)
+{
+    if (hilbert) hilbertDecodeTable(dec, k, x, y, z);
     else { x = gather3(k >> 2); y = gather3(k >> 1); z = gather3(k); }
 }
 
@@ -195,7 +318,7 @@ __host__ __device__ inline int toIntSat(T v)
 }
 
 template <class T>
-__host__ __device__ inline Key gridKey(T x, T y, T z, const GridBox<T> &b, bool hilbert)
+__host__ __device__ inline void gridCell(T x, T y, T z, const GridBox<T> &b, unsigned &cx, unsigned &cy, unsigned &cz)
 {
     int ix = toIntSat<T>(fma(-b.xmin, b.mx, floor(x * b.mx)));
     int iy = toIntSat<T>(fma(-b.ymin, b.my, floor(y * b.my)));
@@ -203,7 +326,22 @@ __host__ __device__ inline Key gridKey(T x, T y, T z, const GridBox<T> &b, bool 
     ix = ix < b.cx - 1 ? ix : b.cx - 1;
     iy = iy < b.cy - 1 ? iy : b.cy - 1;
     iz = iz < b.cz - 1 ? iz : b.cz - 1;
-    return sfcEncode(unsigned(ix), unsigned(iy), unsigned(iz), hilbert);
+    cx = unsigned(ix), cy = unsigned(iy), cz = unsigned(iz);
+}
+template <class T>
+__host__ __device__ inline Key gridKey(T x, T y, T z, const GridBox<T> &b, bool hilbert)
+{
+    unsigned cx, cy, cz;
+    gridCell<T>(x, y, z, b, cx, cy, cz);
+    return sfcEncode(cx, cy, cz, hilbert);
+}
+// The same key through the Hilbert tables (Hilbert trees only).
+template <class T>
+__host__ __device__ inline Key gridKeyHilbertTable(T x, T y, T z, const GridBox<T> &b, const std::uint16_t *enc)
+{
+    unsigned cx, cy, cz;
+    gridCell<T>(x, y, z, b, cx, cy, cz);
+    return hilbertKeyTable(enc, cx, cy, cz);
 }
 
 // ---- binary longest-side keys (binary_node.cuh:43-59) ------------------------
@@ -435,36 +573,66 @@ __device__ inline void lowerBoundPair(const Key *a, std::size_t n, Key v0, Key v
     r1 = b1 + (a[b1] < v1 ? 1 : 0);
 }
 
-// lower_bound(P, n, v0) and lower_bound(P, n, v1) through the index Q.
-__device__ inline void lowerBoundIndexedPair(const Key *P, std::size_t n, const Key *Q, Key v0, Key v1,
-                                             std::size_t &r0, std::size_t &r1)
+// The answers r = lower_bound(a, n, v) given k = lower_bound over the
+// samples a[0], a[S], a[2S], ... (ceil(n/S) of them): r lies in
+// ((k-1) S, k S] (clipped to n), found by branch-free halving inside that
+// window; two searches in lock step.
+__device__ inline void refinePair(const Key *a, std::size_t n, std::size_t S, std::size_t k0, std::size_t k1, Key v0,
+                                  Key v1, std::size_t &r0, std::size_t &r1)
 {
-    const std::size_t nq = indexSize(n);
-    std::size_t j0, j1;
-    lowerBoundPair(Q, nq, v0, v1, j0, j1);
-    // the answer lies in ((j-1) S, j S] (clipped to [0, n])
-    std::size_t b0 = j0 ? (j0 - 1) * kIndexStride + 1 : 0, b1 = j1 ? (j1 - 1) * kIndexStride + 1 : 0;
-    std::size_t l0 = (j0 < nq ? j0 * kIndexStride : n) - b0, l1 = (j1 < nq ? j1 * kIndexStride : n) - b1;
-    if (j0 == 0) l0 = 0;
-    if (j1 == 0) l1 = 0;
-    // lower bound inside [b, b + l]: branch-free halving while the window is > 1
+    const std::size_t m = (n + S - 1) / S;
+    std::size_t b0 = k0 ? (k0 - 1) * S + 1 : 0, b1 = k1 ? (k1 - 1) * S + 1 : 0;
+    std::size_t l0 = k0 ? (k0 < m ? k0 * S : n) - b0 : 0, l1 = k1 ? (k1 < m ? k1 * S : n) - b1 : 0;
     while (l0 > 1 || l1 > 1)
     {
         if (l0 > 1)
         {
             const std::size_t h = l0 >> 1;
-            b0 = P[b0 + h] < v0 ? b0 + h : b0;
+            b0 = a[b0 + h] < v0 ? b0 + h : b0;
             l0 -= h;
         }
         if (l1 > 1)
         {
             const std::size_t h = l1 >> 1;
-            b1 = P[b1 + h] < v1 ? b1 + h : b1;
+            b1 = a[b1 + h] < v1 ? b1 + h : b1;
             l1 -= h;
         }
     }
-    r0 = l0 ? b0 + (P[b0] < v0 ? 1 : 0) : b0;
-    r1 = l1 ? b1 + (P[b1] < v1 ? 1 : 0) : b1;
+    r0 = l0 ? b0 + (a[b0] < v0 ? 1 : 0) : b0;
+    r1 = l1 ? b1 + (a[b1] < v1 ? 1 : 0) : b1;
+}
+
+// lower_bound(P, n, v0) and lower_bound(P, n, v1) through the index Q.
+__device__ inline void lowerBoundIndexedPair(const Key *P, std::size_t n, const Key *Q, Key v0, Key v1,
+                                             std::size_t &r0, std::size_t &r1)
+{
+    std::size_t j0, j1;
+    lowerBoundPair(Q, indexSize(n), v0, v1, j0, j1);
+    refinePair(P, n, kIndexStride, j0, j1, v0, v1, r0, r1);
+}
+
+// Lower bounds of v0 and v1 inside P[lo, hi) when both lie in [lo, hi]:
+// lo + the number of keys of P[lo, hi) below each value.
+__device__ inline void lowerBoundPairIn(const Key *P, std::size_t lo, std::size_t hi, Key v0, Key v1,
+                                        std::size_t &r0, std::size_t &r1)
+{
+    std::size_t n = hi - lo;
+    if (n == 0)
+    {
+        r0 = r1 = lo;
+        return;
+    }
+    std::size_t b0 = lo, b1 = lo;
+    while (n > 1)
+    {
+        const std::size_t half = n >> 1;
+        const Key x0 = P[b0 + half], x1 = P[b1 + half];
+        b0 = x0 < v0 ? b0 + half : b0;
+        b1 = x1 < v1 ? b1 + half : b1;
+        n -= half;
+    }
+    r0 = b0 + (P[b0] < v0 ? 1 : 0);
+    r1 = b1 + (P[b1] < v1 ? 1 : 0);
 }
 
 // ---- device-scope atomics used by the chained scans ---------------------------

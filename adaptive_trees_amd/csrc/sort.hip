@@ -135,10 +135,21 @@ __device__ inline void load4(const double *a, std::size_t i, double v[4])
     v[0] = q0.x, v[1] = q0.y, v[2] = q1.x, v[3] = q1.y;
 }
 
+// The Hilbert encode table (hilbert_table::kTables.enc) staged in LDS by a
+// block that makes Hilbert keys; all threads call it.
+constexpr int kHilbertRows = 48 * 8;
+__device__ inline void stageHilbertEnc(std::uint16_t *lds)
+{
+    for (int i = threadIdx.x; i < kHilbertRows; i += blockDim.x) lds[i] = hilbert_table::kTables.enc[i];
+}
+
 template <class T, KeyKind K, bool FAST>
-__device__ inline Key keyOf(const KeygenParams<T> &p, const std::uint32_t *spread, T x, T y, T z)
+__device__ inline Key keyOf(const KeygenParams<T> &p, const std::uint32_t *spread, const std::uint16_t *henc, T x,
+                            T y, T z)
 {
     if constexpr (FAST) return binaryKeyFast<T>(x, y, z, p.geom, spread);
+    else if constexpr (K == KeyKind::Hilbert)
+        return isEscaped<T>(x, y, z) ? kInvalidKey : gridKeyHilbertTable<T>(x, y, z, p.grid, henc);
     else return coordKey<T, K>(p, x, y, z);
 }
 
@@ -147,8 +158,9 @@ __device__ inline Key keyOf(const KeygenParams<T> &p, const std::uint32_t *sprea
 // groups of 4 consecutive particles (16-byte loads and stores), several
 // groups in flight; otherwise kKeygenUnroll single particles.
 template <class T, KeyKind K, bool FAST>
-__device__ __forceinline__ void keygenPart(const KeygenParams<T> &p, const std::uint32_t *spread, std::uint32_t *h,
-                                           Key *__restrict__ keys, std::size_t begin, std::size_t end, int histShift)
+__device__ __forceinline__ void keygenPart(const KeygenParams<T> &p, const std::uint32_t *spread,
+                                           const std::uint16_t *henc, std::uint32_t *h, Key *__restrict__ keys,
+                                           std::size_t begin, std::size_t end, int histShift)
 {
     const bool vec = ((reinterpret_cast<std::uintptr_t>(p.x) | reinterpret_cast<std::uintptr_t>(p.y) |
                        reinterpret_cast<std::uintptr_t>(p.z) | reinterpret_cast<std::uintptr_t>(keys)) & 15) == 0 &&
@@ -178,7 +190,7 @@ __device__ __forceinline__ void keygenPart(const KeygenParams<T> &p, const std::
             {
                 Key k[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) k[u] = keyOf<T, K, FAST>(p, spread, x[g][u], y[g][u], z[g][u]);
+                for (int u = 0; u < 4; ++u) k[u] = keyOf<T, K, FAST>(p, spread, henc, x[g][u], y[g][u], z[g][u]);
                 ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(keys + i + g * kGroupStep);
                 dst[0] = make_ulonglong2(k[0], k[1]);
                 dst[1] = make_ulonglong2(k[2], k[3]);
@@ -199,7 +211,7 @@ __device__ __forceinline__ void keygenPart(const KeygenParams<T> &p, const std::
             for (int u = 0; u < kKeygenUnroll; ++u)
             {
                 const std::size_t j = i + std::size_t(u) * kSortThreads;
-                k[u] = keyOf<T, K, FAST>(p, spread, p.x[j], p.y[j], p.z[j]);
+                k[u] = keyOf<T, K, FAST>(p, spread, henc, p.x[j], p.y[j], p.z[j]);
             }
 #pragma unroll
             for (int u = 0; u < kKeygenUnroll; ++u)
@@ -211,7 +223,7 @@ __device__ __forceinline__ void keygenPart(const KeygenParams<T> &p, const std::
     }
     for (; i < end; i += kSortThreads)
     {
-        const Key k = keyOf<T, K, FAST>(p, spread, p.x[i], p.y[i], p.z[i]);
+        const Key k = keyOf<T, K, FAST>(p, spread, henc, p.x[i], p.y[i], p.z[i]);
         keys[i] = k;
         atomicAdd(&h[unsigned(k >> histShift) & (kRadix - 1)], 1u);
     }
@@ -230,9 +242,11 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     // LDS atomics on a wave-private table collide least
     __shared__ std::uint32_t hw[kWaves][kRadix];
     __shared__ std::uint32_t spread[K == KeyKind::Binary ? kRadix : 1]; // spread3 of every byte
+    __shared__ std::uint16_t henc[K == KeyKind::Hilbert ? kHilbertRows : 1];
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) hw[w][threadIdx.x] = 0;
     if constexpr (K == KeyKind::Binary) spread[threadIdx.x] = std::uint32_t(spread3(threadIdx.x));
+    if constexpr (K == KeyKind::Hilbert) stageHilbertEnc(henc);
     __syncthreads();
     std::uint32_t *h = hw[threadIdx.x >> 6];
     const std::size_t row = xcdRow(blockIdx.x, gridDim.x);
@@ -244,11 +258,11 @@ __global__ __launch_bounds__(kSortThreads) void keygenKernel(KeygenParams<T> p, 
     const std::size_t end = end0 < rEnd ? end0 : rEnd;
     if constexpr (K == KeyKind::Binary)
     {
-        if (p.geom.fast) keygenPart<T, K, true>(p, spread, h, keys, begin, end, histShift);
-        else keygenPart<T, K, false>(p, spread, h, keys, begin, end, histShift);
+        if (p.geom.fast) keygenPart<T, K, true>(p, spread, henc, h, keys, begin, end, histShift);
+        else keygenPart<T, K, false>(p, spread, henc, h, keys, begin, end, histShift);
     }
     else
-        keygenPart<T, K, false>(p, spread, h, keys, begin, end, histShift);
+        keygenPart<T, K, false>(p, spread, henc, h, keys, begin, end, histShift);
     __syncthreads();
     std::uint32_t t = 0;
 #pragma unroll
@@ -810,7 +824,9 @@ __global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> 
                                                                 std::uint32_t *hist, int histShift)
 {
     __shared__ std::uint32_t h[kRadix];
+    __shared__ std::uint16_t henc[K == KeyKind::Hilbert ? kHilbertRows : 1];
     h[threadIdx.x] = 0;
+    if constexpr (K == KeyKind::Hilbert) stageHilbertEnc(henc);
     __syncthreads();
     const SegRange d = ranges[blockIdx.x];
     const SegCoords<T> c = coords[d.seg];
@@ -819,7 +835,8 @@ __global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> 
     p.z = c.z;
     for (std::uint32_t j = threadIdx.x; j < d.len; j += kSortThreads)
     {
-        const Key k = particleKey<T, K>(p, std::size_t(d.localBase) + j);
+        const std::size_t i = std::size_t(d.localBase) + j;
+        const Key k = keyOf<T, K, false>(p, nullptr, henc, p.x[i], p.y[i], p.z[i]);
         keys[d.begin + j] = k;
         countDigit(h, unsigned(k >> histShift) & (kRadix - 1));
     }

@@ -609,7 +609,7 @@ __device__ void nfCollectBody(const TreeDev &t, int full, unsigned block)
     const Key *tree = t.K[ab];
     const unsigned lane = __lane_id();
     const int wave = threadIdx.x >> 6;
-    std::uint32_t flags = 0; // bits 3r, 3r+1, 3r+2: own, lead, grp of row r
+    std::uint32_t flags = 0; // bits 4r..4r+3: own, lead, grp, grp with the parent's test of row r
 #pragma unroll 1
     for (int r = 0; r < ROWS; ++r)
     {
@@ -634,7 +634,14 @@ __device__ void nfCollectBody(const TreeDev &t, int full, unsigned block)
                 }
             }
         }
-        flags |= (std::uint32_t(own) | std::uint32_t(lead) << 1 | std::uint32_t(grp) << 2) << (3 * r);
+        // a group item whose leader is also a dirty leader does the parent's
+        // merge test too (its 27 parent-level cells are the ones the group's
+        // own stencils are bracketed by)
+        const bool grpLead = grp && lead;
+        lead = lead && !grp;
+        flags |= (std::uint32_t(own) | std::uint32_t(lead) << 1 | std::uint32_t(grp) << 2 |
+                  std::uint32_t(grpLead) << 3)
+                 << (4 * r);
         const std::uint64_t bo = __ballot(own), bl = __ballot(lead), bg = __ballot(grp);
         if (lane == 0)
         {
@@ -665,14 +672,14 @@ __device__ void nfCollectBody(const TreeDev &t, int full, unsigned block)
     for (int r = 0; r < ROWS; ++r)
     {
         const int i = int((block * unsigned(ROWS) + unsigned(r)) * blockDim.x + threadIdx.x);
-        const bool own = (flags >> (3 * r)) & 1u, lead = (flags >> (3 * r + 1)) & 1u,
-                   grp = (flags >> (3 * r + 2)) & 1u;
+        const bool own = (flags >> (4 * r)) & 1u, lead = (flags >> (4 * r + 1)) & 1u,
+                   grp = (flags >> (4 * r + 2)) & 1u, grpLead = (flags >> (4 * r + 3)) & 1u;
         const std::uint64_t bo = __ballot(own), bl = __ballot(lead), bg = __ballot(grp);
         const std::uint32_t baseIdx = blockBase[0] + waveCnt[r][wave][0];
         const std::uint32_t baseGrp = blockBase[1] + waveCnt[r][wave][1];
         if (own) t.nfTasks[baseIdx + __popcll(bo & lt)] = std::uint32_t(i) << 1;
         if (lead) t.nfTasks[baseIdx + __popcll(bo) + __popcll(bl & lt)] = (std::uint32_t(i) << 1) | 1u;
-        if (grp) t.nfGroups[baseGrp + __popcll(bg & lt)] = std::uint32_t(i);
+        if (grp) t.nfGroups[baseGrp + __popcll(bg & lt)] = (std::uint32_t(i) << 1) | (grpLead ? 1u : 0u);
     }
 }
 constexpr int kCollectRows = 8; // rows of kThreads leaves per block (single trees)
@@ -681,32 +688,121 @@ __global__ __launch_bounds__(kThreads) void nfCollectKernel(TreeDev t, int full)
     nfCollectBody<kCollectRows>(t, full, blockIdx.x);
 }
 
-// Near-field count of the level-`level` node at `start`: its own population
-// times the population of the 27 same-level cells, periodic at 2^21 per axis
-// (computeNFCooperative, octree_rebalance.cuh:579-634). One lane per cell in
-// groups of 32 lanes.
-__device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, Key start, unsigned level, bool hilbert,
-                                 unsigned lane32, std::uint64_t &center, std::uint64_t &sum)
+// ---- near-field stencils ---------------------------------------------------------
+// Cell populations are lower-bound differences over P (computeNFCooperative,
+// octree_rebalance.cuh:579-634: the 27 same-level cells around a node,
+// periodic at 2^21 per axis). A level-l cell lies inside one level-(l-1)
+// cell, whose keys form one contiguous run of P: one global search pair per
+// parent-level cell brackets every child-level cell inside it, whose own
+// searches then run inside that (short, cache-near) run. The global searches
+// start in a table of every S-th entry of the sampled index Q held in LDS by
+// each block, then refine through one window of Q and one of P.
+constexpr unsigned kTopEntries = 2048;
+constexpr int kAxis = int(1u << kOctLevels);
+
+struct TopTable
 {
+    const Key *keys;
+    std::uint32_t count, stride;
+};
+
+// The Hilbert tables (hilbert_table::kTables) staged in LDS; every thread
+// of the block calls it (before loadTopTable's barrier).
+struct HilbertLds
+{
+    std::uint16_t enc[48 * 8], dec[48 * 8];
+};
+__device__ void stageHilbert(HilbertLds &h)
+{
+    for (int i = threadIdx.x; i < 48 * 8; i += blockDim.x)
+    {
+        h.enc[i] = hilbert_table::kTables.enc[i];
+        h.dec[i] = hilbert_table::kTables.dec[i];
+    }
+}
+
+// Called by every thread of the block.
+__device__ TopTable loadTopTable(Key *lds, const Key *Q, std::size_t nq)
+{
+    const std::uint32_t stride = nq <= kTopEntries ? 1u : std::uint32_t((nq + kTopEntries - 1) / kTopEntries);
+    const std::uint32_t count = std::uint32_t((nq + stride - 1) / stride);
+    for (std::uint32_t i = threadIdx.x; i < count; i += blockDim.x) lds[i] = Q[std::size_t(i) * stride];
+    __syncthreads();
+    return TopTable{lds, count, stride};
+}
+
+// lower_bound(P, n, v0), lower_bound(P, n, v1) through the LDS table and Q.
+__device__ inline void lowerBoundTopPair(const Key *P, std::size_t n, const Key *Q, const TopTable &tab, Key v0,
+                                         Key v1, std::size_t &r0, std::size_t &r1)
+{
+    std::size_t k0 = 0, k1 = 0;
+    std::uint32_t m = tab.count;
+    if (m)
+    {
+        std::uint32_t b0 = 0, b1 = 0;
+        while (m > 1)
+        {
+            const std::uint32_t h = m >> 1;
+            b0 = tab.keys[b0 + h] < v0 ? b0 + h : b0;
+            b1 = tab.keys[b1 + h] < v1 ? b1 + h : b1;
+            m -= h;
+        }
+        k0 = b0 + (tab.keys[b0] < v0 ? 1u : 0u);
+        k1 = b1 + (tab.keys[b1] < v1 ? 1u : 0u);
+    }
+    const std::size_t nq = indexSize(n);
+    std::size_t j0, j1;
+    refinePair(Q, nq, tab.stride, k0, k1, v0, v1, j0, j1);
+    refinePair(P, n, kIndexStride, j0, j1, v0, v1, r0, r1);
+}
+
+__device__ inline int wrapAxis(int v) { return v < 0 ? v + kAxis : (v >= kAxis ? v - kAxis : v); }
+
+// Near-field count of the level-`level` node at `start` (its own population
+// and the sum over its 27 cells), by groups of 32 lanes: lanes 0..7 search
+// the 2 x 2 x 2 level-(level-1) cells that hold the stencil, lanes 0..26 then
+// count one stencil cell each inside its parent-level run. The root's
+// stencil (27 copies of the root) is searched directly.
+__device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, const TopTable &tab,
+                                 const std::uint16_t *enc, const std::uint16_t *dec, Key start, unsigned level,
+                                 bool hilbert, unsigned lane32, std::uint64_t &center, std::uint64_t &sum)
+{
+    unsigned cx, cy, cz;
+    sfcDecodeT(start, hilbert, dec, cx, cy, cz);
+    const int w = int(1u << (kOctLevels - level));
+    const int u[3] = {int(cx) / w, int(cy) / w, int(cz) / w}; // node index along each axis
+    std::size_t pa = 0, pb = 0;
+    if (level > 0 && lane32 < 8)
+    {
+        int pc[3];
+        for (int a = 0; a < 3; ++a) pc[a] = wrapAxis((((u[a] - 1) >> 1) + int((lane32 >> a) & 1u)) * 2 * w);
+        const Key pk = enclosingBoxCode(sfcEncodeT(unsigned(pc[0]), unsigned(pc[1]), unsigned(pc[2]), hilbert, enc),
+                                        level - 1);
+        lowerBoundTopPair(P, n, Q, tab, pk, pk + nodeRange(level - 1), pa, pb);
+    }
     std::uint64_t pop = 0;
+    int parent = 0;
+    Key ck = start;
     if (lane32 < 27)
     {
-        Key ck = start;
-        if (lane32 != 13)
+        const int d[3] = {int(lane32 % 3) - 1, int((lane32 / 3) % 3) - 1, int(lane32 / 9) - 1};
+        int q[3];
+        for (int a = 0; a < 3; ++a)
         {
-            unsigned cx, cy, cz;
-            sfcDecode(start, hilbert, cx, cy, cz);
-            const int len = int(1u << (kOctLevels - level));
-            const int R = int(1u << kOctLevels);
-            const int dx = int(lane32 % 3) - 1, dy = int((lane32 / 3) % 3) - 1, dz = int(lane32 / 9) - 1;
-            auto pbc = [](int v) { const int w = v < 0 ? v + R : v; return w >= R ? w - R : w; };
-            ck = enclosingBoxCode(sfcEncode(unsigned(pbc(int(cx) + dx * len)), unsigned(pbc(int(cy) + dy * len)),
-                                            unsigned(pbc(int(cz) + dz * len)), hilbert),
-                                  level);
+            const int v = u[a] + d[a];
+            parent |= ((v >> 1) - ((u[a] - 1) >> 1)) << a;
+            q[a] = wrapAxis(v * w);
         }
-        std::size_t a, b;
-        lowerBoundIndexedPair(P, n, Q, ck, ck + nodeRange(level), a, b);
-        pop = std::uint64_t(b - a);
+        if (lane32 != 13)
+            ck = enclosingBoxCode(sfcEncodeT(unsigned(q[0]), unsigned(q[1]), unsigned(q[2]), hilbert, enc), level);
+    }
+    const std::size_t lo = __shfl(pa, parent, 32), hi = __shfl(pb, parent, 32);
+    if (lane32 < 27)
+    {
+        std::size_t r0, r1;
+        if (level > 0) lowerBoundPairIn(P, lo, hi, ck, ck + nodeRange(level), r0, r1);
+        else lowerBoundTopPair(P, n, Q, tab, ck, ck + nodeRange(level), r0, r1);
+        pop = std::uint64_t(r1 - r0);
     }
     std::uint64_t s = pop;
 #pragma unroll
@@ -715,43 +811,65 @@ __device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, Key 
     center = __shfl(pop, 13, 32);
 }
 
-// One wave per complete sibling group at level l: lane c counts the cell
-// (c%4 - 1, c/4%4 - 1, c/16 - 1) of level l around the parent's corner (the
-// 4x4x4 block holding the 27-cell stencils of all 8 children, periodic at
-// 2^21 per axis as nearField), then lanes 0..7 sum their child's 27 cells.
-// Same cells and the same u64 products as nearField per leaf.
-__device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned block, unsigned blocks)
+// One wave per complete sibling group at level l >= 1: lanes 0..26 search
+// the 27 level-(l-1) cells around the parent (the parent's own stencil);
+// lane c then counts the level-l cell (c%4 - 1, c/4%4 - 1, c/16 - 1) around
+// the parent's corner (the 4x4x4 block holding the 27-cell stencils of all
+// 8 children) inside its parent-level run, and lanes 0..7 sum their child's
+// 27 cells. Items flagged for it also get the parent's merge test (its NF
+// from the same 27 parent-level searches). Same cells and the same u64
+// products as the per-node stencil.
+__device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsigned block, unsigned blocks)
 {
+    __shared__ Key topKeys[kTopEntries];
+    __shared__ HilbertLds hl;
     DevState *st = t.st;
     if (st->converged) return;
+    const std::uint32_t total = t.nfCount[1];
+    if (block * (blockDim.x / 64) >= total) return; // no item for this block
+    if (hilbert) stageHilbert(hl);
+    const TopTable tab = loadTopTable(topKeys, t.Pidx, indexSize(t.n));
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
-    const std::uint32_t total = t.nfCount[1];
     const unsigned lane = __lane_id();
     const std::uint32_t waves = blocks * (blockDim.x / 64);
-    constexpr int R = int(1u << kOctLevels);
-    auto pbc = [](int v) { const int w = v < 0 ? v + R : v; return w >= R ? w - R : w; };
     for (std::uint32_t g = block * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += waves)
     {
-        const int leaf0 = int(t.nfGroups[g]);
+        const std::uint32_t item = t.nfGroups[g];
+        const int leaf0 = int(item >> 1);
         const Key start = tree[leaf0];
         const unsigned level = treeLevel(tree[leaf0 + 1] - start);
         const int w = int(1u << (kOctLevels - level));
         unsigned cx, cy, cz;
-        sfcDecode(start, hilbert != 0, cx, cy, cz);
+        sfcDecodeT(start, hilbert != 0, hl.dec, cx, cy, cz);
         const unsigned corner = ~(2u * unsigned(w) - 1u);
         const int bx = int(cx & corner), by = int(cy & corner), bz = int(cz & corner);
+        // the parent-level cells
+        std::size_t pa = 0, pb = 0;
+        if (lane < 27)
+        {
+            const int dx = int(lane % 3) - 1, dy = int((lane / 3) % 3) - 1, dz = int(lane / 9) - 1;
+            const Key pk =
+                enclosingBoxCode(sfcEncodeT(unsigned(wrapAxis(bx + 2 * dx * w)), unsigned(wrapAxis(by + 2 * dy * w)),
+                                            unsigned(wrapAxis(bz + 2 * dz * w)), hilbert != 0, hl.enc),
+                                 level - 1);
+            lowerBoundTopPair(t.P, t.n, t.Pidx, tab, pk, pk + nodeRange(level - 1), pa, pb);
+        }
+        // the child-level cells, each inside its parent-level cell's run
         const int a = int(lane & 3u) - 1, b = int((lane >> 2) & 3u) - 1, c = int(lane >> 4) - 1;
-        const Key ck = enclosingBoxCode(
-            sfcEncode(unsigned(pbc(bx + a * w)), unsigned(pbc(by + b * w)), unsigned(pbc(bz + c * w)), hilbert != 0),
-            level);
-        std::size_t lo, hi;
-        lowerBoundIndexedPair(t.P, t.n, t.Pidx, ck, ck + nodeRange(level), lo, hi);
-        const std::uint32_t pop = std::uint32_t(hi - lo);
+        auto third = [](int o) { return o < 0 ? 0 : (o < 2 ? 1 : 2); };
+        const int parent = third(a) + 3 * third(b) + 9 * third(c);
+        const std::size_t lo = __shfl(pa, parent, 64), hi = __shfl(pb, parent, 64);
+        const Key ck = enclosingBoxCode(sfcEncodeT(unsigned(wrapAxis(bx + a * w)), unsigned(wrapAxis(by + b * w)),
+                                                   unsigned(wrapAxis(bz + c * w)), hilbert != 0, hl.enc),
+                                        level);
+        std::size_t r0, r1;
+        lowerBoundPairIn(t.P, lo, hi, ck, ck + nodeRange(level), r0, r1);
+        const std::uint32_t pop = std::uint32_t(r1 - r0);
         // child `lane` (lanes 0..7) of the group: its offset inside the parent
         const int leaf = leaf0 + int(lane & 7u);
         unsigned x, y, z;
-        sfcDecode(tree[leaf], hilbert != 0, x, y, z);
+        sfcDecodeT(tree[leaf], hilbert != 0, hl.dec, x, y, z);
         const int rx = int(x) - bx >= w ? 1 : 0, ry = int(y) - by >= w ? 1 : 0, rz = int(z) - bz >= w ? 1 : 0;
         std::uint64_t sum = 0;
 #pragma unroll
@@ -768,20 +886,35 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned block, unsig
             t.N[ab][leaf] = own > 0xffffffffull ? 0xffffffffu : unsigned(own);
             t.NF[ab][leaf] = nf > 0xffffffffull ? 0xffffffffu : unsigned(nf);
         }
+        if (item & 1u)
+        {
+            // the parent's merge test: its population times its 27 cells'
+            // (groupCanMerge, compared unclamped)
+            std::uint64_t pp = lane < 27 ? std::uint64_t(pb - pa) : 0;
+            const std::uint64_t pown = __shfl(pp, 13, 64);
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) pp += __shfl_xor(pp, m, 64);
+            if (lane == 0) t.gcm[ab][leaf0] = pown * pp <= std::uint64_t(bucket) ? 1 : 0;
+        }
     }
 }
-__global__ __launch_bounds__(kThreads) void nfGroupKernel(TreeDev t, int hilbert)
+__global__ __launch_bounds__(kThreads) void nfGroupKernel(TreeDev t, int hilbert, unsigned bucket)
 {
-    nfGroupBody(t, hilbert, blockIdx.x, gridDim.x);
+    nfGroupBody(t, hilbert, bucket, blockIdx.x, gridDim.x);
 }
 
 __device__ void nfComputeBody(const TreeDev &t, unsigned bucket, int hilbert, unsigned block, unsigned blocks)
 {
+    __shared__ Key topKeys[kTopEntries];
+    __shared__ HilbertLds hl;
     DevState *st = t.st;
     if (st->converged) return;
+    const std::uint32_t total = t.nfCount[0];
+    if (block * (blockDim.x / 32) >= total) return; // no item for this block
+    if (hilbert) stageHilbert(hl);
+    const TopTable tab = loadTopTable(topKeys, t.Pidx, indexSize(t.n));
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
-    const std::uint32_t total = t.nfCount[0];
     const unsigned lane32 = threadIdx.x & 31;
     const std::uint32_t groups = blocks * (blockDim.x / 32);
     for (std::uint32_t g = block * (blockDim.x / 32) + threadIdx.x / 32; g < total; g += groups)
@@ -793,7 +926,7 @@ __device__ void nfComputeBody(const TreeDev &t, unsigned bucket, int hilbert, un
         unsigned level = treeLevel(tree[leaf + 1] - start);
         if (parent) level -= 1;
         std::uint64_t center, sum;
-        nearField(t.P, t.n, t.Pidx, start, level, hilbert != 0, lane32, center, sum);
+        nearField(t.P, t.n, t.Pidx, tab, hl.enc, hl.dec, start, level, hilbert != 0, lane32, center, sum);
         if (lane32 == 0)
         {
             const std::uint64_t nf = center * sum;
@@ -1266,10 +1399,10 @@ __global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsi
     AT_MULTI_PROLOGUE;
     nfComputeBody(t, bucket, hilbert, block, blocks);
 }
-__global__ __launch_bounds__(kThreads) void multiNfGroupKernel(TreeSet s, int hilbert)
+__global__ __launch_bounds__(kThreads) void multiNfGroupKernel(TreeSet s, int hilbert, unsigned bucket)
 {
     AT_MULTI_PROLOGUE;
-    nfGroupBody(t, hilbert, block, blocks);
+    nfGroupBody(t, hilbert, bucket, block, blocks);
 }
 #undef AT_MULTI_PROLOGUE
 } // namespace
@@ -1341,7 +1474,7 @@ void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, co
     prof::Scope scope("near_field", s);
     hipLaunchKernelGGL(multiNfCollectKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, full ? 1 : 0);
     hipLaunchKernelGGL(multiNfComputeKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.bucket, p.hilbert ? 1 : 0);
-    hipLaunchKernelGGL(multiNfGroupKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.hilbert ? 1 : 0);
+    hipLaunchKernelGGL(multiNfGroupKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.hilbert ? 1 : 0, p.bucket);
 }
 
 std::size_t passTiles(std::size_t cap) { return cap / kPassTile + 1; }
@@ -1388,7 +1521,7 @@ void octRefresh(hipStream_t s, const TreeDev &t, const OctParams &p, bool full, 
     hipLaunchKernelGGL(nfCollectKernel, dim3(ceilDiv(gridLeaves, kThreads * kCollectRows)), dim3(kThreads), 0, s, t,
                        full ? 1 : 0);
     hipLaunchKernelGGL(nfComputeKernel, dim3(2048), dim3(kThreads), 0, s, t, p.bucket, p.hilbert ? 1 : 0);
-    hipLaunchKernelGGL(nfGroupKernel, dim3(2048), dim3(kThreads), 0, s, t, p.hilbert ? 1 : 0);
+    hipLaunchKernelGGL(nfGroupKernel, dim3(2048), dim3(kThreads), 0, s, t, p.hilbert ? 1 : 0, p.bucket);
 }
 void octPass(hipStream_t s, const TreeDev &t, const OctParams &p, long capacity, std::size_t gridLeaves)
 {
