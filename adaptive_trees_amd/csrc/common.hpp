@@ -261,6 +261,65 @@ __host__ __device__ inline void hilbertDecodeTable(const std::uint16_t *dec, Key
     ox = x, oy = y, oz = z;
 }
 
+// Cells by their level: the key of the level-`level` cell holding grid
+// point (x, y, z) (its lower digits zero) and, for Hilbert, the table state
+// below it; a child cell's key from its parent's code and the child's raw
+// octant (the coordinate bits at the child's level); the corner of a cell
+// from its key. Hilbert codes only walk the `level` digits above.
+struct CellCode
+{
+    Key key;
+    unsigned state;
+};
+__host__ __device__ inline CellCode cellCode(unsigned x, unsigned y, unsigned z, unsigned level, bool hilbert,
+                                             const std::uint16_t *enc)
+{
+    if (!hilbert) return CellCode{mortonKey(x, y, z) & ~(nodeRange(level) - 1), 0u};
+    Key key = 0;
+    unsigned s = hilbert_table::kStart;
+    for (unsigned l = 0; l < level; ++l)
+    {
+        const unsigned b = kOctLevels - 1 - l;
+        const unsigned r = ((x >> b) & 1u) << 2 | ((y >> b) & 1u) << 1 | ((z >> b) & 1u);
+        const unsigned e = enc[8 * s + r];
+        key = (key << 3) | Key(e & 7u);
+        s = e >> 3;
+    }
+    return CellCode{level ? key << (3u * (kOctLevels - level)) : Key(0), s};
+}
+__host__ __device__ inline Key childKey(const CellCode &parent, unsigned raw, unsigned level, bool hilbert,
+                                        const std::uint16_t *enc)
+{
+    const unsigned digit = hilbert ? unsigned(enc[8 * parent.state + raw]) & 7u : raw;
+    return parent.key | (Key(digit) << (3u * (kOctLevels - level)));
+}
+// Corner of the level-`level` cell with key `key` (coordinate bits below
+// the level zero) and its Hilbert state.
+__host__ __device__ inline void cellCorner(Key key, unsigned level, bool hilbert, const std::uint16_t *dec,
+                                           unsigned &x, unsigned &y, unsigned &z, unsigned &state)
+{
+    const unsigned low = kOctLevels - level;
+    if (!hilbert)
+    {
+        const std::uint32_t keep = ~((1u << low) - 1u);
+        x = gather3(key >> 2) & keep, y = gather3(key >> 1) & keep, z = gather3(key) & keep;
+        state = 0;
+        return;
+    }
+    unsigned cx = 0, cy = 0, cz = 0, s = hilbert_table::kStart;
+    for (unsigned l = 0; l < level; ++l)
+    {
+        const unsigned d = unsigned(key >> (3u * (kOctLevels - 1 - l))) & 7u;
+        const unsigned e = dec[8 * s + d];
+        cx = (cx << 1) | ((e >> 2) & 1u);
+        cy = (cy << 1) | ((e >> 1) & 1u);
+        cz = (cz << 1) | (e & 1u);
+        s = e >> 3;
+    }
+    x = level ? cx << low : 0u, y = level ? cy << low : 0u, z = level ? cz << low : 0u;
+    state = s;
+}
+
 __host__ __device__ inline Key sfcEncode(unsigned x, unsigned y, unsigned z, bool hilbert)
 {
     return hilbert ? hilbertKey(x, y, z) : mortonKey(x, y, z);

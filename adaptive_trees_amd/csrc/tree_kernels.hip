@@ -761,48 +761,59 @@ __device__ inline int wrapAxis(int v) { return v < 0 ? v + kAxis : (v >= kAxis ?
 // Near-field count of the level-`level` node at `start` (its own population
 // and the sum over its 27 cells), by groups of 32 lanes: lanes 0..7 search
 // the 2 x 2 x 2 level-(level-1) cells that hold the stencil, lanes 0..26 then
-// count one stencil cell each inside its parent-level run. The root's
-// stencil (27 copies of the root) is searched directly.
+// count one stencil cell each inside its parent-level run (a cell's key is
+// its parent's code plus one digit). The root's stencil (27 copies of the
+// root) is searched directly.
 __device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, const TopTable &tab,
                                  const std::uint16_t *enc, const std::uint16_t *dec, Key start, unsigned level,
                                  bool hilbert, unsigned lane32, std::uint64_t &center, std::uint64_t &sum)
 {
-    unsigned cx, cy, cz;
-    sfcDecodeT(start, hilbert, dec, cx, cy, cz);
-    const int w = int(1u << (kOctLevels - level));
-    const int u[3] = {int(cx) / w, int(cy) / w, int(cz) / w}; // node index along each axis
-    std::size_t pa = 0, pb = 0;
-    if (level > 0 && lane32 < 8)
-    {
-        int pc[3];
-        for (int a = 0; a < 3; ++a) pc[a] = wrapAxis((((u[a] - 1) >> 1) + int((lane32 >> a) & 1u)) * 2 * w);
-        const Key pk = enclosingBoxCode(sfcEncodeT(unsigned(pc[0]), unsigned(pc[1]), unsigned(pc[2]), hilbert, enc),
-                                        level - 1);
-        lowerBoundTopPair(P, n, Q, tab, pk, pk + nodeRange(level - 1), pa, pb);
-    }
     std::uint64_t pop = 0;
-    int parent = 0;
-    Key ck = start;
-    if (lane32 < 27)
+    if (level == 0)
     {
-        const int d[3] = {int(lane32 % 3) - 1, int((lane32 / 3) % 3) - 1, int(lane32 / 9) - 1};
-        int q[3];
-        for (int a = 0; a < 3; ++a)
+        if (lane32 < 27)
         {
-            const int v = u[a] + d[a];
-            parent |= ((v >> 1) - ((u[a] - 1) >> 1)) << a;
-            q[a] = wrapAxis(v * w);
+            std::size_t r0, r1;
+            lowerBoundTopPair(P, n, Q, tab, 0, nodeRange(0), r0, r1);
+            pop = std::uint64_t(r1 - r0);
         }
-        if (lane32 != 13)
-            ck = enclosingBoxCode(sfcEncodeT(unsigned(q[0]), unsigned(q[1]), unsigned(q[2]), hilbert, enc), level);
     }
-    const std::size_t lo = __shfl(pa, parent, 32), hi = __shfl(pb, parent, 32);
-    if (lane32 < 27)
+    else
     {
-        std::size_t r0, r1;
-        if (level > 0) lowerBoundPairIn(P, lo, hi, ck, ck + nodeRange(level), r0, r1);
-        else lowerBoundTopPair(P, n, Q, tab, ck, ck + nodeRange(level), r0, r1);
-        pop = std::uint64_t(r1 - r0);
+        unsigned cx, cy, cz, st;
+        cellCorner(start, level, hilbert, dec, cx, cy, cz, st);
+        const int w = int(1u << (kOctLevels - level));
+        const int u[3] = {int(cx) / w, int(cy) / w, int(cz) / w}; // node index along each axis
+        std::size_t pa = 0, pb = 0;
+        CellCode pc{0, 0};
+        if (lane32 < 8)
+        {
+            int q[3];
+            for (int a = 0; a < 3; ++a) q[a] = wrapAxis((((u[a] - 1) >> 1) + int((lane32 >> a) & 1u)) * 2 * w);
+            pc = cellCode(unsigned(q[0]), unsigned(q[1]), unsigned(q[2]), level - 1, hilbert, enc);
+            lowerBoundTopPair(P, n, Q, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
+        }
+        int parent = 0;
+        unsigned raw = 0;
+        if (lane32 < 27)
+        {
+            const int d[3] = {int(lane32 % 3) - 1, int((lane32 / 3) % 3) - 1, int(lane32 / 9) - 1};
+            for (int a = 0; a < 3; ++a)
+            {
+                const int v = u[a] + d[a];
+                parent |= ((v >> 1) - ((u[a] - 1) >> 1)) << a;
+                raw |= unsigned(wrapAxis(v * w) / w & 1) << (2 - a);
+            }
+        }
+        const std::size_t lo = __shfl(pa, parent, 32), hi = __shfl(pb, parent, 32);
+        const CellCode home{__shfl(pc.key, parent, 32), unsigned(__shfl(int(pc.state), parent, 32))};
+        if (lane32 < 27)
+        {
+            const Key ck = childKey(home, raw, level, hilbert, enc);
+            std::size_t r0, r1;
+            lowerBoundPairIn(P, lo, hi, ck, ck + nodeRange(level), r0, r1);
+            pop = std::uint64_t(r1 - r0);
+        }
     }
     std::uint64_t s = pop;
 #pragma unroll
@@ -833,6 +844,7 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsi
     const Key *tree = t.K[ab];
     const unsigned lane = __lane_id();
     const std::uint32_t waves = blocks * (blockDim.x / 64);
+    const bool hil = hilbert != 0;
     for (std::uint32_t g = block * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += waves)
     {
         const std::uint32_t item = t.nfGroups[g];
@@ -840,37 +852,37 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsi
         const Key start = tree[leaf0];
         const unsigned level = treeLevel(tree[leaf0 + 1] - start);
         const int w = int(1u << (kOctLevels - level));
-        unsigned cx, cy, cz;
-        sfcDecodeT(start, hilbert != 0, hl.dec, cx, cy, cz);
-        const unsigned corner = ~(2u * unsigned(w) - 1u);
-        const int bx = int(cx & corner), by = int(cy & corner), bz = int(cz & corner);
-        // the parent-level cells
+        // the parent's corner and its 27 parent-level cells
+        unsigned bxu, byu, bzu, pst;
+        cellCorner(start, level - 1, hil, hl.dec, bxu, byu, bzu, pst);
+        const int bx = int(bxu), by = int(byu), bz = int(bzu);
         std::size_t pa = 0, pb = 0;
+        CellCode pc{0, 0};
         if (lane < 27)
         {
             const int dx = int(lane % 3) - 1, dy = int((lane / 3) % 3) - 1, dz = int(lane / 9) - 1;
-            const Key pk =
-                enclosingBoxCode(sfcEncodeT(unsigned(wrapAxis(bx + 2 * dx * w)), unsigned(wrapAxis(by + 2 * dy * w)),
-                                            unsigned(wrapAxis(bz + 2 * dz * w)), hilbert != 0, hl.enc),
-                                 level - 1);
-            lowerBoundTopPair(t.P, t.n, t.Pidx, tab, pk, pk + nodeRange(level - 1), pa, pb);
+            pc = cellCode(unsigned(wrapAxis(bx + 2 * dx * w)), unsigned(wrapAxis(by + 2 * dy * w)),
+                          unsigned(wrapAxis(bz + 2 * dz * w)), level - 1, hil, hl.enc);
+            lowerBoundTopPair(t.P, t.n, t.Pidx, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
         }
         // the child-level cells, each inside its parent-level cell's run
         const int a = int(lane & 3u) - 1, b = int((lane >> 2) & 3u) - 1, c = int(lane >> 4) - 1;
         auto third = [](int o) { return o < 0 ? 0 : (o < 2 ? 1 : 2); };
         const int parent = third(a) + 3 * third(b) + 9 * third(c);
+        // the child's octant inside its parent: the coordinate bit at level l
+        const unsigned raw = unsigned((a + 2) & 1) << 2 | unsigned((b + 2) & 1) << 1 | unsigned((c + 2) & 1);
         const std::size_t lo = __shfl(pa, parent, 64), hi = __shfl(pb, parent, 64);
-        const Key ck = enclosingBoxCode(sfcEncodeT(unsigned(wrapAxis(bx + a * w)), unsigned(wrapAxis(by + b * w)),
-                                                   unsigned(wrapAxis(bz + c * w)), hilbert != 0, hl.enc),
-                                        level);
+        const CellCode home{__shfl(pc.key, parent, 64), unsigned(__shfl(int(pc.state), parent, 64))};
+        const Key ck = childKey(home, raw, level, hil, hl.enc);
         std::size_t r0, r1;
         lowerBoundPairIn(t.P, lo, hi, ck, ck + nodeRange(level), r0, r1);
         const std::uint32_t pop = std::uint32_t(r1 - r0);
-        // child `lane` (lanes 0..7) of the group: its offset inside the parent
+        // child `lane` (lanes 0..7) of the group: its octant inside the parent,
+        // from its digit at level l and the parent's state
         const int leaf = leaf0 + int(lane & 7u);
-        unsigned x, y, z;
-        sfcDecodeT(tree[leaf], hilbert != 0, hl.dec, x, y, z);
-        const int rx = int(x) - bx >= w ? 1 : 0, ry = int(y) - by >= w ? 1 : 0, rz = int(z) - bz >= w ? 1 : 0;
+        const unsigned digit = unsigned(tree[leaf] >> (3u * (kOctLevels - level))) & 7u;
+        const unsigned oct = hil ? unsigned(hl.dec[8 * pst + digit]) & 7u : digit;
+        const int rx = int(oct >> 2), ry = int((oct >> 1) & 1u), rz = int(oct & 1u);
         std::uint64_t sum = 0;
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)

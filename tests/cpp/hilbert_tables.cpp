@@ -38,6 +38,33 @@ int main()
         check(unsigned(r) & 0x1fffffu, unsigned(r >> 21) & 0x1fffffu, unsigned(r >> 42) & 0x1fffffu);
     }
     check(0x1fffffu, 0x1fffffu, 0x1fffffu);
+    // cell codes by level (both curves): the cell key, a child's key from
+    // its parent's code, and the cell's corner from its key
+    for (int i = 0; i < 300000; ++i)
+    {
+        const std::uint64_t r = rng();
+        const unsigned x = unsigned(r) & 0x1fffffu, y = unsigned(r >> 21) & 0x1fffffu, z = unsigned(r >> 42) & 0x1fffffu;
+        const unsigned level = unsigned(rng() % 22);
+        for (const bool hil : {false, true})
+        {
+            const Key full = sfcEncode(x, y, z, hil);
+            const CellCode c = cellCode(x, y, z, level, hil, T.enc);
+            bool ok = c.key == enclosingBoxCode(full, level);
+            if (level > 0)
+            {
+                const CellCode p = cellCode(x, y, z, level - 1, hil, T.enc);
+                const unsigned b = kOctLevels - level;
+                const unsigned raw = ((x >> b) & 1u) << 2 | ((y >> b) & 1u) << 1 | ((z >> b) & 1u);
+                ok = ok && childKey(p, raw, level, hil, T.enc) == c.key;
+            }
+            unsigned cx, cy, cz, st;
+            cellCorner(c.key, level, hil, T.dec, cx, cy, cz, st);
+            const unsigned keep = level ? ~((1u << (kOctLevels - level)) - 1u) : 0u;
+            ok = ok && cx == (x & keep) && cy == (y & keep) && cz == (z & keep) && (!hil || st == c.state);
+            ++checked;
+            if (!ok && bad++ < 5) std::printf("cell code mismatch at level %u (hilbert %d)\n", level, int(hil));
+        }
+    }
     std::printf("checked %lld bad %lld\n", checked, bad);
     return bad ? 1 : 0;
 }
