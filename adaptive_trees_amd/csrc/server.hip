@@ -104,6 +104,8 @@ struct Resident
     GraphExec graph; // touched only by the tree's worker
     int worker = -1;
     bool admitted = false;
+    bool releasing = false; // a release is queued: no further requests
+    bool failed = false;    // the build failed: no tree, bytes returned
     std::size_t bytes = 0, n = 0;
     int realBytes = 8;
     const void *x = nullptr, *y = nullptr, *z = nullptr;
@@ -134,14 +136,36 @@ public:
         for (int i = 0; i < w; ++i) threads_.emplace_back([this, i] { loop(i); });
     }
 
+    // Requests already on a worker run to completion; builds still waiting
+    // for budget, and the requests held behind them, complete with
+    // AT_ELOGIC. Returns once no thread is inside wait().
     ~Server()
     {
         {
             std::lock_guard<std::mutex> g(m_);
             stopping_ = true;
+            const auto now = Clock::now();
+            auto abandon = [&](Request *r) {
+                r->status = AT_ELOGIC;
+                r->error = "server destroyed before the request was admitted";
+                r->state = kDone;
+                r->start = r->done = now;
+            };
+            for (Request *r : pendingBuilds_) abandon(r);
+            pendingBuilds_.clear();
+            for (auto &kv : trees_)
+            {
+                for (Request *r : kv.second.held) abandon(r);
+                kv.second.held.clear();
+            }
         }
         cv_.notify_all();
+        doneCv_.notify_all();
         for (auto &t : threads_) t.join();
+        {
+            std::unique_lock<std::mutex> g(m_);
+            doneCv_.wait(g, [&] { return waiters_ == 0; });
+        }
         trees_.clear();
         for (auto &wk : workers_) (void)hipStreamDestroy(wk.stream);
         for (auto &kv : requests_) delete kv.second;
@@ -177,7 +201,7 @@ public:
     std::uint64_t submitUpdate(std::uint64_t tree, const void *nx, const void *ny, const void *nz)
     {
         std::lock_guard<std::mutex> g(m_);
-        Resident &t = residentOf(tree);
+        Resident &t = liveResident(tree);
         Request *r = newRequest(kUpdate);
         r->tree = tree;
         r->nx = nx, r->ny = ny, r->nz = nz;
@@ -188,7 +212,8 @@ public:
     std::uint64_t submitRelease(std::uint64_t tree)
     {
         std::lock_guard<std::mutex> g(m_);
-        Resident &t = residentOf(tree);
+        Resident &t = liveResident(tree);
+        t.releasing = true;
         Request *r = newRequest(kRelease);
         r->tree = tree;
         route(t, r);
@@ -199,9 +224,12 @@ public:
     {
         std::unique_lock<std::mutex> g(m_);
         Request *r = requestOf(id);
+        ++waiters_;
         doneCv_.wait(g, [&] { return r->state == kDone; });
-        if (r->status != AT_OK) lastError() = r->error;
-        return r->status;
+        const int status = r->status;
+        if (status != AT_OK) lastError() = r->error;
+        if (--waiters_ == 0 && stopping_) doneCv_.notify_all();
+        return status;
     }
 
     bool poll(std::uint64_t id)
@@ -224,7 +252,8 @@ public:
     TreeBase &tree(std::uint64_t id)
     {
         std::lock_guard<std::mutex> g(m_);
-        Resident &t = residentOf(id);
+        Resident &t = liveResident(id);
+        if (t.failed) throw Error(AT_ELOGIC, "tree failed to build");
         if (!t.tree) throw Error(AT_ELOGIC, "tree is not built yet");
         return *t.tree;
     }
@@ -265,10 +294,11 @@ private:
         if (it == requests_.end()) throw Error(AT_EINVAL, "unknown request id");
         return it->second;
     }
-    Resident &residentOf(std::uint64_t id)
+    // A tree that takes requests: known, and no release queued for it.
+    Resident &liveResident(std::uint64_t id)
     {
         auto it = trees_.find(id);
-        if (it == trees_.end()) throw Error(AT_EINVAL, "unknown or released tree id");
+        if (it == trees_.end() || it->second.releasing) throw Error(AT_EINVAL, "unknown or released tree id");
         return it->second;
     }
 
@@ -296,7 +326,7 @@ private:
         while (!pendingBuilds_.empty())
         {
             Request *r = pendingBuilds_.front();
-            Resident &t = trees_[r->tree];
+            Resident &t = trees_.at(r->tree);
             if (used_ + t.bytes > budget_) break;
             pendingBuilds_.pop_front();
             used_ += t.bytes;
@@ -347,7 +377,9 @@ private:
                 {
                     r->state = kRunning;
                     r->start = now;
-                    Resident &t = trees_[r->tree];
+                    // every queued request's tree is resident: its release
+                    // is queued behind it and refuses later submissions
+                    Resident &t = trees_.at(r->tree);
                     res.push_back(&t);
                     trees.push_back(t.tree.get());
                     wk.work -= std::min(wk.work, t.n);
@@ -414,7 +446,19 @@ private:
                     catch (const Error &e) { fail(i, e.code, e.what()); }
                     catch (const std::exception &e) { fail(i, AT_ENOMEM, e.what()); }
                 }
-                (void)hipStreamSynchronize(wk.stream);
+                // an asynchronous failure belongs to any launched member
+                const hipError_t synced = hipStreamSynchronize(wk.stream);
+                if (synced != hipSuccess)
+                {
+                    const std::string what = std::string("update batch failed: ") + hipGetErrorString(synced);
+                    for (std::size_t i = 0; i < batch.size(); ++i)
+                        if (launched[i])
+                        {
+                            status[i] = AT_EHIP;
+                            errors[i] = what;
+                            launched[i] = false;
+                        }
+                }
                 for (std::size_t i = 0; i < batch.size(); ++i)
                 {
                     if (!launched[i]) continue;
@@ -442,10 +486,17 @@ private:
             {
                 std::lock_guard<std::mutex> g(m_);
                 const auto now = Clock::now();
-                if (head->kind == kBuild)
+                if (head->kind == kBuild && status[0] == AT_OK)
                 {
                     res[0]->tree = std::move(built);
                     res[0]->graph = std::move(builtGraph);
+                }
+                else if (head->kind == kBuild)
+                {
+                    // no tree is kept; its bytes go back to the budget
+                    res[0]->failed = true;
+                    used_ -= std::min(used_, res[0]->bytes);
+                    res[0]->bytes = 0;
                 }
                 if (head->kind == kRelease)
                 {
@@ -477,6 +528,7 @@ private:
     std::mutex m_;
     std::condition_variable cv_, doneCv_;
     bool stopping_ = false;
+    int waiters_ = 0; // threads inside wait()
     std::vector<Worker> workers_;
     std::vector<std::thread> threads_;
     std::map<std::uint64_t, Resident> trees_;

@@ -2020,13 +2020,14 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
     else runPasses(s, pl, ws, keysOut, ws.keysAlt, valsOut, ws.valsAlt, true, true, false);
 }
 
-void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws, bool iota)
+void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws, bool iota,
+               bool msdAllowed)
 {
     if (n == 0) return;
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
     ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
-    const bool msd = g_msd && n >= kMsdMinKeys;
+    const bool msd = msdAllowed && g_msd && n >= kMsdMinKeys;
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
                        msd ? 64 - kRadixBits : 0, ws.rangeHist);
     if (msd) msdSort(s, pl, ws, keys, ws.keysAlt, vals, ws.valsAlt, iota, true, false);

@@ -154,8 +154,11 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
 
 // Stable sort of existing keys (and values) in place. valuesAreIndices:
 // values become the input positions 0..n-1 (vals is output only).
+// msdAllowed = false keeps the LSD passes at any size (sorts of a few
+// million keys with long runs of equal high digits, where the MSD levels'
+// fixed launches cost more than they save).
 void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, SortWorkspace &ws,
-               bool valuesAreIndices);
+               bool valuesAreIndices, bool msdAllowed = true);
 void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws);
 
 // Stable sort of at most kNoAllocMaxKeys pairs, like sortPairs, whose

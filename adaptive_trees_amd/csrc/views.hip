@@ -107,13 +107,27 @@ __global__ __launch_bounds__(kT) void classifyKernel(const Key *K0, const Key *K
             ph = placeholderKey(b & ~(viewRange(lvl, bits) - 1), int(bits * lvl));
         }
     }
+    // the block's candidates take their places with one atomic per block
+    // (the list's order is free: it is sorted next)
+    __shared__ unsigned waveCount[kT / 64], blockBase;
     const std::uint64_t bal = __ballot(emit);
     const unsigned lane = __lane_id();
-    unsigned base = 0;
-    if (lane == 0 && bal) base = atomicAdd(candCount, unsigned(__popcll(bal)));
-    base = __shfl(base, 0, 64);
-    if (emit) cand[base + __popcll(bal & ((std::uint64_t(1) << lane) - 1))] = ph;
+    const int wave = int(threadIdx.x >> 6);
+    if (lane == 0) waveCount[wave] = unsigned(__popcll(bal));
     __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned total = 0;
+        for (int w = 0; w < kT / 64; ++w)
+        {
+            const unsigned c = waveCount[w];
+            waveCount[w] = total;
+            total += c;
+        }
+        blockBase = total ? atomicAdd(candCount, total) : 0u;
+    }
+    __syncthreads();
+    if (emit) cand[blockBase + waveCount[wave] + __popcll(bal & ((std::uint64_t(1) << lane) - 1))] = ph;
     if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&leavesPerLevel[threadIdx.x], hist[threadIdx.x]);
 }
 
@@ -197,10 +211,15 @@ struct Rows
     int *leafToNode;
 };
 
+// An internal node's particles are those of its leaves [first, end) (its
+// start and end keys are leaf boundaries): begin = leafBegin[first], count =
+// leafBegin[end] - begin, with leafBegin[L] = the active count. Equal to
+// lower_bound(P, start) and lower_bound(P, end) - lower_bound(P, start).
 __global__ __launch_bounds__(kT) void internalRowsKernel(const Key *layoutPh, std::size_t icap, const DevState *st,
                                                          unsigned bits, const int *levelOffset, const int *offInternal,
                                                          const int *internalPer, const Key *S, const int *mBySfc,
-                                                         const Key *P, std::size_t n, int withCounts, Rows r)
+                                                         const Key *K0, const Key *K1, const std::uint32_t *leafBegin,
+                                                         int withCounts, Rows r)
 {
     const int idx = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (idx >= activeInternals(int(icap), st, bits)) return;
@@ -218,11 +237,14 @@ __global__ __launch_bounds__(kT) void internalRowsKernel(const Key *layoutPh, st
     for (int s = 0; s < (1 << bits); ++s) r.parent[levelOffset[lvl + 1] + s * M + m] = node;
     if (withCounts)
     {
-        const std::size_t b = lowerBound(P, n, start);
-        const std::size_t e = lowerBound(P, n, start + viewRange(lvl, bits));
-        r.begin[node] = unsigned(b);
-        r.counts[node] = unsigned(e - b);
-        if (lvl == 0) r.counts[st->numNodes] = unsigned(e - b);
+        const Key *K = st->activeBuffer ? K1 : K0;
+        const std::size_t L1 = std::size_t(st->numLeaves) + 1;
+        const std::size_t f = lowerBound(K, L1, start);
+        const std::size_t e = lowerBound(K, L1, start + viewRange(lvl, bits));
+        const std::uint32_t b = leafBegin[f];
+        r.begin[node] = b;
+        r.counts[node] = leafBegin[e] - b;
+        if (lvl == 0) r.counts[st->numNodes] = leafBegin[e] - b;
     }
 }
 
@@ -239,7 +261,7 @@ __global__ __launch_bounds__(kT) void leafBeginKernel(const std::uint32_t *N0, c
     __syncthreads();
     const int L = activeLeaves(cap, st);
     const long base = long(tileShared) * kT * kItems;
-    if (base >= L) return;
+    if (base > L || L == 0) return;
     const std::uint32_t *N = st->activeBuffer ? N1 : N0;
     const long first = base + long(threadIdx.x) * kItems;
     std::uint32_t v[kItems];
@@ -300,7 +322,7 @@ __global__ __launch_bounds__(kT) void leafBeginKernel(const std::uint32_t *N0, c
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
     {
-        if (first + k < L) out[first + k] = std::uint32_t(run);
+        if (first + k <= L) out[first + k] = std::uint32_t(run); // [L]: the total
         run += v[k];
     }
 }
@@ -340,23 +362,56 @@ __global__ __launch_bounds__(kT) void leafRowsKernel(const Key *K0, const Key *K
     }
 }
 
-// fillSfcParticleIndexKernel (tree_view.cu:542-562)
-__global__ __launch_bounds__(kT) void mappingKernel(const Key *K0, const Key *K1, const Key *P, std::size_t n, int cap,
-                                                    const DevState *st, const int *leafToNode, Key *out)
+// fillSfcParticleIndexKernel (tree_view.cu:542-562): particle p (sorted)
+// maps to the view node of the leaf holding P[p], or to 2^64-1 when P[p] is
+// escaped. The leaves partition the active particles [0, A) in order: leaf
+// t holds [leafBegin[t], leafBegin[t+1]), so p's leaf is the last t with
+// leafBegin[t] <= p. Each block takes kMapChunk consecutive particles,
+// finds the leaves [t0, t1] they span, stages those begins in LDS when they
+// fit (else searches them in place), and writes its particles in order;
+// block work is fixed whatever the leaf sizes.
+constexpr int kMapChunk = kT * 8;
+constexpr int kMapStage = 2048;
+__global__ __launch_bounds__(kT) void mappingKernel(std::size_t n, int cap, const DevState *st,
+                                                    const std::uint32_t *leafBegin, const int *leafToNode, Key *out)
 {
+    __shared__ std::uint32_t begins[kMapStage];
+    __shared__ std::uint32_t span[2];
     const int L = activeLeaves(cap, st);
     if (L == 0) return;
-    const std::size_t p = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const Key *K = st->activeBuffer ? K1 : K0;
-    const Key key = P[p];
-    if (key >= kMaxKey)
+    const std::size_t c0 = std::size_t(blockIdx.x) * kMapChunk;
+    if (c0 >= n) return;
+    const std::size_t cEnd = c0 + kMapChunk < n ? c0 + kMapChunk : n;
+    const std::size_t active = leafBegin[L];
+    const std::size_t c1 = cEnd < active ? cEnd : active;
+    if (c0 < c1)
     {
-        out[p] = kInvalidKey;
-        return;
+        if (threadIdx.x == 0)
+        {
+            span[0] = std::uint32_t(upperBound(leafBegin, std::size_t(L), std::uint32_t(c0)) - 1);
+            span[1] = std::uint32_t(upperBound(leafBegin, std::size_t(L), std::uint32_t(c1 - 1)) - 1);
+        }
+        __syncthreads();
+        const std::uint32_t t0 = span[0], m = span[1] - t0 + 1;
+        const bool staged = m <= std::uint32_t(kMapStage);
+        if (staged)
+            for (std::uint32_t j = threadIdx.x; j < m; j += kT) begins[j] = leafBegin[t0 + j];
+        __syncthreads();
+        const std::uint32_t *B = staged ? begins : leafBegin + t0;
+        for (std::size_t p = c0 + threadIdx.x; p < c1; p += kT)
+        {
+            // last t in [0, m) with B[t] <= p (B[0] <= c0 <= p)
+            std::uint32_t lo = 0, len = m;
+            while (len > 1)
+            {
+                const std::uint32_t half = len >> 1;
+                if (B[lo + half] <= std::uint32_t(p)) lo += half;
+                len -= half;
+            }
+            out[p] = Key(std::int64_t(leafToNode[t0 + lo]));
+        }
     }
-    const std::size_t leaf = upperBound(K, std::size_t(L), key) - 1;
-    out[p] = Key(std::int64_t(leafToNode[leaf]));
+    for (std::size_t p = c1 + threadIdx.x; p < cEnd; p += kT) out[p] = kInvalidKey;
 }
 } // namespace
 
@@ -431,7 +486,7 @@ void ViewEngine::prepare(std::size_t leafCapacity, std::size_t numParticles, boo
     m.internalPer.ensure(64, 0, s);
     m.offInternal.ensure(66, 0, s);
     m.leafLevel.ensure(leafCap_, 0, s);
-    m.leafBegin.ensure(leafCap_, 0, s);
+    m.leafBegin.ensure(leafCap_ + 1, 0, s);
     const std::size_t ic = internals ? internals : 1;
     m.cand.ensure(ic, 0, s);
     m.layoutKey.ensure(ic, 0, s);
@@ -481,20 +536,7 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
                        m.leavesPerLevel.p, m.cand.p, m.candCount.p);
     hipLaunchKernelGGL(metadataKernel, dim3(1), dim3(1), 0, s, m.leavesPerLevel.p, m.internalPer.p, m.levelOffset.p,
                        m.offInternal.p, depth_, cap, st, bits_);
-    if (icap > 0)
-    {
-        if (icap > 1) sortKeys(s, m.cand.p, icap, m.sort);
-        const unsigned ib = ceilDiv(icap, kT);
-        hipLaunchKernelGGL(layoutKeysKernel, dim3(ib), dim3(kT), 0, s, m.cand.p, icap, st, bits_, m.layoutKey.p,
-                           m.layoutVal.p);
-        if (icap > 1) sortPairs(s, m.layoutKey.p, m.layoutVal.p, icap, m.sort, false);
-        hipLaunchKernelGGL(materializeKernel, dim3(ib), dim3(kT), 0, s, m.cand.p, m.layoutVal.p, icap, st, bits_,
-                           m.offInternal.p, m.mBySfc.p, m.layoutPh.p);
-        hipLaunchKernelGGL(internalRowsKernel, dim3(ib), dim3(kT), 0, s, m.layoutPh.p, icap, st, bits_,
-                           m.levelOffset.p, m.offInternal.p, m.internalPer.p, m.cand.p, m.mBySfc.p, P, n,
-                           counts ? 1 : 0, rows);
-    }
-    if (counts)
+    if (counts || mapping)
     {
         const std::size_t words = leafCap_ / (kT * 8) + 2;
         hipLaunchKernelGGL(clearScanKernel, dim3(ceilDiv(words, kT)), dim3(kT), 0, s, m.scanStatus.p, words,
@@ -502,11 +544,24 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
         hipLaunchKernelGGL(leafBeginKernel, dim3(unsigned(leafCap_ / (kT * 8) + 1)), dim3(kT), 0, s, N[0], N[1], cap,
                            st, m.leafBegin.p, m.scanStatus.p, m.scanTicket.p);
     }
+    if (icap > 0)
+    {
+        if (icap > 1) sortKeys(s, m.cand.p, icap, m.sort);
+        const unsigned ib = ceilDiv(icap, kT);
+        hipLaunchKernelGGL(layoutKeysKernel, dim3(ib), dim3(kT), 0, s, m.cand.p, icap, st, bits_, m.layoutKey.p,
+                           m.layoutVal.p);
+        if (icap > 1) sortPairs(s, m.layoutKey.p, m.layoutVal.p, icap, m.sort, false, false);
+        hipLaunchKernelGGL(materializeKernel, dim3(ib), dim3(kT), 0, s, m.cand.p, m.layoutVal.p, icap, st, bits_,
+                           m.offInternal.p, m.mBySfc.p, m.layoutPh.p);
+        hipLaunchKernelGGL(internalRowsKernel, dim3(ib), dim3(kT), 0, s, m.layoutPh.p, icap, st, bits_,
+                           m.levelOffset.p, m.offInternal.p, m.internalPer.p, m.cand.p, m.mBySfc.p, K[0], K[1],
+                           m.leafBegin.p, counts ? 1 : 0, rows);
+    }
     hipLaunchKernelGGL(leafRowsKernel, dim3(leafBlocks), dim3(kT), 0, s, K[0], K[1], N[0], N[1], cap, st, bits_,
                        m.leafLevel.p, m.levelOffset.p, m.offInternal.p, m.internalPer.p, m.cand.p, m.mBySfc.p,
                        m.leafBegin.p, counts ? 1 : 0, rows);
     if (mapping && n)
-        hipLaunchKernelGGL(mappingKernel, dim3(ceilDiv(n, kT)), dim3(kT), 0, s, K[0], K[1], P, n, cap, st,
+        hipLaunchKernelGGL(mappingKernel, dim3(ceilDiv(n, kMapChunk)), dim3(kT), 0, s, n, cap, st, m.leafBegin.p,
                            m.leafToNode.p, m.mapping.p);
     AT_HIP(hipGetLastError());
 }

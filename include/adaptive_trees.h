@@ -246,7 +246,10 @@ typedef struct at_server_stats {
     uint64_t batched_updates, update_batches;
 } at_server_stats;
 int at_server_create(const at_server_config *config, at_server **out);
-int at_server_destroy(at_server *server); /* finishes queued requests first */
+/* Finishes the requests already given to a worker first; builds still waiting
+ * for budget, and requests held behind them, complete with AT_ELOGIC (threads
+ * blocked in at_server_wait on them return). */
+int at_server_destroy(at_server *server);
 int at_server_submit_build(at_server *server, int real_bytes, int tree_type, const void *x,
                            const void *y, const void *z, size_t count, const at_config *config,
                            uint64_t *tree_id, uint64_t *request_id);

@@ -115,7 +115,50 @@ def test_request_errors():
     x = DeviceArray(np.zeros(10))
     tid, rid = server.submit_build(x, x, x, 10, make_config(max_particles_per_leaf=0))
     assert server.wait(rid) == 1  # AT_EINVAL from the tree constructor
+    # a failed build keeps no tree and gives its bytes back
+    assert server.stats()["bytes_in_use"] == 0
+    assert server.wait(server.submit_update(tid)) == 2  # AT_ELOGIC: tree failed to build
+    with pytest.raises(TreeError):
+        server.tree(tid).scalars()
+    assert server.wait(server.submit_release(tid)) == 0
     with pytest.raises(TreeError):
         big = DeviceArray(8)
         server.submit_build(big, big, big, 10 ** 9, make_config())  # exceeds the budget
+    # once a release is queued the tree takes no further requests
+    coords = [DeviceArray(np.ascontiguousarray(a, dtype=np.float32)) for a in uniform(1000, seed=7)]
+    tid, rid = server.submit_build(*coords, 1000, make_config(max_particles_per_leaf=32), real=np.float32)
+    release = server.submit_release(tid)
+    with pytest.raises(TreeError):
+        server.submit_update(tid)
+    assert server.wait(rid) == 0 and server.wait(release) == 0
+    assert server.stats()["resident_trees"] == 0 and server.stats()["bytes_in_use"] == 0
     server.close()
+
+
+def test_destroy_completes_builds_waiting_for_budget():
+    import threading
+    import time
+    n = 20_000
+    est = n * 25 + (n // 8 + 64) * 48 + (1 << 20)
+    server = Server(hbm_budget_bytes=est + 1000, workers=1)
+    coords = [[DeviceArray(np.ascontiguousarray(a, dtype=np.float32)) for a in uniform(n, seed=600 + k)]
+              for k in range(2)]
+    cfg = make_config(max_particles_per_leaf=32)
+    first, rid0 = server.submit_build(*coords[0], n, cfg, real=np.float32)
+    second, rid1 = server.submit_build(*coords[1], n, cfg, real=np.float32)
+    held = server.submit_update(second)  # held behind the build that cannot be admitted
+    assert server.wait(rid0) == 0
+    assert server.stats()["waiting_builds"] == 1
+    seen = {}
+
+    def waiter():
+        seen["status"] = server.wait(rid1)
+
+    t = threading.Thread(target=waiter)
+    t.start()
+    time.sleep(0.5)  # the waiter is blocked inside at_server_wait
+    assert not server.poll(rid1) and not server.poll(held)
+    server.close()
+    t.join(timeout=30)
+    assert not t.is_alive()
+    assert seen["status"] == 2  # AT_ELOGIC: destroyed before admission
