@@ -1783,7 +1783,12 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
         const unsigned planCap = level <= 3 ? 2048u : 128u; // one parent per block (8 blocks per CU fit)
         const unsigned segGrid = unsigned(maxSegs < planCap ? maxSegs : planCap);
         const unsigned upGrid = unsigned((maxRanges / kUpRows) < cap ? maxRanges / kUpRows + 1 : cap);
-        const unsigned scatterGrid = level <= 3 ? unsigned(kMsdGrid) : 256u;
+        // levels 2 and 3 (the ones with work on spread keys): about one
+        // block per range (at most n / tile + 256^(level - 1) ranges; the
+        // loop covers any excess), later levels a small looping grid
+        const std::size_t tiles = n / kMsdTile + 1;
+        const unsigned scatterGrid =
+            level == 2 ? unsigned(tiles + 256) : level == 3 ? unsigned(tiles + 4096) : 256u;
         const int set = level & 1;
         const bool srcY = (level - 1) & 1, dstY = level & 1;
         const Key *src = srcY ? Y : X;
@@ -1814,7 +1819,11 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
     }
     {
         prof::Scope scope("msd_local", s);
-        hipLaunchKernelGGL(msdLocalSortKernel, dim3(kMsdGrid), dim3(kLocThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
+        // about one block per chunk (spread keys give fewer than 2 per
+        // kMsdChunk keys; the loop covers any excess): blocks retire and
+        // start independently, so their load, sort and store phases overlap
+        const unsigned localGrid = unsigned(2 * (n / kMsdChunk) + 256);
+        hipLaunchKernelGGL(msdLocalSortKernel, dim3(localGrid), dim3(kLocThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
                            vX, vY, int(finalIsY), maxChunks);
     }
     AT_HIP(hipGetLastError());

@@ -712,12 +712,15 @@ private:
         d.tileSum = direct_.tileSum.p;
         d.total = direct_.total.p;
         d.ticket = direct_.ticket.p;
-        const TreeDev t0 = dev();
-        binDirectCount(stream_, t0, p, d);
-        unsigned long long internals = 0;
-        AT_HIP(hipMemcpyAsync(&internals, direct_.total.p, sizeof(internals), hipMemcpyDeviceToHost, stream_));
-        AT_HIP(hipStreamSynchronize(stream_));
-        const std::size_t L = std::size_t(internals) + 1;
+        binDirectCount(stream_, dev(), p, d);
+        // emitted at the current capacity; when the leaves do not fit, the
+        // buffers grow and the emission runs again (the counts stay valid)
+        direct_.bound.ensure(leafCapacity_ + 1, 0, stream_);
+        d.bound = direct_.bound.p;
+        binDirectWrite(stream_, dev(), p, d, long(leafCapacity_));
+        syncState();
+        const std::size_t L = std::size_t(sth_->numLeaves);
+        if (L <= leafCapacity_) return;
         std::size_t cap = leafCapacity_;
         while (cap < L)
         {
@@ -725,14 +728,13 @@ private:
             constexpr std::size_t maxLeaves = std::size_t(std::numeric_limits<int>::max()) / 2 + 1;
             if (cap > maxLeaves) throw Error(AT_ELENGTH, "adaptive_octree: buffer capacity exceeds node index range");
         }
-        if (cap != leafCapacity_) prepareGraphBuffers(cap);
+        hostLeaves_ = 0; // nothing was emitted: no live entries to keep
+        prepareGraphBuffers(cap);
         direct_.bound.ensure(L + 1, 0, stream_);
         d.bound = direct_.bound.p;
-        const TreeDev t = dev();
-        binDirectWrite(stream_, t, p, d, long(L));
-        binDirectFinish(stream_, t, long(L));
+        binDirectWrite(stream_, dev(), p, d, long(leafCapacity_));
         syncState();
-        if (std::size_t(sth_->numLeaves) != L)
+        if (std::size_t(sth_->numLeaves) != L || L > leafCapacity_)
             throw Error(AT_EHIP, "binary construction produced an inconsistent leaf count");
     }
 

@@ -1203,13 +1203,18 @@ __global__ __launch_bounds__(1024) void binScanTilesKernel(const DevState *st, s
 // Few intervals hold boundaries, so each wave first lists its non-empty
 // intervals (with their output positions) in LDS and then emits them one
 // per lane: the loads of P and delta for a wave's intervals go out together.
+// The leaf count is total + 1; nothing is written when it exceeds capacity
+// (the host grows the buffers and runs the emission again).
 __global__ __launch_bounds__(kThreads) void binEmitKernel(const Key *P, const DevState *st, const std::int8_t *delta,
                                                           const std::uint8_t *cnt, const std::uint32_t *tileOff,
-                                                          Key *K, std::uint32_t *bound, long leaves)
+                                                          Key *K, std::uint32_t *bound,
+                                                          const unsigned long long *total, long capacity)
 {
     constexpr int kW = kThreads / 64;
     __shared__ std::uint32_t wsum[kW];
     __shared__ uint2 list[kW][64 * kItems]; // (interval, first output position)
+    const long leaves = long(*total) + 1;
+    if (leaves > capacity) return;
     const int tid = threadIdx.x;
     const long nact = long(st->numActiveParticles);
     const long q0 = long(blockIdx.x) * kQ;
@@ -1303,14 +1308,18 @@ __global__ __launch_bounds__(kThreads) void binEmitKernel(const Key *P, const De
 }
 
 __global__ __launch_bounds__(kThreads) void binBoundsToCountsKernel(const std::uint32_t *bound, std::uint32_t *N,
-                                                                    long leaves)
+                                                                    const unsigned long long *total, long capacity)
 {
+    const long leaves = long(*total) + 1;
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < leaves) N[i] = bound[i + 1] - bound[i];
+    if (leaves <= capacity && i < leaves) N[i] = bound[i + 1] - bound[i];
 }
 
-__global__ void binDirectStateKernel(DevState *st, int leaves)
+// The state of the constructed tree (its leaf count also when it did not
+// fit the capacity: the host then grows and emits again).
+__global__ void binDirectStateKernel(DevState *st, const unsigned long long *total)
 {
+    const int leaves = int(*total + 1);
     st->changed = 0;
     st->converged = 1;
     st->numLeaves = leaves;
@@ -1688,16 +1697,13 @@ void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDire
                        d.cnt, d.tileSum);
     hipLaunchKernelGGL(binScanTilesKernel, dim3(1), dim3(1024), 0, s, t.st, d.tileSum, d.total);
 }
-void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves)
+void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long capacity)
 {
     prof::Scope scope("bin_direct", s);
     hipLaunchKernelGGL(binEmitKernel, dim3(unsigned(t.n / kQ + 1)), dim3(kThreads), 0, s, t.P, t.st, d.delta, d.cnt,
-                       d.tileSum, t.K[0], d.bound, leaves);
-    hipLaunchKernelGGL(binBoundsToCountsKernel, dim3(ceilDiv(std::size_t(leaves), kThreads)), dim3(kThreads), 0, s,
-                       d.bound, t.N[0], leaves);
-}
-void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves)
-{
-    hipLaunchKernelGGL(binDirectStateKernel, dim3(1), dim3(1), 0, s, t.st, int(leaves));
+                       d.tileSum, t.K[0], d.bound, d.total, capacity);
+    hipLaunchKernelGGL(binBoundsToCountsKernel, dim3(ceilDiv(std::size_t(capacity), kThreads)), dim3(kThreads), 0, s,
+                       d.bound, t.N[0], d.total, capacity);
+    hipLaunchKernelGGL(binDirectStateKernel, dim3(1), dim3(1), 0, s, t.st, d.total);
 }
 } // namespace at

@@ -104,8 +104,9 @@ struct BinDirectScratch
 };
 constexpr unsigned kDirectMaxBucket = 1024;
 void binDirectCount(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d);
-void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long leaves);
-void binDirectFinish(hipStream_t s, const TreeDev &t, long leaves);
+// Emits K and N when the leaf count (total + 1, on the device) fits
+// `capacity`, and writes the tree's state (its leaf count in any case).
+void binDirectWrite(hipStream_t s, const TreeDev &t, const BinParams &p, BinDirectScratch &d, long capacity);
 
 // Leaf-count convergence check (benchmarks/drivers/cstone_check.cu:17-43)
 // over leaf boundaries K[0..leaves] and sorted keys P[0..n), populations
