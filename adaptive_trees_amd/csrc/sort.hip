@@ -25,6 +25,7 @@
 // on the 256 CUs the prefix chain made the look-back cost ~45% of a pass.
 #include "sort.hpp"
 
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -1314,7 +1315,8 @@ constexpr std::uint32_t kRunCap = kMsdChunk / 3 + 1; // runs of 3 or more keys i
 __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCounters *cnt,
                                                                      const MsdChunk *__restrict__ chunks, Key *X,
                                                                      Key *Y, std::uint32_t *vX, std::uint32_t *vY,
-                                                                     int finalIsY, std::uint32_t maxChunks)
+                                                                     int finalIsY, std::uint32_t maxChunks,
+                                                                     const std::uint32_t *fromChunk)
 {
     constexpr int kW = kLocThreads / 64;
     __shared__ Key stageKeys[kMsdChunk];
@@ -1334,6 +1336,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
     Key *kd = finalIsY ? Y : X;
     std::uint32_t *vd = finalIsY ? vY : vX;
     const std::uint32_t count = cnt->chunks < maxChunks ? cnt->chunks : maxChunks;
+    const std::uint32_t from = fromChunk ? *fromChunk : 0u; // chunks before it were sorted by an earlier launch
     const std::uint32_t first = std::uint32_t(wave) * (kLocItems * 64) + lane;
     Key key[kLocItems];
     std::uint32_t val[kLocItems];
@@ -1411,7 +1414,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         __syncthreads();
     };
 
-    for (std::uint32_t c = blockIdx.x; c < count; c += gridDim.x)
+    for (std::uint32_t c = from + blockIdx.x; c < count; c += gridDim.x)
     {
         const MsdChunk ch = chunks[c];
         const Key *ks = ch.parity ? Y : X;
@@ -1752,6 +1755,16 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
     AT_HIP(hipGetLastError());
 }
 
+// Copies a device count into (pinned, coherent) host memory for a polling host.
+constexpr std::uint32_t kCountPending = 0xffffffffu;
+__global__ void publishCountKernel(const std::uint32_t *count, std::uint32_t *host)
+{
+    __atomic_store_n(host, *count, __ATOMIC_RELEASE);
+    __threadfence_system();
+}
+// Records the chunk count (the chunks an early chunk-sort launch takes).
+__global__ void markChunksKernel(MsdCounters *c) { c->pad[0] = c->chunks; }
+
 // Levels 2..8 and the LDS chunk sort, after level 1 has partitioned the
 // keys (X -> Y, next-digit bytes in `digits`) and planned level 2's lists
 // (set 0) and level 1's chunks. Odd levels write Y, even levels X; the result
@@ -1766,14 +1779,44 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
     // are not launched. A captured sort keeps the fixed launch sequence.
     hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
     AT_HIP(hipStreamIsCapturing(s, &capture));
+    // about one block per chunk (spread keys give fewer than 2 per
+    // kMsdChunk keys; the loop covers any excess): blocks retire and start
+    // independently, so their load, sort and store phases overlap
+    const unsigned localGrid = unsigned(2 * (n / kMsdChunk) + 256);
+    auto localSort = [&](const std::uint32_t *fromChunk) {
+        prof::Scope scope("msd_local", s);
+        hipLaunchKernelGGL(msdLocalSortKernel, dim3(localGrid), dim3(kLocThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
+                           vX, vY, int(finalIsY), maxChunks, fromChunk);
+    };
+    const std::uint32_t *from = nullptr;
     for (int level = 2; level <= kSortPasses; ++level)
     {
         if (level == 4 && capture == hipStreamCaptureStatusNone)
         {
-            AT_HIP(hipMemcpyAsync(ws.msdHostCount, &ws.msdCnt->segs[level & 1], sizeof(std::uint32_t),
-                                  hipMemcpyDeviceToHost, s));
-            AT_HIP(hipStreamSynchronize(s));
-            if (*ws.msdHostCount == 0) break;
+            // Levels 4..8 have work only for keys that share 24 high bits in
+            // buckets of more than kMsdChunk. Outside graph capture the
+            // chunks planned so far are sorted at once, and the host reads
+            // level 4's segment count meanwhile (a one-thread kernel stores
+            // it into pinned host memory, polled; a stream synchronisation
+            // after a bound): when it is zero the sort is complete, otherwise
+            // the remaining levels and a chunk-sort launch for their chunks
+            // follow. A captured sort keeps the fixed launch sequence.
+            volatile std::uint32_t *flag = ws.msdHostCount;
+            *flag = kCountPending;
+            hipLaunchKernelGGL(publishCountKernel, dim3(1), dim3(1), 0, s, &ws.msdCnt->segs[level & 1],
+                               ws.msdHostCount);
+            hipLaunchKernelGGL(markChunksKernel, dim3(1), dim3(1), 0, s, ws.msdCnt);
+            localSort(nullptr);
+            AT_HIP(hipGetLastError());
+            const auto t0 = std::chrono::steady_clock::now();
+            while (*flag == kCountPending &&
+                   std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(20))
+            {
+            }
+            if (*flag == kCountPending) AT_HIP(hipStreamSynchronize(s));
+            if (*flag == kCountPending) throw Error(4, "sort: a level count never arrived");
+            if (*flag == 0) return;
+            from = &ws.msdCnt->pad[0];
         }
         prof::Scope scope("msd_level", s);
         // levels past the third see work only for keys that share 24 high
@@ -1817,15 +1860,7 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
                            ws.msdRanges[set ^ 1], ws.msdFirst[set ^ 1], maxSegs, maxRanges, ws.msdChunks,
                            maxChunks, maxSegs);
     }
-    {
-        prof::Scope scope("msd_local", s);
-        // about one block per chunk (spread keys give fewer than 2 per
-        // kMsdChunk keys; the loop covers any excess): blocks retire and
-        // start independently, so their load, sort and store phases overlap
-        const unsigned localGrid = unsigned(2 * (n / kMsdChunk) + 256);
-        hipLaunchKernelGGL(msdLocalSortKernel, dim3(localGrid), dim3(kLocThreads), 0, s, ws.msdCnt, ws.msdChunks, X, Y,
-                           vX, vY, int(finalIsY), maxChunks);
-    }
+    localSort(from);
     AT_HIP(hipGetLastError());
 }
 
@@ -1953,7 +1988,8 @@ void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_
     msdMaxChunks = 16 * (cap / kMsdChunk) + 64 + parents;
     auto alloc = [&](std::size_t bytes) { return poolAlloc(bytes, s); };
     msdCnt = static_cast<MsdCounters *>(alloc(sizeof(MsdCounters)));
-    if (!msdHostCount) AT_HIP(hipHostMalloc(reinterpret_cast<void **>(&msdHostCount), sizeof(std::uint32_t)));
+    if (!msdHostCount)
+        AT_HIP(hipHostMalloc(reinterpret_cast<void **>(&msdHostCount), sizeof(std::uint32_t), hipHostMallocCoherent));
     msdRoot = static_cast<MsdSeg *>(alloc(sizeof(MsdSeg)));
     for (int k = 0; k < 2; ++k)
     {

@@ -97,7 +97,7 @@ struct MsdCounters
     std::uint32_t ranges[2];
     std::uint32_t chunks;
     std::uint32_t overflow;  // list capacity exceeded (never, by the bounds used)
-    std::uint32_t pad[2];
+    std::uint32_t pad[2];    // pad[0]: chunks taken by an early chunk-sort launch
 };
 
 // Device scratch of one sort instance, sized for `capacity` keys.
