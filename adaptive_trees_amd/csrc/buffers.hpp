@@ -37,13 +37,19 @@ struct VecShape
 };
 
 // Device memory of the engine comes from the device's default stream-ordered
-// pool; its release threshold is raised once per process so that freed
-// blocks stay cached for reuse. Allocating and freeing then never
-// synchronises the device (hipFree does), which matters when trees are
-// created and released while others compute. Frees on the null stream are
-// for owners that have synchronised their own stream first.
+// pool (its release threshold raised once per process), in size classes.
+// Allocating and freeing never synchronises the device (hipFree does),
+// which matters when trees are created and released while others compute.
+// A free on the null stream is for owners that have synchronised their own
+// stream first: the block is idle and is kept for the next allocation of
+// its size class without a runtime call. Other frees are ordered on their
+// stream.
 void *poolAlloc(std::size_t bytes, hipStream_t s);
 void poolFree(void *p, hipStream_t s);
+// A word of pinned, coherent host memory (device stores are visible to a
+// polling host), from slabs kept until the process ends.
+std::uint32_t *takePinnedWord();
+void givePinnedWord(std::uint32_t *w);
 
 // A device allocation that grows on request.
 template <class T>

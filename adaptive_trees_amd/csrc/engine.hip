@@ -2,7 +2,9 @@
 #include "engine.hpp"
 #include "tree_impl.hpp"
 
+#include <map>
 #include <mutex>
+#include <vector>
 
 #include <algorithm>
 #include <climits>
@@ -23,7 +25,70 @@ void hipCheck(hipError_t e, const char *what, const char *file, int line)
 namespace
 {
 std::once_flag g_poolOnce[64];
+
+// Idle device blocks kept for reuse. Blocks come from the device's
+// stream-ordered pool in size classes (8 per power of two, at most 12.5%
+// over the request); a block released by an owner that has synchronised
+// its stream (poolFree with a null stream) is idle and goes to this cache,
+// up to kIdleCap bytes per device, and the next request of its class on
+// that device takes it without a runtime call. Trees are created and
+// destroyed by the thousand in a serving loop: hipMallocAsync costs a few
+// microseconds and hipFreeAsync tens (it may trim the pool), per buffer.
+class BlockCache
+{
+public:
+    static constexpr std::size_t kIdleCap = std::size_t(16) << 30;
+    static std::size_t classOf(std::size_t bytes)
+    {
+        if (bytes <= 256) return 256;
+        std::size_t p = std::size_t(1) << (63 - __builtin_clzll(bytes)); // highest power of two <= bytes
+        const std::size_t step = p / 8;
+        return (bytes + step - 1) / step * step;
+    }
+    void *take(int dev, std::size_t cls)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        auto it = idle_.find({dev, cls});
+        if (it == idle_.end() || it->second.empty()) return nullptr;
+        void *p = it->second.back();
+        it->second.pop_back();
+        idleBytes_[dev & 63] -= cls;
+        return p;
+    }
+    void own(void *p, int dev, std::size_t cls)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        owned_[p] = {dev, cls};
+    }
+    // true: kept idle; false: the caller frees it (forgotten here)
+    bool give(void *p, bool idle)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        auto it = owned_.find(p);
+        if (it == owned_.end()) return false;
+        const auto [dev, cls] = it->second;
+        if (idle && idleBytes_[dev & 63] + cls <= kIdleCap)
+        {
+            idle_[{dev, cls}].push_back(p);
+            idleBytes_[dev & 63] += cls;
+            return true;
+        }
+        owned_.erase(it);
+        return false;
+    }
+
+private:
+    std::mutex m_;
+    std::map<void *, std::pair<int, std::size_t>> owned_;
+    std::map<std::pair<int, std::size_t>, std::vector<void *>> idle_;
+    std::size_t idleBytes_[64] = {};
+};
+BlockCache &blockCache()
+{
+    static BlockCache *c = new BlockCache; // never destroyed: blocks stay valid at exit
+    return *c;
 }
+} // namespace
 
 void *poolAlloc(std::size_t bytes, hipStream_t s)
 {
@@ -37,14 +102,20 @@ void *poolAlloc(std::size_t bytes, hipStream_t s)
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
         }
     });
+    const std::size_t cls = BlockCache::classOf(bytes ? bytes : 1);
+    if (void *p = blockCache().take(dev, cls)) return p;
     void *p = nullptr;
-    AT_HIP(hipMallocAsync(&p, bytes ? bytes : 1, s));
+    AT_HIP(hipMallocAsync(&p, cls, s));
+    blockCache().own(p, dev, cls);
     return p;
 }
 
 void poolFree(void *p, hipStream_t s)
 {
-    if (p) (void)hipFreeAsync(p, s);
+    if (!p) return;
+    // a null stream: the owner has synchronised, the block is idle
+    if (blockCache().give(p, s == nullptr)) return;
+    (void)hipFreeAsync(p, s);
 }
 
 namespace
@@ -86,7 +157,47 @@ PinnedStates &pinnedStates()
     static PinnedStates *p = new PinnedStates; // never destroyed: slabs stay valid at exit
     return *p;
 }
+
+class PinnedWords
+{
+public:
+    std::uint32_t *take()
+    {
+        std::lock_guard<std::mutex> g(m_);
+        if (free_.empty())
+        {
+            constexpr std::size_t kSlab = 4096;
+            std::uint32_t *slab = nullptr;
+            AT_HIP(hipHostMalloc(reinterpret_cast<void **>(&slab), kSlab * sizeof(std::uint32_t),
+                                 hipHostMallocCoherent));
+            for (std::size_t i = 0; i < kSlab; ++i) free_.push_back(slab + i);
+        }
+        std::uint32_t *w = free_.back();
+        free_.pop_back();
+        return w;
+    }
+    void give(std::uint32_t *w)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        free_.push_back(w);
+    }
+
+private:
+    std::mutex m_;
+    std::vector<std::uint32_t *> free_;
+};
+PinnedWords &pinnedWords()
+{
+    static PinnedWords *p = new PinnedWords;
+    return *p;
+}
 } // namespace
+
+std::uint32_t *takePinnedWord() { return pinnedWords().take(); }
+void givePinnedWord(std::uint32_t *w)
+{
+    if (w) pinnedWords().give(w);
+}
 
 TreeBase::TreeBase(hipStream_t s, DevState *st, DevState *sth) : stream_(s)
 {

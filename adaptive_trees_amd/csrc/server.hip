@@ -5,9 +5,9 @@
 //              device bytes fit the HBM budget; a release returns its bytes
 //              and admits the waiting builds that now fit.
 //   placement  each admitted tree is pinned to one worker (a host thread
-//              owning one HIP stream), the worker with the least queued
-//              particle work; all requests of a tree run in submission
-//              order on that worker.
+//              owning one HIP stream), the worker holding the fewest
+//              resident particles (then the fewest trees); all requests of
+//              a tree run in submission order on that worker.
 //   batching   a worker takes the next request and, if it is an update, the
 //              updates of other trees queued behind it (up to max_batch):
 //              their new positions are copied in and their update() work is
@@ -16,9 +16,10 @@
 //              (doBuffersNeedResize / resizeBuffers / update) completes it.
 //              Workers run concurrently, so the work of requests admitted
 //              while others are on the GPU overlaps on the device.
-//   graphs     with update_graphs set, each tree's update() is captured
-//              once (thread-local capture on its worker's stream) after its
-//              build and after every resize, and replayed as one launch.
+//   graphs     with update_graphs set, a tree's update() is captured
+//              (thread-local capture on its worker's stream) at its first
+//              update and again at the first update after a resize, and
+//              replayed as one launch.
 // Every request runs the same tree operations it would run alone, on a tree
 // that only its own requests touch, so results do not depend on the mix.
 #include "engine.hpp"
@@ -276,7 +277,9 @@ private:
     {
         hipStream_t stream = nullptr;
         std::deque<Request *> queue;
-        std::size_t work = 0; // queued particles
+        std::size_t work = 0;     // queued particles
+        std::size_t resident = 0; // particles of the trees pinned here
+        std::size_t trees = 0;
     };
 
     Request *newRequest(Kind k)
@@ -333,8 +336,13 @@ private:
             peak_ = std::max(peak_, used_);
             int best = 0;
             for (int w = 1; w < int(workers_.size()); ++w)
-                if (workers_[std::size_t(w)].work < workers_[std::size_t(best)].work) best = w;
+            {
+                const Worker &a = workers_[std::size_t(w)], &b = workers_[std::size_t(best)];
+                if (a.resident < b.resident || (a.resident == b.resident && a.trees < b.trees)) best = w;
+            }
             t.worker = best;
+            workers_[std::size_t(best)].resident += t.n;
+            ++workers_[std::size_t(best)].trees;
             t.admitted = true;
             enqueue(best, r, t.n);
             while (!t.held.empty())
@@ -398,7 +406,6 @@ private:
             };
             Request *head = batch[0];
             std::unique_ptr<TreeBase> built;
-            GraphExec builtGraph;
             if (head->kind == kBuild)
             {
                 try
@@ -408,7 +415,6 @@ private:
                     built.reset(createTree(head->realBytes, head->type, head->x, head->y, head->z, head->n, c));
                     built->build();
                     (void)built->needsResize();
-                    if (graphs_) builtGraph.g = captureUpdate(*built, wk.stream);
                 }
                 catch (const Error &e) { fail(0, e.code, e.what()); }
                 catch (const std::exception &e) { fail(0, AT_ENOMEM, e.what()); }
@@ -437,6 +443,7 @@ private:
                             AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->z), r->nz, bytes,
                                                   hipMemcpyDeviceToDevice, wk.stream));
                         }
+                        if (graphs_ && !res[i]->graph.g) res[i]->graph.g = captureUpdate(*trees[i], wk.stream);
                         if (res[i]->graph.g)
                             AT_HIP(hipGraphLaunch(res[i]->graph.g, wk.stream));
                         else
@@ -468,14 +475,9 @@ private:
                         while (trees[i]->needsResize())
                         {
                             if (++attempts > 64) throw Error(AT_ELENGTH, "update did not fit after 64 resizes");
-                            res[i]->graph.reset(); // the resize moves the captured buffers
+                            res[i]->graph.reset(); // the resize moves the captured buffers: captured anew next time
                             trees[i]->resizeBuffers();
                             trees[i]->update();
-                        }
-                        if (attempts && graphs_)
-                        {
-                            (void)trees[i]->needsResize(); // refresh host sizes before capturing
-                            res[i]->graph.g = captureUpdate(*trees[i], wk.stream);
                         }
                     }
                     catch (const Error &e) { fail(i, e.code, e.what()); }
@@ -486,22 +488,25 @@ private:
             {
                 std::lock_guard<std::mutex> g(m_);
                 const auto now = Clock::now();
-                if (head->kind == kBuild && status[0] == AT_OK)
-                {
-                    res[0]->tree = std::move(built);
-                    res[0]->graph = std::move(builtGraph);
-                }
+                if (head->kind == kBuild && status[0] == AT_OK) res[0]->tree = std::move(built);
                 else if (head->kind == kBuild)
                 {
                     // no tree is kept; its bytes go back to the budget
                     res[0]->failed = true;
                     used_ -= std::min(used_, res[0]->bytes);
                     res[0]->bytes = 0;
+                    wk.resident -= std::min(wk.resident, res[0]->n);
+                    wk.trees -= std::min<std::size_t>(wk.trees, 1);
                 }
                 if (head->kind == kRelease)
                 {
                     auto it = trees_.find(head->tree);
                     used_ -= std::min(used_, it->second.bytes);
+                    if (!it->second.failed)
+                    {
+                        wk.resident -= std::min(wk.resident, it->second.n);
+                        wk.trees -= std::min<std::size_t>(wk.trees, 1);
+                    }
                     it->second.graph.reset();
                     dropped = std::move(it->second.tree);
                     trees_.erase(it);

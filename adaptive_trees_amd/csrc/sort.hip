@@ -1955,7 +1955,7 @@ SortPlan makeSortPlan(std::size_t n)
 void SortWorkspace::allocate(std::size_t cap, bool withValues, hipStream_t s)
 {
     if (cap <= capacity && (!withValues || values)) return;
-    release();
+    release(s);
     capacity = cap < 1 ? 1 : cap;
     // the smallest tile gives the largest range count for any tuning
     maxRanges = (capacity + 8 * kSortThreads - 1) / (8 * kSortThreads);
@@ -1988,8 +1988,7 @@ void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_
     msdMaxChunks = 16 * (cap / kMsdChunk) + 64 + parents;
     auto alloc = [&](std::size_t bytes) { return poolAlloc(bytes, s); };
     msdCnt = static_cast<MsdCounters *>(alloc(sizeof(MsdCounters)));
-    if (!msdHostCount)
-        AT_HIP(hipHostMalloc(reinterpret_cast<void **>(&msdHostCount), sizeof(std::uint32_t), hipHostMallocCoherent));
+    if (!msdHostCount) msdHostCount = takePinnedWord();
     msdRoot = static_cast<MsdSeg *>(alloc(sizeof(MsdSeg)));
     for (int k = 0; k < 2; ++k)
     {
@@ -2003,7 +2002,7 @@ void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_
     msdChunks = static_cast<MsdChunk *>(alloc(msdMaxChunks * sizeof(MsdChunk)));
 }
 
-void SortWorkspace::release()
+void SortWorkspace::release(hipStream_t s)
 {
     if (borrowed)
     {
@@ -2012,22 +2011,25 @@ void SortWorkspace::release()
         capacity = 0; maxRanges = 0; values = false; borrowed = false;
         return;
     }
-    // owners synchronise their stream before the workspace goes away
+    // ordered on s; s = null: the owner has synchronised (destruction)
     for (void *p : {(void *)keysAlt, (void *)valsAlt, (void *)digits, (void *)rangeHist, (void *)rangeOff,
                     (void *)digitTotal})
-        poolFree(p, nullptr);
+        poolFree(p, s);
     keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
     digitTotal = nullptr;
     capacity = 0; maxRanges = 0; values = false;
     for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
                     (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
                     (void *)msdSubTotals, (void *)msdChunks})
-        poolFree(p, nullptr);
+        poolFree(p, s);
     msdCnt = nullptr; msdRoot = nullptr; msdSegs[0] = msdSegs[1] = nullptr; msdRanges[0] = msdRanges[1] = nullptr;
     msdFirst[0] = msdFirst[1] = nullptr; msdHist = msdOff = msdSubTotals = nullptr; msdChunks = nullptr;
     msdCapacity = msdMaxSegs = msdStride = msdMaxChunks = msdParents = 0;
-    if (msdHostCount) (void)hipHostFree(msdHostCount);
-    msdHostCount = nullptr;
+    if (!s)
+    {
+        givePinnedWord(msdHostCount); // kept across regrowth: the host may poll it until the stream ends
+        msdHostCount = nullptr;
+    }
 }
 
 template <class T>

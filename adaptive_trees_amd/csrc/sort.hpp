@@ -130,7 +130,9 @@ struct SortWorkspace
     // MSD lists for sorts of up to cap keys whose level 1 has `parents`
     // parent buckets besides the whole array (segmented sorts)
     void allocateMsd(std::size_t cap, std::size_t parents, hipStream_t s);
-    void release();
+    // Frees ordered on s; s = null when the owner has synchronised its
+    // stream (the destructor's case).
+    void release(hipStream_t s = nullptr);
     ~SortWorkspace() { release(); }
 };
 

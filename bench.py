@@ -575,6 +575,21 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
     batch_s = time.perf_counter() - t0
     exact = all(np.array_equal(batch.tree(k).get(n), plain[k].get(n))
                 for k in range(trees) for n in ("particleKeys_d", "perm_d", "cornerstone_d", "leafCounts_d"))
+
+    def build_s(fn, reps=5):
+        times = []
+        for _ in range(reps):
+            load(cur_b, 0)
+            load(cur_p, 0)
+            stream.sync()
+            t0 = time.perf_counter()
+            fn()
+            stream.sync()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times))
+
+    batch_build = build_s(batch.build)
+    plain_build = build_s(lambda: [t.build() for t in plain])
     work = total * (rounds + 1)
     return {
         "workload": "%d octrees, sizes log-uniform [1e3, 2e6] (total %d), FP32 uniform/Plummer, Morton, "
@@ -584,6 +599,8 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
         "plain_s": round(plain_s, 4),
         "batched_s": round(batch_s, 4),
         "speedup": round(plain_s / batch_s, 3),
+        "build_only_ms": {"batched": round(1e3 * batch_build, 2), "plain": round(1e3 * plain_build, 2),
+                          "speedup": round(plain_build / batch_build, 2), "statistic": "median of 5"},
         "batched_Mparticles_per_s": round(work / batch_s / 1e6, 1),
         "plain_Mparticles_per_s": round(work / plain_s / 1e6, 1),
         "resizes": resizes,
