@@ -33,6 +33,7 @@ struct SetTables
     std::size_t cap = 0;
     TreeDev *dTrees = nullptr, *hTrees = nullptr;
     std::uint32_t *dMaps = nullptr, *hMaps = nullptr;
+    hipEvent_t uploaded = nullptr; // after the last copy out of the pinned tables
     ~SetTables() { release(); }
     void release()
     {
@@ -40,9 +41,18 @@ struct SetTables
         if (dMaps) (void)hipFree(dMaps);
         if (hTrees) (void)hipHostFree(hTrees);
         if (hMaps) (void)hipHostFree(hMaps);
+        if (uploaded) (void)hipEventDestroy(uploaded);
         dTrees = hTrees = nullptr;
         dMaps = hMaps = nullptr;
+        uploaded = nullptr;
         cap = 0;
+    }
+    // The pinned tables may be rewritten once their last copy has run (the
+    // device tables are rewritten in stream order, after the kernels that
+    // read them).
+    void waitWritable()
+    {
+        if (uploaded) AT_HIP(hipEventSynchronize(uploaded));
     }
     void reserve(std::size_t n)
     {
@@ -53,6 +63,7 @@ struct SetTables
         AT_HIP(hipMalloc(&dMaps, kShapes * (n + 1) * sizeof(std::uint32_t)));
         AT_HIP(hipHostMalloc(&hTrees, n * sizeof(TreeDev)));
         AT_HIP(hipHostMalloc(&hMaps, kShapes * (n + 1) * sizeof(std::uint32_t)));
+        AT_HIP(hipEventCreateWithFlags(&uploaded, hipEventDisableTiming));
     }
     // hTrees[0..count) and the per-member block counts are filled by the
     // caller through `blocks`; writes the prefixes and copies both tables.
@@ -72,6 +83,7 @@ struct SetTables
         if (!count) return;
         AT_HIP(hipMemcpyAsync(dTrees, hTrees, count * sizeof(TreeDev), hipMemcpyHostToDevice, s));
         AT_HIP(hipMemcpyAsync(dMaps, hMaps, kShapes * (cap + 1) * sizeof(std::uint32_t), hipMemcpyHostToDevice, s));
+        AT_HIP(hipEventRecord(uploaded, s));
     }
 };
 
@@ -250,7 +262,7 @@ private:
     template <class F>
     void stage(SetTables &tab, const std::vector<std::size_t> &members, bool building, F blocks, TreeSet out[kShapes])
     {
-        AT_HIP(hipStreamSynchronize(stream_)); // the staging buffers are reused
+        tab.waitWritable();
         for (std::size_t i = 0; i < members.size(); ++i)
         {
             TreeDev t = trees_[members[i]]->device();
