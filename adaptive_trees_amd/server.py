@@ -6,12 +6,29 @@ under an HBM budget, each tree is pinned to one worker stream, and updates
 of different trees queued on a worker are launched together. Requests return
 ids at once; ``wait`` blocks until one is done and returns its status.
 """
+import atexit
 import ctypes
+import weakref
 
 import numpy as np
 
 from . import _ffi
 from .tree import OCTREE, _ptr
+
+# Servers still open when the interpreter exits are closed first: their worker
+# threads keep launching kernels on borrowed device arrays, which interpreter
+# teardown may free (and the HIP runtime's own exit handlers must not run
+# under them). Closing drains every queued request and joins the workers.
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_servers():
+    for s in list(_LIVE):
+        try:
+            s.close()
+        except Exception:
+            pass
 
 
 class ServerTree:
@@ -43,11 +60,21 @@ class Server:
         self.lib.call("server_create", ctypes.byref(cfg), ctypes.byref(h))
         self.handle = h.value
         self._keep = []
+        _LIVE.add(self)
 
     def close(self):
+        """Finishes the requests given to workers, then destroys the server;
+        the device arrays it borrowed may be freed afterwards."""
         if getattr(self, "handle", None):
             self.lib.fn("server_destroy")(self.handle)
             self.handle = None
+        _LIVE.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

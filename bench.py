@@ -674,37 +674,44 @@ def serving_benchmark(rates, requests, seed=5, workers=4):
             else:
                 prep.append(None)
         info = {r[2]: r for r in trace if r[0] == "build"}
-        server = Server(hbm_budget_bytes=200 << 30, workers=workers, max_batch=16)
-        ids, turn, reqs = {}, {}, []
-        start = time.perf_counter()
-        for r, bufs in zip(trace, prep):
-            delay = r[1] - (time.perf_counter() - start)
-            if delay > 0:
-                time.sleep(delay)
-            if r[0] == "build":
-                _, _, tid, p, kind, real, cfg = r
-                ids[tid], rid = server.submit_build(*bufs[0], len(pool[p][0][0]), make_config(**cfg), real=real,
-                                                    kind=kind)
-                turn[tid] = 0
-                reqs.append((rid, len(pool[p][0][0])))
-            elif r[0] == "update":
-                tid = r[2]
-                _, _, _, p, kind, real, cfg = info[tid]
-                turn[tid] += 1
-                reqs.append((server.submit_update(ids[tid], new=snapshot(p, real, turn[tid] % 2)),
-                             len(pool[p][0][0])))
-            else:
-                reqs.append((server.submit_release(ids[r[2]]), 0))
-        lat, first, last, work = [], None, 0.0, 0
-        for rid, n in reqs:
-            assert server.wait(rid) == 0
-            s_ms, b_ms, d_ms = server.times(rid)
-            lat.append(d_ms - s_ms)
-            first = s_ms if first is None else min(first, s_ms)
-            last = max(last, d_ms)
-            work += n
-        stats = server.stats()
-        server.close()
+        # closed before anything else on every exit path: its workers read the
+        # device arrays of this function until their queues are drained
+        with Server(hbm_budget_bytes=200 << 30, workers=workers, max_batch=16) as server:
+            ids, turn, reqs = {}, {}, []
+            start = time.perf_counter()
+            for r, bufs in zip(trace, prep):
+                delay = r[1] - (time.perf_counter() - start)
+                if delay > 0:
+                    time.sleep(delay)
+                if r[0] == "build":
+                    _, _, tid, p, kind, real, cfg = r
+                    ids[tid], rid = server.submit_build(*bufs[0], len(pool[p][0][0]), make_config(**cfg),
+                                                        real=real, kind=kind)
+                    turn[tid] = 0
+                    reqs.append((rid, len(pool[p][0][0])))
+                elif r[0] == "update":
+                    tid = r[2]
+                    _, _, _, p, kind, real, cfg = info[tid]
+                    turn[tid] += 1
+                    reqs.append((server.submit_update(ids[tid], new=snapshot(p, real, turn[tid] % 2)),
+                                 len(pool[p][0][0])))
+                else:
+                    reqs.append((server.submit_release(ids[r[2]]), 0))
+            lat, first, last, work, failed = [], None, 0.0, 0, []
+            for rid, n in reqs:
+                st = server.wait(rid)
+                if st != 0:
+                    failed.append((rid, st))
+                    continue
+                s_ms, b_ms, d_ms = server.times(rid)
+                lat.append(d_ms - s_ms)
+                first = s_ms if first is None else min(first, s_ms)
+                last = max(last, d_ms)
+                work += n
+            stats = server.stats()
+        if failed:
+            raise RuntimeError("serving rate %d/s: %d requests failed (first: id %d, status %d)"
+                               % (rate, len(failed), failed[0][0], failed[0][1]))
         served_s = (last - first) / 1e3
         plain, turn2, slat = {}, {}, []
         start = time.perf_counter()
