@@ -328,9 +328,19 @@ inline std::size_t upperBound(const Key *a, std::size_t n, Key v)
 
 // ---------------------------------------------------------------------------
 // A thrust::device_vector stand-in that tracks the logical size and the
-// allocation capacity with thrust's growth rules (reserve(n) allocates n;
-// growing resize allocates max(2*size, n, 2*capacity)); the octree's
-// leafCapacity depends on these capacities (octree.cu:453-457).
+// allocation capacity; the octree's leafCapacity depends on these capacities
+// (octree.cu:453-457). Thrust is a dependency of the reference that is not
+// vendored: CCCL as shipped with CUDA 13.3, the reference's toolchain
+// (benchmarks/env/environment.txt:4; CMakeLists.txt:15 finds Thrust). Its
+// published vector_base algorithm (thrust/detail/vector_base.inl):
+//   reserve(n), n > capacity: allocates exactly n elements;
+//   growing to n elements without room (append / fill_insert, the path of
+//   resize(n) and resize(n, no_init)): new capacity = size + max(size, n -
+//   size), and at least 2 * capacity, i.e. max(2 size, n, 2 capacity);
+//   shrinking or growing within the capacity keeps the allocation.
+// The reference's call sites: resizeWithHeadroom (helpers.cuh:122-141),
+// prepareGraphBuffers (octree.cu:182-241) and the build loop's temporaries
+// (octree.cu:159-180).
 // ---------------------------------------------------------------------------
 template <class T>
 struct TVec
