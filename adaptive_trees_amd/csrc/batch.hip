@@ -10,14 +10,23 @@
 //                 table of member TreeDevs plus, per launch shape, the
 //                 prefix of per-member block counts, so small and large
 //                 trees share one grid with no idle blocks.
-// The build runs the members' pass loops in lock step (a member leaves the
-// loop when its pass changes nothing, exactly where its own build() would
-// stop); update() has fixed capacity-sized tables and is graph-capturable.
+// The build runs every member's passes on the device with no host read-back
+// in between: each member's buffers are reserved for a leaf count its build
+// is not expected to exceed, and before each step of a pass a one-block plan
+// kernel turns the members' device states into the next launches' block
+// maps (members that converged get none). A member leaves where its own
+// build() would stop, the pass that changes nothing; the host reads every
+// state once at the end and replays the committed passes' leaf counts into
+// the mirrored buffer capacities. A member whose pass would overflow its
+// reservation (rejected on the device, its tree unchanged) finishes with
+// host-driven lock-step passes. update() has fixed capacity-sized tables and
+// is graph-capturable.
 // Results are those of each member's own build()/update(): the per-tree
 // arithmetic is unchanged, only the launches are shared.
 #include "batch.hpp"
 #include "tree_impl.hpp"
 
+#include <algorithm>
 #include <climits>
 
 namespace at
@@ -138,19 +147,94 @@ public:
         AT_HIP(hipStreamSynchronize(stream_));
         built_ = false;
         const std::size_t T = trees_.size();
-        sort_.keygenAndSort<Real>(stream_, kind_, kp_, coords_.p, P_.p, perm_.p);
-        std::vector<std::size_t> active(T);
+        sort_.keygenAndSort<Real>(stream_, kind_, kp_, coords_.p, P_.p, perm_.p, false);
+        std::vector<std::size_t> all(T);
         for (std::size_t k = 0; k < T; ++k)
         {
-            active[k] = k;
+            all[k] = k;
             trees_[k]->buildBegin();
+            trees_[k]->reserveDeviceBuild(trees_[k]->deviceBuildCapacity());
         }
+        // ---- every pass on the device: tables staged once, block maps planned per step
         TreeSet set[kShapes];
-        stage(build_, active, true, [&](Shape sh, std::size_t k) { return blocksFor(sh, k, 1); }, set);
+        const std::size_t logStride = maxDepth_ + 1;
+        passLog_.ensure(T * logStride + 1, 0, stream_);
+        stageWith(build_, all, [&](std::size_t i, TreeDev &t) {
+            t.capacity = trees_[all[i]]->deviceBuildLimit();
+            t.passLog = passLog_.p + all[i] * logStride;
+        }, [&](Shape sh, std::size_t k) { return blocksFor(sh, k, 1); }, set);
         multiBuildSearchIndex(stream_, set[kIndex]);
         multiOctInit(stream_, set[kClear]);
-        for (unsigned pass = 0; pass < maxDepth_ && !active.empty(); ++pass)
+        if (T)
         {
+            planStart_.ensure(std::size_t(kPlanShapes) * (T + 1), 0, stream_);
+            const TreeDev *dt = build_.dTrees;
+            const int count = int(T);
+            const TreeSet tiles = plannedSet(dt, count, planStart_.p, kPlanTiles);
+            const TreeSet proc = plannedSet(dt, count, planStart_.p, kPlanProcess);
+            const TreeSet commit = plannedSet(dt, count, planStart_.p, kPlanCommit);
+            const TreeSet fresh = plannedSet(dt, count, planStart_.p, kPlanLeaves);
+            const TreeSet nfWork = plannedSet(dt, count, planStart_.p, kPlanNf);
+            for (unsigned pass = 0; pass < maxDepth_; ++pass)
+            {
+                multiPlan(stream_, dt, count, 0, params_.nf, planStart_.p);
+                multiOctDecideScan(stream_, tiles, params_);
+                multiPlan(stream_, dt, count, 1, params_.nf, planStart_.p);
+                multiOctProcess(stream_, proc, params_);
+                multiCommit(stream_, commit, false);
+                multiOctRefresh(stream_, fresh, nfWork, params_, false);
+            }
+        }
+        // ---- the one read-back: states and the committed passes' leaf counts
+        logHost_.resize(T * logStride + 1);
+        if (T)
+        {
+            AT_HIP(hipMemcpyAsync(sth_, st_, T * sizeof(DevState), hipMemcpyDeviceToHost, stream_));
+            AT_HIP(hipMemcpyAsync(logHost_.data(), passLog_.p, T * logStride * sizeof(std::uint32_t),
+                                  hipMemcpyDeviceToHost, stream_));
+        }
+        AT_HIP(hipStreamSynchronize(stream_));
+        std::vector<std::size_t> stalled;
+        std::vector<unsigned> decided(T, 0);
+        for (std::size_t k = 0; k < T; ++k)
+        {
+            const DevState &st = sth_[k];
+            const unsigned passes = std::min(st.passes, maxDepth_);
+            for (unsigned j = 0; j < passes; ++j) trees_[k]->replayBuildPass(logHost_[k * logStride + j]);
+            decided[k] = passes;
+            if (st.needsResize)
+            {
+                sth_[k].needsResize = 0;
+                sth_[k].converged = 0;
+                stalled.push_back(k);
+            }
+        }
+        if (!stalled.empty()) hostPasses(stalled, decided);
+        for (auto &t : trees_) t->buildEnd(false);
+        for (std::size_t k = 0; k < T; ++k) Member::finishState(sth_[k], trees_[k]->buildLeaves());
+        if (T) AT_HIP(hipMemcpyAsync(st_, sth_, T * sizeof(DevState), hipMemcpyHostToDevice, stream_));
+        built_ = true;
+        refreshUpdateTables();
+        for (std::size_t k = 0; k < T; ++k) (void)trees_[k]->applyState(sth_[k]);
+        for (auto &t : trees_)
+            if (t->configuredViews().any()) t->buildViews(t->configuredViews());
+        // completes the buffers' growth copies and the uploads above
+        AT_HIP(hipStreamSynchronize(stream_));
+    }
+
+    // Host-driven lock-step passes for `members` (their device states with
+    // cleared flags are in sth_), each until its pass changes nothing or
+    // maxDepth decisions; decided[k] = decisions already made.
+    void hostPasses(std::vector<std::size_t> active, std::vector<unsigned> &decided)
+    {
+        AT_HIP(hipMemcpyAsync(st_, sth_, trees_.size() * sizeof(DevState), hipMemcpyHostToDevice, stream_));
+        TreeSet set[kShapes];
+        for (;;)
+        {
+            active.erase(std::remove_if(active.begin(), active.end(),
+                                        [&](std::size_t k) { return decided[k] >= maxDepth_; }),
+                         active.end());
+            if (active.empty()) break;
             for (std::size_t k : active) trees_[k]->buildPrePass();
             stage(build_, active, true,
                   [&](Shape sh, std::size_t i) { return blocksFor(sh, active[i], trees_[active[i]]->buildLeaves()); },
@@ -160,7 +244,10 @@ public:
             readStates();
             std::vector<std::size_t> next;
             for (std::size_t k : active)
+            {
+                ++decided[k];
                 if (trees_[k]->buildAfterDecide(sth_[k])) next.push_back(k);
+            }
             if (next.empty()) break;
             // the process grid covers the old leaves, the refresh the new ones
             TreeSet proc[kShapes];
@@ -177,16 +264,7 @@ public:
             for (std::size_t k : next) trees_[k]->buildAfterProcess();
             active.swap(next);
         }
-        for (auto &t : trees_) t->buildEnd(false);
         readStates();
-        for (std::size_t k = 0; k < T; ++k) Member::finishState(sth_[k], trees_[k]->buildLeaves());
-        if (T) AT_HIP(hipMemcpyAsync(st_, sth_, T * sizeof(DevState), hipMemcpyHostToDevice, stream_));
-        AT_HIP(hipStreamSynchronize(stream_));
-        built_ = true;
-        refreshUpdateTables();
-        for (auto &t : trees_)
-            if (t->configuredViews().any()) t->buildViews(t->configuredViews());
-        (void)needsResize(nullptr);
     }
 
     void update() override
@@ -257,19 +335,28 @@ private:
         }
     }
 
-    // Stages the TreeDevs of `members` (build: capacity unlimited) and the
-    // block maps; one copy of each table.
-    template <class F>
-    void stage(SetTables &tab, const std::vector<std::size_t> &members, bool building, F blocks, TreeSet out[kShapes])
+    // Stages the TreeDevs of `members` (each adjusted by `adjust(i, t)`) and
+    // the block maps; one copy of each table.
+    template <class A, class F>
+    void stageWith(SetTables &tab, const std::vector<std::size_t> &members, A adjust, F blocks,
+                   TreeSet out[kShapes])
     {
         tab.waitWritable();
         for (std::size_t i = 0; i < members.size(); ++i)
         {
             TreeDev t = trees_[members[i]]->device();
-            if (building) t.capacity = LONG_MAX;
+            adjust(i, t);
             tab.hTrees[i] = t;
         }
         tab.upload(stream_, members.size(), blocks, out);
+    }
+    // build: capacity unlimited (host-driven passes grow the buffers)
+    template <class F>
+    void stage(SetTables &tab, const std::vector<std::size_t> &members, bool building, F blocks, TreeSet out[kShapes])
+    {
+        stageWith(tab, members, [&](std::size_t, TreeDev &t) {
+            if (building) t.capacity = LONG_MAX;
+        }, blocks, out);
     }
 
     void refreshUpdateTables()
@@ -279,7 +366,6 @@ private:
         stage(upd_, all, false,
               [&](Shape sh, std::size_t k) { return blocksFor(sh, k, std::size_t(trees_[k]->device().capacity)); },
               updSet_);
-        AT_HIP(hipStreamSynchronize(stream_));
     }
 
     void readStates()
@@ -304,6 +390,9 @@ private:
     OctParams params_{};
     SetTables build_, aux_, upd_;
     TreeSet updSet_[kShapes];
+    DevBuf<std::uint32_t> planStart_; // [kPlanShapes][T + 1] block maps planned on the device
+    DevBuf<std::uint32_t> passLog_;   // [T][maxDepth + 1] leaf counts of the committed build passes
+    std::vector<std::uint32_t> logHost_;
 };
 
 template <class Real>

@@ -1769,8 +1769,10 @@ __global__ void markChunksKernel(MsdCounters *c) { c->pad[0] = c->chunks; }
 // keys (X -> Y, next-digit bytes in `digits`) and planned level 2's lists
 // (set 0) and level 1's chunks. Odd levels write Y, even levels X; the result
 // ends in Y when finalIsY, else in X.
+// hostPoll = false keeps the fixed launch sequence outside capture too (the
+// device-driven batch build, whose host waits once, after its last pass).
 void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *digits, Key *X, Key *Y,
-               std::uint32_t *vX, std::uint32_t *vY, bool finalIsY)
+               std::uint32_t *vX, std::uint32_t *vY, bool finalIsY, bool hostPoll = true)
 {
     const std::uint32_t maxSegs = std::uint32_t(ws.msdMaxSegs), maxRanges = std::uint32_t(ws.msdStride),
                         maxChunks = std::uint32_t(ws.msdMaxChunks);
@@ -1791,7 +1793,7 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
     const std::uint32_t *from = nullptr;
     for (int level = 2; level <= kSortPasses; ++level)
     {
-        if (level == 4 && capture == hipStreamCaptureStatusNone)
+        if (level == 4 && hostPoll && capture == hipStreamCaptureStatusNone)
         {
             // Levels 4..8 have work only for keys that share 24 high bits in
             // buckets of more than kMsdChunk. Outside graph capture the
@@ -2324,7 +2326,7 @@ void SegmentedSort::plan(const std::vector<std::size_t> &counts, hipStream_t s)
 
 template <class T>
 void SegmentedSort::keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &shared,
-                                  const SegCoords<T> *coords, Key *keys, std::uint32_t *vals)
+                                  const SegCoords<T> *coords, Key *keys, std::uint32_t *vals, bool hostPoll)
 {
     if (!ranges_) return;
     const dim3 rgrid{unsigned(ranges_)}, block{unsigned(kSortThreads)};
@@ -2368,7 +2370,7 @@ void SegmentedSort::keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParam
                            msd_.msdRanges[0], msd_.msdFirst[0], std::uint32_t(msd_.msdMaxSegs),
                            std::uint32_t(msd_.msdStride), msd_.msdChunks, std::uint32_t(msd_.msdMaxChunks),
                            std::uint32_t(segments_));
-        msdLevels(s, msd_, total_, digits_.p, keysAlt_.p, keys, valsAlt_.p, vals, true);
+        msdLevels(s, msd_, total_, digits_.p, keysAlt_.p, keys, valsAlt_.p, vals, true, hostPoll);
         return;
     }
     Key *kA = keys, *kB = keysAlt_.p;
@@ -2401,9 +2403,9 @@ void SegmentedSort::keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParam
 }
 
 template void SegmentedSort::keygenAndSort<float>(hipStream_t, KeyKind, const KeygenParams<float> &,
-                                                  const SegCoords<float> *, Key *, std::uint32_t *);
+                                                  const SegCoords<float> *, Key *, std::uint32_t *, bool);
 template void SegmentedSort::keygenAndSort<double>(hipStream_t, KeyKind, const KeygenParams<double> &,
-                                                   const SegCoords<double> *, Key *, std::uint32_t *);
+                                                   const SegCoords<double> *, Key *, std::uint32_t *, bool);
 
 template void keygenAndSort<float>(hipStream_t, KeyKind, const KeygenParams<float> &, std::size_t, Key *,
                                    std::uint32_t *, SortWorkspace &);

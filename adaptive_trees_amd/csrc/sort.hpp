@@ -194,9 +194,11 @@ public:
     std::size_t ranges() const { return ranges_; }
     // Keys of every segment (particle j of segment s from coords[s]) into
     // keys[segBegin[s] + ...], sorted per segment with vals = slot in segment.
+    // hostPoll = false: the MSD levels keep their fixed launch sequence (no
+    // host read of a level count).
     template <class T>
     void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &shared, const SegCoords<T> *coords,
-                       Key *keys, std::uint32_t *vals);
+                       Key *keys, std::uint32_t *vals, bool hostPoll = true);
 
 private:
     std::size_t total_ = 0, ranges_ = 0, segments_ = 0, rangeKeys_ = 0;

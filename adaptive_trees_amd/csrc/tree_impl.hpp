@@ -222,6 +222,41 @@ public:
         activeBuffer_ ^= 1;
         buildL_ = buildNewL_;
     }
+    // ---- a build whose passes run on the device without host read-backs
+    // (batches): the buffers are reserved up front for `deviceBuildCapacity`
+    // leaves, a pass that would exceed them is rejected on the device (the
+    // tree stays as it was), and the host replays the committed passes'
+    // leaf counts afterwards.
+    // LeafCount: level jumps make at most 4096 leaves per 512 * bucket
+    // particles, plain splits fewer; near field: up to one leaf per two
+    // particles. A tree that needs more continues with host-driven passes.
+    std::size_t deviceBuildCapacity() const
+    {
+        const std::size_t base = nf_ ? n_ / 2 : 8 * (n_ / std::max(bucket_, 1u));
+        return std::max<std::size_t>(base + 4096 + 64, vK_.cap);
+    }
+    void reserveDeviceBuild(std::size_t leaves) { ensurePhysical(leaves, 0); }
+    // The capacity a device pass may fill: the reserved halves and scan buffers.
+    long deviceBuildLimit() const
+    {
+        std::size_t c = std::min(K_[0].cap, K_[1].cap);
+        c = c ? c - 1 : 0;
+        c = std::min({c, N_[0].cap, N_[1].cap, offs_.cap ? offs_.cap - 1 : 0});
+        if (nf_) c = std::min({c, NF_[0].cap, NF_[1].cap, gcm_[0].cap, gcm_[1].cap, nfDirty_.cap, groupDirty_.cap,
+                               tasks_.cap / 2, groups_.cap ? (groups_.cap - 1) * 8 : 0});
+        else c = std::min(c, countDirty_.cap);
+        return long(c);
+    }
+    // buildAfterDecide + buildAfterProcess of one committed device pass, on
+    // the mirrored capacities only.
+    void replayBuildPass(std::size_t newLeaves)
+    {
+        buildNewL_ = newLeaves;
+        vTmp_.resizeHeadroom(newLeaves + 1);
+        std::swap(vK_, vTmp_);
+        activeBuffer_ ^= 1;
+        buildL_ = newLeaves;
+    }
     // Half 0 becomes the active one, as the reference keeps the tree in K_d;
     // capacity from the vector capacities reached (octree.cu:182-241,453-457).
     // sync = false leaves the stream synchronisation to the caller.

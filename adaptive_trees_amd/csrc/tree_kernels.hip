@@ -465,13 +465,16 @@ __device__ void binProcessBody(const TreeDev &t, unsigned block)
 __global__ __launch_bounds__(kThreads) void binProcessKernel(TreeDev t) { binProcessBody(t, blockIdx.x); }
 
 // Commit the pass and clear the chained-scan status for the next one.
+// passLog (device-driven batch builds): the new leaf count of each committed
+// pass, for the host's mirror of the build's buffer capacities.
 __device__ void commitBody(DevState *st, int binary, std::uint64_t *status, std::size_t words, std::uint32_t *ticket,
-                           std::uint32_t *nfCount, unsigned block, unsigned blocks)
+                           std::uint32_t *nfCount, unsigned block, unsigned blocks, std::uint32_t *passLog = nullptr)
 {
     if (block == 0 && threadIdx.x == 0)
     {
         if (!st->converged)
         {
+            if (passLog) passLog[st->passes] = std::uint32_t(st->newNumLeaves);
             st->numLeaves = st->newNumLeaves;
             if (binary) st->numNodes = (st->numLeaves - 1) * 2 + 1;
             st->activeBuffer ^= 1;
@@ -1356,76 +1359,164 @@ __device__ inline int treeOfBlock(const TreeSet &s, unsigned b)
     return lo;
 }
 
-#define AT_MULTI_PROLOGUE                                                                                              \
-    const int k = treeOfBlock(s, blockIdx.x);                                                                          \
-    const TreeDev t = s.trees[k];                                                                                      \
-    const unsigned block = blockIdx.x - s.start[k];                                                                    \
-    const unsigned blocks = s.start[k + 1] - s.start[k];                                                               \
-    (void)block;                                                                                                       \
-    (void)blocks
+// Every block walks the virtual blocks b = blockIdx.x, + gridDim.x, ... below
+// the set's total (one step when the grid equals a host-planned total): the
+// tree k that owns b, its TreeDev t and b's index `block` among the tree's
+// `blocks`. b is block-uniform, and each step ends with a barrier, so the
+// bodies' shared memory is free again for the next step.
+#define AT_MULTI_LOOP(BODY)                                                                                            \
+    const unsigned virtualBlocks = s.total ? *s.total : s.blocks;                                                      \
+    for (unsigned vb = blockIdx.x; vb < virtualBlocks; vb += gridDim.x)                                                \
+    {                                                                                                                  \
+        const int k = treeOfBlock(s, vb);                                                                              \
+        const TreeDev t = s.trees[k];                                                                                  \
+        const unsigned block = vb - s.start[k];                                                                        \
+        const unsigned blocks = s.start[k + 1] - s.start[k];                                                           \
+        (void)block;                                                                                                   \
+        (void)blocks;                                                                                                  \
+        BODY;                                                                                                          \
+        __syncthreads();                                                                                               \
+    }
 
-__global__ void multiOctInitKernel(TreeSet s)
-{
-    AT_MULTI_PROLOGUE;
-    octInitBody(t, t.statusWords, block, blocks);
-}
+__global__ void multiOctInitKernel(TreeSet s) { AT_MULTI_LOOP(octInitBody(t, t.statusWords, block, blocks)); }
 __global__ void multiBuildIndexKernel(TreeSet s)
 {
-    AT_MULTI_PROLOGUE;
-    buildIndexBody(t.P, t.n, const_cast<Key *>(t.Pidx), block, blocks);
+    AT_MULTI_LOOP(buildIndexBody(t.P, t.n, const_cast<Key *>(t.Pidx), block, blocks));
 }
 __global__ void multiResetKernel(TreeSet s)
 {
-    AT_MULTI_PROLOGUE;
-    resetBody(t.st, t.P, t.n, t.scanStatus, t.statusWords, t.scanTicket, t.nfCount, block, blocks);
+    AT_MULTI_LOOP(resetBody(t.st, t.P, t.n, t.scanStatus, t.statusWords, t.scanTicket, t.nfCount, block, blocks));
 }
 // Clears the chained-scan status words and ticket of each tree (the build
 // loop's per-pass reset, without a memset per tree).
-__global__ void multiClearScanKernel(TreeSet s)
+__device__ void clearScanBody(const TreeDev &t, unsigned block, unsigned blocks)
 {
-    AT_MULTI_PROLOGUE;
     if (block == 0 && threadIdx.x == 0) *t.scanTicket = 0;
     for (std::size_t w = std::size_t(block) * blockDim.x + threadIdx.x; w < t.statusWords;
          w += std::size_t(blocks) * blockDim.x)
         t.scanStatus[w] = 0;
 }
-__global__ __launch_bounds__(kThreads) void multiOctDecideScanKernel(TreeSet s, unsigned bucket, int nf)
+__global__ void multiClearScanKernel(TreeSet s) { AT_MULTI_LOOP(clearScanBody(t, block, blocks)); }
+__device__ void multiOctDecideBody(const TreeDev &t, unsigned bucket, int nf)
 {
-    AT_MULTI_PROLOGUE;
     const OctDecider d{{t.K[0], t.K[1]}, {t.N[0], t.N[1]}, {t.NF[0], t.NF[1]}, {t.gcm[0], t.gcm[1]}, bucket, nf != 0};
     decideScanBody<kThreads>(d, t.st, t.capacity, t.offs, t.scanStatus, t.scanTicket);
 }
+__global__ __launch_bounds__(kThreads) void multiOctDecideScanKernel(TreeSet s, unsigned bucket, int nf)
+{
+    AT_MULTI_LOOP(multiOctDecideBody(t, bucket, nf));
+}
 __global__ __launch_bounds__(kThreads) void multiOctProcessKernel(TreeSet s, int nf)
 {
-    AT_MULTI_PROLOGUE;
-    octProcessBody(t, nf != 0, block);
+    AT_MULTI_LOOP(octProcessBody(t, nf != 0, block));
 }
 __global__ void multiCommitKernel(TreeSet s, int binary)
 {
-    AT_MULTI_PROLOGUE;
-    commitBody(t.st, binary, t.scanStatus, t.statusWords, t.scanTicket, t.nfCount, block, blocks);
+    AT_MULTI_LOOP(commitBody(t.st, binary, t.scanStatus, t.statusWords, t.scanTicket, t.nfCount, block, blocks,
+                             t.passLog));
 }
 __global__ __launch_bounds__(kThreads) void multiLeafCountsKernel(TreeSet s, int full)
 {
-    AT_MULTI_PROLOGUE;
-    leafCountsBody(t, block, full);
+    AT_MULTI_LOOP(leafCountsBody(t, block, full));
 }
 __global__ __launch_bounds__(kThreads) void multiNfCollectKernel(TreeSet s, int full)
 {
-    AT_MULTI_PROLOGUE;
-    nfCollectBody<1>(t, full, block);
+    AT_MULTI_LOOP(nfCollectBody<1>(t, full, block));
 }
 __global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsigned bucket, int hilbert)
 {
-    AT_MULTI_PROLOGUE;
-    nfComputeBody(t, bucket, hilbert, block, blocks);
+    AT_MULTI_LOOP(nfComputeBody(t, bucket, hilbert, block, blocks));
 }
 __global__ __launch_bounds__(kThreads) void multiNfGroupKernel(TreeSet s, int hilbert, unsigned bucket)
 {
-    AT_MULTI_PROLOGUE;
-    nfGroupBody(t, hilbert, bucket, block, blocks);
+    AT_MULTI_LOOP(nfGroupBody(t, hilbert, bucket, block, blocks));
 }
-#undef AT_MULTI_PROLOGUE
+#undef AT_MULTI_LOOP
+
+// Device versions of the per-tree block counts (leafBlocksFor, nfBlocksFor,
+// tileBlocksFor below).
+__device__ inline std::uint32_t dLeafBlocks(std::uint32_t leaves)
+{
+    const std::uint32_t b = (leaves + kThreads - 1) / kThreads;
+    return b < 1 ? 1u : b;
+}
+__device__ inline std::uint32_t dNfBlocks(std::uint32_t leaves)
+{
+    const std::uint32_t b = (leaves + 63) / 64;
+    return b < 1 ? 1u : (b > 2048 ? 2048u : b);
+}
+
+constexpr int kPlanThreads = 1024;
+__global__ __launch_bounds__(kPlanThreads) void multiPlanKernel(const TreeDev *trees, int count, int phase, int nf,
+                                                                std::uint32_t *start)
+{
+    __shared__ std::uint32_t waveSum[kPlanShapes][kPlanThreads / 64];
+    __shared__ std::uint32_t carry[kPlanShapes];
+    const unsigned lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    if (threadIdx.x < kPlanShapes) carry[threadIdx.x] = 0;
+    __syncthreads();
+    const std::size_t stride = std::size_t(count) + 1;
+    for (int base = 0; base < count; base += kPlanThreads)
+    {
+        const int k = base + int(threadIdx.x);
+        std::uint32_t c[kPlanShapes] = {0u, 0u, 0u, 0u, 0u};
+        if (k < count)
+        {
+            const DevState *st = trees[k].st;
+            if (!st->converged)
+            {
+                const std::uint32_t L = std::uint32_t(st->numLeaves);
+                if (phase == 0)
+                    c[kPlanTiles] = L / kPassTile + 1;
+                else
+                {
+                    const std::uint32_t newL = std::uint32_t(st->newNumLeaves);
+                    c[kPlanProcess] = dLeafBlocks(L);
+                    c[kPlanCommit] = 1;
+                    c[kPlanLeaves] = dLeafBlocks(newL);
+                    c[kPlanNf] = nf ? dNfBlocks(newL) : 0u;
+                }
+            }
+        }
+        std::uint32_t inc[kPlanShapes];
+#pragma unroll
+        for (int sh = 0; sh < kPlanShapes; ++sh)
+        {
+            inc[sh] = c[sh];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1)
+            {
+                const std::uint32_t v = __shfl_up(inc[sh], o, 64);
+                if (lane >= unsigned(o)) inc[sh] += v;
+            }
+            if (lane == 63) waveSum[sh][wave] = inc[sh];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int sh = 0; sh < kPlanShapes; ++sh)
+        {
+            std::uint32_t before = carry[sh];
+            std::uint32_t all = 0;
+            for (int w = 0; w < kPlanThreads / 64; ++w)
+            {
+                if (w < wave) before += waveSum[sh][w];
+                all += waveSum[sh][w];
+            }
+            const bool mine = phase == 0 ? sh == kPlanTiles : sh != kPlanTiles;
+            if (mine && k < count) start[sh * stride + std::size_t(k)] = before + inc[sh] - c[sh];
+            __syncthreads(); // every thread has read carry[sh]
+            if (threadIdx.x == 0) carry[sh] += all;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kPlanShapes)
+    {
+        const int sh = int(threadIdx.x);
+        const bool mine = phase == 0 ? sh == kPlanTiles : sh != kPlanTiles;
+        if (mine) start[sh * stride + std::size_t(count)] = carry[sh];
+    }
+}
 } // namespace
 
 unsigned clearBlocksFor(std::size_t statusWords) { return clearBlocks(statusWords); }
@@ -1439,9 +1530,14 @@ void buildSearchIndex(hipStream_t s, const TreeDev &t)
     if (t.n) hipLaunchKernelGGL(buildIndexKernel, dim3(indexBlocksFor(t.n)), dim3(256), 0, s, t.P, t.n,
                                 const_cast<Key *>(t.Pidx));
 }
+// Grid of a multi-tree launch: the host-planned total, or a fixed grid that
+// loops over a device-planned one (8 blocks of 256 threads per CU).
+constexpr unsigned kPlannedGrid = 2048;
+inline unsigned multiGrid(const TreeSet &set) { return set.total ? kPlannedGrid : set.blocks; }
+
 void multiBuildSearchIndex(hipStream_t s, const TreeSet &index)
 {
-    if (index.blocks) hipLaunchKernelGGL(multiBuildIndexKernel, dim3(index.blocks), dim3(256), 0, s, index);
+    if (multiGrid(index)) hipLaunchKernelGGL(multiBuildIndexKernel, dim3(multiGrid(index)), dim3(256), 0, s, index);
 }
 unsigned tileBlocksFor(std::size_t gridLeaves) { return unsigned(gridLeaves / kPassTile + 1); }
 unsigned leafBlocksFor(std::size_t gridLeaves)
@@ -1457,45 +1553,67 @@ unsigned nfBlocksFor(std::size_t gridLeaves)
 
 void multiOctInit(hipStream_t s, const TreeSet &clear)
 {
-    if (clear.blocks) hipLaunchKernelGGL(multiOctInitKernel, dim3(clear.blocks), dim3(256), 0, s, clear);
+    if (multiGrid(clear)) hipLaunchKernelGGL(multiOctInitKernel, dim3(multiGrid(clear)), dim3(256), 0, s, clear);
 }
 void multiReset(hipStream_t s, const TreeSet &clear)
 {
-    if (clear.blocks) hipLaunchKernelGGL(multiResetKernel, dim3(clear.blocks), dim3(256), 0, s, clear);
+    if (multiGrid(clear)) hipLaunchKernelGGL(multiResetKernel, dim3(multiGrid(clear)), dim3(256), 0, s, clear);
 }
 void multiClearScan(hipStream_t s, const TreeSet &clear)
 {
-    if (clear.blocks) hipLaunchKernelGGL(multiClearScanKernel, dim3(clear.blocks), dim3(256), 0, s, clear);
+    if (multiGrid(clear)) hipLaunchKernelGGL(multiClearScanKernel, dim3(multiGrid(clear)), dim3(256), 0, s, clear);
 }
 void multiOctDecideScan(hipStream_t s, const TreeSet &tiles, const OctParams &p)
 {
     prof::Scope scope("decide_scan", s);
-    if (tiles.blocks)
-        hipLaunchKernelGGL(multiOctDecideScanKernel, dim3(tiles.blocks), dim3(kThreads), 0, s, tiles, p.bucket,
+    if (multiGrid(tiles))
+        hipLaunchKernelGGL(multiOctDecideScanKernel, dim3(multiGrid(tiles)), dim3(kThreads), 0, s, tiles, p.bucket,
                            p.nf ? 1 : 0);
 }
 void multiOctProcess(hipStream_t s, const TreeSet &leaves, const OctParams &p)
 {
-    if (leaves.blocks)
-        hipLaunchKernelGGL(multiOctProcessKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, p.nf ? 1 : 0);
+    if (multiGrid(leaves))
+        hipLaunchKernelGGL(multiOctProcessKernel, dim3(multiGrid(leaves)), dim3(kThreads), 0, s, leaves,
+                           p.nf ? 1 : 0);
 }
 void multiCommit(hipStream_t s, const TreeSet &clear, bool binary)
 {
-    if (clear.blocks) hipLaunchKernelGGL(multiCommitKernel, dim3(clear.blocks), dim3(256), 0, s, clear, binary ? 1 : 0);
+    if (multiGrid(clear))
+        hipLaunchKernelGGL(multiCommitKernel, dim3(multiGrid(clear)), dim3(256), 0, s, clear, binary ? 1 : 0);
 }
 void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, const OctParams &p, bool full)
 {
-    if (!leaves.blocks) return;
+    if (!multiGrid(leaves)) return;
     if (!p.nf)
     {
         prof::Scope scope("leaf_counts", s);
-        hipLaunchKernelGGL(multiLeafCountsKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, full ? 1 : 0);
+        hipLaunchKernelGGL(multiLeafCountsKernel, dim3(multiGrid(leaves)), dim3(kThreads), 0, s, leaves,
+                           full ? 1 : 0);
         return;
     }
     prof::Scope scope("near_field", s);
-    hipLaunchKernelGGL(multiNfCollectKernel, dim3(leaves.blocks), dim3(kThreads), 0, s, leaves, full ? 1 : 0);
-    hipLaunchKernelGGL(multiNfComputeKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.bucket, p.hilbert ? 1 : 0);
-    hipLaunchKernelGGL(multiNfGroupKernel, dim3(nf.blocks), dim3(kThreads), 0, s, nf, p.hilbert ? 1 : 0, p.bucket);
+    hipLaunchKernelGGL(multiNfCollectKernel, dim3(multiGrid(leaves)), dim3(kThreads), 0, s, leaves, full ? 1 : 0);
+    if (multiGrid(nf))
+    {
+        hipLaunchKernelGGL(multiNfComputeKernel, dim3(multiGrid(nf)), dim3(kThreads), 0, s, nf, p.bucket,
+                           p.hilbert ? 1 : 0);
+        hipLaunchKernelGGL(multiNfGroupKernel, dim3(multiGrid(nf)), dim3(kThreads), 0, s, nf, p.hilbert ? 1 : 0,
+                           p.bucket);
+    }
+}
+
+void multiPlan(hipStream_t s, const TreeDev *trees, int count, int phase, bool nf, std::uint32_t *start)
+{
+    hipLaunchKernelGGL(multiPlanKernel, dim3(1), dim3(kPlanThreads), 0, s, trees, count, phase, nf ? 1 : 0, start);
+}
+TreeSet plannedSet(const TreeDev *trees, int count, const std::uint32_t *start, PlanShape shape)
+{
+    TreeSet set;
+    set.trees = trees;
+    set.start = start + std::size_t(shape) * (std::size_t(count) + 1);
+    set.count = count;
+    set.total = set.start + count;
+    return set;
 }
 
 std::size_t passTiles(std::size_t cap) { return cap / kPassTile + 1; }

@@ -29,18 +29,34 @@ struct TreeDev
     std::uint32_t *tileFirst = nullptr;          // full leaf counts by tiles of P: n / kBoundTile + 2 entries
     std::size_t n = 0;                           // particle slots
     long capacity = 0;                           // leaf capacity of a pass (multi-tree launches)
+    std::uint32_t *passLog = nullptr;            // device-driven batch builds: new leaf count of each committed pass
 };
 
-// Trees driven by one launch. Block b belongs to the tree k with
-// start[k] <= b < start[k+1]; start is a host-computed prefix of per-tree
-// block counts (one table per launch shape), copied to the device.
+// Trees driven by one launch. Virtual block b belongs to the tree k with
+// start[k] <= b < start[k+1]; start is a prefix of per-tree block counts (one
+// table per launch shape), either computed on the host and copied to the
+// device (blocks = start[count], one launched block per virtual block) or
+// planned on the device from the trees' states (total points at
+// start[count]; a fixed grid loops over the virtual blocks).
 struct TreeSet
 {
     const TreeDev *trees = nullptr;       // device array [count]
     const std::uint32_t *start = nullptr; // device array [count + 1]
     int count = 0;
-    unsigned blocks = 0;                  // start[count]
+    unsigned blocks = 0;                  // start[count] (host-planned tables)
+    const std::uint32_t *total = nullptr; // device start[count] (device-planned tables)
 };
+
+// Launch shapes planned on the device between the steps of a pass.
+enum PlanShape { kPlanTiles = 0, kPlanProcess, kPlanCommit, kPlanLeaves, kPlanNf, kPlanShapes };
+// Per-tree block counts from the trees' device states, written as the
+// prefixes start[shape * (count + 1) + k]. phase 0 (before a decision):
+// decision tiles of the trees not converged. phase 1 (after it): process
+// blocks over the old leaves, one commit block, refresh blocks over the new
+// leaves and near-field work groups (nf), for the trees whose pass changed
+// the tree and fitted its capacity.
+void multiPlan(hipStream_t s, const TreeDev *trees, int count, int phase, bool nf, std::uint32_t *start);
+TreeSet plannedSet(const TreeDev *trees, int count, const std::uint32_t *start, PlanShape shape);
 
 // Per-tree block counts of each multi-tree launch shape.
 unsigned clearBlocksFor(std::size_t statusWords); // status-word clearing (init, reset, commit)

@@ -128,3 +128,38 @@ def run_batch(real, sfc, crit, bucket, SIZES):
         b.build()
         assert not b.needsResize().any()
         same(b.tree(0), o, "oracle member %d" % k, nf)
+
+
+def test_batch_build_beyond_reservation():
+    """Members whose build outgrows the leaves reserved for the device-driven
+    passes (bucket 1, pairs of nearly coincident particles: chains of 7 empty
+    leaves per level down to level 21) finish with host-driven passes; every
+    member equals its plain tree and one the oracle."""
+    rng = np.random.default_rng(31)
+    sizes = [400, 0, 1000, 64, 600]
+    host = []
+    for n in sizes:
+        base = rng.random((n // 2, 3))
+        pts = np.concatenate([base, base + 1e-7 * rng.random((n // 2, 3))])[:n] if n else np.zeros((0, 3))
+        host.append([np.ascontiguousarray(pts[:, a], dtype=np.float64) for a in range(3)])
+    dev = [[DeviceArray(a) for a in m] for m in host]
+    cfg = dict(max_particles_per_leaf=1, sfc_kind=MORTON, split_criterion=LEAF_COUNT)
+    s = stream()
+    batch = Batch([m[0] for m in dev], [m[1] for m in dev], [m[2] for m in dev], sizes,
+                  make_config(stream=s, **cfg), real=np.float64)
+    print("batch build beyond reservation", flush=True)
+    batch.build()
+    plain = [Tree(*m, n, make_config(**cfg), real=np.float64, kind=OCTREE) for m, n in zip(dev, sizes)]
+    for t in plain:
+        t.build()
+    leaves = [t.scalars().num_leaves for t in plain]
+    assert leaves[2] > 8 * sizes[2] + 4160, "the test needs a member beyond its reservation: %s" % leaves
+    for k, t in enumerate(plain):
+        same(batch.tree(k), t, "member %d" % k, False)
+    o = OracleTree(*[h.copy() for h in host[2]], sizes[2], make_config(**cfg), real=np.float64, kind=OCTREE)
+    o.build()
+    same(batch.tree(2), o, "oracle member 2", False)
+    # a second build of the same batch reuses the grown buffers
+    batch.build()
+    for k, t in enumerate(plain):
+        same(batch.tree(k), t, "rebuild member %d" % k, False)
