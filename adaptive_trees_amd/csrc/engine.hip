@@ -514,6 +514,7 @@ int at_tune_set(const char *key, int value)
         else if (k == "sort_digit_array") at::setSortDigitArray(value);
         else if (k == "sort_prefetch") at::setSortPrefetch(value);
         else if (k == "sort_msd") at::setSortMsd(value);
+        else if (k == "sort_digit_bits") at::setSortDigitBits(value);
         else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
     });
 }
@@ -559,6 +560,7 @@ int at_tune_get(const char *key, int *value)
         else if (k == "sort_digit_array") *value = at::sortDigitArray();
         else if (k == "sort_prefetch") *value = at::sortPrefetch();
         else if (k == "sort_msd") *value = at::sortMsd();
+        else if (k == "sort_digit_bits") *value = at::sortDigitBits();
         else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
     });
 }

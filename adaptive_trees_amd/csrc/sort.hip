@@ -46,6 +46,7 @@ int g_targetRanges = 8192; // one 4096-key tile per range at 2^25 keys
 int g_digitArray = 1; // scatter writes the next pass's digit bytes for the histogram
 int g_prefetch = 0;   // scatter loads the next tile while working on the current one
 int g_msd = 1;        // MSD levels + LDS chunk sort for large (key, value) sorts
+int g_digitBits = 8;  // LSD digit width below kMsdMinKeys keys (8 or 11)
 constexpr std::size_t kMsdMinKeys = std::size_t(1) << 20;
 
 __device__ inline unsigned laneId() { return __lane_id(); }
@@ -816,6 +817,246 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
         body(xcdRow(blockIdx.x, ranges)); // gridDim.x == ranges
 }
 
+// ---- wide digits: LSD passes of 11 bits ------------------------------------------
+// The digit width the autotuner may pick for LSD sorts (below kMsdMinKeys
+// keys): 6 passes of 11 bits instead of 8 of 8 cover the 64-bit key, 18
+// launches instead of 24. One tile of 2048 keys per range; per pass a
+// histogram of each range (digit-major rows), one scan block per digit over
+// the ranges, and a scatter that ranks each key within its wave by 11
+// ballots, stages the tile in output order in LDS and writes it out.
+// Stable, like the 8-bit passes: ties keep their input order.
+constexpr int kWideBits = 11;
+constexpr int kWideRadix = 1 << kWideBits;
+constexpr int kWideThreads = 256, kWideItems = 8;
+constexpr int kWideTile = kWideThreads * kWideItems;
+constexpr int kWidePasses = (64 + kWideBits - 1) / kWideBits; // even: the result ends where it started
+constexpr int kWideDigitsPerThread = kWideRadix / kWideThreads;
+static_assert(kWidePasses % 2 == 0, "an even pass count leaves the result in the input buffers");
+
+template <int BITS>
+__device__ inline std::uint64_t laneMatchBits(unsigned dg)
+{
+    std::uint32_t lo = ~0u, hi = ~0u;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b)
+    {
+        const std::uint32_t mask = std::uint32_t(__builtin_amdgcn_sbfe(int(dg), b, 1));
+        const std::uint64_t ballot = __ballot(int(mask));
+        lo &= ~(std::uint32_t(ballot) ^ mask);
+        hi &= ~(std::uint32_t(ballot >> 32) ^ mask);
+    }
+    return (std::uint64_t(hi) << 32) | lo;
+}
+
+// Exclusive scan of one value per thread of a kWideThreads block; total =
+// the sum over the block. All threads must call it.
+__device__ inline std::uint32_t wideBlockScan(std::uint32_t v, std::uint32_t *waveTotals, std::uint32_t &total)
+{
+    const unsigned lane = laneId();
+    const int wave = threadIdx.x >> 6;
+    std::uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const std::uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= unsigned(o)) inc += t;
+    }
+    if (lane == 63) waveTotals[wave] = inc;
+    __syncthreads();
+    std::uint32_t before = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kWideThreads / 64; ++w)
+    {
+        if (w < wave) before += waveTotals[w];
+        total += waveTotals[w];
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+// hist[d * ranges + r] = keys of range r (keys [r * kWideTile, ...)) whose
+// digit at `shift` is d.
+__global__ __launch_bounds__(kWideThreads) void wideUpsweepKernel(const Key *__restrict__ keys, std::size_t n,
+                                                                  int shift, std::uint32_t *__restrict__ hist,
+                                                                  std::uint32_t ranges)
+{
+    __shared__ std::uint32_t h[kWideRadix];
+    for (int d = threadIdx.x; d < kWideRadix; d += kWideThreads) h[d] = 0;
+    __syncthreads();
+    const std::size_t base = std::size_t(blockIdx.x) * kWideTile;
+#pragma unroll
+    for (int j = 0; j < kWideItems; ++j)
+    {
+        const std::size_t i = base + std::size_t(j) * kWideThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[unsigned(keys[i] >> shift) & (kWideRadix - 1)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kWideRadix; d += kWideThreads) hist[std::size_t(d) * ranges + blockIdx.x] = h[d];
+}
+
+// Block d: exclusive prefix of digit d's counts over the ranges, and its total.
+__global__ __launch_bounds__(kWideThreads) void wideScanKernel(const std::uint32_t *__restrict__ hist,
+                                                               std::uint32_t *__restrict__ off,
+                                                               std::uint32_t *__restrict__ total,
+                                                               std::uint32_t ranges)
+{
+    __shared__ std::uint32_t wt[kWideThreads / 64];
+    const std::size_t row = std::size_t(blockIdx.x) * ranges;
+    std::uint32_t carry = 0;
+    for (std::uint32_t b = 0; b < ranges; b += kWideThreads)
+    {
+        const std::uint32_t r = b + threadIdx.x;
+        const std::uint32_t v = r < ranges ? hist[row + r] : 0u;
+        std::uint32_t all;
+        const std::uint32_t ex = wideBlockScan(v, wt, all);
+        if (r < ranges) off[row + r] = carry + ex;
+        carry += all;
+    }
+    if (threadIdx.x == 0) total[blockIdx.x] = carry;
+}
+
+// Block r scatters range r (one tile). kIota: the values are the keys'
+// input positions (the first pass of a sort by slot).
+template <bool kVals, bool kIota>
+__global__ __launch_bounds__(kWideThreads) void wideScatterKernel(const Key *__restrict__ kin, Key *__restrict__ kout,
+                                                                  const std::uint32_t *__restrict__ vin,
+                                                                  std::uint32_t *__restrict__ vout, std::size_t n,
+                                                                  int shift, const std::uint32_t *__restrict__ off,
+                                                                  const std::uint32_t *__restrict__ total,
+                                                                  std::uint32_t ranges)
+{
+    constexpr int kW = kWideThreads / 64;
+    constexpr int kD = kWideDigitsPerThread;
+    constexpr bool kLoadVals = kVals && !kIota;
+    __shared__ Key stageKeys[kWideTile];
+    __shared__ std::uint32_t stageVals[kVals ? kWideTile : 1];
+    __shared__ std::uint32_t waveCount[kW][kWideRadix]; // per-wave digit counts, then tile positions
+    __shared__ std::uint32_t sink[kW][64];              // ranking adds of non-leader lanes
+    __shared__ std::uint32_t adjust[kWideRadix];        // output index of a staged position, per digit
+    __shared__ std::uint32_t wt[kW];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const unsigned lane = laneId();
+    const std::uint32_t r = blockIdx.x;
+    const std::size_t rBegin = std::size_t(r) * kWideTile;
+    const std::uint32_t rLen = std::uint32_t(n - rBegin < std::size_t(kWideTile) ? n - rBegin : kWideTile);
+
+    // the range's start of every digit (thread t: digits kD t .. kD t + kD - 1)
+    std::uint32_t tot[kD], sum = 0;
+#pragma unroll
+    for (int q = 0; q < kD; ++q)
+    {
+        tot[q] = total[kD * tid + q];
+        sum += tot[q];
+    }
+    std::uint32_t all;
+    std::uint32_t run = wideBlockScan(sum, wt, all);
+#pragma unroll
+    for (int q = 0; q < kD; ++q)
+    {
+        const int d = kD * tid + q;
+        adjust[d] = run + off[std::size_t(d) * ranges + r];
+        run += tot[q];
+    }
+    for (int i = tid; i < kW * kWideRadix; i += kWideThreads) (&waveCount[0][0])[i] = 0;
+
+    // keys (wave-striped: item j of lane l is element j * 64 + l of the
+    // wave's 512); padding ranks last with the largest digit
+    Key key[kWideItems];
+    std::uint32_t val[kLoadVals ? kWideItems : 1];
+    const std::uint32_t waveFirst = std::uint32_t(wave) * (kWideItems * 64) + lane;
+#pragma unroll
+    for (int j = 0; j < kWideItems; ++j)
+    {
+        const std::uint32_t idx = waveFirst + std::uint32_t(j) * 64;
+        const std::uint32_t c = idx < rLen ? idx : rLen - 1;
+        const Key k = kin[rBegin + c];
+        key[j] = idx < rLen ? k : kInvalidKey;
+        if constexpr (kLoadVals) val[j] = vin[rBegin + c];
+    }
+    __syncthreads();
+
+    // rank within the wave: the lowest lane of each digit group adds the
+    // group's size to the wave's counter (the others add 0 to a slot of
+    // their own; one wave's LDS operations execute in order)
+    const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
+    std::uint32_t rank[kWideItems], old[kWideItems];
+    int leader[kWideItems];
+#pragma unroll
+    for (int j = 0; j < kWideItems; ++j)
+    {
+        const unsigned dg = unsigned(key[j] >> shift) & (kWideRadix - 1);
+        const std::uint64_t match = laneMatchBits<kWideBits>(dg);
+        const std::uint32_t lower = __popcll(match & ltMask);
+        const bool lead = lower == 0;
+        std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
+        old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
+        leader[j] = __builtin_ctzll(match);
+        rank[j] = lower;
+    }
+#pragma unroll
+    for (int j = 0; j < kWideItems; ++j) rank[j] += std::uint32_t(__shfl(int(old[j]), leader[j], 64));
+    __syncthreads();
+
+    // per digit: wave offsets and the tile count, then the digits' starts in the tile
+    std::uint32_t cnt[kD], tsum = 0;
+#pragma unroll
+    for (int q = 0; q < kD; ++q)
+    {
+        const int d = kD * tid + q;
+        std::uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kW; ++w)
+        {
+            const std::uint32_t x = waveCount[w][d];
+            waveCount[w][d] = c;
+            c += x;
+        }
+        cnt[q] = c;
+        tsum += c;
+    }
+    std::uint32_t tstart = wideBlockScan(tsum, wt, all);
+#pragma unroll
+    for (int q = 0; q < kD; ++q)
+    {
+        const int d = kD * tid + q;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) waveCount[w][d] += tstart;
+        adjust[d] -= tstart;
+        tstart += cnt[q];
+    }
+    __syncthreads();
+
+    // stage in output order, then write
+#pragma unroll
+    for (int j = 0; j < kWideItems; ++j)
+    {
+        const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kWideRadix - 1)];
+        stageKeys[pos] = key[j];
+        if constexpr (kVals)
+        {
+            std::uint32_t v;
+            if constexpr (kIota) v = std::uint32_t(rBegin) + waveFirst + std::uint32_t(j) * 64;
+            else v = val[j];
+            stageVals[pos] = v;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kWideItems; ++q)
+    {
+        const int i = q * kWideThreads + tid;
+        if (std::uint32_t(i) < rLen)
+        {
+            const Key k = stageKeys[i];
+            const std::uint32_t dst = adjust[unsigned(k >> shift) & (kWideRadix - 1)] + std::uint32_t(i);
+            kout[dst] = k;
+            if constexpr (kVals) vout[dst] = stageVals[i];
+        }
+    }
+}
+
 // ---- segmented sort kernels --------------------------------------------------
 constexpr int kSegThreads = 512, kSegItems = 8; // tile shape of the segmented scatter
 
@@ -900,7 +1141,6 @@ constexpr int kLocThreads = 256, kLocItems = 8;       // LDS chunk sort block sh
 constexpr std::uint32_t kMsdChunk = kLocThreads * kLocItems; // LDS chunk capacity (keys)
 constexpr int kMsdThreads = 512, kMsdItems = 8;       // scatter tile of levels >= 2
 constexpr std::uint32_t kMsdTile = kMsdThreads * kMsdItems;
-constexpr int kMsdGrid = 1024;                        // grid of the looping level kernels (multiple of 8)
 constexpr unsigned kMsdScanRows = 16;                 // chunk rows of the level scan's grid (x 256 digits)
 constexpr unsigned kRootPlanBlocks = 32;              // blocks sharing the range writes of the root plan
 
@@ -1755,6 +1995,45 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
     AT_HIP(hipGetLastError());
 }
 
+// The 11-bit LSD passes when they are chosen and the workspace holds their
+// histograms (fewer than kMsdMinKeys keys).
+bool widePasses(const SortWorkspace &ws, std::size_t n)
+{
+    return g_digitBits == kWideBits && ws.wideHist && (n + kWideTile - 1) / kWideTile <= ws.wideRanges;
+}
+void runWidePasses(hipStream_t s, std::size_t n, SortWorkspace &ws, Key *kA, Key *kB, std::uint32_t *vA,
+                   std::uint32_t *vB, bool vals, bool iota)
+{
+    const std::uint32_t ranges = std::uint32_t((n + kWideTile - 1) / kWideTile);
+    for (int pass = 0; pass < kWidePasses; ++pass)
+    {
+        const Key *src = (pass & 1) ? kB : kA;
+        Key *dst = (pass & 1) ? kA : kB;
+        const std::uint32_t *vsrc = (pass & 1) ? vB : vA;
+        std::uint32_t *vdst = (pass & 1) ? vA : vB;
+        const int shift = pass * kWideBits;
+        {
+            prof::Scope scope("sort_upsweep", s);
+            hipLaunchKernelGGL(wideUpsweepKernel, dim3(ranges), dim3(kWideThreads), 0, s, src, n, shift, ws.wideHist,
+                               ranges);
+            hipLaunchKernelGGL(wideScanKernel, dim3(kWideRadix), dim3(kWideThreads), 0, s, ws.wideHist, ws.wideOff,
+                               ws.wideTotal, ranges);
+        }
+        prof::Scope scope("sort_pass", s);
+        const dim3 grid(ranges), block(kWideThreads);
+        if (!vals)
+            hipLaunchKernelGGL((wideScatterKernel<false, false>), grid, block, 0, s, src, dst, nullptr, nullptr, n,
+                               shift, ws.wideOff, ws.wideTotal, ranges);
+        else if (iota && pass == 0)
+            hipLaunchKernelGGL((wideScatterKernel<true, true>), grid, block, 0, s, src, dst, nullptr, vdst, n, shift,
+                               ws.wideOff, ws.wideTotal, ranges);
+        else
+            hipLaunchKernelGGL((wideScatterKernel<true, false>), grid, block, 0, s, src, dst, vsrc, vdst, n, shift,
+                               ws.wideOff, ws.wideTotal, ranges);
+    }
+    AT_HIP(hipGetLastError());
+}
+
 // Copies a device count into (pinned, coherent) host memory for a polling host.
 constexpr std::uint32_t kCountPending = 0xffffffffu;
 __global__ void publishCountKernel(const std::uint32_t *count, std::uint32_t *host)
@@ -1916,6 +2195,12 @@ void setSortMsd(int on)
     g_msd = on;
 }
 std::size_t sortMsdMinKeys() { return kMsdMinKeys; }
+int sortDigitBits() { return g_digitBits; }
+void setSortDigitBits(int bits)
+{
+    if (bits != 8 && bits != kWideBits) throw Error(1, "sort_digit_bits must be 8 or 11");
+    g_digitBits = bits;
+}
 int sortDigitArray() { return g_digitArray; }
 void setSortDigitArray(int on)
 {
@@ -1970,6 +2255,13 @@ void SortWorkspace::allocate(std::size_t cap, bool withValues, hipStream_t s)
     rangeHist = static_cast<std::uint32_t *>(poolAlloc(histRows * kRadix * sizeof(std::uint32_t), s));
     rangeOff = static_cast<std::uint32_t *>(poolAlloc(maxRanges * kRadix * sizeof(std::uint32_t), s));
     digitTotal = static_cast<std::uint32_t *>(poolAlloc(kRadix * sizeof(std::uint32_t), s));
+    if (capacity < kMsdMinKeys)
+    {
+        wideRanges = (capacity + kWideTile - 1) / kWideTile;
+        wideHist = static_cast<std::uint32_t *>(poolAlloc(wideRanges * kWideRadix * sizeof(std::uint32_t), s));
+        wideOff = static_cast<std::uint32_t *>(poolAlloc(wideRanges * kWideRadix * sizeof(std::uint32_t), s));
+        wideTotal = static_cast<std::uint32_t *>(poolAlloc(kWideRadix * sizeof(std::uint32_t), s));
+    }
 }
 
 void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_t s)
@@ -2009,16 +2301,16 @@ void SortWorkspace::release(hipStream_t s)
     if (borrowed)
     {
         keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
-        digitTotal = nullptr;
+        digitTotal = nullptr; wideHist = wideOff = wideTotal = nullptr; wideRanges = 0;
         capacity = 0; maxRanges = 0; values = false; borrowed = false;
         return;
     }
     // ordered on s; s = null: the owner has synchronised (destruction)
     for (void *p : {(void *)keysAlt, (void *)valsAlt, (void *)digits, (void *)rangeHist, (void *)rangeOff,
-                    (void *)digitTotal})
+                    (void *)digitTotal, (void *)wideHist, (void *)wideOff, (void *)wideTotal})
         poolFree(p, s);
     keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
-    digitTotal = nullptr;
+    digitTotal = nullptr; wideHist = wideOff = wideTotal = nullptr; wideRanges = 0;
     capacity = 0; maxRanges = 0; values = false;
     for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
                     (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
@@ -2066,6 +2358,7 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
         }
     }
     if (msd) msdSort(s, pl, ws, ws.keysAlt, keysOut, ws.valsAlt, valsOut, true, false, true);
+    else if (widePasses(ws, n)) runWidePasses(s, n, ws, keysOut, ws.keysAlt, valsOut, ws.valsAlt, true, true);
     else runPasses(s, pl, ws, keysOut, ws.keysAlt, valsOut, ws.valsAlt, true, true, false);
 }
 
@@ -2077,6 +2370,11 @@ void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, Sor
     ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
     const bool msd = msdAllowed && g_msd && n >= kMsdMinKeys;
+    if (!msd && widePasses(ws, n))
+    {
+        runWidePasses(s, n, ws, keys, ws.keysAlt, vals, ws.valsAlt, true, iota);
+        return;
+    }
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
                        msd ? 64 - kRadixBits : 0, ws.rangeHist);
     if (msd) msdSort(s, pl, ws, keys, ws.keysAlt, vals, ws.valsAlt, iota, true, false);
@@ -2088,6 +2386,11 @@ void sortKeys(hipStream_t s, Key *keys, std::size_t n, SortWorkspace &ws)
     if (n == 0) return;
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
     ws.allocate(n, false, s);
+    if (widePasses(ws, n))
+    {
+        runWidePasses(s, n, ws, keys, ws.keysAlt, nullptr, nullptr, false, false);
+        return;
+    }
     const SortPlan pl = makeSortPlan(n);
     hipLaunchKernelGGL(upsweepKeysKernel, dim3(unsigned(pl.ranges)), dim3(kSortThreads), 0, s, keys, n, pl.rangeKeys,
                        0, ws.rangeHist);
@@ -2258,6 +2561,36 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
             if (t < best) best = t, bm = m;
         }
     g_msd = bm;
+    // the LSD digit width, on a sort below the MSD threshold (its own
+    // workspace: the 11-bit histograms exist only for such sizes)
+    const std::size_t m = n < kMsdMinKeys ? n : kMsdMinKeys - 1;
+    SortWorkspace small;
+    small.allocate(m, true, s);
+    auto timeLsd = [&]() {
+        float b = 1e30f;
+        for (int rep = 0; rep < 6; ++rep)
+        {
+            AT_HIP(hipMemcpyAsync(keys.p, src.p, m * sizeof(Key), hipMemcpyDeviceToDevice, s));
+            AT_HIP(hipEventRecord(e0, s));
+            sortPairs(s, keys.p, vals.p, m, small, true, false);
+            AT_HIP(hipEventRecord(e1, s));
+            AT_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            AT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0 && ms < b) b = ms;
+        }
+        return b;
+    };
+    float lsd[2] = {0.f, 0.f};
+    int bb = 8;
+    float bestLsd = 1e30f;
+    for (int i = 0; i < 2; ++i)
+    {
+        g_digitBits = i ? kWideBits : 8;
+        lsd[i] = timeLsd();
+        if (lsd[i] < bestLsd) bestLsd = lsd[i], bb = g_digitBits;
+    }
+    g_digitBits = bb;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     AT_HIP(hipStreamSynchronize(s));
@@ -2265,7 +2598,10 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
            ", \"sort_prefetch\": " + std::to_string(bp) +
            ", \"sort_ranges\": " + std::to_string(br) + ", \"sort_digit_array\": " + std::to_string(bd) +
            ", \"sort_msd\": " + std::to_string(bm) +
-           ", \"keys\": " + std::to_string(n) + ", \"sort_ms\": " + std::to_string(best) + "}";
+           ", \"keys\": " + std::to_string(n) + ", \"sort_ms\": " + std::to_string(best) +
+           ", \"sort_digit_bits\": " + std::to_string(bb) + ", \"digit_bits_keys\": " + std::to_string(m) +
+           ", \"lsd_ms_8_bits\": " + std::to_string(lsd[0]) + ", \"lsd_ms_11_bits\": " + std::to_string(lsd[1]) +
+           "}";
 }
 
 // ---- SegmentedSort -------------------------------------------------------------

@@ -38,9 +38,14 @@ void setSortPrefetch(int on);
 int sortMsd();
 void setSortMsd(int on);
 std::size_t sortMsdMinKeys();
+// Digit width of the LSD passes of sorts below sortMsdMinKeys() keys: 8
+// (8 passes) or 11 (6 passes of 2048 digits). Results are identical.
+int sortDigitBits();
+void setSortDigitBits(int bits);
 
 // Times every tile shape, then range count, then histogram source on a
-// stable sort of n random 63-bit (key, index) pairs and keeps the fastest
+// stable sort of n random 63-bit (key, index) pairs, then the LSD digit
+// width on a sort below sortMsdMinKeys() keys, and keeps the fastest
 // settings (process-wide). Returns them as a JSON object.
 std::string autotuneSort(std::size_t n, hipStream_t s);
 
@@ -109,6 +114,10 @@ struct SortWorkspace
     std::uint32_t *rangeHist = nullptr;  // [kRadix][rows] digit counts per range (or range part)
     std::uint32_t *rangeOff = nullptr;   // [kRadix][ranges] exclusive offset of each range within its digit
     std::uint32_t *digitTotal = nullptr; // [kRadix]
+    // 11-bit LSD passes (workspaces of fewer than 2^20 keys): [2048][wideRanges]
+    // histograms and offsets, digit totals [2048]
+    std::uint32_t *wideHist = nullptr, *wideOff = nullptr, *wideTotal = nullptr;
+    std::size_t wideRanges = 0;
     std::size_t capacity = 0;
     std::size_t maxRanges = 0;
     bool values = false;

@@ -2,14 +2,15 @@
 
 Every setting gives bit-identical trees; only the sort's schedule changes
 (tile shape, key ranges per pass, histogram source, MSD levels or LSD
-passes for large sorts).
+passes for large sorts, digit width of the LSD passes of smaller sorts).
 """
 import ctypes
 import json
 
 from . import _ffi
 
-KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch", "sort_msd")
+KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch", "sort_msd",
+        "sort_digit_bits")
 
 
 def get():

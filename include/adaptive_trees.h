@@ -279,14 +279,19 @@ int at_server_get_stats(at_server *server, at_server_stats *out);
  * "sort_msd":         1 = sorts of at least 2^20 (key, value) pairs
  *                     partition by the high digits first and finish buckets
  *                     of up to 4096 keys in LDS, 0 = eight LSD passes.
+ * "sort_digit_bits":  digit width of the LSD passes of sorts below 2^20
+ *                     keys: 8 (eight passes) or 11 (six passes of 2048
+ *                     digits).
  * A tile (sort_threads x sort_items) holds at most 8192 keys; a larger one
  * is rejected when a sort is planned. */
 int at_tune_set(const char *key, int value);
 int at_tune_get(const char *key, int *value);
 /* Autotuner: times the tile shapes, then the range counts, then the
- * histogram source on a sort of n random keys (on `stream`), keeps the
- * fastest settings and writes them as JSON into report (may be NULL). Every
- * setting gives identical results; only the schedule changes. */
+ * histogram source on a sort of n random keys (on `stream`), then the LSD
+ * digit width on a sort of min(n, 2^20 - 1) keys; keeps the fastest
+ * settings and writes them as JSON into report (may be NULL). Every setting
+ * gives identical results; only the schedule changes. Replaces the fixed
+ * digit width and tile shape of CUB's SortPairs at src/octree/octree.cu:301-304. */
 int at_tune_auto(size_t n, void *stream, char *report, size_t report_len);
 
 /* ---- device radix sort ---------------------------------------------------

@@ -120,3 +120,36 @@ def test_sort_pairs_rejects_null_pointers():
     lib = _ffi.hip_lib()
     with pytest.raises(_ffi.InvalidArgument):
         lib.call("sort_pairs", None, None, 10, 1, None)
+
+
+@pytest.fixture
+def wide_digits():
+    saved = tune.get()
+    tune.set(sort_digit_bits=11)
+    yield
+    tune.set(**saved)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["uniform", "invalid_mix", "few", "dup_runs", "clustered", "low_bits"])
+@pytest.mark.parametrize("n", [1, 1000, 2047, 2048, 2049, 300_001, (1 << 20) - 1])
+def test_wide_digit_lsd_matches_stable_argsort(wide_digits, kind, n):
+    """Six passes of 11-bit digits (sorts below 2^20 keys), slot values."""
+    print("wide sort", kind, n, flush=True)
+    rng = np.random.default_rng(zlib.crc32(("wide %s %d" % (kind, n)).encode()))
+    keys = keys_of(kind, n, rng)
+    k, v = gpu_sort(keys)
+    order = np.argsort(keys, kind="stable").astype(np.uint32)
+    assert np.array_equal(v, order), "%s n=%d: permutation differs at %s" % (kind, n, np.nonzero(v != order)[0][:4])
+    assert np.array_equal(k, keys[order])
+
+
+@pytest.mark.gpu
+def test_wide_digit_lsd_carries_values(wide_digits):
+    rng = np.random.default_rng(92)
+    n = 777_777
+    keys = keys_of("dup_runs", n, rng)
+    vals = rng.integers(0, 2**32, n, dtype=np.uint32)
+    k, v = gpu_sort(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])

@@ -60,7 +60,7 @@ def test_tunings_are_bit_exact(tuning, cfg):
 def test_invalid_tunings_are_rejected():
     lib = _ffi.hip_lib()
     for k, v in (("sort_threads", 300), ("sort_items", 7), ("sort_ranges", 1), ("sort_digit_array", 2),
-                 ("sort_prefetch", 3), ("no_such_key", 1)):
+                 ("sort_prefetch", 3), ("sort_digit_bits", 9), ("no_such_key", 1)):
         with pytest.raises(_ffi.InvalidArgument):
             lib.call("tune_set", k.encode(), v)
 
@@ -84,6 +84,8 @@ def test_autotuner_picks_a_valid_setting(tuning):
     assert r["sort_items"] in (8, 12, 16)
     assert r["sort_ranges"] in (2048, 4096, 8192, 16384, 32768)
     assert r["sort_digit_array"] in (0, 1)
+    assert r["sort_digit_bits"] in (8, 11) and r["digit_bits_keys"] == (1 << 20) - 1
+    assert r["lsd_ms_8_bits"] > 0 and r["lsd_ms_11_bits"] > 0
     assert r["sort_ms"] > 0
     assert {k: r[k] for k in tune.KEYS} == tune.get()
     # the picked setting is bit-exact as well
@@ -92,3 +94,26 @@ def test_autotuner_picks_a_valid_setting(tuning):
     p.build()
     p.compare("autotuned")
     tune.set(**saved)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bits", [8, 11])
+def test_digit_width_trees_bit_exact(bits):
+    """Trees whose sorts take the LSD passes (below 2^20 keys) are the
+    oracle's at either digit width: build, update, views."""
+    from adaptive_trees_amd import tune
+    saved = tune.get()
+    tune.set(sort_digit_bits=bits)
+    try:
+        print("digit bits", bits, flush=True)
+        x, y, z = plummer(250_000, seed=74)
+        x[::89], y[::89], z[::89] = 0.125, 0.5, 0.875
+        for kind, real, bucket, sfc in ((OCTREE, np.float32, 64, HILBERT), (OCTREE, np.float64, 16, MORTON),
+                                        (BINARY, np.float32, 32, MORTON)):
+            p = Pair(x, y, z, kind=kind, real=real, max_particles_per_leaf=bucket, sfc_kind=sfc, views=(1, 1, 1))
+            p.build()
+            p.compare("build bits=%d kind=%d" % (bits, kind))
+            p.update()
+            p.compare("update bits=%d kind=%d" % (bits, kind))
+    finally:
+        tune.set(**saved)
