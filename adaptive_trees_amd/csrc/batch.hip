@@ -123,7 +123,7 @@ public:
             trees_.emplace_back(new Member(x[k], y[k], z[k], counts[k], c, maxDepth, iterations, &slot));
         }
         coords_.ensure(T ? T : 1, 0, stream_);
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         if (T) AT_HIP(hipMemcpy(coords_.p, coords.data(), T * sizeof(SegCoords<Real>), hipMemcpyHostToDevice));
         kp_ = KeygenParams<Real>{};
         kp_.grid = impl::makeGridBox<Real>(c.box);
@@ -136,7 +136,7 @@ public:
 
     ~OctreeBatch() override
     {
-        (void)hipStreamSynchronize(stream_);
+        (void)syncStreamStatus(stream_);
         trees_.clear();
         if (st_) (void)hipFree(st_);
         if (sth_) (void)hipHostFree(sth_);
@@ -144,7 +144,7 @@ public:
 
     void build() override
     {
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         built_ = false;
         const std::size_t T = trees_.size();
         sort_.keygenAndSort<Real>(stream_, kind_, kp_, coords_.p, P_.p, perm_.p, false);
@@ -193,7 +193,7 @@ public:
             AT_HIP(hipMemcpyAsync(logHost_.data(), passLog_.p, T * logStride * sizeof(std::uint32_t),
                                   hipMemcpyDeviceToHost, stream_));
         }
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         std::vector<std::size_t> stalled;
         std::vector<unsigned> decided(T, 0);
         for (std::size_t k = 0; k < T; ++k)
@@ -219,7 +219,7 @@ public:
         for (auto &t : trees_)
             if (t->configuredViews().any()) t->buildViews(t->configuredViews());
         // completes the buffers' growth copies and the uploads above
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
     // Host-driven lock-step passes for `members` (their device states with
@@ -310,7 +310,7 @@ public:
                 trees_[k]->growForResize();
                 AT_HIP(hipMemsetAsync(&st_[k].needsResize, 0, sizeof(int), stream_));
             }
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         refreshUpdateTables();
     }
 
@@ -372,7 +372,7 @@ private:
     {
         if (!trees_.empty())
             AT_HIP(hipMemcpyAsync(sth_, st_, trees_.size() * sizeof(DevState), hipMemcpyDeviceToHost, stream_));
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
     hipStream_t stream_ = nullptr;

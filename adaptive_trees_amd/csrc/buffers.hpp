@@ -40,12 +40,19 @@ struct VecShape
 // pool (its release threshold raised once per process), in size classes.
 // Allocating and freeing never synchronises the device (hipFree does),
 // which matters when trees are created and released while others compute.
-// A free on the null stream is for owners that have synchronised their own
-// stream first: the block is idle and is kept for the next allocation of
-// its size class without a runtime call. Other frees are ordered on their
-// stream.
+//   poolFree(p, s)  p may still be used by work queued on s: it is kept
+//                   pending and becomes reusable when the library next
+//                   synchronises s (syncStream);
+//   poolRelease(p)  the owner has synchronised every stream that used p:
+//                   reusable at once, by the next allocation of its size
+//                   class, without a runtime call.
 void *poolAlloc(std::size_t bytes, hipStream_t s);
 void poolFree(void *p, hipStream_t s);
+void poolRelease(void *p);
+// hipStreamSynchronize, after which the blocks freed on s before the call
+// are reusable. syncStream throws on a HIP error; syncStreamStatus returns it.
+void syncStream(hipStream_t s);
+hipError_t syncStreamStatus(hipStream_t s);
 // A word of pinned, coherent host memory (device stores are visible to a
 // polling host), from slabs kept until the process ends.
 std::uint32_t *takePinnedWord();
@@ -60,10 +67,11 @@ struct DevBuf
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
+    // the owner has synchronised its stream (destruction)
     ~DevBuf() { release(); }
     void release()
     {
-        poolFree(p, nullptr);
+        poolRelease(p);
         p = nullptr;
         cap = 0;
     }
@@ -74,7 +82,7 @@ struct DevBuf
         T *q = static_cast<T *>(poolAlloc((n ? n : 1) * sizeof(T), s));
         if (p && keep)
             AT_HIP(hipMemcpyAsync(q, p, (keep < cap ? keep : cap) * sizeof(T), hipMemcpyDeviceToDevice, s));
-        poolFree(p, s); // ordered after the copy and every earlier use on s
+        poolFree(p, s); // reusable once s has run past the copy and every earlier use
         p = q;
         cap = n;
     }

@@ -28,16 +28,20 @@ std::once_flag g_poolOnce[64];
 
 // Idle device blocks kept for reuse. Blocks come from the device's
 // stream-ordered pool in size classes (8 per power of two, at most 12.5%
-// over the request); a block released by an owner that has synchronised
-// its stream (poolFree with a null stream) is idle and goes to this cache,
-// up to kIdleCap bytes per device, and the next request of its class on
-// that device takes it without a runtime call. Trees are created and
-// destroyed by the thousand in a serving loop: hipMallocAsync costs a few
+// over the request) and stay owned by this cache:
+//   idle      free for the next request of their class on their device,
+//             taken without a runtime call (up to kIdleCap bytes per device);
+//   pending   freed in the order of a stream whose work may still use them:
+//             they become idle when the library next synchronises that
+//             stream (syncStream), having taken the list before the wait.
+// Trees are created and destroyed by the thousand in a serving loop and
+// grow buffers pass by pass in a build: hipMallocAsync costs a few
 // microseconds and hipFreeAsync tens (it may trim the pool), per buffer.
 class BlockCache
 {
 public:
     static constexpr std::size_t kIdleCap = std::size_t(16) << 30;
+    static constexpr std::size_t kPendingCap = std::size_t(4) << 30; // per stream; beyond: hipFreeAsync
     static std::size_t classOf(std::size_t bytes)
     {
         if (bytes <= 256) return 256;
@@ -60,14 +64,58 @@ public:
         std::lock_guard<std::mutex> g(m_);
         owned_[p] = {dev, cls};
     }
-    // true: kept idle; false: the caller frees it (forgotten here)
-    bool give(void *p, bool idle)
+    // The owner has synchronised: true = kept idle; false = the caller frees
+    // it (forgotten here).
+    bool release(void *p)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        return idleLocked(p);
+    }
+    // Freed in stream s's order: true = pending until s is synchronised;
+    // false = the caller frees it on s (forgotten here).
+    bool defer(void *p, hipStream_t s)
     {
         std::lock_guard<std::mutex> g(m_);
         auto it = owned_.find(p);
         if (it == owned_.end()) return false;
+        Pending &q = pending_[s];
+        if (q.bytes + it->second.second > kPendingCap)
+        {
+            owned_.erase(it);
+            return false;
+        }
+        q.blocks.push_back(p);
+        q.bytes += it->second.second;
+        return true;
+    }
+    std::vector<void *> takePending(hipStream_t s)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        auto it = pending_.find(s);
+        if (it == pending_.end()) return {};
+        std::vector<void *> out;
+        out.swap(it->second.blocks);
+        pending_.erase(it);
+        return out;
+    }
+    // Blocks whose stream has been synchronised: idle; those beyond the idle
+    // cap are returned for the caller to free.
+    std::vector<void *> settle(const std::vector<void *> &blocks)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        std::vector<void *> drop;
+        for (void *p : blocks)
+            if (!idleLocked(p)) drop.push_back(p);
+        return drop;
+    }
+
+private:
+    bool idleLocked(void *p)
+    {
+        auto it = owned_.find(p);
+        if (it == owned_.end()) return false;
         const auto [dev, cls] = it->second;
-        if (idle && idleBytes_[dev & 63] + cls <= kIdleCap)
+        if (idleBytes_[dev & 63] + cls <= kIdleCap)
         {
             idle_[{dev, cls}].push_back(p);
             idleBytes_[dev & 63] += cls;
@@ -76,11 +124,15 @@ public:
         owned_.erase(it);
         return false;
     }
-
-private:
+    struct Pending
+    {
+        std::vector<void *> blocks;
+        std::size_t bytes = 0;
+    };
     std::mutex m_;
     std::map<void *, std::pair<int, std::size_t>> owned_;
     std::map<std::pair<int, std::size_t>, std::vector<void *>> idle_;
+    std::map<hipStream_t, Pending> pending_;
     std::size_t idleBytes_[64] = {};
 };
 BlockCache &blockCache()
@@ -113,10 +165,36 @@ void *poolAlloc(std::size_t bytes, hipStream_t s)
 void poolFree(void *p, hipStream_t s)
 {
     if (!p) return;
-    // a null stream: the owner has synchronised, the block is idle
-    if (blockCache().give(p, s == nullptr)) return;
+    if (blockCache().defer(p, s)) return;
     (void)hipFreeAsync(p, s);
 }
+
+void poolRelease(void *p)
+{
+    if (!p) return;
+    if (blockCache().release(p)) return;
+    (void)hipFreeAsync(p, nullptr); // idle: no stream uses it
+}
+
+namespace
+{
+hipError_t syncAndSettle(hipStream_t s)
+{
+    std::vector<void *> blocks = blockCache().takePending(s);
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+    {
+        // the stream's work may not have finished: its blocks are freed in its order
+        for (void *p : blocks) (void)hipFreeAsync(p, s);
+        return e;
+    }
+    for (void *p : blockCache().settle(blocks)) (void)hipFreeAsync(p, s);
+    return hipSuccess;
+}
+} // namespace
+
+void syncStream(hipStream_t s) { AT_HIP(syncAndSettle(s)); }
+hipError_t syncStreamStatus(hipStream_t s) { return syncAndSettle(s); }
 
 namespace
 {
@@ -215,16 +293,16 @@ TreeBase::TreeBase(hipStream_t s, DevState *st, DevState *sth) : stream_(s)
 
 TreeBase::~TreeBase()
 {
-    (void)hipStreamSynchronize(stream_);
+    (void)syncStreamStatus(stream_);
     if (!ownsState_) return;
-    poolFree(st_, nullptr);
+    poolRelease(st_);
     pinnedStates().give(sth_);
 }
 
 void TreeBase::readState()
 {
     AT_HIP(hipMemcpyAsync(sth_, st_, sizeof(DevState), hipMemcpyDeviceToHost, stream_));
-    AT_HIP(hipStreamSynchronize(stream_));
+    syncStream(stream_);
 }
 
 using impl::makeTree;
@@ -382,7 +460,7 @@ int at_tree_update_passes(at_tree *t, uint32_t *passes)
         at::TreeBase &tree = treeOf(t);
         at::DevState h{};
         AT_HIP(hipMemcpyAsync(&h, tree.deviceState(), sizeof(h), hipMemcpyDeviceToHost, tree.stream()));
-        AT_HIP(hipStreamSynchronize(tree.stream()));
+        at::syncStream(tree.stream());
         *passes = h.passes;
     });
 }
@@ -412,7 +490,7 @@ int at_tree_leaf_count_violations(at_tree *t, uint32_t bucket, uint64_t *overful
                             sc.tree_type == AT_TREE_BINARY ? 1 : 3, counts.p);
         unsigned long long h[2] = {0, 0};
         AT_HIP(hipMemcpyAsync(h, counts.p, sizeof(h), hipMemcpyDeviceToHost, s));
-        AT_HIP(hipStreamSynchronize(s));
+        at::syncStream(s);
         *overfull = h[0];
         *mergeable = h[1];
     });
@@ -454,8 +532,14 @@ int at_stream_create(void **s)
     *s = h;
     return r;
 }
-int at_stream_destroy(void *s) { return hipStatus(hipStreamDestroy(hipStream_t(s))); }
-int at_stream_synchronize(void *s) { return hipStatus(hipStreamSynchronize(hipStream_t(s))); }
+int at_stream_destroy(void *s)
+{
+    // blocks still pending on the stream become reusable before it goes
+    const hipError_t e = at::syncStreamStatus(hipStream_t(s));
+    if (e != hipSuccess) return hipStatus(e);
+    return hipStatus(hipStreamDestroy(hipStream_t(s)));
+}
+int at_stream_synchronize(void *s) { return hipStatus(at::syncStreamStatus(hipStream_t(s))); }
 int at_device_synchronize(void) { return hipStatus(hipDeviceSynchronize()); }
 int at_event_create(void **e)
 {
@@ -525,7 +609,7 @@ int at_sort_pairs(void *keys, void *vals, size_t n, int slot_values, void *strea
         at::SortWorkspace ws;
         at::sortPairs(hipStream_t(stream), static_cast<at::Key *>(keys), static_cast<std::uint32_t *>(vals), n, ws,
                       slot_values != 0);
-        AT_HIP(hipStreamSynchronize(hipStream_t(stream))); // before the workspace is released
+        at::syncStream(hipStream_t(stream)); // before the workspace is released
     });
 }
 int at_sort_pairs_noalloc(void *keys, void *vals, size_t n, int slot_values, void *stream)

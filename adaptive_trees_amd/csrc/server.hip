@@ -18,8 +18,16 @@
 //              while others are on the GPU overlaps on the device.
 //   graphs     with update_graphs set, a tree's update() is captured
 //              (thread-local capture on its worker's stream) at its first
-//              update and again at the first update after a resize, and
-//              replayed as one launch.
+//              update and again at the first update after a resize, one
+//              capture at a time in the process, and replayed as one
+//              launch. While any stream is being captured, HIP fails
+//              operations on the legacy default stream in every thread of
+//              the process (hipErrorStreamCaptureImplicit), so captures
+//              run only inside requests and only when the caller asks.
+//   release    a tree with nothing queued or running is released at once:
+//              the request completes at submission and the worker frees
+//              the tree afterwards (its bytes return to the budget then);
+//              otherwise the release runs after the tree's requests.
 // Every request runs the same tree operations it would run alone, on a tree
 // that only its own requests touch, so results do not depend on the mix.
 #include "engine.hpp"
@@ -44,6 +52,7 @@ struct Request
 {
     std::uint64_t id = 0, tree = 0;
     Kind kind = kBuild;
+    bool internal = false; // the deferred destruction of a tree released at submission
     // build
     int realBytes = 8, type = AT_TREE_OCTREE;
     const void *x = nullptr, *y = nullptr, *z = nullptr;
@@ -79,8 +88,11 @@ struct GraphExec
 };
 
 // Records tree.update() on stream s; null when the capture is refused.
+// Captures and instantiations of all servers run one at a time.
+std::mutex g_captureMu;
 hipGraphExec_t captureUpdate(TreeBase &tree, hipStream_t s)
 {
+    std::lock_guard<std::mutex> lock(g_captureMu);
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     bool ok = true;
     try
@@ -106,6 +118,7 @@ struct Resident
     int worker = -1;
     bool admitted = false;
     bool releasing = false; // a release is queued: no further requests
+    int outstanding = 0;    // its requests queued on or running at its worker
     bool failed = false;    // the build failed: no tree, bytes returned
     std::size_t bytes = 0, n = 0;
     int realBytes = 8;
@@ -168,7 +181,11 @@ public:
             doneCv_.wait(g, [&] { return waiters_ == 0; });
         }
         trees_.clear();
-        for (auto &wk : workers_) (void)hipStreamDestroy(wk.stream);
+        for (auto &wk : workers_)
+        {
+            (void)syncStreamStatus(wk.stream); // blocks pending on it become reusable
+            (void)hipStreamDestroy(wk.stream);
+        }
         for (auto &kv : requests_) delete kv.second;
     }
 
@@ -217,6 +234,19 @@ public:
         t.releasing = true;
         Request *r = newRequest(kRelease);
         r->tree = tree;
+        if (t.admitted && t.outstanding == 0)
+        {
+            // nothing of the tree is queued or running: done now, freed by its worker
+            r->state = kDone;
+            r->start = r->done = r->submit;
+            ++completed_;
+            Request *d = newRequest(kRelease);
+            d->tree = tree;
+            d->internal = true;
+            enqueue(t.worker, d, 0);
+            doneCv_.notify_all();
+            return r->id;
+        }
         route(t, r);
         return r->id;
     }
@@ -320,6 +350,7 @@ private:
     {
         workers_[std::size_t(w)].queue.push_back(r);
         workers_[std::size_t(w)].work += n;
+        ++trees_.at(r->tree).outstanding;
         cv_.notify_all();
     }
 
@@ -454,7 +485,7 @@ private:
                     catch (const std::exception &e) { fail(i, AT_ENOMEM, e.what()); }
                 }
                 // an asynchronous failure belongs to any launched member
-                const hipError_t synced = hipStreamSynchronize(wk.stream);
+                const hipError_t synced = syncStreamStatus(wk.stream);
                 if (synced != hipSuccess)
                 {
                     const std::string what = std::string("update batch failed: ") + hipGetErrorString(synced);
@@ -488,6 +519,8 @@ private:
             {
                 std::lock_guard<std::mutex> g(m_);
                 const auto now = Clock::now();
+                for (std::size_t i = 0; i < batch.size(); ++i)
+                    if (batch[i]->kind != kRelease) --res[i]->outstanding;
                 if (head->kind == kBuild && status[0] == AT_OK) res[0]->tree = std::move(built);
                 else if (head->kind == kBuild)
                 {
@@ -517,7 +550,7 @@ private:
                     batch[i]->error = errors[i];
                     batch[i]->state = kDone;
                     batch[i]->done = now;
-                    ++completed_;
+                    if (!batch[i]->internal) ++completed_;
                 }
                 admit();
             }

@@ -2094,7 +2094,7 @@ void msdLevels(hipStream_t s, SortWorkspace &ws, std::size_t n, std::uint8_t *di
                    std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(20))
             {
             }
-            if (*flag == kCountPending) AT_HIP(hipStreamSynchronize(s));
+            if (*flag == kCountPending) syncStream(s);
             if (*flag == kCountPending) throw Error(4, "sort: a level count never arrived");
             if (*flag == 0) return;
             from = &ws.msdCnt->pad[0];
@@ -2242,7 +2242,7 @@ SortPlan makeSortPlan(std::size_t n)
 void SortWorkspace::allocate(std::size_t cap, bool withValues, hipStream_t s)
 {
     if (cap <= capacity && (!withValues || values)) return;
-    release(s);
+    releaseOn(s);
     capacity = cap < 1 ? 1 : cap;
     // the smallest tile gives the largest range count for any tuning
     maxRanges = (capacity + 8 * kSortThreads - 1) / (8 * kSortThreads);
@@ -2296,34 +2296,47 @@ void SortWorkspace::allocateMsd(std::size_t cap, std::size_t parents, hipStream_
     msdChunks = static_cast<MsdChunk *>(alloc(msdMaxChunks * sizeof(MsdChunk)));
 }
 
-void SortWorkspace::release(hipStream_t s)
+// Frees every buffer through `give` (stream-ordered or after a
+// synchronisation); the pinned count word is kept across regrowth (the host
+// may poll it until the stream's work ends) and returned at destruction.
+template <class Give>
+void releaseWith(SortWorkspace &w, Give give, bool final)
 {
-    if (borrowed)
+    if (w.borrowed)
     {
-        keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
-        digitTotal = nullptr; wideHist = wideOff = wideTotal = nullptr; wideRanges = 0;
-        capacity = 0; maxRanges = 0; values = false; borrowed = false;
+        w.keysAlt = nullptr; w.valsAlt = nullptr; w.digits = nullptr; w.rangeHist = nullptr; w.rangeOff = nullptr;
+        w.digitTotal = nullptr; w.wideHist = w.wideOff = w.wideTotal = nullptr; w.wideRanges = 0;
+        w.capacity = 0; w.maxRanges = 0; w.values = false; w.borrowed = false;
         return;
     }
-    // ordered on s; s = null: the owner has synchronised (destruction)
-    for (void *p : {(void *)keysAlt, (void *)valsAlt, (void *)digits, (void *)rangeHist, (void *)rangeOff,
-                    (void *)digitTotal, (void *)wideHist, (void *)wideOff, (void *)wideTotal})
-        poolFree(p, s);
-    keysAlt = nullptr; valsAlt = nullptr; digits = nullptr; rangeHist = nullptr; rangeOff = nullptr;
-    digitTotal = nullptr; wideHist = wideOff = wideTotal = nullptr; wideRanges = 0;
-    capacity = 0; maxRanges = 0; values = false;
-    for (void *p : {(void *)msdCnt, (void *)msdRoot, (void *)msdSegs[0], (void *)msdSegs[1], (void *)msdRanges[0],
-                    (void *)msdRanges[1], (void *)msdFirst[0], (void *)msdFirst[1], (void *)msdHist, (void *)msdOff,
-                    (void *)msdSubTotals, (void *)msdChunks})
-        poolFree(p, s);
-    msdCnt = nullptr; msdRoot = nullptr; msdSegs[0] = msdSegs[1] = nullptr; msdRanges[0] = msdRanges[1] = nullptr;
-    msdFirst[0] = msdFirst[1] = nullptr; msdHist = msdOff = msdSubTotals = nullptr; msdChunks = nullptr;
-    msdCapacity = msdMaxSegs = msdStride = msdMaxChunks = msdParents = 0;
-    if (!s)
+    for (void *p : {(void *)w.keysAlt, (void *)w.valsAlt, (void *)w.digits, (void *)w.rangeHist, (void *)w.rangeOff,
+                    (void *)w.digitTotal, (void *)w.wideHist, (void *)w.wideOff, (void *)w.wideTotal})
+        give(p);
+    w.keysAlt = nullptr; w.valsAlt = nullptr; w.digits = nullptr; w.rangeHist = nullptr; w.rangeOff = nullptr;
+    w.digitTotal = nullptr; w.wideHist = w.wideOff = w.wideTotal = nullptr; w.wideRanges = 0;
+    w.capacity = 0; w.maxRanges = 0; w.values = false;
+    for (void *p : {(void *)w.msdCnt, (void *)w.msdRoot, (void *)w.msdSegs[0], (void *)w.msdSegs[1],
+                    (void *)w.msdRanges[0], (void *)w.msdRanges[1], (void *)w.msdFirst[0], (void *)w.msdFirst[1],
+                    (void *)w.msdHist, (void *)w.msdOff, (void *)w.msdSubTotals, (void *)w.msdChunks})
+        give(p);
+    w.msdCnt = nullptr; w.msdRoot = nullptr; w.msdSegs[0] = w.msdSegs[1] = nullptr;
+    w.msdRanges[0] = w.msdRanges[1] = nullptr; w.msdFirst[0] = w.msdFirst[1] = nullptr;
+    w.msdHist = w.msdOff = w.msdSubTotals = nullptr; w.msdChunks = nullptr;
+    w.msdCapacity = w.msdMaxSegs = w.msdStride = w.msdMaxChunks = w.msdParents = 0;
+    if (final)
     {
-        givePinnedWord(msdHostCount); // kept across regrowth: the host may poll it until the stream ends
-        msdHostCount = nullptr;
+        givePinnedWord(w.msdHostCount);
+        w.msdHostCount = nullptr;
     }
+}
+
+void SortWorkspace::release()
+{
+    releaseWith(*this, [](void *p) { poolRelease(p); }, true);
+}
+void SortWorkspace::releaseOn(hipStream_t s)
+{
+    releaseWith(*this, [s](void *p) { poolFree(p, s); }, false);
 }
 
 template <class T>
@@ -2593,7 +2606,7 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
     g_digitBits = bb;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    AT_HIP(hipStreamSynchronize(s));
+    syncStream(s);
     return "{\"sort_threads\": " + std::to_string(bt) + ", \"sort_items\": " + std::to_string(bi) +
            ", \"sort_prefetch\": " + std::to_string(bp) +
            ", \"sort_ranges\": " + std::to_string(br) + ", \"sort_digit_array\": " + std::to_string(bd) +
@@ -2648,7 +2661,7 @@ void SegmentedSort::plan(const std::vector<std::size_t> &counts, hipStream_t s)
         for (std::size_t g = 0; g < segments_; ++g)
             parents.push_back(MsdSeg{std::uint64_t(segBegin_[g]), std::uint32_t(counts[g]), 0, 0, 0});
     }
-    AT_HIP(hipStreamSynchronize(s));
+    syncStream(s);
     if (ranges_) AT_HIP(hipMemcpy(range_.p, rs.data(), ranges_ * sizeof(SegRange), hipMemcpyHostToDevice));
     AT_HIP(hipMemcpy(segRange_.p, first.data(), first.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice));
     AT_HIP(hipMemcpy(segBeginD_.p, begins.data(), begins.size() * sizeof(std::uint64_t), hipMemcpyHostToDevice));

@@ -139,9 +139,11 @@ struct SortWorkspace
     // MSD lists for sorts of up to cap keys whose level 1 has `parents`
     // parent buckets besides the whole array (segmented sorts)
     void allocateMsd(std::size_t cap, std::size_t parents, hipStream_t s);
-    // Frees ordered on s; s = null when the owner has synchronised its
-    // stream (the destructor's case).
-    void release(hipStream_t s = nullptr);
+    // release(): the owner has synchronised its stream (destruction);
+    // releaseOn(s): frees in stream s's order (regrowth while s may still
+    // use the buffers).
+    void release();
+    void releaseOn(hipStream_t s);
     ~SortWorkspace() { release(); }
 };
 

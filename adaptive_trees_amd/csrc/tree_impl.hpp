@@ -156,15 +156,15 @@ public:
         idx_.ensure(indexSize(n) + 1, 0, stream_);
         vK_.resize(2);
         ensurePhysical(8, 0);
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
-    // members free their memory on the null stream: finish this tree's work first
-    ~OctreeImpl() override { (void)hipStreamSynchronize(stream_); }
+    // members release their memory as idle blocks: finish this tree's work first
+    ~OctreeImpl() override { (void)syncStreamStatus(stream_); }
 
     void build() override
     {
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         computeParticleKeys();
         buildBegin();
         TreeDev t = dev();
@@ -331,7 +331,7 @@ public:
         (void)needsResize();
         prepareGraphBuffers(growBy25(leafCapacity_), hostLeaves_);
         AT_HIP(hipMemsetAsync(&st_->needsResize, 0, sizeof(int), stream_));
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
     void buffer(int id, const void **d, std::size_t *s, std::size_t *c) override
@@ -519,13 +519,13 @@ private:
         vTmp_.resize(leaves + 1);
         view_.syncSizes(leaves, numActive_);
         hostLeaves_ = leaves;
-        if (sync) AT_HIP(hipStreamSynchronize(stream_));
+        if (sync) syncStream(stream_);
     }
 
     void finishBuildState(std::size_t L) // finishBuildStateKernel (octree.cu:40-47)
     {
         AT_HIP(hipMemcpyAsync(sth_, st_, sizeof(DevState), hipMemcpyDeviceToHost, stream_));
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         DevState s = *sth_;
         s.numLeaves = s.newNumLeaves = int(L);
         s.changed = s.converged = s.needsResize = 0;
@@ -533,7 +533,7 @@ private:
         s.activeBuffer = 0;
         *sth_ = s;
         AT_HIP(hipMemcpyAsync(st_, sth_, sizeof(DevState), hipMemcpyHostToDevice, stream_));
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
     void buildTreeView(const ViewFlags &v) // octree.cu:309-318
@@ -596,14 +596,14 @@ public:
         sort_.allocate(n ? n : 1, true, stream_);
         idx_.ensure(indexSize(n) + 1, 0, stream_);
         tileFirst_.ensure(n / kBoundTile + 2, 0, stream_);
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
-    ~BinaryImpl() override { (void)hipStreamSynchronize(stream_); }
+    ~BinaryImpl() override { (void)syncStreamStatus(stream_); }
 
     void build() override
     {
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         built_ = false;
         hostLeaves_ = 1;
         prepareGraphBuffers(std::max<std::size_t>(2, leafCapacity_));
@@ -652,7 +652,7 @@ public:
         if (!built_) throw Error(AT_ELOGIC, "adaptive_octree: call build() before resizeBuffers()");
         syncState();
         growBuffers();
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
     void buffer(int id, const void **d, std::size_t *s, std::size_t *c) override
@@ -689,7 +689,7 @@ public:
     unsigned maxAchievedDepth() override // kdtree3d.cu:404-415
     {
         if (views_.any()) return view_.maxAchievedDepth(stream_);
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
         std::vector<Key> k(hostLeaves_ + 1);
         AT_HIP(hipMemcpy(k.data(), K_[activeBuffer_].p, k.size() * sizeof(Key), hipMemcpyDeviceToHost));
         unsigned d = 0;
@@ -831,7 +831,7 @@ private:
         ticket_.ensure(1, 0, stream_);
         if (built_) prepareViews();
         syncSizes(hostLeaves_);
-        AT_HIP(hipStreamSynchronize(stream_));
+        syncStream(stream_);
     }
 
     void growBuffers()

@@ -435,7 +435,7 @@ ViewEngine::ViewEngine(unsigned levelBits) : impl_(new Impl), bits_(levelBits), 
 ViewEngine::~ViewEngine()
 {
     delete impl_;
-    poolFree(depth_, nullptr);
+    poolRelease(depth_);
 }
 
 void ViewEngine::prepare(std::size_t leafCapacity, std::size_t numParticles, bool structure, bool counts,
@@ -570,7 +570,7 @@ unsigned ViewEngine::maxAchievedDepth(hipStream_t s) const
 {
     if (!depth_) return 0;
     std::uint8_t v = 0;
-    AT_HIP(hipStreamSynchronize(s));
+    syncStream(s);
     AT_HIP(hipMemcpy(&v, depth_, 1, hipMemcpyDeviceToHost));
     return v;
 }

@@ -53,7 +53,11 @@ class ServerTree:
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes, workers=4, max_batch=16, update_graphs=True):
+    def __init__(self, hbm_budget_bytes, workers=4, max_batch=16, update_graphs=False):
+        """update_graphs: capture each tree's update() at its first update and
+        replay it as one launch. While a capture runs, HIP fails operations on
+        the legacy default stream in every thread of the process, so enable it
+        only where no thread uses that stream while requests are in flight."""
         self.lib = _ffi.hip_lib()
         cfg = _ffi.ServerConfig(int(hbm_budget_bytes), int(workers), int(max_batch), 1 if update_graphs else 0, 0)
         h = ctypes.c_void_p()

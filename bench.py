@@ -757,7 +757,8 @@ def serving_benchmark(rates, requests, seed=5, workers=4):
         progress("serving rate %d/s done" % rate)
     return {"workload": "Poisson trace, %d requests per rate, sizes log-uniform [1e4, 8e6], 70%% octree / 30%% "
                         "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; %d worker streams, "
-                        "200 GB budget; serial = plain Tree calls in arrival order on one thread"
+                        "200 GB budget, updates launched directly (no graph captures); serial = plain Tree calls "
+                        "in arrival order on one thread"
                         % (requests, workers),
             "rates": results}
 

@@ -237,7 +237,10 @@ typedef struct at_server_config {
     uint64_t hbm_budget_bytes;
     int32_t workers;       /* worker threads, one HIP stream each (default 4) */
     int32_t max_batch;     /* updates launched per synchronisation (default 16) */
-    int32_t update_graphs; /* nonzero: replay each tree's update from a HIP graph */
+    int32_t update_graphs; /* nonzero: replay each tree's update from a HIP graph,
+                              captured at its first update; while a capture
+                              runs, HIP fails legacy-default-stream operations
+                              in every thread of the process */
     int32_t reserved0;
 } at_server_config;
 typedef struct at_server_stats {
