@@ -140,12 +140,14 @@ BlockCache &blockCache()
     static BlockCache *c = new BlockCache; // never destroyed: blocks stay valid at exit
     return *c;
 }
-} // namespace
 
-void *poolAlloc(std::size_t bytes, hipStream_t s)
+// Idle blocks beyond the cache's cap are freed on a non-blocking stream of
+// their device (never the legacy default stream: while any stream of the
+// process is being captured, HIP fails legacy-stream work in every thread).
+hipStream_t g_reclaim[64] = {};
+
+void setupDevice(int dev)
 {
-    int dev = 0;
-    AT_HIP(hipGetDevice(&dev));
     std::call_once(g_poolOnce[dev & 63], [dev] {
         hipMemPool_t pool = nullptr;
         if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool)
@@ -153,7 +155,22 @@ void *poolAlloc(std::size_t bytes, hipStream_t s)
             std::uint64_t threshold = ~std::uint64_t(0);
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
         }
+        (void)hipStreamCreateWithFlags(&g_reclaim[dev & 63], hipStreamNonBlocking);
     });
+}
+hipStream_t reclaimStream(void *p)
+{
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) return nullptr;
+    return g_reclaim[a.device & 63];
+}
+} // namespace
+
+void *poolAlloc(std::size_t bytes, hipStream_t s)
+{
+    int dev = 0;
+    AT_HIP(hipGetDevice(&dev));
+    setupDevice(dev);
     const std::size_t cls = BlockCache::classOf(bytes ? bytes : 1);
     if (void *p = blockCache().take(dev, cls)) return p;
     void *p = nullptr;
@@ -173,7 +190,7 @@ void poolRelease(void *p)
 {
     if (!p) return;
     if (blockCache().release(p)) return;
-    (void)hipFreeAsync(p, nullptr); // idle: no stream uses it
+    (void)hipFreeAsync(p, reclaimStream(p)); // idle: no stream uses it
 }
 
 namespace
@@ -188,7 +205,7 @@ hipError_t syncAndSettle(hipStream_t s)
         for (void *p : blocks) (void)hipFreeAsync(p, s);
         return e;
     }
-    for (void *p : blockCache().settle(blocks)) (void)hipFreeAsync(p, s);
+    for (void *p : blockCache().settle(blocks)) (void)hipFreeAsync(p, reclaimStream(p)); // idle now
     return hipSuccess;
 }
 } // namespace
