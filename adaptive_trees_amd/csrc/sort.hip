@@ -1116,16 +1116,33 @@ __global__ __launch_bounds__(kSortThreads) void segScanKernel(const std::uint32_
     __shared__ std::uint32_t waveTotals[kWaves];
     const unsigned d = threadIdx.x;
     const std::uint32_t r0 = segRange[blockIdx.x], r1 = segRange[blockIdx.x + 1];
+    // eight ranges' counts in flight at a time (a member of 2M keys has ~500 ranges)
+    constexpr std::uint32_t kAhead = 8;
     std::uint32_t run = 0;
-    for (std::uint32_t r = r0; r < r1; ++r)
+    for (std::uint32_t r = r0; r < r1; r += kAhead)
     {
-        const std::uint32_t c = hist[std::size_t(r) * kRadix + d];
-        off[std::size_t(r) * kRadix + d] = run;
-        run += c;
+        std::uint32_t c[kAhead];
+#pragma unroll
+        for (std::uint32_t u = 0; u < kAhead; ++u) c[u] = r + u < r1 ? hist[std::size_t(r + u) * kRadix + d] : 0u;
+#pragma unroll
+        for (std::uint32_t u = 0; u < kAhead; ++u)
+            if (r + u < r1)
+            {
+                off[std::size_t(r + u) * kRadix + d] = run;
+                run += c[u];
+            }
     }
     if (segTotals) segTotals[std::size_t(blockIdx.x) * kRadix + d] = run; // the segment's count of digit d
     const std::uint32_t base = std::uint32_t(segBegin[blockIdx.x]) + blockExclusiveScan(run, waveTotals);
-    for (std::uint32_t r = r0; r < r1; ++r) off[std::size_t(r) * kRadix + d] += base;
+    for (std::uint32_t r = r0; r < r1; r += kAhead)
+    {
+        std::uint32_t c[kAhead];
+#pragma unroll
+        for (std::uint32_t u = 0; u < kAhead; ++u) c[u] = r + u < r1 ? off[std::size_t(r + u) * kRadix + d] : 0u;
+#pragma unroll
+        for (std::uint32_t u = 0; u < kAhead; ++u)
+            if (r + u < r1) off[std::size_t(r + u) * kRadix + d] = c[u] + base;
+    }
 }
 
 // ---- MSD sort ----------------------------------------------------------------

@@ -1406,9 +1406,21 @@ __global__ __launch_bounds__(kThreads) void multiOctDecideScanKernel(TreeSet s, 
 {
     AT_MULTI_LOOP(multiOctDecideBody(t, bucket, nf));
 }
+// Leaf-shaped multi-tree launches: a virtual block covers kLeafGroup rows of
+// kThreads leaves of one tree (the member lookup and the TreeDev load are
+// paid once per 2048 leaves).
+constexpr unsigned kLeafGroup = 8;
+__device__ void multiProcessGroup(const TreeDev &t, int nf, unsigned block)
+{
+    for (unsigned r = 0; r < kLeafGroup; ++r) octProcessBody(t, nf != 0, block * kLeafGroup + r);
+}
+__device__ void multiLeafCountsGroup(const TreeDev &t, unsigned block, int full)
+{
+    for (unsigned r = 0; r < kLeafGroup; ++r) leafCountsBody(t, block * kLeafGroup + r, full);
+}
 __global__ __launch_bounds__(kThreads) void multiOctProcessKernel(TreeSet s, int nf)
 {
-    AT_MULTI_LOOP(octProcessBody(t, nf != 0, block));
+    AT_MULTI_LOOP(multiProcessGroup(t, nf, block));
 }
 __global__ void multiCommitKernel(TreeSet s, int binary)
 {
@@ -1417,11 +1429,11 @@ __global__ void multiCommitKernel(TreeSet s, int binary)
 }
 __global__ __launch_bounds__(kThreads) void multiLeafCountsKernel(TreeSet s, int full)
 {
-    AT_MULTI_LOOP(leafCountsBody(t, block, full));
+    AT_MULTI_LOOP(multiLeafCountsGroup(t, block, full));
 }
 __global__ __launch_bounds__(kThreads) void multiNfCollectKernel(TreeSet s, int full)
 {
-    AT_MULTI_LOOP(nfCollectBody<1>(t, full, block));
+    AT_MULTI_LOOP(nfCollectBody<kLeafGroup>(t, full, block));
 }
 __global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsigned bucket, int hilbert)
 {
@@ -1437,7 +1449,8 @@ __global__ __launch_bounds__(kThreads) void multiNfGroupKernel(TreeSet s, int hi
 // tileBlocksFor below).
 __device__ inline std::uint32_t dLeafBlocks(std::uint32_t leaves)
 {
-    const std::uint32_t b = (leaves + kThreads - 1) / kThreads;
+    constexpr std::uint32_t kLeaves = kThreads * kLeafGroup;
+    const std::uint32_t b = (leaves + kLeaves - 1) / kLeaves;
     return b < 1 ? 1u : b;
 }
 __device__ inline std::uint32_t dNfBlocks(std::uint32_t leaves)
@@ -1542,7 +1555,7 @@ void multiBuildSearchIndex(hipStream_t s, const TreeSet &index)
 unsigned tileBlocksFor(std::size_t gridLeaves) { return unsigned(gridLeaves / kPassTile + 1); }
 unsigned leafBlocksFor(std::size_t gridLeaves)
 {
-    const unsigned b = ceilDiv(gridLeaves, kThreads);
+    const unsigned b = ceilDiv(gridLeaves, kThreads * kLeafGroup);
     return b < 1 ? 1 : b;
 }
 unsigned nfBlocksFor(std::size_t gridLeaves)

@@ -61,7 +61,7 @@ TreeSet plannedSet(const TreeDev *trees, int count, const std::uint32_t *start, 
 // Per-tree block counts of each multi-tree launch shape.
 unsigned clearBlocksFor(std::size_t statusWords); // status-word clearing (init, reset, commit)
 unsigned tileBlocksFor(std::size_t gridLeaves);   // decision + chained scan tiles
-unsigned leafBlocksFor(std::size_t gridLeaves);   // one thread per leaf
+unsigned leafBlocksFor(std::size_t gridLeaves);   // one thread per leaf, 8 rows of 256 leaves per block
 unsigned nfBlocksFor(std::size_t gridLeaves);     // near-field work groups
 
 constexpr int kPassTile = 2048;                  // leaves per chained-scan tile
