@@ -34,7 +34,7 @@ namespace at
 namespace
 {
 // Launch shapes of one TreeSet table.
-enum Shape { kClear = 0, kTiles, kLeaves, kNf, kIndex, kShapes };
+enum Shape { kClear = 0, kTiles, kLeaves, kNf, kIndex, kTileFirst, kTilesOfP, kShapes };
 
 // Device + pinned host staging of a TreeDev table and its block maps.
 struct SetTables
@@ -273,7 +273,12 @@ public:
         sort_.keygenAndSort<Real>(stream_, kind_, kp_, coords_.p, P_.p, perm_.p);
         multiBuildSearchIndex(stream_, updSet_[kIndex]);
         multiReset(stream_, updSet_[kClear]);
-        multiOctRefresh(stream_, updSet_[kLeaves], updSet_[kNf], params_, true);
+        // every member recounts all leaves: by tiles of P for small LeafCount
+        // leaves (the members' own update() does the same)
+        if (!params_.nf && params_.bucket <= kTileCountBucket)
+            multiLeafCountsByTiles(stream_, updSet_[kTileFirst], updSet_[kTilesOfP], updSet_[kLeaves]);
+        else
+            multiOctRefresh(stream_, updSet_[kLeaves], updSet_[kNf], params_, true);
         for (unsigned i = 0; i < iterations_; ++i)
         {
             multiOctDecideScan(stream_, updSet_[kTiles], params_);
@@ -331,6 +336,8 @@ private:
         case kTiles: return tileBlocksFor(leaves);
         case kLeaves: return leafBlocksFor(leaves);
         case kIndex: return indexBlocksFor(counts_[k]);
+        case kTileFirst: return tileFirstBlocksFor(counts_[k]);
+        case kTilesOfP: return tileBlocksOfP(counts_[k]);
         default: return params_.nf ? nfBlocksFor(leaves) : 0u;
         }
     }

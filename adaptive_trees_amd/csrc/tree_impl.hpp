@@ -154,6 +154,7 @@ public:
             permp_ = perm_.p;
         }
         idx_.ensure(indexSize(n) + 1, 0, stream_);
+        if (!nf_) tileFirst_.ensure(n / kBoundTile + 2, 0, stream_);
         vK_.resize(2);
         ensurePhysical(8, 0);
         syncStream(stream_);
@@ -249,6 +250,11 @@ public:
     }
     // buildAfterDecide + buildAfterProcess of one committed device pass, on
     // the mirrored capacities only.
+    // Large LeafCount trees with small leaves recount by tiles of P (nearly
+    // every line of P would be searched anyway: 16M Plummer particles, bucket
+    // 64: 0.84 vs 0.90 ms per update); below 4M particles the three launches
+    // cost more than the searches. Near-field trees refresh N with NF.
+    bool fullCountsByTiles() const { return !nf_ && bucket_ <= kTileCountBucket && n_ >= (std::size_t(1) << 22); }
     void replayBuildPass(std::size_t newLeaves)
     {
         buildNewL_ = newLeaves;
@@ -306,7 +312,9 @@ public:
         const TreeDev t = dev();
         const OctParams p = params();
         updateReset(stream_, t);
-        octRefresh(stream_, t, p, true, leafCapacity_);
+        // the particles moved: count every leaf (small leaves: one read of P)
+        if (fullCountsByTiles()) leafCountsByTiles(stream_, t);
+        else octRefresh(stream_, t, p, true, leafCapacity_);
         for (unsigned i = 0; i < iterations_; ++i) octPass(stream_, t, p, long(leafCapacity_), leafCapacity_);
         buildTreeView(views_);
         AT_HIP(hipGetLastError());
@@ -412,6 +420,7 @@ private:
         t.st = st_;
         t.P = Pp_;
         t.Pidx = idx_.p;
+        t.tileFirst = nf_ ? nullptr : tileFirst_.p;
         t.n = n_;
         t.capacity = long(leafCapacity_);
         return t;
@@ -561,6 +570,7 @@ private:
     DevBuf<Key> P_;
     DevBuf<std::uint32_t> perm_;
     DevBuf<Key> idx_;               // sampled search index of P
+    DevBuf<std::uint32_t> tileFirst_; // LeafCount: per tile of P, the first leaf boundary it owns (full counts)
     Key *Pp_ = nullptr;             // own P_ or the batch's segment
     std::uint32_t *permp_ = nullptr;
     DevBuf<Key> K_[2];

@@ -1346,6 +1346,68 @@ unsigned clearBlocks(std::size_t words)
     return b < 1 ? 1 : (b > 1024 ? 1024 : b);
 }
 
+// Tile b (kBoundTile keys of P) owns the leaf boundaries i with
+// P[b * kBoundTile - 1] < K[i] <= P[(b + 1) * kBoundTile - 1]; tileFirst[b]
+// is its first. Entries b = block * kThreads + lane, striding by blocks.
+__device__ void tileFirstBody(const TreeDev &t, unsigned block, unsigned blocks)
+{
+    const DevState *st = t.st;
+    if (st->converged || t.n == 0) return;
+    const std::size_t tiles = (t.n + kBoundTile - 1) / kBoundTile;
+    const std::uint32_t L = std::uint32_t(st->numLeaves);
+    const Key *K = t.K[st->activeBuffer];
+    for (std::size_t b = std::size_t(block) * kThreads + threadIdx.x; b <= tiles; b += std::size_t(blocks) * kThreads)
+        t.tileFirst[b] = b == 0 ? 0u
+                         : b == tiles ? L + 1
+                                      : std::uint32_t(upperBound(K, std::size_t(L) + 1, t.P[b * kBoundTile - 1]));
+}
+__global__ __launch_bounds__(kThreads) void tileFirstKernel(TreeDev t) { tileFirstBody(t, blockIdx.x, gridDim.x); }
+// Tile `tile`: its keys in LDS, then the lower bound of each boundary it owns
+// (into t.offs, read as 32-bit words).
+__device__ void tileBoundsBody(const TreeDev &t, unsigned tile)
+{
+    __shared__ Key tp[kBoundTile];
+    const DevState *st = t.st;
+    if (st->converged) return;
+    const Key *K = t.K[st->activeBuffer];
+    const std::size_t s = std::size_t(tile) * kBoundTile;
+    const std::size_t len = t.n - s < kBoundTile ? t.n - s : kBoundTile;
+    {
+        constexpr int kPer = kBoundTile / kThreads;
+        Key v[kPer]; // all loads in flight before the first LDS store
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            const std::size_t i = threadIdx.x + std::size_t(k) * kThreads;
+            v[k] = i < len ? t.P[s + i] : Key(0);
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+        {
+            const std::size_t i = threadIdx.x + std::size_t(k) * kThreads;
+            if (i < len) tp[i] = v[k];
+        }
+    }
+    __syncthreads();
+    std::uint32_t *lb = reinterpret_cast<std::uint32_t *>(t.offs);
+    const std::uint32_t i1 = t.tileFirst[tile + 1];
+    for (std::uint32_t i = t.tileFirst[tile] + threadIdx.x; i < i1; i += kThreads)
+        lb[i] = std::uint32_t(s + lowerBound(tp, len, K[i]));
+}
+__global__ __launch_bounds__(kThreads) void tileBoundsKernel(TreeDev t) { tileBoundsBody(t, blockIdx.x); }
+// Leaves i = block * kThreads + lane, striding by blocks: N[i] from the bounds.
+__device__ void tileCountsBody(const TreeDev &t, unsigned block, unsigned blocks)
+{
+    const DevState *st = t.st;
+    if (st->converged) return;
+    const int L = st->numLeaves;
+    const std::uint32_t *lb = reinterpret_cast<const std::uint32_t *>(t.offs);
+    std::uint32_t *N = t.N[st->activeBuffer];
+    for (int i = int(block * kThreads + threadIdx.x); i < L; i += int(blocks * kThreads))
+        N[i] = t.n == 0 ? 0u : lb[i + 1] - lb[i];
+}
+__global__ __launch_bounds__(kThreads) void tileCountsKernel(TreeDev t) { tileCountsBody(t, blockIdx.x, gridDim.x); }
+
 // ---- multi-tree kernels: block -> (tree, block within the tree) ------------------
 __device__ inline int treeOfBlock(const TreeSet &s, unsigned b)
 {
@@ -1442,6 +1504,37 @@ __global__ __launch_bounds__(kThreads) void multiNfComputeKernel(TreeSet s, unsi
 __global__ __launch_bounds__(kThreads) void multiNfGroupKernel(TreeSet s, int hilbert, unsigned bucket)
 {
     AT_MULTI_LOOP(nfGroupBody(t, hilbert, bucket, block, blocks));
+}
+// Full leaf counts by tiles of P (small leaves): kTileFirst blocks of
+// kThreads first-boundary searches, one block per tile, then leaf-shaped counts.
+__global__ __launch_bounds__(kThreads) void multiTileFirstKernel(TreeSet s)
+{
+    AT_MULTI_LOOP(tileFirstBody(t, block, blocks));
+}
+// kTileGroup tiles per virtual block (the member lookup paid once per group)
+constexpr unsigned kTileGroup = 8;
+__device__ void multiTileBoundsGroup(const TreeDev &t, unsigned block)
+{
+    const unsigned tiles = unsigned((t.n + kBoundTile - 1) / kBoundTile);
+    for (unsigned r = 0; r < kTileGroup; ++r)
+    {
+        const unsigned tile = block * kTileGroup + r;
+        if (tile >= tiles) break; // block-uniform
+        tileBoundsBody(t, tile);
+        __syncthreads(); // the tile's LDS copy is rewritten by the next one
+    }
+}
+__global__ __launch_bounds__(kThreads) void multiTileBoundsKernel(TreeSet s)
+{
+    AT_MULTI_LOOP(multiTileBoundsGroup(t, block));
+}
+__device__ void multiTileCountsGroup(const TreeDev &t, unsigned block, unsigned blocks)
+{
+    tileCountsBody(t, block, blocks);
+}
+__global__ __launch_bounds__(kThreads) void multiTileCountsKernel(TreeSet s)
+{
+    AT_MULTI_LOOP(multiTileCountsGroup(t, block, blocks));
 }
 #undef AT_MULTI_LOOP
 
@@ -1615,6 +1708,20 @@ void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, co
     }
 }
 
+unsigned tileFirstBlocksFor(std::size_t n)
+{
+    return n ? ceilDiv((n + kBoundTile - 1) / kBoundTile + 1, kThreads) : 0u;
+}
+unsigned tileBlocksOfP(std::size_t n) { return unsigned(((n + kBoundTile - 1) / kBoundTile + kTileGroup - 1) / kTileGroup); }
+void multiLeafCountsByTiles(hipStream_t s, const TreeSet &first, const TreeSet &tiles, const TreeSet &leaves)
+{
+    prof::Scope scope("leaf_counts", s);
+    if (multiGrid(first)) hipLaunchKernelGGL(multiTileFirstKernel, dim3(multiGrid(first)), dim3(kThreads), 0, s, first);
+    if (multiGrid(tiles)) hipLaunchKernelGGL(multiTileBoundsKernel, dim3(multiGrid(tiles)), dim3(kThreads), 0, s, tiles);
+    if (multiGrid(leaves))
+        hipLaunchKernelGGL(multiTileCountsKernel, dim3(multiGrid(leaves)), dim3(kThreads), 0, s, leaves);
+}
+
 void multiPlan(hipStream_t s, const TreeDev *trees, int count, int phase, bool nf, std::uint32_t *start)
 {
     hipLaunchKernelGGL(multiPlanKernel, dim3(1), dim3(kPlanThreads), 0, s, trees, count, phase, nf ? 1 : 0, start);
@@ -1700,65 +1807,15 @@ void binProcessCommit(hipStream_t s, const TreeDev &t, std::size_t gridLeaves)
 // holds its lower bound in P: tile b owns the boundaries with
 // P[bT - 1] < K[i] <= P[(b+1)T - 1] (the last tile also those above every
 // key), i.e. tileFirst[b] <= i < tileFirst[b + 1].
-__global__ __launch_bounds__(kThreads) void tileFirstKernel(TreeDev t, std::size_t tiles)
-{
-    const DevState *st = t.st;
-    if (st->converged) return;
-    const std::uint32_t L = std::uint32_t(st->numLeaves);
-    const Key *K = t.K[st->activeBuffer];
-    for (std::size_t b = std::size_t(blockIdx.x) * kThreads + threadIdx.x; b <= tiles;
-         b += std::size_t(gridDim.x) * kThreads)
-        t.tileFirst[b] = b == 0 ? 0u
-                         : b == tiles ? L + 1
-                                      : std::uint32_t(upperBound(K, std::size_t(L) + 1, t.P[b * kBoundTile - 1]));
-}
-// Block b: the tile in LDS, then the lower bound of each boundary it owns.
-__global__ __launch_bounds__(kThreads) void tileBoundsKernel(TreeDev t)
-{
-    __shared__ Key tp[kBoundTile];
-    const DevState *st = t.st;
-    if (st->converged) return;
-    const Key *K = t.K[st->activeBuffer];
-    const std::size_t s = std::size_t(blockIdx.x) * kBoundTile;
-    const std::size_t len = t.n - s < kBoundTile ? t.n - s : kBoundTile;
-    {
-        constexpr int kPer = kBoundTile / kThreads;
-        Key v[kPer]; // all loads in flight before the first LDS store
-#pragma unroll
-        for (int k = 0; k < kPer; ++k)
-        {
-            const std::size_t i = threadIdx.x + std::size_t(k) * kThreads;
-            v[k] = i < len ? t.P[s + i] : Key(0);
-        }
-#pragma unroll
-        for (int k = 0; k < kPer; ++k)
-        {
-            const std::size_t i = threadIdx.x + std::size_t(k) * kThreads;
-            if (i < len) tp[i] = v[k];
-        }
-    }
-    __syncthreads();
-    std::uint32_t *lb = reinterpret_cast<std::uint32_t *>(t.offs);
-    const std::uint32_t i1 = t.tileFirst[blockIdx.x + 1];
-    for (std::uint32_t i = t.tileFirst[blockIdx.x] + threadIdx.x; i < i1; i += kThreads)
-        lb[i] = std::uint32_t(s + lowerBound(tp, len, K[i]));
-}
-__global__ __launch_bounds__(kThreads) void tileCountsKernel(TreeDev t)
-{
-    const DevState *st = t.st;
-    if (st->converged) return;
-    const int L = st->numLeaves;
-    const std::uint32_t *lb = reinterpret_cast<const std::uint32_t *>(t.offs);
-    std::uint32_t *N = t.N[st->activeBuffer];
-    for (int i = int(blockIdx.x * kThreads + threadIdx.x); i < L; i += int(gridDim.x * kThreads))
-        N[i] = lb[i + 1] - lb[i];
-}
 void leafCountsByTiles(hipStream_t s, const TreeDev &t)
 {
     prof::Scope scope("leaf_counts", s);
     const std::size_t tiles = (t.n + kBoundTile - 1) / kBoundTile;
-    hipLaunchKernelGGL(tileFirstKernel, dim3(ceilDiv(tiles + 1, kThreads)), dim3(kThreads), 0, s, t, tiles);
-    hipLaunchKernelGGL(tileBoundsKernel, dim3(unsigned(tiles)), dim3(kThreads), 0, s, t);
+    if (tiles)
+    {
+        hipLaunchKernelGGL(tileFirstKernel, dim3(ceilDiv(tiles + 1, kThreads)), dim3(kThreads), 0, s, t);
+        hipLaunchKernelGGL(tileBoundsKernel, dim3(unsigned(tiles)), dim3(kThreads), 0, s, t);
+    }
     hipLaunchKernelGGL(tileCountsKernel, dim3(2048), dim3(kThreads), 0, s, t);
 }
 

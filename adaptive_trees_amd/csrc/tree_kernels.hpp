@@ -146,4 +146,11 @@ void multiOctDecideScan(hipStream_t s, const TreeSet &tiles, const OctParams &p)
 void multiOctProcess(hipStream_t s, const TreeSet &leaves, const OctParams &p);
 void multiCommit(hipStream_t s, const TreeSet &clear, bool binary);
 void multiOctRefresh(hipStream_t s, const TreeSet &leaves, const TreeSet &nf, const OctParams &p, bool full);
+// Full leaf counts of LeafCount trees by tiles of P (leafCountsByTiles per
+// member; t.tileFirst and t.offs as scratch): `first` with
+// tileFirstBlocksFor(n) blocks per tree, `tiles` with tileBlocksOfP(n),
+// `leaves` leaf-shaped.
+unsigned tileFirstBlocksFor(std::size_t n);
+unsigned tileBlocksOfP(std::size_t n);
+void multiLeafCountsByTiles(hipStream_t s, const TreeSet &first, const TreeSet &tiles, const TreeSet &leaves);
 } // namespace at
