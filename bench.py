@@ -609,6 +609,41 @@ def batch_benchmark(trees=512, rounds=5, seed=4):
     }
 
 
+def batch_profile(trees=512, rounds=5, seed=4, repeats=3):
+    """The batched path alone: `repeats` times a build() and `rounds` graph-replayed updates of the
+    512-tree workload, each timed on its own (stream synchronised around it); medians over the
+    repeats after the first."""
+    lib = _lib()
+    sizes, first, moved = batch_workload(trees, seed)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    cur = [DeviceArray(a) for a in first]
+    snaps = [[DeviceArray(a) for a in first], [DeviceArray(a) for a in moved]]
+    stream = Stream()
+    batch = Batch(*([cur[a].ptr + 4 * int(off[k]) for k in range(trees)] for a in range(3)), sizes,
+                  make_config(stream=stream.p, max_particles_per_leaf=64), real=np.float32)
+    graph, builds, updates = None, [], []
+    for rep in range(repeats):
+        copy_into(cur, snaps[0], stream)
+        stream.sync()
+        t0 = time.perf_counter()
+        batch.build()
+        stream.sync()
+        builds.append(time.perf_counter() - t0)
+        graph = graph or batch.capture_update(stream.p)
+        for r in range(1, rounds + 1):
+            copy_into(cur, snaps[r % 2], stream)
+            stream.sync()
+            t0 = time.perf_counter()
+            lib.call("graph_launch", graph, stream.p)
+            stream.sync()
+            updates.append(time.perf_counter() - t0)
+            if batch.needsResize().any():
+                raise RuntimeError("batch_profile: a member needs a resize")
+    lib.call("graph_destroy", graph)
+    return {"particles": int(sizes.sum()), "build_ms": round(1e3 * float(np.median(builds[1:])), 3),
+            "update_ms": round(1e3 * float(np.median(updates[rounds:])), 3), "repeats": repeats, "rounds": rounds}
+
+
 # ---- serving -------------------------------------------------------------------------------------------
 def serving_trace(rate, requests, pool, seed):
     """Poisson arrivals at `rate`/s. The first 8 requests build resident trees; afterwards 30%
@@ -880,6 +915,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default sort tuning")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched-vs-plain measurement")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="run only the batched path (build + 5 graph-replayed updates, three times) for a kernel "
+                         "trace, and print its phase times")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the other single-tree workloads")
     ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the serving loop")
@@ -906,6 +944,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     rates = tuple(int(r) for r in args.serve_rates.split(","))
+    if args.batch_profile:
+        print(json.dumps(batch_profile()))
+        return
     if args.only_batch:
         print(json.dumps({"batched": batch_benchmark()}))
         return
