@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libadaptive_trees_hip.so")
 ARCH = os.environ.get("AT_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["sort.hip", "tree_kernels.hip", "views.hip", "engine.hip", "batch.hip", "server.hip", "profile.hip",
-           "io.cpp", "print.cpp"]
+           "io.cpp", "print.cpp", "workloads.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

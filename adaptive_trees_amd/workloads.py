@@ -1,8 +1,10 @@
 """Seeded synthetic particle sets for the benchmark and test workloads.
 
-All generators draw from numpy's PCG64 (np.random.default_rng(seed)) in
-float64 and cast to the requested precision at the end, so the same seed
-gives the same coordinates on every host.
+The generators are the library's at_workload_* (host code): one xoshiro256**
+stream per call, seeded through splitmix64(seed); uniform doubles
+(u >> 11) * 2^-53, Gaussians by Box-Muller in double; coordinates are made in
+float64 and cast to the requested precision at the end (round to nearest).
+The same host arrays feed the GPU trees and the oracle.
 
   uniform_cube   uniform in a box (default [0,1]^3)
   plummer        Plummer sphere, scale radius a, truncated at 20a,
@@ -10,75 +12,78 @@ gives the same coordinates on every host.
   two_clusters   4:1:1 box, 40%/40% isotropic Gaussians at x = 1 and 3
                  (sigma = 0.02 of the long side), 20% uniform, clipped
   batch_sizes    log-uniform tree sizes for batches
+  batch_kinds    one bit per tree (uniform or Plummer)
   reflect_step   Gaussian displacement with reflecting walls
 """
+import ctypes
+
 import numpy as np
+
+from . import _ffi
 
 UNIT_BOX = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
 ANISO_BOX = (0.0, 4.0, 0.0, 1.0, 0.0, 1.0)
 
+_DP = ctypes.POINTER(ctypes.c_double)
 
-def _split(p, dtype):
-    p = np.asarray(p, dtype=np.float64)
-    return tuple(np.ascontiguousarray(p[:, a], dtype=dtype) for a in range(3))
+
+def _lib():
+    return _ffi.hip_lib()
+
+
+def _box(box):
+    return (ctypes.c_double * 6)(*[float(v) for v in box])
+
+
+def _split(xyz, n, dtype):
+    return tuple(np.ascontiguousarray(xyz[a * n:(a + 1) * n], dtype=dtype) for a in range(3))
+
+
+def _call(name, *args):
+    lib = _lib()
+    f = lib.fn(name)
+    lib.check(f(*args))
 
 
 def uniform_cube(n, seed=1, dtype=np.float64, box=UNIT_BOX):
-    rng = np.random.default_rng(seed)
-    lo = np.array(box[0::2])
-    hi = np.array(box[1::2])
-    return _split(lo + rng.random((n, 3)) * (hi - lo), dtype)
+    out = np.empty(3 * n, dtype=np.float64)
+    _call("workload_uniform", ctypes.c_uint64(seed), ctypes.c_size_t(n), _box(box), out.ctypes.data_as(_DP))
+    return _split(out, n, dtype)
 
 
 def plummer(n, seed=2, a=0.05, dtype=np.float32):
-    rng = np.random.default_rng(seed)
-    chunks, have = [], 0
-    while have < n:
-        m = max(n - have, 1024)
-        u = rng.random(m)
-        u = np.where(u <= 0, np.finfo(float).tiny, u)
-        r = a / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
-        r = r[r <= 20 * a]
-        cz = 2 * rng.random(len(r)) - 1
-        ph = 2 * np.pi * rng.random(len(r))
-        s = np.sqrt(1 - cz * cz)
-        chunks.append(np.stack([r * s * np.cos(ph), r * s * np.sin(ph), r * cz], axis=1))
-        have += len(r)
-    p = np.clip(np.concatenate(chunks)[:n] + 0.5, 0.0, 1.0)
-    return _split(p, dtype)
+    out = np.empty(3 * n, dtype=np.float64)
+    _call("workload_plummer", ctypes.c_uint64(seed), ctypes.c_size_t(n), ctypes.c_double(a), out.ctypes.data_as(_DP))
+    return _split(out, n, dtype)
 
 
 def two_clusters(n, seed=3, dtype=np.float32, box=ANISO_BOX):
-    rng = np.random.default_rng(seed)
-    lx = box[1] - box[0]
-    sigma = 0.02 * lx
-    n1 = int(0.4 * n)
-    n2 = int(0.4 * n)
-    n3 = n - n1 - n2
-    c1 = np.array([box[0] + 0.25 * lx, 0.5 * (box[2] + box[3]), 0.5 * (box[4] + box[5])])
-    c2 = np.array([box[0] + 0.75 * lx, 0.5 * (box[2] + box[3]), 0.5 * (box[4] + box[5])])
-    lo = np.array(box[0::2])
-    hi = np.array(box[1::2])
-    p = np.concatenate([c1 + sigma * rng.standard_normal((n1, 3)),
-                        c2 + sigma * rng.standard_normal((n2, 3)),
-                        lo + rng.random((n3, 3)) * (hi - lo)])
-    p = np.clip(p, lo, hi)
-    return _split(p, dtype)
+    out = np.empty(3 * n, dtype=np.float64)
+    _call("workload_two_clusters", ctypes.c_uint64(seed), ctypes.c_size_t(n), _box(box), ctypes.c_double(0.02),
+          out.ctypes.data_as(_DP))
+    return _split(out, n, dtype)
 
 
 def batch_sizes(count, seed=4, lo=1000, hi=2_000_000):
-    rng = np.random.default_rng(seed)
-    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=count))).astype(np.int64)
+    out = np.empty(count, dtype=np.int64)
+    _call("workload_log_uniform_sizes", ctypes.c_uint64(seed), ctypes.c_size_t(count), ctypes.c_double(lo),
+          ctypes.c_double(hi), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return out
+
+
+def batch_kinds(count, seed=4):
+    """One bit per tree (the stream's lowest bit): 1 = Plummer, 0 = uniform."""
+    out = np.empty(count, dtype=np.uint64)
+    _call("workload_bits", ctypes.c_uint64(seed), ctypes.c_size_t(count),
+          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    return (out & np.uint64(1)).astype(np.int64)
 
 
 def reflect_step(coords, sigma, seed, box=UNIT_BOX, dtype=None):
     """x += N(0, (sigma * side)^2) per axis, reflected at the walls, clamped."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for a, c in enumerate(coords):
-        lo, hi = box[2 * a], box[2 * a + 1]
-        v = np.asarray(c, dtype=np.float64) + rng.normal(0.0, sigma * (hi - lo), size=len(c))
-        v = np.where(v < lo, 2 * lo - v, v)
-        v = np.where(v > hi, 2 * hi - v, v)
-        out.append(np.ascontiguousarray(np.clip(v, lo, hi), dtype=dtype or np.asarray(c).dtype))
-    return tuple(out)
+    n = len(coords[0])
+    src = np.concatenate([np.asarray(c, dtype=np.float64) for c in coords]) if n else np.zeros(0)
+    out = np.empty(3 * n, dtype=np.float64)
+    _call("workload_reflect_step", ctypes.c_uint64(seed), ctypes.c_size_t(n), _box(box), ctypes.c_double(sigma),
+          src.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
+    return _split(out, n, dtype or np.asarray(coords[0]).dtype)

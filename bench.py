@@ -489,8 +489,7 @@ def batch_workload(trees, seed):
     """512 octree inputs: sizes log-uniform in [1e3, 2e6], uniform or Plummer per tree by a seed
     bit, FP32, plus one reflected-Gaussian moved snapshot (sigma = 1e-3 of the unit box)."""
     sizes = workloads.batch_sizes(trees, seed=seed)
-    rng = np.random.default_rng(seed)
-    kinds = rng.integers(0, 2, size=trees)
+    kinds = workloads.batch_kinds(trees, seed=seed)
     first, moved = [[], [], []], [[], [], []]
     for k, n in enumerate(sizes):
         gen = workloads.plummer if kinds[k] else workloads.uniform_cube
@@ -676,8 +675,7 @@ def serving_trace(rate, requests, pool, seed):
 def serving_benchmark(rates, requests, seed=5, workers=4):
     """The serving loop (worker streams, 200 GB budget) against serial plain calls on the same
     trace; latency = result ready - arrival, per request."""
-    rng = np.random.default_rng(seed)
-    sizes = np.rint(np.exp(rng.uniform(np.log(1e4), np.log(8e6), size=20))).astype(np.int64)
+    sizes = workloads.batch_sizes(20, seed=seed, lo=10_000, hi=8_000_000)
     pool = []  # per entry: two snapshots, host float64
     for k, n in enumerate(sizes):
         a = (workloads.plummer if k % 2 else workloads.uniform_cube)(int(n), seed=700 + k, dtype=np.float64)

@@ -348,6 +348,27 @@ int at_io_manifest_frame(const void *manifest, size_t i, int *index, long *id, c
                          char *path, size_t path_len);
 void at_io_manifest_close(void *manifest);
 
+/* ---- synthetic workloads (host memory) -------------------------------------
+ * No counterpart in the reference, which reads recorded datasets: the seeded
+ * inputs of the benchmark workloads. One xoshiro256** stream per call, seeded
+ * by splitmix64(seed); uniform doubles (u >> 11) * 2^-53, Gaussians by
+ * Box-Muller in double. Coordinates column-major in xyz[3 n]; box =
+ * {xmin, xmax, ymin, ymax, zmin, zmax}. The draw order is documented in
+ * adaptive_trees_amd/csrc/workloads.cpp. */
+int at_workload_uniform(uint64_t seed, size_t n, const double *box, double *xyz);
+/* Plummer sphere of scale radius a around (0.5, 0.5, 0.5), truncated at 20 a,
+ * clipped to the unit box. */
+int at_workload_plummer(uint64_t seed, size_t n, double a, double *xyz);
+/* 40% / 40% isotropic Gaussians (sigma = sigma_of_long_side * (xmax - xmin))
+ * at 1/4 and 3/4 of x, 20% uniform, clipped to the box. */
+int at_workload_two_clusters(uint64_t seed, size_t n, const double *box, double sigma_of_long_side, double *xyz);
+/* out = in + N(0, (sigma_of_side * side)^2) per axis, reflected at the box
+ * walls and clamped to it. */
+int at_workload_reflect_step(uint64_t seed, size_t n, const double *box, double sigma_of_side, const double *in,
+                             double *out);
+int at_workload_log_uniform_sizes(uint64_t seed, size_t count, double lo, double hi, int64_t *out);
+int at_workload_bits(uint64_t seed, size_t count, uint64_t *out); /* the raw stream */
+
 #ifdef __cplusplus
 }
 #endif

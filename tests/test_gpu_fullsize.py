@@ -190,8 +190,7 @@ def test_c3_binary_32M_two_clusters_graph_updates():
 def test_c4_batch_512_trees():
     trees, rounds, seed = 512, 5, 4
     sizes = workloads.batch_sizes(trees, seed=seed)
-    rng = np.random.default_rng(seed)
-    kinds = rng.integers(0, 2, size=trees)
+    kinds = workloads.batch_kinds(trees, seed=seed)
     snaps = []  # per member: rounds + 1 snapshots (host FP32)
     for k, n in enumerate(sizes):
         gen = workloads.plummer if kinds[k] else workloads.uniform_cube
@@ -294,8 +293,7 @@ def test_c4_batch_512_trees():
 
 
 def test_c5_served_mix_up_to_8M():
-    rng = np.random.default_rng(5)
-    sizes = list(np.rint(np.exp(rng.uniform(np.log(1e4), np.log(8e6), size=9))).astype(int)) + [8_000_000]
+    sizes = [int(v) for v in workloads.batch_sizes(9, seed=5, lo=10_000, hi=8_000_000)] + [8_000_000]
     specs = []
     for k, n in enumerate(sizes):
         kind = OCTREE if k % 10 < 7 else BINARY
