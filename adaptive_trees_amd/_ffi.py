@@ -97,7 +97,7 @@ class ServerConfig(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("bytes_in_use", "bytes_peak", "hbm_budget", "resident_trees",
                                                 "waiting_builds", "completed", "batched_updates",
-                                                "update_batches")]
+                                                "update_batches", "grouped_builds", "grouped_updates")]
 
 
 _ERRORS = {AT_EINVAL: InvalidArgument, AT_ELOGIC: LogicError, AT_ELENGTH: LengthError, AT_EIO: InputError}

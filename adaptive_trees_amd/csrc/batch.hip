@@ -423,7 +423,449 @@ BatchBase *makeBatch(int type, std::size_t T, const void *const *x, const void *
     return new OctreeBatch<Real>(T, reinterpret_cast<const Real *const *>(x), reinterpret_cast<const Real *const *>(y),
                                  reinterpret_cast<const Real *const *>(z), counts, c, md, it);
 }
+
+// ---------------------------------------------------------------------------
+// Groups: members that keep their own buffers
+// ---------------------------------------------------------------------------
+// A member's segment of the group's sorted keys and the copy blocks that
+// write it to the member's P and Perm.
+struct SegOut
+{
+    Key *P;
+    std::uint32_t *perm;
+    std::uint64_t begin;      // first key of the segment in the group arrays
+    std::uint32_t n;
+    std::uint32_t firstBlock; // first copy block of the member
+};
+constexpr int kCopyThreads = 256, kCopyItems = 8, kCopyChunk = kCopyThreads * kCopyItems;
+
+__global__ __launch_bounds__(kCopyThreads) void scatterSegmentsKernel(const SegOut *segs, int count, const Key *P,
+                                                                       const std::uint32_t *perm)
+{
+    __shared__ int member;
+    if (threadIdx.x == 0)
+    {
+        // the last member whose first block is at most this block
+        int lo = 0, hi = count - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (segs[mid].firstBlock <= blockIdx.x) lo = mid;
+            else hi = mid - 1;
+        }
+        member = lo;
+    }
+    __syncthreads();
+    const SegOut s = segs[member];
+    const std::size_t base = std::size_t(blockIdx.x - s.firstBlock) * kCopyChunk;
+#pragma unroll
+    for (int j = 0; j < kCopyItems; ++j)
+    {
+        const std::size_t i = base + std::size_t(j) * kCopyThreads + threadIdx.x;
+        if (i < s.n)
+        {
+            s.P[i] = P[s.begin + i];
+            s.perm[i] = perm[s.begin + i];
+        }
+    }
+}
+
+__global__ void gatherStatesKernel(const TreeDev *trees, int count, DevState *out)
+{
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < count) out[k] = *trees[k].st;
+}
+
+__global__ void scatterStatesKernel(const DevState *in, const TreeDev *trees, int count)
+{
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < count) *trees[k].st = in[k];
+}
+
+// A table written in pinned host memory and copied to the device in stream
+// order. Growing it waits for the stream (kernels may still read it).
+template <class T>
+struct Staged
+{
+    T *d = nullptr, *h = nullptr;
+    std::size_t cap = 0;
+    hipEvent_t copied = nullptr; // after the last copy out of h
+    ~Staged() { release(); }
+    void release()
+    {
+        if (d) (void)hipFree(d);
+        if (h) (void)hipHostFree(h);
+        if (copied) (void)hipEventDestroy(copied);
+        d = h = nullptr;
+        copied = nullptr;
+        cap = 0;
+    }
+    T *host(std::size_t n, hipStream_t s)
+    {
+        if (n > cap)
+        {
+            syncStream(s);
+            release();
+            cap = n;
+            AT_HIP(hipMalloc(&d, n * sizeof(T)));
+            AT_HIP(hipHostMalloc(&h, n * sizeof(T)));
+            AT_HIP(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+        }
+        else if (copied)
+            AT_HIP(hipEventSynchronize(copied));
+        return h;
+    }
+    void upload(hipStream_t s, std::size_t n)
+    {
+        if (!n) return;
+        AT_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+        AT_HIP(hipEventRecord(copied, s));
+    }
+};
+
+// Members of one class larger than this, or groups beyond this many
+// particles, run on their own: launch overhead is what a group saves, and
+// the workspace grows with the group.
+constexpr std::size_t kGroupMaxTree = std::size_t(1) << 22;
+constexpr std::size_t kGroupMaxTotal = std::size_t(1) << 24;
 } // namespace
+
+struct GroupRunner::Impl
+{
+    hipStream_t s = nullptr;
+    SegmentedSort sort;
+    DevBuf<Key> P;
+    DevBuf<std::uint32_t> perm;
+    Staged<SegCoords<float>> coords4;
+    Staged<SegCoords<double>> coords8;
+    Staged<SegOut> outs;
+    SetTables tab;
+    DevBuf<std::uint32_t> planStart, passLog;
+    std::vector<std::uint32_t> logHost;
+    // gathered member states: device [round], pinned host copy
+    DevBuf<DevState> states;
+    DevState *statesH = nullptr;
+    std::size_t statesHCap = 0;
+    struct Pending
+    {
+        int realBytes;
+        std::vector<TreeBase *> members;
+        std::size_t offset; // of the members' states in `states`
+    };
+    std::vector<Pending> pending;
+    std::size_t pendingStates = 0;
+
+    ~Impl()
+    {
+        (void)syncStreamStatus(s);
+        if (statesH) (void)hipHostFree(statesH);
+    }
+
+    DevState *hostStates(std::size_t n)
+    {
+        if (n > statesHCap)
+        {
+            if (statesH) AT_HIP(hipHostFree(statesH));
+            statesH = nullptr;
+            AT_HIP(hipHostMalloc(&statesH, n * sizeof(DevState)));
+            statesHCap = n;
+        }
+        return statesH;
+    }
+    template <class Real>
+    Staged<SegCoords<Real>> &coordTable()
+    {
+        if constexpr (sizeof(Real) == 4) return coords4;
+        else return coords8;
+    }
+
+    template <class Real>
+    std::vector<impl::OctreeImpl<Real> *> cast(const std::vector<TreeBase *> &members) const
+    {
+        using M = impl::OctreeImpl<Real>;
+        std::vector<M *> out;
+        out.reserve(members.size());
+        for (TreeBase *t : members)
+        {
+            M *m = dynamic_cast<M *>(t);
+            if (!m) throw Error(AT_EINVAL, "group members are octrees of one precision");
+            if (m->stream() != s) throw Error(AT_EINVAL, "group members run on the group's stream");
+            if (m->particles() == 0) throw Error(AT_EINVAL, "group members hold particles");
+            if (!out.empty())
+            {
+                const M &a = *out[0];
+                const OctParams p = a.parameters(), q = m->parameters();
+                const GridBox<Real> &g = a.gridBox(), &h = m->gridBox();
+                const bool same = p.bucket == q.bucket && p.nf == q.nf && p.hilbert == q.hilbert &&
+                                  a.depthLimit() == m->depthLimit() && a.updateIterations() == m->updateIterations() &&
+                                  g.xmin == h.xmin && g.ymin == h.ymin && g.zmin == h.zmin && g.mx == h.mx &&
+                                  g.my == h.my && g.mz == h.mz && g.cx == h.cx && g.cy == h.cy && g.cz == h.cz;
+                if (!same) throw Error(AT_EINVAL, "group members share one configuration");
+            }
+            out.push_back(m);
+        }
+        return out;
+    }
+
+    template <class M>
+    std::uint32_t blocksFor(Shape sh, M &m, std::size_t leaves)
+    {
+        switch (sh)
+        {
+        case kClear: return clearBlocksFor(m.device().statusWords);
+        case kTiles: return tileBlocksFor(leaves);
+        case kLeaves: return leafBlocksFor(leaves);
+        case kIndex: return indexBlocksFor(m.particles());
+        case kTileFirst: return tileFirstBlocksFor(m.particles());
+        case kTilesOfP: return tileBlocksOfP(m.particles());
+        default: return m.parameters().nf ? nfBlocksFor(leaves) : 0u;
+        }
+    }
+
+    // The members' keys sorted together, each segment written to its
+    // member's P and Perm.
+    template <class Real>
+    void sortInto(const std::vector<impl::OctreeImpl<Real> *> &m, bool hostPoll)
+    {
+        const std::size_t G = m.size();
+        std::vector<std::size_t> counts(G);
+        for (std::size_t k = 0; k < G; ++k) counts[k] = m[k]->particles();
+        sort.plan(counts, s);
+        const std::size_t total = sort.total();
+        P.ensure(total, 0, s);
+        perm.ensure(total, 0, s);
+        Staged<SegCoords<Real>> &ct = coordTable<Real>();
+        SegCoords<Real> *hc = ct.host(G, s);
+        for (std::size_t k = 0; k < G; ++k) hc[k] = m[k]->coordinates();
+        ct.upload(s, G);
+        KeygenParams<Real> kp{};
+        kp.grid = m[0]->gridBox();
+        const KeyKind kind = m[0]->parameters().hilbert ? KeyKind::Hilbert : KeyKind::Morton;
+        sort.keygenAndSort<Real>(s, kind, kp, ct.d, P.p, perm.p, hostPoll);
+        SegOut *ho = outs.host(G, s);
+        std::uint32_t blocks = 0;
+        for (std::size_t k = 0; k < G; ++k)
+        {
+            ho[k] = SegOut{m[k]->sortedKeys(), m[k]->sortedPerm(), std::uint64_t(sort.segBegin(k)),
+                           std::uint32_t(counts[k]), blocks};
+            blocks += std::uint32_t((counts[k] + kCopyChunk - 1) / kCopyChunk);
+        }
+        outs.upload(s, G);
+        hipLaunchKernelGGL(scatterSegmentsKernel, dim3(blocks), dim3(kCopyThreads), 0, s, outs.d, int(G), P.p,
+                           perm.p);
+        AT_HIP(hipGetLastError());
+    }
+
+    // TreeDev table of the members (adjusted by `adjust`) and block maps
+    // from `leaves(k)`.
+    template <class M, class A, class L>
+    void stage(const std::vector<M *> &m, A adjust, L leaves, TreeSet set[kShapes])
+    {
+        if (m.size() > tab.cap)
+        {
+            syncStream(s); // earlier launches may still read the tables
+            tab.reserve(m.size());
+        }
+        tab.waitWritable();
+        for (std::size_t k = 0; k < m.size(); ++k)
+        {
+            TreeDev t = m[k]->device();
+            adjust(k, t);
+            tab.hTrees[k] = t;
+        }
+        tab.upload(s, m.size(), [&](Shape sh, std::size_t k) { return blocksFor(sh, *m[k], leaves(k)); }, set);
+    }
+
+    void gather(std::size_t G, std::size_t offset)
+    {
+        states.ensure(offset + G, offset, s);
+        hipLaunchKernelGGL(gatherStatesKernel, dim3(unsigned((G + 63) / 64)), dim3(64), 0, s, tab.dTrees, int(G),
+                           states.p + offset);
+        AT_HIP(hipGetLastError());
+    }
+
+    // OctreeBatch::build over members with their own buffers.
+    template <class Real>
+    void build(const std::vector<TreeBase *> &members)
+    {
+        if (!pending.empty()) throw Error(AT_ELOGIC, "group build with updates pending");
+        const auto m = cast<Real>(members);
+        const std::size_t G = m.size();
+        if (!G) return;
+        syncStream(s);
+        sortInto<Real>(m, false);
+        for (auto *t : m)
+        {
+            t->buildBegin();
+            t->reserveDeviceBuild(t->deviceBuildCapacity());
+        }
+        const unsigned maxDepth = m[0]->depthLimit();
+        const OctParams p = m[0]->parameters();
+        const std::size_t logStride = maxDepth + 1;
+        passLog.ensure(G * logStride + 1, 0, s);
+        TreeSet set[kShapes];
+        stage(m, [&](std::size_t k, TreeDev &t) {
+            t.capacity = m[k]->deviceBuildLimit();
+            t.passLog = passLog.p + k * logStride;
+        }, [](std::size_t) { return std::size_t(1); }, set);
+        multiBuildSearchIndex(s, set[kIndex]);
+        multiOctInit(s, set[kClear]);
+        planStart.ensure(std::size_t(kPlanShapes) * (G + 1), 0, s);
+        const TreeDev *dt = tab.dTrees;
+        const int count = int(G);
+        const TreeSet tiles = plannedSet(dt, count, planStart.p, kPlanTiles);
+        const TreeSet proc = plannedSet(dt, count, planStart.p, kPlanProcess);
+        const TreeSet commit = plannedSet(dt, count, planStart.p, kPlanCommit);
+        const TreeSet fresh = plannedSet(dt, count, planStart.p, kPlanLeaves);
+        const TreeSet nfWork = plannedSet(dt, count, planStart.p, kPlanNf);
+        for (unsigned pass = 0; pass < maxDepth; ++pass)
+        {
+            multiPlan(s, dt, count, 0, p.nf, planStart.p);
+            multiOctDecideScan(s, tiles, p);
+            multiPlan(s, dt, count, 1, p.nf, planStart.p);
+            multiOctProcess(s, proc, p);
+            multiCommit(s, commit, false);
+            multiOctRefresh(s, fresh, nfWork, p, false);
+        }
+        // the one read-back: states and the committed passes' leaf counts
+        gather(G, 0);
+        DevState *sh = hostStates(G);
+        logHost.resize(G * logStride);
+        AT_HIP(hipMemcpyAsync(sh, states.p, G * sizeof(DevState), hipMemcpyDeviceToHost, s));
+        AT_HIP(hipMemcpyAsync(logHost.data(), passLog.p, G * logStride * sizeof(std::uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        syncStream(s);
+        std::vector<std::size_t> stalled;
+        std::vector<unsigned> decided(G, 0);
+        for (std::size_t k = 0; k < G; ++k)
+        {
+            const unsigned passes = std::min(sh[k].passes, maxDepth);
+            for (unsigned j = 0; j < passes; ++j) m[k]->replayBuildPass(logHost[k * logStride + j]);
+            decided[k] = passes;
+            if (sh[k].needsResize) stalled.push_back(k);
+        }
+        std::vector<DevState> stalledStates;
+        for (std::size_t k : stalled) stalledStates.push_back(sh[k]);
+        for (std::size_t k = 0; k < G; ++k)
+        {
+            if (sh[k].needsResize) continue;
+            m[k]->buildEnd(false);
+            impl::OctreeImpl<Real>::finishState(sh[k], m[k]->buildLeaves());
+        }
+        AT_HIP(hipMemcpyAsync(states.p, sh, G * sizeof(DevState), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(scatterStatesKernel, dim3(unsigned((G + 63) / 64)), dim3(64), 0, s, states.p, tab.dTrees,
+                           count);
+        AT_HIP(hipGetLastError());
+        for (std::size_t k = 0; k < G; ++k)
+        {
+            if (sh[k].needsResize) continue;
+            if (m[k]->configuredViews().any()) m[k]->buildViews(m[k]->configuredViews());
+            (void)m[k]->adoptState(sh[k]);
+        }
+        // completes the buffers' growth copies and the uploads above
+        syncStream(s);
+        // members whose pass outgrew the reservation finish on their own
+        for (std::size_t i = 0; i < stalled.size(); ++i) m[stalled[i]]->continueBuild(stalledStates[i], decided[stalled[i]]);
+    }
+
+    // OctreeBatch::update over members with their own buffers; the states
+    // are gathered for readBack.
+    template <class Real>
+    void launchUpdate(const std::vector<TreeBase *> &members)
+    {
+        const auto m = cast<Real>(members);
+        const std::size_t G = m.size();
+        if (!G) return;
+        for (auto *t : m)
+            if (!t->isBuilt()) throw Error(AT_ELOGIC, "adaptive_octree: call build() before update()");
+        sortInto<Real>(m, true);
+        TreeSet set[kShapes];
+        stage(m, [](std::size_t, TreeDev &) {}, [&](std::size_t k) { return std::size_t(m[k]->device().capacity); },
+              set);
+        const OctParams p = m[0]->parameters();
+        multiBuildSearchIndex(s, set[kIndex]);
+        multiReset(s, set[kClear]);
+        // every member recounts all leaves: by tiles of P for small LeafCount leaves
+        if (!p.nf && p.bucket <= kTileCountBucket)
+            multiLeafCountsByTiles(s, set[kTileFirst], set[kTilesOfP], set[kLeaves]);
+        else
+            multiOctRefresh(s, set[kLeaves], set[kNf], p, true);
+        for (unsigned i = 0; i < m[0]->updateIterations(); ++i)
+        {
+            multiOctDecideScan(s, set[kTiles], p);
+            multiOctProcess(s, set[kLeaves], p);
+            multiCommit(s, set[kClear], false);
+            multiOctRefresh(s, set[kLeaves], set[kNf], p, false);
+        }
+        for (auto *t : m)
+            if (t->configuredViews().any()) t->buildViews(t->configuredViews());
+        gather(G, pendingStates);
+        pending.push_back(Pending{int(sizeof(Real)), members, pendingStates});
+        pendingStates += G;
+    }
+
+    template <class Real>
+    void finish(const Pending &pd, std::vector<char> &flags)
+    {
+        const auto m = cast<Real>(pd.members);
+        flags.assign(m.size(), 0);
+        for (std::size_t k = 0; k < m.size(); ++k) flags[k] = m[k]->adoptState(statesH[pd.offset + k]) ? 1 : 0;
+    }
+};
+
+GroupRunner::GroupRunner(hipStream_t s) : impl_(new Impl)
+{
+    impl_->s = s;
+}
+
+GroupRunner::~GroupRunner() = default;
+
+bool GroupRunner::groupable(int type, const at_config &c, std::size_t n)
+{
+    return type == AT_TREE_OCTREE && n > 0 && n <= kGroupMaxTree && c.max_particles_per_leaf > 0;
+}
+
+std::size_t GroupRunner::maxParticles() { return kGroupMaxTotal; }
+
+void GroupRunner::build(const std::vector<TreeBase *> &members)
+{
+    if (members.empty()) return;
+    if (dynamic_cast<impl::OctreeImpl<float> *>(members[0])) impl_->build<float>(members);
+    else impl_->build<double>(members);
+}
+
+void GroupRunner::launchUpdate(const std::vector<TreeBase *> &members)
+{
+    if (members.empty()) return;
+    if (dynamic_cast<impl::OctreeImpl<float> *>(members[0])) impl_->launchUpdate<float>(members);
+    else impl_->launchUpdate<double>(members);
+}
+
+void GroupRunner::readBack()
+{
+    Impl &g = *impl_;
+    if (!g.pendingStates) return;
+    DevState *h = g.hostStates(g.pendingStates);
+    AT_HIP(hipMemcpyAsync(h, g.states.p, g.pendingStates * sizeof(DevState), hipMemcpyDeviceToHost, g.s));
+}
+
+void GroupRunner::finishUpdates(std::vector<std::vector<char>> &needsResize)
+{
+    Impl &g = *impl_;
+    needsResize.assign(g.pending.size(), {});
+    for (std::size_t i = 0; i < g.pending.size(); ++i)
+    {
+        if (g.pending[i].realBytes == 4) g.finish<float>(g.pending[i], needsResize[i]);
+        else g.finish<double>(g.pending[i], needsResize[i]);
+    }
+    discardUpdates();
+}
+
+void GroupRunner::discardUpdates()
+{
+    impl_->pending.clear();
+    impl_->pendingStates = 0;
+}
 
 BatchBase::~BatchBase() = default;
 

@@ -8,14 +8,20 @@
 //              owning one HIP stream), the worker holding the fewest
 //              resident particles (then the fewest trees); all requests of
 //              a tree run in submission order on that worker.
-//   batching   a worker takes the next request and, if it is an update, the
-//              updates of other trees queued behind it (up to max_batch):
-//              their new positions are copied in and their update() work is
-//              launched back to back on the stream before one
-//              synchronisation, then each tree's resize protocol
-//              (doBuffersNeedResize / resizeBuffers / update) completes it.
-//              Workers run concurrently, so the work of requests admitted
-//              while others are on the GPU overlaps on the device.
+//   rounds     a worker takes its queued requests in rounds: in queue
+//              order, at most one per tree and none behind a request of its
+//              tree left queued (up to max_batch). The round's builds run
+//              first, then its updates (new positions copied in), launched
+//              on the stream before one synchronisation; then each tree's
+//              resize protocol (doBuffersNeedResize / resizeBuffers /
+//              update) completes it, and the round's releases free their
+//              trees. Workers run concurrently, so rounds of different
+//              workers overlap on the device.
+//   groups     octrees of one configuration (precision, split criterion,
+//              leaf size, key kind, box, limits) of up to 4M particles in a
+//              round share their launches (GroupRunner: one segmented sort,
+//              multi-tree pass kernels; builds with device-driven passes),
+//              in groups of up to 16M particles; the rest run tree by tree.
 //   graphs     with update_graphs set, a tree's update() is captured
 //              (thread-local capture on its worker's stream) at its first
 //              update and again at the first update after a resize, one
@@ -30,14 +36,18 @@
 //              otherwise the release runs after the tree's requests.
 // Every request runs the same tree operations it would run alone, on a tree
 // that only its own requests touch, so results do not depend on the mix.
+#include "batch.hpp"
 #include "engine.hpp"
 
+#include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <map>
 #include <mutex>
 #include <thread>
+#include <tuple>
 
 namespace at
 {
@@ -121,9 +131,32 @@ struct Resident
     int outstanding = 0;    // its requests queued on or running at its worker
     bool failed = false;    // the build failed: no tree, bytes returned
     std::size_t bytes = 0, n = 0;
-    int realBytes = 8;
+    int realBytes = 8, type = AT_TREE_OCTREE;
+    at_config cfg{};
     const void *x = nullptr, *y = nullptr, *z = nullptr;
     std::deque<Request *> held; // requests submitted before the build was admitted
+};
+
+// Trees whose requests may share launches: same precision, type and
+// configuration (the stream aside).
+struct ClassKey
+{
+    int realBytes, type, split, sfc, maxDepth, iterations;
+    unsigned bucket;
+    std::array<std::uint8_t, 3> views;
+    std::array<double, 6> box;
+    explicit ClassKey(int rb, int t, const at_config &c)
+        : realBytes(rb), type(t), split(c.split_criterion), sfc(c.sfc_kind), maxDepth(c.max_depth),
+          iterations(c.update_iterations), bucket(c.max_particles_per_leaf),
+          views{c.views[0], c.views[1], c.views[2]},
+          box{c.box.xmin, c.box.xmax, c.box.ymin, c.box.ymax, c.box.zmin, c.box.zmax}
+    {
+    }
+    bool operator<(const ClassKey &o) const
+    {
+        return std::tie(realBytes, type, split, sfc, maxDepth, iterations, bucket, views, box) <
+               std::tie(o.realBytes, o.type, o.split, o.sfc, o.maxDepth, o.iterations, o.bucket, o.views, o.box);
+    }
 };
 
 std::size_t estimateBytes(std::size_t n, int realBytes, const at_config &c)
@@ -183,6 +216,7 @@ public:
         trees_.clear();
         for (auto &wk : workers_)
         {
+            wk.groups.reset();
             (void)syncStreamStatus(wk.stream); // blocks pending on it become reusable
             (void)hipStreamDestroy(wk.stream);
         }
@@ -208,6 +242,8 @@ public:
         Resident &t = trees_[r->tree];
         t.n = n;
         t.realBytes = realBytes;
+        t.type = type;
+        t.cfg = cfg;
         t.x = x, t.y = y, t.z = z;
         t.bytes = bytes;
         pendingBuilds_.push_back(r);
@@ -300,6 +336,8 @@ public:
         o->completed = completed_;
         o->batched_updates = batchedUpdates_;
         o->update_batches = updateBatches_;
+        o->grouped_builds = groupedBuilds_;
+        o->grouped_updates = groupedUpdates_;
     }
 
 private:
@@ -310,7 +348,9 @@ private:
         std::size_t work = 0;     // queued particles
         std::size_t resident = 0; // particles of the trees pinned here
         std::size_t trees = 0;
+        std::unique_ptr<GroupRunner> groups; // made and used by the worker's thread
     };
+    static constexpr std::size_t kRoundScan = 4096; // queued requests looked at per round
 
     Request *newRequest(Kind k)
     {
@@ -384,33 +424,67 @@ private:
         }
     }
 
+    // The next requests to run together: in queue order, at most one per
+    // tree, and none behind a request of its tree that stays queued (each
+    // tree's requests keep their order); up to max_batch of them.
+    void takeRound(Worker &wk, std::vector<Request *> &round)
+    {
+        std::vector<std::uint64_t> seen; // trees with a request taken or passed over
+        std::size_t scanned = 0;
+        for (auto it = wk.queue.begin();
+             it != wk.queue.end() && int(round.size()) < maxBatch_ && scanned < kRoundScan; ++scanned)
+        {
+            Request *q = *it;
+            if (std::find(seen.begin(), seen.end(), q->tree) != seen.end())
+            {
+                ++it;
+                continue;
+            }
+            seen.push_back(q->tree);
+            round.push_back(q);
+            it = wk.queue.erase(it);
+        }
+    }
+
+    // Members of one class in groups of at most GroupRunner::maxParticles()
+    // particles; a group of one runs on its own.
+    template <class Size>
+    static void formGroups(const std::map<ClassKey, std::vector<std::size_t>> &byClass, Size sizeOf,
+                           std::vector<std::vector<std::size_t>> &groups, std::vector<std::size_t> &alone)
+    {
+        for (const auto &kv : byClass)
+        {
+            std::vector<std::size_t> g;
+            std::size_t total = 0;
+            auto flush = [&] {
+                if (g.size() > 1) groups.push_back(g);
+                else if (g.size() == 1) alone.push_back(g[0]);
+                g.clear();
+                total = 0;
+            };
+            for (std::size_t i : kv.second)
+            {
+                if (total + sizeOf(i) > GroupRunner::maxParticles()) flush();
+                g.push_back(i);
+                total += sizeOf(i);
+            }
+            flush();
+        }
+    }
+
     void loop(int w)
     {
         Worker &wk = workers_[std::size_t(w)];
         for (;;)
         {
             std::vector<Request *> batch;
-            std::vector<TreeBase *> trees;
             std::vector<Resident *> res;
+            std::size_t updates = 0;
             {
                 std::unique_lock<std::mutex> g(m_);
                 cv_.wait(g, [&] { return stopping_ || !wk.queue.empty(); });
                 if (wk.queue.empty()) return; // stopping and drained
-                batch.push_back(wk.queue.front());
-                wk.queue.pop_front();
-                if (batch[0]->kind == kUpdate)
-                {
-                    // later updates of other trees join the batch
-                    for (auto it = wk.queue.begin(); it != wk.queue.end() && int(batch.size()) < maxBatch_;)
-                    {
-                        Request *q = *it;
-                        bool distinct = q->kind == kUpdate;
-                        for (Request *b : batch) distinct = distinct && b->tree != q->tree;
-                        if (!distinct) break; // keep per-tree order: stop at the first conflict
-                        batch.push_back(q);
-                        it = wk.queue.erase(it);
-                    }
-                }
+                takeRound(wk, batch);
                 const auto now = Clock::now();
                 for (Request *r : batch)
                 {
@@ -420,131 +494,225 @@ private:
                     // is queued behind it and refuses later submissions
                     Resident &t = trees_.at(r->tree);
                     res.push_back(&t);
-                    trees.push_back(t.tree.get());
-                    wk.work -= std::min(wk.work, t.n);
+                    wk.work -= std::min(wk.work, r->internal ? std::size_t(0) : t.n);
+                    updates += r->kind == kUpdate ? 1 : 0;
                 }
-                if (batch[0]->kind == kUpdate && batch.size() > 1)
+                if (updates > 1)
                 {
                     ++updateBatches_;
-                    batchedUpdates_ += batch.size();
+                    batchedUpdates_ += updates;
                 }
             }
-            std::vector<int> status(batch.size(), AT_OK);
-            std::vector<std::string> errors(batch.size());
+            const std::size_t R = batch.size();
+            std::vector<int> status(R, AT_OK);
+            std::vector<std::string> errors(R);
             auto fail = [&](std::size_t i, int code, const char *what) {
                 status[i] = code;
                 errors[i] = what;
             };
-            Request *head = batch[0];
-            std::unique_ptr<TreeBase> built;
-            if (head->kind == kBuild)
-            {
+            auto guarded = [&](std::size_t i, auto &&f) {
                 try
                 {
-                    at_config c = head->cfg;
+                    f();
+                    return true;
+                }
+                catch (const Error &e) { fail(i, e.code, e.what()); }
+                catch (const std::exception &e) { fail(i, AT_ENOMEM, e.what()); }
+                return false;
+            };
+            if (!wk.groups) wk.groups.reset(new GroupRunner(wk.stream));
+            GroupRunner &runner = *wk.groups;
+
+            // ---- builds: every tree made, then groups of one class built together
+            std::vector<std::unique_ptr<TreeBase>> built(R);
+            std::map<ClassKey, std::vector<std::size_t>> byClass;
+            std::vector<std::size_t> alone;
+            for (std::size_t i = 0; i < R; ++i)
+            {
+                Request *r = batch[i];
+                if (r->kind != kBuild) continue;
+                const bool made = guarded(i, [&] {
+                    at_config c = r->cfg;
                     c.stream = wk.stream;
-                    built.reset(createTree(head->realBytes, head->type, head->x, head->y, head->z, head->n, c));
-                    built->build();
-                    (void)built->needsResize();
-                }
-                catch (const Error &e) { fail(0, e.code, e.what()); }
-                catch (const std::exception &e) { fail(0, AT_ENOMEM, e.what()); }
+                    built[i].reset(createTree(r->realBytes, r->type, r->x, r->y, r->z, r->n, c));
+                });
+                if (!made) continue;
+                if (!graphs_ && GroupRunner::groupable(r->type, r->cfg, r->n))
+                    byClass[ClassKey(r->realBytes, r->type, r->cfg)].push_back(i);
+                else
+                    alone.push_back(i);
             }
-            else if (head->kind == kRelease)
+            std::vector<std::vector<std::size_t>> groups;
+            formGroups(byClass, [&](std::size_t i) { return batch[i]->n; }, groups, alone);
+            std::size_t groupedBuilds = 0, groupedUpdates = 0;
+            for (const auto &g : groups)
             {
-                // the tree is destroyed under the lock below
+                groupedBuilds += g.size();
+                std::vector<TreeBase *> members;
+                for (std::size_t i : g) members.push_back(built[i].get());
+                bool ok = false;
+                std::string what;
+                int code = AT_OK;
+                try
+                {
+                    runner.build(members);
+                    ok = true;
+                }
+                catch (const Error &e) { code = e.code, what = e.what(); }
+                catch (const std::exception &e) { code = AT_ENOMEM, what = e.what(); }
+                if (!ok)
+                    for (std::size_t i : g) fail(i, code, what.c_str());
             }
-            else
+            for (std::size_t i : alone)
+                (void)guarded(i, [&] {
+                    built[i]->build();
+                    (void)built[i]->needsResize();
+                });
+
+            // ---- updates: new positions copied in, groups and single trees
+            // launched, one synchronisation, then each tree's resize protocol
+            std::vector<bool> launched(R, false);
+            std::vector<char> resize(R, 0);
+            byClass.clear();
+            alone.clear();
+            for (std::size_t i = 0; i < R; ++i)
             {
-                // copy the new positions and launch every update, then sync once
-                std::vector<bool> launched(batch.size(), false);
-                for (std::size_t i = 0; i < batch.size(); ++i)
-                {
-                    Request *r = batch[i];
-                    try
+                Request *r = batch[i];
+                if (r->kind != kUpdate) continue;
+                const bool copied = guarded(i, [&] {
+                    if (!res[i]->tree) throw Error(AT_ELOGIC, "tree failed to build");
+                    const std::size_t bytes = res[i]->n * std::size_t(res[i]->realBytes);
+                    if (r->nx && bytes)
                     {
-                        if (!trees[i]) throw Error(AT_ELOGIC, "tree failed to build");
-                        const std::size_t bytes = res[i]->n * std::size_t(res[i]->realBytes);
-                        if (r->nx && bytes)
-                        {
-                            AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->x), r->nx, bytes,
-                                                  hipMemcpyDeviceToDevice, wk.stream));
-                            AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->y), r->ny, bytes,
-                                                  hipMemcpyDeviceToDevice, wk.stream));
-                            AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->z), r->nz, bytes,
-                                                  hipMemcpyDeviceToDevice, wk.stream));
-                        }
-                        if (graphs_ && !res[i]->graph.g) res[i]->graph.g = captureUpdate(*trees[i], wk.stream);
-                        if (res[i]->graph.g)
-                            AT_HIP(hipGraphLaunch(res[i]->graph.g, wk.stream));
-                        else
-                            trees[i]->update();
-                        launched[i] = true;
+                        AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->x), r->nx, bytes, hipMemcpyDeviceToDevice,
+                                              wk.stream));
+                        AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->y), r->ny, bytes, hipMemcpyDeviceToDevice,
+                                              wk.stream));
+                        AT_HIP(hipMemcpyAsync(const_cast<void *>(res[i]->z), r->nz, bytes, hipMemcpyDeviceToDevice,
+                                              wk.stream));
                     }
-                    catch (const Error &e) { fail(i, e.code, e.what()); }
-                    catch (const std::exception &e) { fail(i, AT_ENOMEM, e.what()); }
-                }
-                // an asynchronous failure belongs to any launched member
-                const hipError_t synced = syncStreamStatus(wk.stream);
-                if (synced != hipSuccess)
-                {
-                    const std::string what = std::string("update batch failed: ") + hipGetErrorString(synced);
-                    for (std::size_t i = 0; i < batch.size(); ++i)
-                        if (launched[i])
-                        {
-                            status[i] = AT_EHIP;
-                            errors[i] = what;
-                            launched[i] = false;
-                        }
-                }
-                for (std::size_t i = 0; i < batch.size(); ++i)
-                {
-                    if (!launched[i]) continue;
-                    try
-                    {
-                        int attempts = 0;
-                        while (trees[i]->needsResize())
-                        {
-                            if (++attempts > 64) throw Error(AT_ELENGTH, "update did not fit after 64 resizes");
-                            res[i]->graph.reset(); // the resize moves the captured buffers: captured anew next time
-                            trees[i]->resizeBuffers();
-                            trees[i]->update();
-                        }
-                    }
-                    catch (const Error &e) { fail(i, e.code, e.what()); }
-                    catch (const std::exception &e) { fail(i, AT_ENOMEM, e.what()); }
-                }
+                });
+                if (!copied) continue;
+                if (!graphs_ && GroupRunner::groupable(res[i]->type, res[i]->cfg, res[i]->n))
+                    byClass[ClassKey(res[i]->realBytes, res[i]->type, res[i]->cfg)].push_back(i);
+                else
+                    alone.push_back(i);
             }
-            std::unique_ptr<TreeBase> dropped;
+            groups.clear();
+            formGroups(byClass, [&](std::size_t i) { return res[i]->n; }, groups, alone);
+            std::vector<std::vector<std::size_t>> launchedGroups;
+            for (const auto &g : groups)
+            {
+                std::vector<TreeBase *> members;
+                for (std::size_t i : g) members.push_back(res[i]->tree.get());
+                std::string what;
+                int code = AT_OK;
+                try
+                {
+                    runner.launchUpdate(members);
+                    groupedUpdates += g.size();
+                    launchedGroups.push_back(g);
+                    for (std::size_t i : g) launched[i] = true;
+                    continue;
+                }
+                catch (const Error &e) { code = e.code, what = e.what(); }
+                catch (const std::exception &e) { code = AT_ENOMEM, what = e.what(); }
+                for (std::size_t i : g) fail(i, code, what.c_str());
+            }
+            for (std::size_t i : alone)
+                launched[i] = guarded(i, [&] {
+                    TreeBase *t = res[i]->tree.get();
+                    if (graphs_ && !res[i]->graph.g) res[i]->graph.g = captureUpdate(*t, wk.stream);
+                    if (res[i]->graph.g) AT_HIP(hipGraphLaunch(res[i]->graph.g, wk.stream));
+                    else t->update();
+                    resize[i] = 1; // asks needsResize after the synchronisation
+                });
+            bool readBack = true;
+            try
+            {
+                runner.readBack();
+            }
+            catch (const std::exception &)
+            {
+                readBack = false;
+            }
+            // an asynchronous failure belongs to any launched request
+            const hipError_t synced = syncStreamStatus(wk.stream);
+            if (synced != hipSuccess || !readBack)
+            {
+                runner.discardUpdates();
+                const std::string what = synced != hipSuccess
+                                             ? std::string("update batch failed: ") + hipGetErrorString(synced)
+                                             : std::string("update batch failed: state read-back");
+                for (std::size_t i = 0; i < R; ++i)
+                    if (launched[i])
+                    {
+                        status[i] = synced != hipSuccess ? AT_EHIP : AT_ENOMEM;
+                        errors[i] = what;
+                        launched[i] = false;
+                    }
+            }
+            else if (!launchedGroups.empty())
+            {
+                std::vector<std::vector<char>> flags;
+                runner.finishUpdates(flags);
+                for (std::size_t g = 0; g < launchedGroups.size(); ++g)
+                    for (std::size_t k = 0; k < launchedGroups[g].size(); ++k)
+                        resize[launchedGroups[g][k]] = flags[g][k];
+            }
+            for (std::size_t i = 0; i < R; ++i)
+            {
+                if (!launched[i] || !resize[i]) continue;
+                (void)guarded(i, [&] {
+                    TreeBase *t = res[i]->tree.get();
+                    int attempts = 0;
+                    while (t->needsResize())
+                    {
+                        if (++attempts > 64) throw Error(AT_ELENGTH, "update did not fit after 64 resizes");
+                        res[i]->graph.reset(); // the resize moves the captured buffers: captured anew next time
+                        t->resizeBuffers();
+                        t->update();
+                    }
+                });
+            }
+
+            // ---- completion: trees kept, failed builds and releases returned
+            std::vector<std::unique_ptr<TreeBase>> dropped;
             {
                 std::lock_guard<std::mutex> g(m_);
                 const auto now = Clock::now();
-                for (std::size_t i = 0; i < batch.size(); ++i)
-                    if (batch[i]->kind != kRelease) --res[i]->outstanding;
-                if (head->kind == kBuild && status[0] == AT_OK) res[0]->tree = std::move(built);
-                else if (head->kind == kBuild)
+                for (std::size_t i = 0; i < R; ++i)
                 {
-                    // no tree is kept; its bytes go back to the budget
-                    res[0]->failed = true;
-                    used_ -= std::min(used_, res[0]->bytes);
-                    res[0]->bytes = 0;
-                    wk.resident -= std::min(wk.resident, res[0]->n);
-                    wk.trees -= std::min<std::size_t>(wk.trees, 1);
-                }
-                if (head->kind == kRelease)
-                {
-                    auto it = trees_.find(head->tree);
-                    used_ -= std::min(used_, it->second.bytes);
-                    if (!it->second.failed)
+                    Request *r = batch[i];
+                    if (r->kind != kRelease) --res[i]->outstanding;
+                    if (r->kind == kBuild && status[i] == AT_OK) res[i]->tree = std::move(built[i]);
+                    else if (r->kind == kBuild)
                     {
-                        wk.resident -= std::min(wk.resident, it->second.n);
+                        // no tree is kept; its bytes go back to the budget
+                        dropped.push_back(std::move(built[i]));
+                        res[i]->failed = true;
+                        used_ -= std::min(used_, res[i]->bytes);
+                        res[i]->bytes = 0;
+                        wk.resident -= std::min(wk.resident, res[i]->n);
                         wk.trees -= std::min<std::size_t>(wk.trees, 1);
                     }
-                    it->second.graph.reset();
-                    dropped = std::move(it->second.tree);
-                    trees_.erase(it);
+                    else if (r->kind == kRelease)
+                    {
+                        auto it = trees_.find(r->tree);
+                        used_ -= std::min(used_, it->second.bytes);
+                        if (!it->second.failed)
+                        {
+                            wk.resident -= std::min(wk.resident, it->second.n);
+                            wk.trees -= std::min<std::size_t>(wk.trees, 1);
+                        }
+                        it->second.graph.reset();
+                        dropped.push_back(std::move(it->second.tree));
+                        trees_.erase(it);
+                    }
                 }
-                for (std::size_t i = 0; i < batch.size(); ++i)
+                groupedBuilds_ += groupedBuilds;
+                groupedUpdates_ += groupedUpdates;
+                for (std::size_t i = 0; i < R; ++i)
                 {
                     batch[i]->status = status[i];
                     batch[i]->error = errors[i];
@@ -554,7 +722,7 @@ private:
                 }
                 admit();
             }
-            dropped.reset(); // free device memory outside the lock
+            dropped.clear(); // free device memory outside the lock
             doneCv_.notify_all();
         }
     }
@@ -574,6 +742,7 @@ private:
     std::deque<Request *> pendingBuilds_;
     std::uint64_t lastTree_ = 0, lastRequest_ = 0;
     std::uint64_t completed_ = 0, batchedUpdates_ = 0, updateBatches_ = 0;
+    std::uint64_t groupedBuilds_ = 0, groupedUpdates_ = 0;
 };
 } // namespace at
 

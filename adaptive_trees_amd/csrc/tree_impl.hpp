@@ -168,27 +168,40 @@ public:
         syncStream(stream_);
         computeParticleKeys();
         buildBegin();
-        TreeDev t = dev();
-        octInitRoot(stream_, t);
-        const OctParams p = params();
-        for (unsigned i = 0; i < maxDepth_; ++i)
-        {
-            buildPrePass();
-            AT_HIP(hipMemsetAsync(status_.p, 0, status_.cap * sizeof(std::uint64_t), stream_));
-            AT_HIP(hipMemsetAsync(ticket_.p, 0, sizeof(std::uint32_t), stream_));
-            t = dev();
-            octDecideScan(stream_, t, p, LONG_MAX, buildL_);
-            readState();
-            if (!buildAfterDecide(*sth_)) break;
-            t = dev();
-            octProcessCommit(stream_, t, p, buildL_);
-            octRefresh(stream_, t, p, false, buildNewL_);
-            buildAfterProcess();
-        }
-        buildEnd(true);
-        finishBuildState(buildL_);
-        buildTreeView(views_);
-        (void)needsResize();
+        octInitRoot(stream_, dev());
+        buildPasses(0);
+        finishBuild();
+    }
+
+    // A group member whose device-driven build stopped at a pass beyond its
+    // reservation: `s` is its device state after `decided` committed passes
+    // (mirrored on the host); the remaining passes run as in build().
+    void continueBuild(DevState s, unsigned decided)
+    {
+        s.needsResize = 0;
+        s.converged = 0;
+        *sth_ = s;
+        AT_HIP(hipMemcpyAsync(st_, sth_, sizeof(DevState), hipMemcpyHostToDevice, stream_));
+        buildPasses(decided);
+        finishBuild();
+    }
+
+    // ---- members of a group keep their own buffers: the group sorts the
+    // members' keys together and writes each member's segment to its P and
+    // Perm, then drives every member through one launch per step ----
+    SegCoords<Real> coordinates() const { return SegCoords<Real>{x_, y_, z_}; }
+    std::size_t particles() const { return n_; }
+    Key *sortedKeys() { return Pp_; }
+    std::uint32_t *sortedPerm() { return permp_; }
+    unsigned depthLimit() const { return maxDepth_; }
+    unsigned updateIterations() const { return iterations_; }
+    const GridBox<Real> &gridBox() const { return grid_; }
+    // A state the group read back: the pinned host copy and the host sizes;
+    // returns needsResize.
+    bool adoptState(const DevState &s)
+    {
+        *sth_ = s;
+        return applyState(s);
     }
 
     // ---- the build loop of octree.cu:159-180 in steps, so that a batch can
@@ -396,6 +409,34 @@ public:
 
 private:
     OctParams params() const { return OctParams{bucket_, nf_, hilbert_}; }
+
+    // Host-driven passes from decision `first` on (octree.cu:159-180): each
+    // decision is read back; the loop ends at the pass that changes nothing.
+    void buildPasses(unsigned first)
+    {
+        const OctParams p = params();
+        for (unsigned i = first; i < maxDepth_; ++i)
+        {
+            buildPrePass();
+            AT_HIP(hipMemsetAsync(status_.p, 0, status_.cap * sizeof(std::uint64_t), stream_));
+            AT_HIP(hipMemsetAsync(ticket_.p, 0, sizeof(std::uint32_t), stream_));
+            TreeDev t = dev();
+            octDecideScan(stream_, t, p, LONG_MAX, buildL_);
+            readState();
+            if (!buildAfterDecide(*sth_)) break;
+            t = dev();
+            octProcessCommit(stream_, t, p, buildL_);
+            octRefresh(stream_, t, p, false, buildNewL_);
+            buildAfterProcess();
+        }
+    }
+    void finishBuild()
+    {
+        buildEnd(true);
+        finishBuildState(buildL_);
+        buildTreeView(views_);
+        (void)needsResize();
+    }
 
     TreeDev dev()
     {

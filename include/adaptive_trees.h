@@ -229,24 +229,30 @@ int at_batch_graph_capture_update(at_batch *batch, void *stream, void **graph_ex
  * coordinates before its update) or NULLs when they were changed in place
  * (only safe when no request of that tree is in flight). Build requests are
  * admitted in arrival order while their estimated bytes fit hbm_budget
- * (a release frees them); updates of different trees queued on one worker
- * are launched together (up to max_batch) before one synchronisation. Tree
- * buffers may be read once the tree's last request is done. */
+ * (a release frees them). Each worker runs its queued requests in rounds:
+ * up to max_batch requests, at most one per tree and in each tree's order;
+ * the round's octree builds, then its octree updates, of one configuration
+ * and up to 4M particles per tree share their launches (16M particles per
+ * group); other requests run tree by tree; the round synchronises once
+ * after its updates. Tree buffers may be read once the tree's last request
+ * is done. */
 typedef struct at_server at_server;
 typedef struct at_server_config {
     uint64_t hbm_budget_bytes;
     int32_t workers;       /* worker threads, one HIP stream each (default 4) */
-    int32_t max_batch;     /* updates launched per synchronisation (default 16) */
+    int32_t max_batch;     /* requests per worker round (default 16) */
     int32_t update_graphs; /* nonzero: replay each tree's update from a HIP graph,
-                              captured at its first update; while a capture
-                              runs, HIP fails legacy-default-stream operations
-                              in every thread of the process */
+                              captured at its first update (trees then run
+                              one by one, no shared launches); while a
+                              capture runs, HIP fails legacy-default-stream
+                              operations in every thread of the process */
     int32_t reserved0;
 } at_server_config;
 typedef struct at_server_stats {
     uint64_t bytes_in_use, bytes_peak, hbm_budget;
     uint64_t resident_trees, waiting_builds, completed;
-    uint64_t batched_updates, update_batches;
+    uint64_t batched_updates, update_batches; /* updates in rounds of several updates; such rounds */
+    uint64_t grouped_builds, grouped_updates; /* requests that shared launches with other trees */
 } at_server_stats;
 int at_server_create(const at_server_config *config, at_server **out);
 /* Finishes the requests already given to a worker first; builds still waiting

@@ -162,3 +162,109 @@ def test_destroy_completes_builds_waiting_for_budget():
     t.join(timeout=30)
     assert not t.is_alive()
     assert seen["status"] == 2  # AT_ELOGIC: destroyed before admission
+
+
+def pair_chains(n, seed):
+    """Pairs of nearly coincident particles: with one particle per leaf, chains
+    of empty leaves down to the deepest level (builds beyond a group's
+    reservation for the device-driven passes)."""
+    rng = np.random.default_rng(seed)
+    base = rng.random((n // 2, 3))
+    pts = np.concatenate([base, base + 1e-7 * rng.random((n // 2, 3))])[:n]
+    return [np.ascontiguousarray(pts[:, a]) for a in range(3)]
+
+
+# classes of several trees each: their builds and updates share launches
+GROUPED = [
+    (OCTREE, np.float32, dict(sfc_kind=MORTON, max_particles_per_leaf=64), [3000, 50_000, 120_000, 7, 4_500_000]),
+    (OCTREE, np.float64, dict(sfc_kind=HILBERT, split_criterion=NF_COUNT, max_particles_per_leaf=4096),
+     [20_000, 60_000, 9_000]),
+    (OCTREE, np.float32, dict(sfc_kind=HILBERT, max_particles_per_leaf=16, views=(1, 1, 1)), [5_000, 40_000]),
+    (OCTREE, np.float64, dict(sfc_kind=MORTON, max_particles_per_leaf=1), [400, 1000, 64]),
+    (BINARY, np.float32, dict(max_particles_per_leaf=32), [30_000, 80_000]),
+]
+
+
+@pytest.mark.parametrize("workers", [1, 3])
+def test_grouped_requests_equal_results_alone(workers):
+    """Builds and updates of octrees of one configuration run as groups (one
+    segmented sort and multi-tree passes over trees that keep their own
+    buffers); every tree equals its requests run alone on a plain Tree,
+    including bucket-1 pair chains that outgrow the group build's
+    reservation, trees with views, a tree too large for a group and binary
+    trees in the same rounds."""
+    trees = []
+    for c, (kind, real, cfg, sizes) in enumerate(GROUPED):
+        for j, n in enumerate(sizes):
+            seed = 1000 + 10 * c + j
+            if cfg["max_particles_per_leaf"] == 1:
+                base = pair_chains(n, seed)
+                sigma = 1e-9
+            else:
+                base = (plummer if j % 2 else uniform)(n, seed=seed)
+                sigma = 2e-3
+            snaps = [base]
+            for u in range(UPDATES):
+                snaps.append(jitter(snaps[-1], sigma, seed=seed + 500 + u))
+            dsnaps = [[DeviceArray(np.ascontiguousarray(a, dtype=real)) for a in s] for s in snaps]
+            coords = [DeviceArray(max(n, 1) * np.dtype(real).itemsize) for _ in range(3)]
+            copy_into(coords, dsnaps[0])
+            trees.append((kind, real, cfg, n, dsnaps, coords))
+    print("grouped mix: %d trees, workers %d" % (len(trees), workers), flush=True)
+
+    def estimate(n, cfg):  # the server's per-tree estimate
+        views = any(cfg.get("views", (0, 0, 0)))
+        return n * 25 + (n // 8 + 64) * 48 + (n * 8 + (n // 8) * 26 if views else 0) + (1 << 20)
+
+    # a blocker tree holds the budget while every request is submitted; its
+    # release admits all builds at once (each followed by its held updates),
+    # so the first round of a worker holds all of its builds
+    nb = 8_000_000
+    budget = estimate(nb, {}) + 1000
+    assert sum(estimate(t[3], t[2]) for t in trees) <= budget
+    blocker = [DeviceArray(np.ascontiguousarray(a, dtype=np.float32)) for a in uniform(nb, seed=999)]
+    with Server(hbm_budget_bytes=budget, workers=workers, max_batch=64) as server:
+        bid, brid = server.submit_build(*blocker, nb, make_config(max_particles_per_leaf=64), real=np.float32)
+        assert server.wait(brid) == 0
+        ids, reqs = [], []
+        for kind, real, cfg, n, snaps, coords in trees:
+            tid, rid = server.submit_build(*coords, n, make_config(**cfg), real=real, kind=kind)
+            ids.append(tid)
+            reqs.append(rid)
+        for u in range(1, UPDATES + 1):
+            for tid, t in zip(ids, trees):
+                reqs.append(server.submit_update(tid, new=t[4][u]))
+        assert server.stats()["waiting_builds"] == len(trees)
+        assert server.wait(server.submit_release(bid)) == 0
+        for r in reqs:
+            assert server.wait(r) == 0
+        st = server.stats()
+        print("grouped builds %d, grouped updates %d" % (st["grouped_builds"], st["grouped_updates"]), flush=True)
+        assert st["completed"] == len(reqs) + 2
+        if workers == 1:
+            # every octree of up to 4M particles is in a group of its class
+            assert st["grouped_builds"] == 12 and st["grouped_updates"] == 12 * UPDATES
+        else:
+            assert st["grouped_builds"] > 0 and st["grouped_updates"] > 0
+        for tid, (kind, real, cfg, n, snaps, coords) in zip(ids, trees):
+            alone_coords = [DeviceArray(max(n, 1) * np.dtype(real).itemsize) for _ in range(3)]
+            copy_into(alone_coords, snaps[0])
+            alone = Tree(*alone_coords, n, make_config(**cfg), real=real, kind=kind)
+            alone.build()
+            for u in range(1, UPDATES + 1):
+                copy_into(alone_coords, snaps[u])
+                alone.update()
+                while alone.doBuffersNeedResize():
+                    alone.resizeBuffers()
+                    alone.update()
+            mine = server.tree(tid)
+            sa, sb = mine.scalars(), alone.scalars()
+            for f in ("num_particles", "num_active_particles", "num_nodes", "num_leaves", "leaf_capacity",
+                      "active_buffer"):
+                assert getattr(sa, f) == getattr(sb, f), (tid, n, f)
+            for name in ARRAYS:
+                if name == "nearFieldCounts_d" and (kind == BINARY or sa.split_criterion == LEAF_COUNT):
+                    continue
+                a, b = mine.get(name), alone.get(name)
+                assert a.shape == b.shape and np.array_equal(a, b), (tid, n, name)
+            alone.close()
