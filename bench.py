@@ -672,9 +672,11 @@ def serving_trace(rate, requests, pool, seed):
     return trace
 
 
-def serving_benchmark(rates, requests, seed=5, workers=4):
+def serving_benchmark(rates, requests, seed=5, workers=4, saturated=800):
     """The serving loop (worker streams, 200 GB budget) against serial plain calls on the same
-    trace; latency = result ready - arrival, per request."""
+    trace; latency = result ready - arrival, per request. Besides the Poisson rates, a saturated
+    trace submits all `saturated` requests at once (capacity: requests/s of the whole trace); each
+    of its passes runs twice and the second is reported."""
     sizes = workloads.batch_sizes(20, seed=seed, lo=10_000, hi=8_000_000)
     pool = []  # per entry: two snapshots, host float64
     for k, n in enumerate(sizes):
@@ -689,24 +691,7 @@ def serving_benchmark(rates, requests, seed=5, workers=4):
             dev_pool[key] = [DeviceArray(np.ascontiguousarray(v, dtype=real)) for v in pool[p][which]]
         return dev_pool[key]
 
-    results = []
-    for rate in rates:
-        trace = serving_trace(rate, requests, pool, seed + rate)
-        prep = []
-        for r in trace:
-            if r[0] == "build":
-                _, _, tid, p, kind, real, cfg = r
-                n = len(pool[p][0][0])
-                bufs = [[DeviceArray(n * np.dtype(real).itemsize) for _ in range(3)] for _ in range(2)]
-                for b in bufs:
-                    for d, s in zip(b, snapshot(p, real, 0)):
-                        _lib().call("memcpy", d.ptr, s.ptr, s.nbytes, 3)
-                prep.append(bufs)
-                for w in (0, 1):
-                    snapshot(p, real, w)
-            else:
-                prep.append(None)
-        info = {r[2]: r for r in trace if r[0] == "build"}
+    def served_pass(trace, prep, info, rate):
         # closed before anything else on every exit path: its workers read the
         # device arrays of this function until their queues are drained
         with Server(hbm_budget_bytes=200 << 30, workers=workers, max_batch=16) as server:
@@ -743,9 +728,11 @@ def serving_benchmark(rates, requests, seed=5, workers=4):
                 work += n
             stats = server.stats()
         if failed:
-            raise RuntimeError("serving rate %d/s: %d requests failed (first: id %d, status %d)"
+            raise RuntimeError("serving rate %s: %d requests failed (first: id %d, status %d)"
                                % (rate, len(failed), failed[0][0], failed[0][1]))
-        served_s = (last - first) / 1e3
+        return lat, (last - first) / 1e3, work, stats
+
+    def serial_pass(trace, prep, info):
         plain, turn2, slat = {}, {}, []
         start = time.perf_counter()
         for r, bufs in zip(trace, prep):
@@ -776,23 +763,51 @@ def serving_benchmark(rates, requests, seed=5, workers=4):
         serial_s = time.perf_counter() - start - trace[0][1]
         for t, _ in plain.values():
             t.close()
+        return slat, serial_s
+
+    results = []
+    runs = [(rate, requests) for rate in rates] + ([("saturated", saturated)] if saturated else [])
+    for rate, count in runs:
+        trace = serving_trace(1e9 if rate == "saturated" else rate, count, pool,
+                              seed + (2000 if rate == "saturated" else rate))
+        prep = []
+        for r in trace:
+            if r[0] == "build":
+                _, _, tid, p, kind, real, cfg = r
+                n = len(pool[p][0][0])
+                bufs = [[DeviceArray(n * np.dtype(real).itemsize) for _ in range(3)] for _ in range(2)]
+                for b in bufs:
+                    for d, s in zip(b, snapshot(p, real, 0)):
+                        _lib().call("memcpy", d.ptr, s.ptr, s.nbytes, 3)
+                prep.append(bufs)
+                for w in (0, 1):
+                    snapshot(p, real, w)
+            else:
+                prep.append(None)
+        info = {r[2]: r for r in trace if r[0] == "build"}
+        for _ in range(2 if rate == "saturated" else 1):
+            lat, served_s, work, stats = served_pass(trace, prep, info, rate)
+            slat, serial_s = serial_pass(trace, prep, info)
+        n_req = len(trace)
         results.append({
-            "rate_per_s": rate, "requests": len(reqs),
+            "rate_per_s": rate, "requests": n_req,
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "serial_p50_ms": round(float(np.percentile(slat, 50)), 3),
             "serial_p99_ms": round(float(np.percentile(slat, 99)), 3),
-            "requests_per_s": round(len(reqs) / served_s, 1),
+            "requests_per_s": round(n_req / served_s, 1),
             "Mparticles_per_s": round(work / served_s / 1e6, 1),
-            "serial_requests_per_s": round(len(reqs) / serial_s, 1),
+            "serial_requests_per_s": round(n_req / serial_s, 1),
             "served_over_serial": round(serial_s / served_s, 3),
-            "batched_updates": int(stats["batched_updates"]), "peak_GB": round(stats["bytes_peak"] / 1e9, 3),
+            "batched_updates": int(stats["batched_updates"]), "grouped_builds": int(stats["grouped_builds"]),
+            "grouped_updates": int(stats["grouped_updates"]), "peak_GB": round(stats["bytes_peak"] / 1e9, 3),
         })
-        progress("serving rate %d/s done" % rate)
+        progress("serving rate %s done" % rate)
     return {"workload": "Poisson trace, %d requests per rate, sizes log-uniform [1e4, 8e6], 70%% octree / 30%% "
-                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; %d worker streams, "
-                        "200 GB budget, updates launched directly (no graph captures); serial = plain Tree calls "
-                        "in arrival order on one thread"
-                        % (requests, workers),
+                        "binary, FP32/FP64, Hilbert/Morton, LeafCount 64 / NFCount 4096; saturated: %d requests "
+                        "submitted at once (second of two passes); %d worker streams, 200 GB budget, rounds of "
+                        "up to 16 requests, octrees of one configuration grouped, no graph captures; serial = "
+                        "plain Tree calls in arrival order on one thread"
+                        % (requests, saturated, workers),
             "rates": results}
 
 
@@ -921,6 +936,8 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=4, help="worker streams of the serving loop")
     ap.add_argument("--serve-rates", default="50,100,200,500,1000,2000", help="arrival rates (requests/s)")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests per arrival rate")
+    ap.add_argument("--serve-saturated", type=int, default=800,
+                    help="requests of the saturated trace (all submitted at once; 0 skips it)")
     ap.add_argument("--only-configs", action="store_true", help="run only the other single-tree workloads")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving-loop measurement")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -949,7 +966,8 @@ def main():
         print(json.dumps({"batched": batch_benchmark()}))
         return
     if args.only_serve:
-        print(json.dumps({"serving": serving_benchmark(rates, args.serve_requests, workers=args.serve_workers)}))
+        print(json.dumps({"serving": serving_benchmark(rates, args.serve_requests, workers=args.serve_workers,
+                                                       saturated=args.serve_saturated)}))
         return
     if args.only_configs:
         print(json.dumps({"other_configs": other_configs(not args.no_cpu)}))
@@ -969,7 +987,8 @@ def main():
         out["batched"] = batch_benchmark()
         progress("batch done")
     if not args.no_serve:
-        out["serving"] = serving_benchmark(rates, args.serve_requests, workers=args.serve_workers)
+        out["serving"] = serving_benchmark(rates, args.serve_requests, workers=args.serve_workers,
+                                           saturated=args.serve_saturated)
     if not args.no_cpu:
         base, exact_build, exact_update = cpu_baseline(A, B, box, after_build, after_update)
         out["cpu_baseline"] = base
