@@ -616,6 +616,7 @@ int at_tune_set(const char *key, int value)
         else if (k == "sort_prefetch") at::setSortPrefetch(value);
         else if (k == "sort_msd") at::setSortMsd(value);
         else if (k == "sort_digit_bits") at::setSortDigitBits(value);
+        else if (k == "sort_msd_min_keys") at::setSortMsdMinKeys(value < 0 ? 0 : std::size_t(value));
         else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
     });
 }
@@ -662,6 +663,7 @@ int at_tune_get(const char *key, int *value)
         else if (k == "sort_prefetch") *value = at::sortPrefetch();
         else if (k == "sort_msd") *value = at::sortMsd();
         else if (k == "sort_digit_bits") *value = at::sortDigitBits();
+        else if (k == "sort_msd_min_keys") *value = int(at::sortMsdMinKeys());
         else throw at::Error(AT_EINVAL, "unknown tuning parameter: " + k);
     });
 }

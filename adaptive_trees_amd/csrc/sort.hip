@@ -46,8 +46,14 @@ int g_targetRanges = 8192; // one 4096-key tile per range at 2^25 keys
 int g_digitArray = 1; // scatter writes the next pass's digit bytes for the histogram
 int g_prefetch = 0;   // scatter loads the next tile while working on the current one
 int g_msd = 1;        // MSD levels + LDS chunk sort for large (key, value) sorts
-int g_digitBits = 8;  // LSD digit width below kMsdMinKeys keys (8 or 11)
-constexpr std::size_t kMsdMinKeys = std::size_t(1) << 20;
+int g_digitBits = 8;  // LSD digit width below the MSD threshold (8 or 11)
+// sorts of at least this many (key, value) pairs take the MSD levels: from
+// 4096 keys a partition level and the LDS chunk sort (about 14 launches and
+// one host read) beat eight LSD passes of three launches each (requests of
+// 1e4..1e6 particles 25-30% faster on a mixed serving trace)
+std::size_t g_msdMinKeys = 4096;
+// workspaces below this capacity hold the 11-bit LSD histograms
+constexpr std::size_t kWideMaxKeys = std::size_t(1) << 20;
 
 __device__ inline unsigned laneId() { return __lane_id(); }
 
@@ -818,8 +824,8 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
 }
 
 // ---- wide digits: LSD passes of 11 bits ------------------------------------------
-// The digit width the autotuner may pick for LSD sorts (below kMsdMinKeys
-// keys): 6 passes of 11 bits instead of 8 of 8 cover the 64-bit key, 18
+// The digit width the autotuner may pick for LSD sorts (below the MSD
+// threshold and kWideMaxKeys keys): 6 passes of 11 bits instead of 8 of 8 cover the 64-bit key, 18
 // launches instead of 24. One tile of 2048 keys per range; per pass a
 // histogram of each range (digit-major rows), one scan block per digit over
 // the ranges, and a scatter that ranks each key within its wave by 11
@@ -2013,7 +2019,7 @@ void runPasses(hipStream_t s, const SortPlan &pl, SortWorkspace &ws, Key *kA, Ke
 }
 
 // The 11-bit LSD passes when they are chosen and the workspace holds their
-// histograms (fewer than kMsdMinKeys keys).
+// histograms (fewer than kWideMaxKeys keys).
 bool widePasses(const SortWorkspace &ws, std::size_t n)
 {
     return g_digitBits == kWideBits && ws.wideHist && (n + kWideTile - 1) / kWideTile <= ws.wideRanges;
@@ -2211,7 +2217,13 @@ void setSortMsd(int on)
     if (on != 0 && on != 1) throw Error(1, "sort_msd must be 0 or 1");
     g_msd = on;
 }
-std::size_t sortMsdMinKeys() { return kMsdMinKeys; }
+std::size_t sortMsdMinKeys() { return g_msdMinKeys; }
+void setSortMsdMinKeys(std::size_t n)
+{
+    if (n < kMsdChunk || n >= (std::size_t(1) << 31))
+        throw Error(1, "sort_msd_min_keys must be in [" + std::to_string(kMsdChunk) + ", 2^31)");
+    g_msdMinKeys = n;
+}
 int sortDigitBits() { return g_digitBits; }
 void setSortDigitBits(int bits)
 {
@@ -2272,7 +2284,7 @@ void SortWorkspace::allocate(std::size_t cap, bool withValues, hipStream_t s)
     rangeHist = static_cast<std::uint32_t *>(poolAlloc(histRows * kRadix * sizeof(std::uint32_t), s));
     rangeOff = static_cast<std::uint32_t *>(poolAlloc(maxRanges * kRadix * sizeof(std::uint32_t), s));
     digitTotal = static_cast<std::uint32_t *>(poolAlloc(kRadix * sizeof(std::uint32_t), s));
-    if (capacity < kMsdMinKeys)
+    if (capacity < kWideMaxKeys)
     {
         wideRanges = (capacity + kWideTile - 1) / kWideTile;
         wideHist = static_cast<std::uint32_t *>(poolAlloc(wideRanges * kWideRadix * sizeof(std::uint32_t), s));
@@ -2365,7 +2377,7 @@ void keygenAndSort(hipStream_t s, KeyKind kind, const KeygenParams<T> &p, std::s
     ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
     // MSD: keys into the alternate buffer, histogram of the top digit
-    const bool msd = g_msd && n >= kMsdMinKeys;
+    const bool msd = g_msd && n >= g_msdMinKeys;
     Key *keys = msd ? ws.keysAlt : keysOut;
     const int histShift = msd ? 64 - kRadixBits : 0;
     {
@@ -2399,7 +2411,7 @@ void sortPairs(hipStream_t s, Key *keys, std::uint32_t *vals, std::size_t n, Sor
     if (n >= (std::size_t(1) << 32)) throw Error(3, "radix sort supports fewer than 2^32 keys");
     ws.allocate(n, true, s);
     const SortPlan pl = makeSortPlan(n);
-    const bool msd = msdAllowed && g_msd && n >= kMsdMinKeys;
+    const bool msd = msdAllowed && g_msd && n >= g_msdMinKeys;
     if (!msd && widePasses(ws, n))
     {
         runWidePasses(s, n, ws, keys, ws.keysAlt, vals, ws.valsAlt, true, iota);
@@ -2583,7 +2595,7 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
     }
     g_digitArray = bd;
     int bm = g_msd;
-    if (n >= kMsdMinKeys)
+    if (n >= g_msdMinKeys)
         for (int m : {0, 1})
         {
             g_msd = m;
@@ -2593,7 +2605,7 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
     g_msd = bm;
     // the LSD digit width, on a sort below the MSD threshold (its own
     // workspace: the 11-bit histograms exist only for such sizes)
-    const std::size_t m = n < kMsdMinKeys ? n : kMsdMinKeys - 1;
+    const std::size_t m = n < kWideMaxKeys ? n : kWideMaxKeys - 1;
     SortWorkspace small;
     small.allocate(m, true, s);
     auto timeLsd = [&]() {
@@ -2627,7 +2639,7 @@ std::string autotuneSort(std::size_t n, hipStream_t s)
     return "{\"sort_threads\": " + std::to_string(bt) + ", \"sort_items\": " + std::to_string(bi) +
            ", \"sort_prefetch\": " + std::to_string(bp) +
            ", \"sort_ranges\": " + std::to_string(br) + ", \"sort_digit_array\": " + std::to_string(bd) +
-           ", \"sort_msd\": " + std::to_string(bm) +
+           ", \"sort_msd\": " + std::to_string(bm) + ", \"sort_msd_min_keys\": " + std::to_string(g_msdMinKeys) +
            ", \"keys\": " + std::to_string(n) + ", \"sort_ms\": " + std::to_string(best) +
            ", \"sort_digit_bits\": " + std::to_string(bb) + ", \"digit_bits_keys\": " + std::to_string(m) +
            ", \"lsd_ms_8_bits\": " + std::to_string(lsd[0]) + ", \"lsd_ms_11_bits\": " + std::to_string(lsd[1]) +
@@ -2667,7 +2679,7 @@ void SegmentedSort::plan(const std::vector<std::size_t> &counts, hipStream_t s)
     keysAlt_.ensure(total_ ? total_ : 1, 0, s);
     valsAlt_.ensure(total_ ? total_ : 1, 0, s);
     digits_.ensure(total_ ? total_ : 1, 0, s);
-    msdReady_ = total_ >= kMsdMinKeys;
+    msdReady_ = total_ >= g_msdMinKeys;
     std::vector<MsdSeg> parents;
     if (msdReady_)
     {

@@ -33,19 +33,22 @@ void setSortDigitArray(int on);
 int sortPrefetch();
 void setSortPrefetch(int on);
 // 1: sorts of at least sortMsdMinKeys() (key, value) pairs partition by the
-// high digits first (MSD levels) and finish buckets of up to 4096 keys in
+// high digits first (MSD levels) and finish buckets of up to 2048 keys in
 // LDS; 0: LSD passes over all 8 digits. Results are identical.
 int sortMsd();
 void setSortMsd(int on);
+// The MSD threshold (default 4096 pairs; at least 2048).
 std::size_t sortMsdMinKeys();
-// Digit width of the LSD passes of sorts below sortMsdMinKeys() keys: 8
-// (8 passes) or 11 (6 passes of 2048 digits). Results are identical.
+void setSortMsdMinKeys(std::size_t n);
+// Digit width of the LSD passes of sorts below 2^20 keys that do not take
+// the MSD levels: 8 (8 passes) or 11 (6 passes of 2048 digits). Results are
+// identical.
 int sortDigitBits();
 void setSortDigitBits(int bits);
 
 // Times every tile shape, then range count, then histogram source on a
 // stable sort of n random 63-bit (key, index) pairs, then the LSD digit
-// width on a sort below sortMsdMinKeys() keys, and keeps the fastest
+// width on an LSD sort of min(n, 2^20 - 1) keys, and keeps the fastest
 // settings (process-wide). Returns them as a JSON object.
 std::string autotuneSort(std::size_t n, hipStream_t s);
 

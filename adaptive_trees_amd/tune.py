@@ -10,7 +10,7 @@ import json
 from . import _ffi
 
 KEYS = ("sort_threads", "sort_items", "sort_ranges", "sort_digit_array", "sort_prefetch", "sort_msd",
-        "sort_digit_bits")
+        "sort_digit_bits", "sort_msd_min_keys")
 
 
 def get():

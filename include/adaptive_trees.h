@@ -285,12 +285,13 @@ int at_server_get_stats(at_server *server, at_server_stats *out);
  *                     the keys;
  * "sort_prefetch":    1 = the scatter loads the next tile while it works on
  *                     the current one (tiles 256/512 x 8/12);
- * "sort_msd":         1 = sorts of at least 2^20 (key, value) pairs
- *                     partition by the high digits first and finish buckets
- *                     of up to 4096 keys in LDS, 0 = eight LSD passes.
- * "sort_digit_bits":  digit width of the LSD passes of sorts below 2^20
- *                     keys: 8 (eight passes) or 11 (six passes of 2048
- *                     digits).
+ * "sort_msd":         1 = sorts of at least sort_msd_min_keys (key, value)
+ *                     pairs partition by the high digits first and finish
+ *                     buckets of up to 2048 keys in LDS, 0 = LSD passes.
+ * "sort_msd_min_keys": that threshold (2048 .. 2^31 - 1; default 4096).
+ * "sort_digit_bits":  digit width of the LSD passes of sorts below 2^20 keys
+ *                     (with sort_msd 0, or below the MSD threshold): 8
+ *                     (eight passes) or 11 (six passes of 2048 digits).
  * A tile (sort_threads x sort_items) holds at most 8192 keys; a larger one
  * is rejected when a sort is planned. */
 int at_tune_set(const char *key, int value);

@@ -1,7 +1,8 @@
 """Large sorts: MSD levels + LDS chunk sort against LSD passes and the oracle.
 
-Sorts of at least 2^20 (key, slot) pairs partition by the high digits first
-and finish buckets of up to 4096 keys in LDS. Key distributions here drive
+Sorts from the MSD threshold on (4096 (key, slot) pairs by default)
+partition by the high digits first and finish buckets of up to 2048 keys in
+LDS. Key distributions here drive
 the level planning through its cases: clustered keys (three levels), few
 distinct keys and long duplicate runs (all eight levels, equal-key runs
 larger than a chunk), escaped particles (the invalid key ~0), anisotropic

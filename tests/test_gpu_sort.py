@@ -1,7 +1,8 @@
 """The device radix sort on its own (at_sort_pairs) against numpy's stable sort.
 
-Sizes below 2^20 take the LSD passes; larger ones the MSD levels (when
-sort_msd is on) with the LDS chunk sort. Distributions cover what the level
+Sorts take the MSD levels with the LDS chunk sort from sort_msd_min_keys
+pairs on (4096 by default; 2^20 and 4096 are both tested), the LSD passes
+below it and when sort_msd is off. Distributions cover what the level
 planning and the chunk sort branch on: uniform 64-bit keys, 63-bit keys with
 the invalid key ~0 mixed in, a handful of distinct values, long duplicate
 runs, clustered high bits, keys whose high bits repeat in runs for the chunk
@@ -83,10 +84,13 @@ def gpu_sort(keys, vals=None):
     return dk.download(), dv.download()
 
 
-@pytest.fixture(params=[1, 0], ids=["msd", "lsd"])
+@pytest.fixture(params=[1, 0, 2], ids=["msd_from_2e20", "lsd", "msd_from_4096"])
 def msd(request):
+    """1: MSD levels from 2^20 keys (LSD passes below); 0: LSD passes only;
+    2: MSD levels from 4096 keys (small sorts through the partition levels
+    and the chunk sort)."""
     saved = tune.get()
-    tune.set(sort_msd=request.param)
+    tune.set(sort_msd=1 if request.param else 0, sort_msd_min_keys=4096 if request.param == 2 else 1 << 20)
     yield request.param
     tune.set(**saved)
 
@@ -125,7 +129,7 @@ def test_sort_pairs_rejects_null_pointers():
 @pytest.fixture
 def wide_digits():
     saved = tune.get()
-    tune.set(sort_digit_bits=11)
+    tune.set(sort_digit_bits=11, sort_msd_min_keys=1 << 20)
     yield
     tune.set(**saved)
 

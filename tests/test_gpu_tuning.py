@@ -60,7 +60,8 @@ def test_tunings_are_bit_exact(tuning, cfg):
 def test_invalid_tunings_are_rejected():
     lib = _ffi.hip_lib()
     for k, v in (("sort_threads", 300), ("sort_items", 7), ("sort_ranges", 1), ("sort_digit_array", 2),
-                 ("sort_prefetch", 3), ("sort_digit_bits", 9), ("no_such_key", 1)):
+                 ("sort_prefetch", 3), ("sort_digit_bits", 9), ("sort_msd_min_keys", 2047),
+                 ("sort_msd_min_keys", -1), ("no_such_key", 1)):
         with pytest.raises(_ffi.InvalidArgument):
             lib.call("tune_set", k.encode(), v)
 
@@ -99,11 +100,12 @@ def test_autotuner_picks_a_valid_setting(tuning):
 @pytest.mark.gpu
 @pytest.mark.parametrize("bits", [8, 11])
 def test_digit_width_trees_bit_exact(bits):
-    """Trees whose sorts take the LSD passes (below 2^20 keys) are the
-    oracle's at either digit width: build, update, views."""
+    """Trees whose sorts take the LSD passes (below 2^20 keys, MSD threshold
+    raised to 2^20) are the oracle's at either digit width: build, update,
+    views."""
     from adaptive_trees_amd import tune
     saved = tune.get()
-    tune.set(sort_digit_bits=bits)
+    tune.set(sort_digit_bits=bits, sort_msd_min_keys=1 << 20)
     try:
         print("digit bits", bits, flush=True)
         x, y, z = plummer(250_000, seed=74)
@@ -115,5 +117,32 @@ def test_digit_width_trees_bit_exact(bits):
             p.compare("build bits=%d kind=%d" % (bits, kind))
             p.update()
             p.compare("update bits=%d kind=%d" % (bits, kind))
+    finally:
+        tune.set(**saved)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("min_keys", [2048, 1 << 14])
+def test_msd_threshold_trees_bit_exact(min_keys):
+    """Trees whose sorts take the MSD levels and the LDS chunk sort at small
+    sizes (thresholds below the default 4096 and above it) are the oracle's:
+    build, update, views."""
+    from adaptive_trees_amd import tune
+    saved = tune.get()
+    tune.set(sort_msd=1, sort_msd_min_keys=min_keys)
+    try:
+        assert tune.get()["sort_msd_min_keys"] == min_keys
+        print("msd threshold", min_keys, flush=True)
+        for n, seed in ((250_000, 74), (3000, 75), (20_000, 76)):
+            x, y, z = plummer(n, seed=seed)
+            x[::89], y[::89], z[::89] = 0.125, 0.5, 0.875
+            for kind, real, bucket, sfc in ((OCTREE, np.float32, 64, HILBERT), (OCTREE, np.float64, 16, MORTON),
+                                            (BINARY, np.float32, 32, MORTON)):
+                p = Pair(x, y, z, kind=kind, real=real, max_particles_per_leaf=bucket, sfc_kind=sfc,
+                         views=(1, 1, 1))
+                p.build()
+                p.compare("build n=%d min=%d kind=%d" % (n, min_keys, kind))
+                p.update()
+                p.compare("update n=%d min=%d kind=%d" % (n, min_keys, kind))
     finally:
         tune.set(**saved)
