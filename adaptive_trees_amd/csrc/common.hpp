@@ -694,6 +694,55 @@ __device__ inline void lowerBoundPairIn(const Key *P, std::size_t lo, std::size_
     r1 = b1 + (P[b1] < v1 ? 1 : 0);
 }
 
+// 32-bit index forms of refinePair and lowerBoundPairIn (arrays of fewer than
+// 2^32 keys): the same steps in half the registers.
+__device__ inline void refinePair32(const Key *a, std::uint32_t n, std::uint32_t S, std::uint32_t k0,
+                                    std::uint32_t k1, Key v0, Key v1, std::uint32_t &r0, std::uint32_t &r1)
+{
+    const std::uint32_t m = std::uint32_t((std::uint64_t(n) + S - 1) / S);
+    std::uint32_t b0 = k0 ? (k0 - 1) * S + 1 : 0, b1 = k1 ? (k1 - 1) * S + 1 : 0;
+    std::uint32_t l0 = k0 ? (k0 < m ? k0 * S : n) - b0 : 0, l1 = k1 ? (k1 < m ? k1 * S : n) - b1 : 0;
+    while (l0 > 1 || l1 > 1)
+    {
+        if (l0 > 1)
+        {
+            const std::uint32_t h = l0 >> 1;
+            b0 = a[b0 + h] < v0 ? b0 + h : b0;
+            l0 -= h;
+        }
+        if (l1 > 1)
+        {
+            const std::uint32_t h = l1 >> 1;
+            b1 = a[b1 + h] < v1 ? b1 + h : b1;
+            l1 -= h;
+        }
+    }
+    r0 = l0 ? b0 + (a[b0] < v0 ? 1u : 0u) : b0;
+    r1 = l1 ? b1 + (a[b1] < v1 ? 1u : 0u) : b1;
+}
+
+__device__ inline void lowerBoundPairIn32(const Key *P, std::uint32_t lo, std::uint32_t hi, Key v0, Key v1,
+                                          std::uint32_t &r0, std::uint32_t &r1)
+{
+    std::uint32_t n = hi - lo;
+    if (n == 0)
+    {
+        r0 = r1 = lo;
+        return;
+    }
+    std::uint32_t b0 = lo, b1 = lo;
+    while (n > 1)
+    {
+        const std::uint32_t half = n >> 1;
+        const Key x0 = P[b0 + half], x1 = P[b1 + half];
+        b0 = x0 < v0 ? b0 + half : b0;
+        b1 = x1 < v1 ? b1 + half : b1;
+        n -= half;
+    }
+    r0 = b0 + (P[b0] < v0 ? 1u : 0u);
+    r1 = b1 + (P[b1] < v1 ? 1u : 0u);
+}
+
 // ---- device-scope atomics used by the chained scans ---------------------------
 __device__ inline std::uint32_t ldAcq(const std::uint32_t *p)
 {

@@ -759,6 +759,31 @@ __device__ inline void lowerBoundTopPair(const Key *P, std::size_t n, const Key 
     refinePair(P, n, kIndexStride, j0, j1, v0, v1, r0, r1);
 }
 
+// The same with 32-bit positions (trees hold fewer than 2^32 particles).
+__device__ inline void lowerBoundTopPair32(const Key *P, std::uint32_t n, const Key *Q, const TopTable &tab, Key v0,
+                                           Key v1, std::uint32_t &r0, std::uint32_t &r1)
+{
+    std::uint32_t k0 = 0, k1 = 0;
+    std::uint32_t m = tab.count;
+    if (m)
+    {
+        std::uint32_t b0 = 0, b1 = 0;
+        while (m > 1)
+        {
+            const std::uint32_t h = m >> 1;
+            b0 = tab.keys[b0 + h] < v0 ? b0 + h : b0;
+            b1 = tab.keys[b1 + h] < v1 ? b1 + h : b1;
+            m -= h;
+        }
+        k0 = b0 + (tab.keys[b0] < v0 ? 1u : 0u);
+        k1 = b1 + (tab.keys[b1] < v1 ? 1u : 0u);
+    }
+    const std::uint32_t nq = std::uint32_t(indexSize(n));
+    std::uint32_t j0, j1;
+    refinePair32(Q, nq, tab.stride, k0, k1, v0, v1, j0, j1);
+    refinePair32(P, n, std::uint32_t(kIndexStride), j0, j1, v0, v1, r0, r1);
+}
+
 __device__ inline int wrapAxis(int v) { return v < 0 ? v + kAxis : (v >= kAxis ? v - kAxis : v); }
 
 // Near-field count of the level-`level` node at `start` (its own population
@@ -772,12 +797,13 @@ __device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, cons
                                  bool hilbert, unsigned lane32, std::uint64_t &center, std::uint64_t &sum)
 {
     std::uint64_t pop = 0;
+    const std::uint32_t n32 = std::uint32_t(n);
     if (level == 0)
     {
         if (lane32 < 27)
         {
-            std::size_t r0, r1;
-            lowerBoundTopPair(P, n, Q, tab, 0, nodeRange(0), r0, r1);
+            std::uint32_t r0, r1;
+            lowerBoundTopPair32(P, n32, Q, tab, 0, nodeRange(0), r0, r1);
             pop = std::uint64_t(r1 - r0);
         }
     }
@@ -787,14 +813,14 @@ __device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, cons
         cellCorner(start, level, hilbert, dec, cx, cy, cz, st);
         const int w = int(1u << (kOctLevels - level));
         const int u[3] = {int(cx) / w, int(cy) / w, int(cz) / w}; // node index along each axis
-        std::size_t pa = 0, pb = 0;
+        std::uint32_t pa = 0, pb = 0;
         CellCode pc{0, 0};
         if (lane32 < 8)
         {
             int q[3];
             for (int a = 0; a < 3; ++a) q[a] = wrapAxis((((u[a] - 1) >> 1) + int((lane32 >> a) & 1u)) * 2 * w);
             pc = cellCode(unsigned(q[0]), unsigned(q[1]), unsigned(q[2]), level - 1, hilbert, enc);
-            lowerBoundTopPair(P, n, Q, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
+            lowerBoundTopPair32(P, n32, Q, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
         }
         int parent = 0;
         unsigned raw = 0;
@@ -808,13 +834,13 @@ __device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, cons
                 raw |= unsigned(wrapAxis(v * w) / w & 1) << (2 - a);
             }
         }
-        const std::size_t lo = __shfl(pa, parent, 32), hi = __shfl(pb, parent, 32);
+        const std::uint32_t lo = __shfl(pa, parent, 32), hi = __shfl(pb, parent, 32);
         const CellCode home{__shfl(pc.key, parent, 32), unsigned(__shfl(int(pc.state), parent, 32))};
         if (lane32 < 27)
         {
             const Key ck = childKey(home, raw, level, hilbert, enc);
-            std::size_t r0, r1;
-            lowerBoundPairIn(P, lo, hi, ck, ck + nodeRange(level), r0, r1);
+            std::uint32_t r0, r1;
+            lowerBoundPairIn32(P, lo, hi, ck, ck + nodeRange(level), r0, r1);
             pop = std::uint64_t(r1 - r0);
         }
     }
@@ -848,6 +874,7 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsi
     const unsigned lane = __lane_id();
     const std::uint32_t waves = blocks * (blockDim.x / 64);
     const bool hil = hilbert != 0;
+    const std::uint32_t n32 = std::uint32_t(t.n);
     for (std::uint32_t g = block * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += waves)
     {
         const std::uint32_t item = t.nfGroups[g];
@@ -859,14 +886,14 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsi
         unsigned bxu, byu, bzu, pst;
         cellCorner(start, level - 1, hil, hl.dec, bxu, byu, bzu, pst);
         const int bx = int(bxu), by = int(byu), bz = int(bzu);
-        std::size_t pa = 0, pb = 0;
+        std::uint32_t pa = 0, pb = 0;
         CellCode pc{0, 0};
         if (lane < 27)
         {
             const int dx = int(lane % 3) - 1, dy = int((lane / 3) % 3) - 1, dz = int(lane / 9) - 1;
             pc = cellCode(unsigned(wrapAxis(bx + 2 * dx * w)), unsigned(wrapAxis(by + 2 * dy * w)),
                           unsigned(wrapAxis(bz + 2 * dz * w)), level - 1, hil, hl.enc);
-            lowerBoundTopPair(t.P, t.n, t.Pidx, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
+            lowerBoundTopPair32(t.P, n32, t.Pidx, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
         }
         // the child-level cells, each inside its parent-level cell's run
         const int a = int(lane & 3u) - 1, b = int((lane >> 2) & 3u) - 1, c = int(lane >> 4) - 1;
@@ -874,12 +901,12 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsi
         const int parent = third(a) + 3 * third(b) + 9 * third(c);
         // the child's octant inside its parent: the coordinate bit at level l
         const unsigned raw = unsigned((a + 2) & 1) << 2 | unsigned((b + 2) & 1) << 1 | unsigned((c + 2) & 1);
-        const std::size_t lo = __shfl(pa, parent, 64), hi = __shfl(pb, parent, 64);
+        const std::uint32_t lo = __shfl(pa, parent, 64), hi = __shfl(pb, parent, 64);
         const CellCode home{__shfl(pc.key, parent, 64), unsigned(__shfl(int(pc.state), parent, 64))};
         const Key ck = childKey(home, raw, level, hil, hl.enc);
-        std::size_t r0, r1;
-        lowerBoundPairIn(t.P, lo, hi, ck, ck + nodeRange(level), r0, r1);
-        const std::uint32_t pop = std::uint32_t(r1 - r0);
+        std::uint32_t r0, r1;
+        lowerBoundPairIn32(t.P, lo, hi, ck, ck + nodeRange(level), r0, r1);
+        const std::uint32_t pop = r1 - r0;
         // child `lane` (lanes 0..7) of the group: its octant inside the parent,
         // from its digit at level l and the parent's state
         const int leaf = leaf0 + int(lane & 7u);
@@ -1421,17 +1448,32 @@ __device__ inline int treeOfBlock(const TreeSet &s, unsigned b)
     return lo;
 }
 
+// The member's TreeDev in LDS for the block: the bodies index its buffer
+// pairs by the active half at run time, which a copy in registers would
+// place in per-lane scratch memory. Every thread of the block calls it.
+__device__ inline void stageTreeDev(TreeDev &lds, const TreeDev &src)
+{
+    static_assert(sizeof(TreeDev) % 4 == 0, "TreeDev is copied as 32-bit words");
+    const std::uint32_t *from = reinterpret_cast<const std::uint32_t *>(&src);
+    std::uint32_t *to = reinterpret_cast<std::uint32_t *>(&lds);
+    for (unsigned w = threadIdx.x; w < sizeof(TreeDev) / 4; w += blockDim.x) to[w] = from[w];
+    __syncthreads();
+}
+
 // Every block walks the virtual blocks b = blockIdx.x, + gridDim.x, ... below
 // the set's total (one step when the grid equals a host-planned total): the
 // tree k that owns b, its TreeDev t and b's index `block` among the tree's
 // `blocks`. b is block-uniform, and each step ends with a barrier, so the
 // bodies' shared memory is free again for the next step.
 #define AT_MULTI_LOOP(BODY)                                                                                            \
+    __shared__ alignas(alignof(TreeDev)) unsigned char multiTreeBytes[sizeof(TreeDev)];                               \
+    TreeDev &multiTree = *reinterpret_cast<TreeDev *>(multiTreeBytes);                                                 \
     const unsigned virtualBlocks = s.total ? *s.total : s.blocks;                                                      \
     for (unsigned vb = blockIdx.x; vb < virtualBlocks; vb += gridDim.x)                                                \
     {                                                                                                                  \
         const int k = treeOfBlock(s, vb);                                                                              \
-        const TreeDev t = s.trees[k];                                                                                  \
+        stageTreeDev(multiTree, s.trees[k]);                                                                           \
+        const TreeDev &t = multiTree;                                                                                  \
         const unsigned block = vb - s.start[k];                                                                        \
         const unsigned blocks = s.start[k + 1] - s.start[k];                                                           \
         (void)block;                                                                                                   \
