@@ -851,14 +851,17 @@ __device__ inline void nearField(const Key *P, std::size_t n, const Key *Q, cons
     center = __shfl(pop, 13, 32);
 }
 
-// One wave per complete sibling group at level l >= 1: lanes 0..26 search
-// the 27 level-(l-1) cells around the parent (the parent's own stencil);
-// lane c then counts the level-l cell (c%4 - 1, c/4%4 - 1, c/16 - 1) around
-// the parent's corner (the 4x4x4 block holding the 27-cell stencils of all
-// 8 children) inside its parent-level run, and lanes 0..7 sum their child's
-// 27 cells. Items flagged for it also get the parent's merge test (its NF
-// from the same 27 parent-level searches). Same cells and the same u64
-// products as the per-node stencil.
+// Two complete sibling groups (level l >= 1) per wave, one per half-wave:
+// lanes h*32 + 0..26 name the 27 level-(l-1) cells around group h's parent
+// (the parent's own stencil) and search them. The second group's cells that
+// are also the first group's (consecutive groups lie close in key order)
+// take the first group's bounds instead of searching again. Then, for each
+// group in turn, lane c counts the level-l cell (c%4 - 1, c/4%4 - 1,
+// c/16 - 1) around the parent's corner (the 4x4x4 block holding the 27-cell
+// stencils of all 8 children) inside its parent-level run, and lanes 0..7
+// sum their child's 27 cells. Items flagged for it also get the parent's
+// merge test (its NF from the same 27 parent-level searches). Same cells and
+// the same u64 products as the per-node stencil.
 __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsigned block, unsigned blocks)
 {
     __shared__ Key topKeys[kTopEntries];
@@ -866,18 +869,22 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsi
     DevState *st = t.st;
     if (st->converged) return;
     const std::uint32_t total = t.nfCount[1];
-    if (block * (blockDim.x / 64) >= total) return; // no item for this block
+    const std::uint32_t wavesPerBlock = blockDim.x / 64;
+    if (2 * block * wavesPerBlock >= total) return; // no item for this block
     if (hilbert) stageHilbert(hl);
     const TopTable tab = loadTopTable(topKeys, t.Pidx, indexSize(t.n));
     const int ab = st->activeBuffer;
     const Key *tree = t.K[ab];
     const unsigned lane = __lane_id();
-    const std::uint32_t waves = blocks * (blockDim.x / 64);
+    const unsigned half = lane >> 5, hlane = lane & 31u;
+    const std::uint32_t waves = blocks * wavesPerBlock;
     const bool hil = hilbert != 0;
     const std::uint32_t n32 = std::uint32_t(t.n);
-    for (std::uint32_t g = block * (blockDim.x / 64) + (threadIdx.x >> 6); g < total; g += waves)
+    for (std::uint32_t g2 = 2 * (block * wavesPerBlock + (threadIdx.x >> 6)); g2 < total; g2 += 2 * waves)
     {
-        const std::uint32_t item = t.nfGroups[g];
+        // group `half` of the pair (the second half repeats the first when the pair has one group)
+        const bool hasB = g2 + 1 < total;
+        const std::uint32_t item = t.nfGroups[half && hasB ? g2 + 1 : g2];
         const int leaf0 = int(item >> 1);
         const Key start = tree[leaf0];
         const unsigned level = treeLevel(tree[leaf0 + 1] - start);
@@ -886,57 +893,89 @@ __device__ void nfGroupBody(const TreeDev &t, int hilbert, unsigned bucket, unsi
         unsigned bxu, byu, bzu, pst;
         cellCorner(start, level - 1, hil, hl.dec, bxu, byu, bzu, pst);
         const int bx = int(bxu), by = int(byu), bz = int(bzu);
-        std::uint32_t pa = 0, pb = 0;
+        const int dx = int(hlane % 3) - 1, dy = int((hlane / 3) % 3) - 1, dz = int(hlane / 9) - 1;
         CellCode pc{0, 0};
-        if (lane < 27)
-        {
-            const int dx = int(lane % 3) - 1, dy = int((lane / 3) % 3) - 1, dz = int(lane / 9) - 1;
+        if (hlane < 27)
             pc = cellCode(unsigned(wrapAxis(bx + 2 * dx * w)), unsigned(wrapAxis(by + 2 * dy * w)),
                           unsigned(wrapAxis(bz + 2 * dz * w)), level - 1, hil, hl.enc);
-            lowerBoundTopPair32(t.P, n32, t.Pidx, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
+        // a cell of the second group that is also one of the first group's:
+        // its lane in the first half (same level; per axis the offset from
+        // the first parent, over the level-(l-1) grid of G cells per axis)
+        int from = -1;
+        const unsigned levelA = __shfl(level, 0, 64);
+        const int axA = __shfl(bx, 0, 64), ayA = __shfl(by, 0, 64), azA = __shfl(bz, 0, 64);
+        if (half && hasB && hlane < 27 && level == levelA)
+        {
+            const int G = kAxis / (2 * w);
+            auto near = [&](int cB, int cA, int d) {
+                // the offset e in -1..1 with (cA/2w + e) = (cB/2w + d) mod G, or 2 if none
+                const int target = ((cB / (2 * w) + d) % G + G) % G;
+                for (int e = -1; e <= 1; ++e)
+                    if ((((cA / (2 * w) + e) % G + G) % G) == target) return e;
+                return 2;
+            };
+            const int ex = near(bx, axA, dx), ey = near(by, ayA, dy), ez = near(bz, azA, dz);
+            if (ex != 2 && ey != 2 && ez != 2) from = (ex + 1) + 3 * (ey + 1) + 9 * (ez + 1);
         }
-        // the child-level cells, each inside its parent-level cell's run
+        std::uint32_t pa = 0, pb = 0;
+        if (hlane < 27 && from < 0 && (half == 0 || hasB))
+            lowerBoundTopPair32(t.P, n32, t.Pidx, tab, pc.key, pc.key + nodeRange(level - 1), pa, pb);
+        {
+            const int src = from >= 0 ? from : int(lane);
+            const std::uint32_t qa = __shfl(pa, src, 64), qb = __shfl(pb, src, 64);
+            pa = qa;
+            pb = qb;
+        }
+        // each group in turn: the child-level cells, each inside its
+        // parent-level cell's run
         const int a = int(lane & 3u) - 1, b = int((lane >> 2) & 3u) - 1, c = int(lane >> 4) - 1;
         auto third = [](int o) { return o < 0 ? 0 : (o < 2 ? 1 : 2); };
-        const int parent = third(a) + 3 * third(b) + 9 * third(c);
         // the child's octant inside its parent: the coordinate bit at level l
         const unsigned raw = unsigned((a + 2) & 1) << 2 | unsigned((b + 2) & 1) << 1 | unsigned((c + 2) & 1);
-        const std::uint32_t lo = __shfl(pa, parent, 64), hi = __shfl(pb, parent, 64);
-        const CellCode home{__shfl(pc.key, parent, 64), unsigned(__shfl(int(pc.state), parent, 64))};
-        const Key ck = childKey(home, raw, level, hil, hl.enc);
-        std::uint32_t r0, r1;
-        lowerBoundPairIn32(t.P, lo, hi, ck, ck + nodeRange(level), r0, r1);
-        const std::uint32_t pop = r1 - r0;
-        // child `lane` (lanes 0..7) of the group: its octant inside the parent,
-        // from its digit at level l and the parent's state
-        const int leaf = leaf0 + int(lane & 7u);
-        const unsigned digit = unsigned(tree[leaf] >> (3u * (kOctLevels - level))) & 7u;
-        const unsigned oct = hil ? unsigned(hl.dec[8 * pst + digit]) & 7u : digit;
-        const int rx = int(oct >> 2), ry = int((oct >> 1) & 1u), rz = int(oct & 1u);
-        std::uint64_t sum = 0;
-#pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                for (int dx = 0; dx < 3; ++dx)
-                    sum += __shfl(pop, (rx + dx) + 4 * (ry + dy) + 16 * (rz + dz), 64);
-        const std::uint64_t own = __shfl(pop, (rx + 1) + 4 * (ry + 1) + 16 * (rz + 1), 64);
-        if (lane < 8)
+        for (unsigned h = 0; h < (hasB ? 2u : 1u); ++h)
         {
-            const std::uint64_t nf = own * sum;
-            t.N[ab][leaf] = own > 0xffffffffull ? 0xffffffffu : unsigned(own);
-            t.NF[ab][leaf] = nf > 0xffffffffull ? 0xffffffffu : unsigned(nf);
-        }
-        if (item & 1u)
-        {
-            // the parent's merge test: its population times its 27 cells'
-            // (groupCanMerge, compared unclamped)
-            std::uint64_t pp = lane < 27 ? std::uint64_t(pb - pa) : 0;
-            const std::uint64_t pown = __shfl(pp, 13, 64);
+            const int base = int(32 * h);
+            const std::uint32_t itemH = __shfl(item, base, 64);
+            const int leaf0H = int(itemH >> 1);
+            const unsigned levelH = __shfl(level, base, 64), pstH = __shfl(pst, base, 64);
+            const int parent = base + third(a) + 3 * third(b) + 9 * third(c);
+            const std::uint32_t lo = __shfl(pa, parent, 64), hi = __shfl(pb, parent, 64);
+            const CellCode home{__shfl(pc.key, parent, 64), unsigned(__shfl(int(pc.state), parent, 64))};
+            const Key ck = childKey(home, raw, levelH, hil, hl.enc);
+            std::uint32_t r0, r1;
+            lowerBoundPairIn32(t.P, lo, hi, ck, ck + nodeRange(levelH), r0, r1);
+            const std::uint32_t pop = r1 - r0;
+            // child `lane` (lanes 0..7) of the group: its octant inside the
+            // parent, from its digit at level l and the parent's state
+            const int leaf = leaf0H + int(lane & 7u);
+            const unsigned digit = unsigned(tree[leaf] >> (3u * (kOctLevels - levelH))) & 7u;
+            const unsigned oct = hil ? unsigned(hl.dec[8 * pstH + digit]) & 7u : digit;
+            const int rx = int(oct >> 2), ry = int((oct >> 1) & 1u), rz = int(oct & 1u);
+            std::uint64_t sum = 0;
 #pragma unroll
-            for (int m = 32; m >= 1; m >>= 1) pp += __shfl_xor(pp, m, 64);
-            if (lane == 0) t.gcm[ab][leaf0] = pown * pp <= std::uint64_t(bucket) ? 1 : 0;
+            for (int ez = 0; ez < 3; ++ez)
+#pragma unroll
+                for (int ey = 0; ey < 3; ++ey)
+#pragma unroll
+                    for (int ex = 0; ex < 3; ++ex)
+                        sum += __shfl(pop, (rx + ex) + 4 * (ry + ey) + 16 * (rz + ez), 64);
+            const std::uint64_t own = __shfl(pop, (rx + 1) + 4 * (ry + 1) + 16 * (rz + 1), 64);
+            if (lane < 8)
+            {
+                const std::uint64_t nf = own * sum;
+                t.N[ab][leaf] = own > 0xffffffffull ? 0xffffffffu : unsigned(own);
+                t.NF[ab][leaf] = nf > 0xffffffffull ? 0xffffffffu : unsigned(nf);
+            }
+            if (itemH & 1u)
+            {
+                // the parent's merge test: its population times its 27 cells'
+                // (groupCanMerge, compared unclamped)
+                std::uint64_t pp = half == h && hlane < 27 ? std::uint64_t(pb - pa) : 0;
+                const std::uint64_t pown = __shfl(pp, base + 13, 64);
+#pragma unroll
+                for (int m = 32; m >= 1; m >>= 1) pp += __shfl_xor(pp, m, 64);
+                if (lane == 0) t.gcm[ab][leaf0H] = pown * pp <= std::uint64_t(bucket) ? 1 : 0;
+            }
         }
     }
 }
