@@ -169,6 +169,7 @@ public:
         computeParticleKeys();
         buildBegin();
         octInitRoot(stream_, dev());
+        clearedStatus_ = status_.p;
         buildPasses(0);
         finishBuild();
     }
@@ -182,6 +183,7 @@ public:
         s.converged = 0;
         *sth_ = s;
         AT_HIP(hipMemcpyAsync(st_, sth_, sizeof(DevState), hipMemcpyHostToDevice, stream_));
+        clearedStatus_ = nullptr; // the rejected decision left its scan status behind
         buildPasses(decided);
         finishBuild();
     }
@@ -418,14 +420,21 @@ private:
         for (unsigned i = first; i < maxDepth_; ++i)
         {
             buildPrePass();
-            AT_HIP(hipMemsetAsync(status_.p, 0, status_.cap * sizeof(std::uint64_t), stream_));
-            AT_HIP(hipMemsetAsync(ticket_.p, 0, sizeof(std::uint32_t), stream_));
+            // the scan status and ticket are zero after the root's init and
+            // after every commit; a status buffer regrown since is cleared
+            if (status_.p != clearedStatus_)
+            {
+                AT_HIP(hipMemsetAsync(status_.p, 0, status_.cap * sizeof(std::uint64_t), stream_));
+                AT_HIP(hipMemsetAsync(ticket_.p, 0, sizeof(std::uint32_t), stream_));
+                clearedStatus_ = status_.p;
+            }
             TreeDev t = dev();
             octDecideScan(stream_, t, p, LONG_MAX, buildL_);
             readState();
             if (!buildAfterDecide(*sth_)) break;
             t = dev();
             octProcessCommit(stream_, t, p, buildL_);
+            clearedStatus_ = status_.p;
             octRefresh(stream_, t, p, false, buildNewL_);
             buildAfterProcess();
         }
@@ -619,6 +628,7 @@ private:
     DevBuf<std::uint8_t> gcm_[2], nfDirty_, groupDirty_, countDirty_;
     DevBuf<std::int64_t> offs_;
     DevBuf<std::uint64_t> status_;
+    std::uint64_t *clearedStatus_ = nullptr; // status_ as last cleared by the root's init, a commit or a memset
     DevBuf<std::uint32_t> ticket_, tasks_, groups_, taskCount_;
     ViewEngine view_;
 };
