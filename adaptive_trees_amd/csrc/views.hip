@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kT) void layoutKeysKernel(const Key *S, std::size_t
 
 __global__ __launch_bounds__(kT) void materializeKernel(const Key *S, const std::uint32_t *layoutVal,
                                                         std::size_t icap, const DevState *st, unsigned bits,
-                                                        const int *offInternal, int *mBySfc, Key *layoutPh)
+                                                        const int *offInternal, int *mBySfc)
 {
     const int li = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (li >= activeInternals(int(icap), st, bits)) return;
@@ -188,7 +188,6 @@ __global__ __launch_bounds__(kT) void materializeKernel(const Key *S, const std:
     const Key ph = S[sp];
     const unsigned lvl = phLevel(ph, bits);
     mBySfc[sp] = li - offInternal[lvl];
-    layoutPh[li] = ph;
 }
 
 __device__ inline int finalNodeIndex(Key ph, unsigned lvl, unsigned bits, const int *levelOffset,
@@ -215,15 +214,18 @@ struct Rows
 // start and end keys are leaf boundaries): begin = leafBegin[first], count =
 // leafBegin[end] - begin, with leafBegin[L] = the active count. Equal to
 // lower_bound(P, start) and lower_bound(P, end) - lower_bound(P, start).
-__global__ __launch_bounds__(kT) void internalRowsKernel(const Key *layoutPh, std::size_t icap, const DevState *st,
-                                                         unsigned bits, const int *levelOffset, const int *offInternal,
+// Threads take the internal nodes in S order (by level, key order inside a
+// level), so a wave's searches in S and K walk neighbouring keys; the rows
+// land at the layout positions.
+__global__ __launch_bounds__(kT) void internalRowsKernel(std::size_t icap, const DevState *st, unsigned bits,
+                                                         const int *levelOffset, const int *offInternal,
                                                          const int *internalPer, const Key *S, const int *mBySfc,
                                                          const Key *K0, const Key *K1, const std::uint32_t *leafBegin,
                                                          int withCounts, Rows r)
 {
-    const int idx = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (idx >= activeInternals(int(icap), st, bits)) return;
-    const Key ph = layoutPh[idx];
+    const int sp = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (sp >= activeInternals(int(icap), st, bits)) return;
+    const Key ph = S[sp];
     const unsigned lvl = phLevel(ph, bits);
     const Key start = phStart(ph, lvl, bits);
     const int node = finalNodeIndex(ph, lvl, bits, levelOffset, offInternal, internalPer, S, mBySfc);
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(kT) void internalRowsKernel(const Key *layoutPh, st
     r.depth[node] = std::uint8_t(lvl);
     r.split[node] = 1;
     if (lvl == 0) r.parent[node] = -1;
-    const int m = idx - offInternal[lvl];
+    const int m = mBySfc[sp];
     const int M = internalPer[lvl];
     r.child[node] = levelOffset[lvl + 1] + m;
     for (int s = 0; s < (1 << bits); ++s) r.parent[levelOffset[lvl + 1] + s * M + m] = node;
@@ -425,7 +427,7 @@ struct ViewEngine::Impl
     // scratch
     DevBuf<int> leafToNode, leavesPerLevel, internalPer, offInternal, mBySfc;
     DevBuf<std::uint8_t> leafLevel;
-    DevBuf<Key> cand, layoutKey, layoutPh;
+    DevBuf<Key> cand, layoutKey;
     DevBuf<std::uint32_t> layoutVal, leafBegin, candCount, scanTicket;
     DevBuf<std::uint64_t> scanStatus;
     SortWorkspace sort;
@@ -491,7 +493,6 @@ void ViewEngine::prepare(std::size_t leafCapacity, std::size_t numParticles, boo
     m.cand.ensure(ic, 0, s);
     m.layoutKey.ensure(ic, 0, s);
     m.layoutVal.ensure(ic, 0, s);
-    m.layoutPh.ensure(ic, 0, s);
     m.mBySfc.ensure(ic, 0, s);
     m.candCount.ensure(1, 0, s);
     m.scanTicket.ensure(1, 0, s);
@@ -552,9 +553,8 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
                            m.layoutVal.p);
         if (icap > 1) sortPairs(s, m.layoutKey.p, m.layoutVal.p, icap, m.sort, false, false);
         hipLaunchKernelGGL(materializeKernel, dim3(ib), dim3(kT), 0, s, m.cand.p, m.layoutVal.p, icap, st, bits_,
-                           m.offInternal.p, m.mBySfc.p, m.layoutPh.p);
-        hipLaunchKernelGGL(internalRowsKernel, dim3(ib), dim3(kT), 0, s, m.layoutPh.p, icap, st, bits_,
-                           m.levelOffset.p, m.offInternal.p, m.internalPer.p, m.cand.p, m.mBySfc.p, K[0], K[1],
+                           m.offInternal.p, m.mBySfc.p);
+        hipLaunchKernelGGL(internalRowsKernel, dim3(ib), dim3(kT), 0, s, icap, st, bits_, m.levelOffset.p, m.offInternal.p, m.internalPer.p, m.cand.p, m.mBySfc.p, K[0], K[1],
                            m.leafBegin.p, counts ? 1 : 0, rows);
     }
     hipLaunchKernelGGL(leafRowsKernel, dim3(leafBlocks), dim3(kT), 0, s, K[0], K[1], N[0], N[1], cap, st, bits_,
