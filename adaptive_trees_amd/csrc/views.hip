@@ -250,10 +250,16 @@ __global__ __launch_bounds__(kT) void internalRowsKernel(std::size_t icap, const
     }
 }
 
+// Particles per block of the particle -> leaf-node mapping.
+constexpr int kMapChunk = kT * 8;
+
 // Exclusive scan of N over the live leaves (chained scan, 2048 per tile).
+// With chunkLeaf, each leaf t with particles [b, e) also records itself as
+// the leaf of every mapping chunk that starts inside it: chunkLeaf[c] = t
+// for c * kMapChunk in [b, e).
 __global__ __launch_bounds__(kT) void leafBeginKernel(const std::uint32_t *N0, const std::uint32_t *N1, int cap,
                                                       const DevState *st, std::uint32_t *out, std::uint64_t *status,
-                                                      std::uint32_t *ticket)
+                                                      std::uint32_t *ticket, std::uint32_t *chunkLeaf)
 {
     constexpr int kItems = 8;
     __shared__ std::uint32_t tileShared;
@@ -325,6 +331,9 @@ __global__ __launch_bounds__(kT) void leafBeginKernel(const std::uint32_t *N0, c
     for (int k = 0; k < kItems; ++k)
     {
         if (first + k <= L) out[first + k] = std::uint32_t(run); // [L]: the total
+        if (chunkLeaf && first + k < L)
+            for (std::uint64_t c = (run + kMapChunk - 1) / kMapChunk; c * kMapChunk < run + v[k]; ++c)
+                chunkLeaf[c] = std::uint32_t(first + k);
         run += v[k];
     }
 }
@@ -368,14 +377,16 @@ __global__ __launch_bounds__(kT) void leafRowsKernel(const Key *K0, const Key *K
 // maps to the view node of the leaf holding P[p], or to 2^64-1 when P[p] is
 // escaped. The leaves partition the active particles [0, A) in order: leaf
 // t holds [leafBegin[t], leafBegin[t+1]), so p's leaf is the last t with
-// leafBegin[t] <= p. Each block takes kMapChunk consecutive particles,
-// finds the leaves [t0, t1] they span, stages those begins in LDS when they
-// fit (else searches them in place), and writes its particles in order;
-// block work is fixed whatever the leaf sizes.
-constexpr int kMapChunk = kT * 8;
+// leafBegin[t] <= p. Each block takes kMapChunk consecutive particles; the
+// leaves [t0, t1] they span come from chunkLeaf (the leaf holding the
+// chunk's first particle, and the one holding the next chunk's, or the
+// last leaf); it stages those begins in LDS when they fit (else searches
+// them in place), and writes its particles in order; block work is fixed
+// whatever the leaf sizes.
 constexpr int kMapStage = 2048;
 __global__ __launch_bounds__(kT) void mappingKernel(std::size_t n, int cap, const DevState *st,
-                                                    const std::uint32_t *leafBegin, const int *leafToNode, Key *out)
+                                                    const std::uint32_t *leafBegin, const std::uint32_t *chunkLeaf,
+                                                    const int *leafToNode, Key *out)
 {
     __shared__ std::uint32_t begins[kMapStage];
     __shared__ std::uint32_t span[2];
@@ -390,8 +401,9 @@ __global__ __launch_bounds__(kT) void mappingKernel(std::size_t n, int cap, cons
     {
         if (threadIdx.x == 0)
         {
-            span[0] = std::uint32_t(upperBound(leafBegin, std::size_t(L), std::uint32_t(c0)) - 1);
-            span[1] = std::uint32_t(upperBound(leafBegin, std::size_t(L), std::uint32_t(c1 - 1)) - 1);
+            // c0 < active: a leaf holds c0; c1 < active: a leaf holds c1 = the next chunk's start
+            span[0] = chunkLeaf[blockIdx.x];
+            span[1] = c1 < active ? chunkLeaf[blockIdx.x + 1] : std::uint32_t(L - 1);
         }
         __syncthreads();
         const std::uint32_t t0 = span[0], m = span[1] - t0 + 1;
@@ -429,6 +441,7 @@ struct ViewEngine::Impl
     DevBuf<std::uint8_t> leafLevel;
     DevBuf<Key> cand, layoutKey;
     DevBuf<std::uint32_t> layoutVal, leafBegin, candCount, scanTicket;
+    DevBuf<std::uint32_t> chunkLeaf; // per mapping chunk: the leaf holding its first particle
     DevBuf<std::uint64_t> scanStatus;
     SortWorkspace sort;
 };
@@ -482,7 +495,11 @@ void ViewEngine::prepare(std::size_t leafCapacity, std::size_t numParticles, boo
         m.begin.ensure(m.shape[Begin].cap, m.begin.cap, s);
         m.counts.ensure(m.shape[Counts].cap, m.counts.cap, s);
     }
-    if (prepMapping_) m.mapping.ensure(m.shape[Mapping].cap, m.mapping.cap, s);
+    if (prepMapping_)
+    {
+        m.mapping.ensure(m.shape[Mapping].cap, m.mapping.cap, s);
+        m.chunkLeaf.ensure(particleCap_ / kMapChunk + 2, 0, s);
+    }
     m.leafToNode.ensure(leafCap_, m.leafToNode.cap, s);
     m.leavesPerLevel.ensure(64, 0, s);
     m.internalPer.ensure(64, 0, s);
@@ -543,7 +560,7 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
         hipLaunchKernelGGL(clearScanKernel, dim3(ceilDiv(words, kT)), dim3(kT), 0, s, m.scanStatus.p, words,
                            m.scanTicket.p);
         hipLaunchKernelGGL(leafBeginKernel, dim3(unsigned(leafCap_ / (kT * 8) + 1)), dim3(kT), 0, s, N[0], N[1], cap,
-                           st, m.leafBegin.p, m.scanStatus.p, m.scanTicket.p);
+                           st, m.leafBegin.p, m.scanStatus.p, m.scanTicket.p, mapping ? m.chunkLeaf.p : nullptr);
     }
     if (icap > 0)
     {
@@ -562,7 +579,7 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
                        m.leafBegin.p, counts ? 1 : 0, rows);
     if (mapping && n)
         hipLaunchKernelGGL(mappingKernel, dim3(ceilDiv(n, kMapChunk)), dim3(kT), 0, s, n, cap, st, m.leafBegin.p,
-                           m.leafToNode.p, m.mapping.p);
+                           m.chunkLeaf.p, m.leafToNode.p, m.mapping.p);
     AT_HIP(hipGetLastError());
 }
 
