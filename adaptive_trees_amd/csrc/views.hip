@@ -8,7 +8,7 @@
 //   classify   leaf levels, per-level histogram, and one placeholder key per
 //              internal node (the boundary between its child 0 and child 1
 //              subtrees, tree_view.cu:151-203), appended in any order;
-//   metadata   level offsets from the histogram (one thread, <= 64 levels);
+//   metadata   level offsets from the histogram (one wave, <= 64 levels);
 //   sort S     internal placeholder keys: sorted = grouped by level, SFC
 //              order inside a level (replaces the stable level sort);
 //   layout     sort (level | reversed path) keys to get m for every node;
@@ -131,34 +131,51 @@ __global__ __launch_bounds__(kT) void classifyKernel(const Key *K0, const Key *K
     if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&leavesPerLevel[threadIdx.x], hist[threadIdx.x]);
 }
 
-// buildLevelMetadataKernel (tree_view.cu:211-275)
-__global__ void metadataKernel(const int *leavesPerLevel, int *internalPer, int *levelOffset, int *offInternal,
-                               std::uint8_t *maxDepth, int cap, const DevState *st, unsigned bits)
+// buildLevelMetadataKernel (tree_view.cu:211-275), one wave: lane l holds
+// level l. The node counts per level follow from the deepest level up
+// (internal[l] = nodes[l+1] / children, nodes[l] = leaves[l] + internal[l]),
+// one lane over LDS; the offsets are prefix sums across the wave.
+__global__ __launch_bounds__(64) void metadataKernel(const int *leavesPerLevel, int *internalPer, int *levelOffset,
+                                                     int *offInternal, std::uint8_t *maxDepth, int cap,
+                                                     const DevState *st, unsigned bits)
 {
+    __shared__ int leaves[64], internal[64], nodes[64];
     if (activeLeaves(cap, st) == 0) return;
     const unsigned kMax = 63u / bits;
     const int children = 1 << bits;
-    int nodesPer[64];
-    unsigned maxLevel = 0;
-    for (unsigned l = 0; l <= kMax; ++l)
+    const unsigned l = threadIdx.x;
+    const int lv = l <= kMax ? leavesPerLevel[l] : 0;
+    leaves[l] = lv;
+    internal[l] = 0;
+    nodes[l] = 0;
+    const std::uint64_t nonEmpty = __ballot(lv != 0);
+    const unsigned maxLevel = nonEmpty ? 63u - unsigned(__clzll((long long)nonEmpty)) : 0u;
+    __syncthreads();
+    if (l == 0)
     {
-        if (leavesPerLevel[l] != 0) maxLevel = l;
-        internalPer[l] = 0;
-        nodesPer[l] = 0;
+        *maxDepth = std::uint8_t(maxLevel);
+        nodes[maxLevel] = leaves[maxLevel];
+        for (int k = int(maxLevel) - 1; k >= 0; --k)
+        {
+            internal[k] = nodes[k + 1] / children;
+            nodes[k] = leaves[k] + internal[k];
+        }
     }
-    *maxDepth = std::uint8_t(maxLevel);
-    nodesPer[maxLevel] = leavesPerLevel[maxLevel];
-    for (int l = int(maxLevel) - 1; l >= 0; --l)
+    __syncthreads();
+    int n = nodes[l], in = internal[l];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
     {
-        internalPer[l] = nodesPer[l + 1] / children;
-        nodesPer[l] = leavesPerLevel[l] + internalPer[l];
+        const int tn = __shfl_up(n, o, 64), ti = __shfl_up(in, o, 64);
+        if (l >= unsigned(o)) n += tn, in += ti;
     }
-    levelOffset[0] = 0;
-    for (unsigned l = 0; l <= maxLevel; ++l) levelOffset[l + 1] = levelOffset[l] + nodesPer[l];
-    for (unsigned l = maxLevel + 1; l <= kMax; ++l) levelOffset[l + 1] = levelOffset[maxLevel + 1];
-    offInternal[0] = 0;
-    for (unsigned l = 0; l <= maxLevel; ++l) offInternal[l + 1] = offInternal[l] + internalPer[l];
-    for (unsigned l = maxLevel + 1; l <= kMax; ++l) offInternal[l + 1] = offInternal[maxLevel + 1];
+    if (l <= kMax)
+    {
+        internalPer[l] = internal[l];
+        levelOffset[l + 1] = n;
+        offInternal[l + 1] = in;
+    }
+    if (l == 0) levelOffset[0] = offInternal[0] = 0;
 }
 
 __global__ __launch_bounds__(kT) void layoutKeysKernel(const Key *S, std::size_t icap, const DevState *st,
@@ -552,7 +569,7 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
                        int(maxLevel_ + 1), m.candCount.p, m.cand.p, icap);
     hipLaunchKernelGGL(classifyKernel, dim3(leafBlocks), dim3(kT), 0, s, K[0], K[1], st, cap, bits_, m.leafLevel.p,
                        m.leavesPerLevel.p, m.cand.p, m.candCount.p);
-    hipLaunchKernelGGL(metadataKernel, dim3(1), dim3(1), 0, s, m.leavesPerLevel.p, m.internalPer.p, m.levelOffset.p,
+    hipLaunchKernelGGL(metadataKernel, dim3(1), dim3(64), 0, s, m.leavesPerLevel.p, m.internalPer.p, m.levelOffset.p,
                        m.offInternal.p, depth_, cap, st, bits_);
     if (counts || mapping)
     {
