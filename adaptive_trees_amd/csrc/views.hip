@@ -70,64 +70,93 @@ __global__ void prepKernel(int *leavesPerLevel, int levels, unsigned *candCount,
     for (std::size_t j = i; j < icap; j += std::size_t(gridDim.x) * blockDim.x) cand[j] = kInvalidKey;
 }
 
+// Leaf levels and their histogram, and one placeholder key per internal
+// node (emitted at the boundary between its child-0 and child-1 subtrees).
+// A block takes kClassifyRows rows of kT leaves: each wave adds a row's
+// levels to the block histogram with one LDS atomic per distinct level, and
+// the block's candidates take their places with one atomic per block (the
+// list's order is free: it is sorted next).
+constexpr int kClassifyRows = 8;
 __global__ __launch_bounds__(kT) void classifyKernel(const Key *K0, const Key *K1, DevState *st, int cap,
                                                      unsigned bits, std::uint8_t *leafLevel, int *leavesPerLevel,
                                                      Key *cand, unsigned *candCount)
 {
     __shared__ int hist[64];
+    __shared__ unsigned waveCount[kClassifyRows][kT / 64], blockBase;
     if (threadIdx.x < 64) hist[threadIdx.x] = 0;
     __syncthreads();
     const Key *K = st->activeBuffer ? K1 : K0;
     const int L = activeLeaves(cap, st);
-    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i == 0)
+    if (blockIdx.x == 0 && threadIdx.x == 0)
     {
         if (st->numLeaves > cap) st->needsResize = 1;
         else st->numNodes = L + int(unsigned(L - 1) / ((1u << bits) - 1));
     }
-    bool emit = false;
-    Key ph = 0;
-    if (i < L)
-    {
-        const unsigned lvl = viewLevel(K[i + 1] - K[i], bits);
-        leafLevel[i] = std::uint8_t(lvl);
-        atomicAdd(&hist[lvl], 1);
-    }
-    if (i + 1 < L)
-    {
-        const Key a = K[i], b = K[i + 1];
-        const int cp = clz64(a ^ b) - 1;
-        const unsigned maxLevel = 63u / bits;
-        unsigned lvl = unsigned(cp) / bits;
-        if (lvl > maxLevel) lvl = maxLevel;
-        const unsigned slot = unsigned(b >> (63 - bits * (lvl + 1))) & ((1u << bits) - 1);
-        if (slot == 1)
-        {
-            emit = true;
-            ph = placeholderKey(b & ~(viewRange(lvl, bits) - 1), int(bits * lvl));
-        }
-    }
-    // the block's candidates take their places with one atomic per block
-    // (the list's order is free: it is sorted next)
-    __shared__ unsigned waveCount[kT / 64], blockBase;
-    const std::uint64_t bal = __ballot(emit);
     const unsigned lane = __lane_id();
     const int wave = int(threadIdx.x >> 6);
-    if (lane == 0) waveCount[wave] = unsigned(__popcll(bal));
+    const unsigned maxLevel = 63u / bits;
+    Key ph[kClassifyRows];
+    std::uint32_t emitRows = 0;
+#pragma unroll
+    for (int r = 0; r < kClassifyRows; ++r)
+    {
+        const int i = int(blockIdx.x) * kT * kClassifyRows + r * kT + int(threadIdx.x);
+        int lvl = -1;
+        if (i < L)
+        {
+            lvl = int(viewLevel(K[i + 1] - K[i], bits));
+            leafLevel[i] = std::uint8_t(lvl);
+        }
+        std::uint64_t pending = __ballot(lvl >= 0);
+        while (pending)
+        {
+            const int leader = __ffsll((long long)pending) - 1;
+            const int l0 = __shfl(lvl, leader, 64);
+            const std::uint64_t same = __ballot(lvl == l0) & pending;
+            if (int(lane) == leader) atomicAdd(&hist[l0], __popcll(same));
+            pending &= ~same;
+        }
+        ph[r] = 0;
+        bool emit = false;
+        if (i + 1 < L)
+        {
+            const Key a = K[i], b = K[i + 1];
+            const int cp = clz64(a ^ b) - 1;
+            unsigned level = unsigned(cp) / bits;
+            if (level > maxLevel) level = maxLevel;
+            const unsigned slot = unsigned(b >> (63 - bits * (level + 1))) & ((1u << bits) - 1);
+            if (slot == 1)
+            {
+                emit = true;
+                ph[r] = placeholderKey(b & ~(viewRange(level, bits) - 1), int(bits * level));
+            }
+        }
+        emitRows |= emit ? 1u << r : 0u;
+        const std::uint64_t bal = __ballot(emit);
+        if (lane == 0) waveCount[r][wave] = unsigned(__popcll(bal));
+    }
     __syncthreads();
     if (threadIdx.x == 0)
     {
         unsigned total = 0;
-        for (int w = 0; w < kT / 64; ++w)
-        {
-            const unsigned c = waveCount[w];
-            waveCount[w] = total;
-            total += c;
-        }
+        for (int r = 0; r < kClassifyRows; ++r)
+            for (int w = 0; w < kT / 64; ++w)
+            {
+                const unsigned c = waveCount[r][w];
+                waveCount[r][w] = total;
+                total += c;
+            }
         blockBase = total ? atomicAdd(candCount, total) : 0u;
     }
     __syncthreads();
-    if (emit) cand[blockBase + waveCount[wave] + __popcll(bal & ((std::uint64_t(1) << lane) - 1))] = ph;
+    const std::uint64_t below = (std::uint64_t(1) << lane) - 1;
+#pragma unroll
+    for (int r = 0; r < kClassifyRows; ++r)
+    {
+        const bool emit = (emitRows >> r) & 1u;
+        const std::uint64_t bal = __ballot(emit);
+        if (emit) cand[blockBase + waveCount[r][wave] + __popcll(bal & below)] = ph[r];
+    }
     if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&leavesPerLevel[threadIdx.x], hist[threadIdx.x]);
 }
 
@@ -567,7 +596,8 @@ void ViewEngine::build(hipStream_t s, Key *const K[2], std::uint32_t *const N[2]
     const unsigned prepBlocks = ceilDiv(icap > 64 ? icap : 64, kT);
     hipLaunchKernelGGL(prepKernel, dim3(prepBlocks > 1024 ? 1024 : prepBlocks), dim3(kT), 0, s, m.leavesPerLevel.p,
                        int(maxLevel_ + 1), m.candCount.p, m.cand.p, icap);
-    hipLaunchKernelGGL(classifyKernel, dim3(leafBlocks), dim3(kT), 0, s, K[0], K[1], st, cap, bits_, m.leafLevel.p,
+    hipLaunchKernelGGL(classifyKernel, dim3(ceilDiv(leafCap_, kT * kClassifyRows)), dim3(kT), 0, s, K[0], K[1], st,
+                       cap, bits_, m.leafLevel.p,
                        m.leavesPerLevel.p, m.cand.p, m.candCount.p);
     hipLaunchKernelGGL(metadataKernel, dim3(1), dim3(64), 0, s, m.leavesPerLevel.p, m.internalPer.p, m.levelOffset.p,
                        m.offInternal.p, depth_, cap, st, bits_);
