@@ -161,22 +161,25 @@ struct OctDecider
     unsigned bucket;
     bool nf;
 
+    // the active half by selects (an index into the pairs would put them in
+    // per-lane scratch memory)
     __device__ int operator()(int ab, int i, int L) const
     {
-        const Key *tree = K[ab];
+        const Key *tree = ab ? K[1] : K[0];
+        const std::uint32_t *n = ab ? N[1] : N[0];
         unsigned level = 0;
         const int s = siblingIndex(tree, L, i, level);
         if (!nf)
         {
             if (s > 0)
             {
-                const std::uint32_t *g = N[ab] + i - s;
+                const std::uint32_t *g = n + i - s;
                 std::uint64_t parent = 0;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) parent += g[k];
                 if (parent <= std::uint64_t(bucket)) return 0;
             }
-            const std::uint32_t c = N[ab][i];
+            const std::uint32_t c = n[i];
             // 32-bit threshold products, as the reference computes them
             if (c > bucket * 512u && level + 3 < kOctLevels) return 4096;
             if (c > bucket * 64u && level + 2 < kOctLevels) return 512;
@@ -187,9 +190,9 @@ struct OctDecider
         if (s >= 0)
         {
             const int first = i - s;
-            if (first + 8 <= L && gcm[ab][first]) return s == 0 ? 1 : 0;
+            if (first + 8 <= L && (ab ? gcm[1] : gcm[0])[first]) return s == 0 ? 1 : 0;
         }
-        const std::uint64_t c = NF[ab][i];
+        const std::uint64_t c = (ab ? NF[1] : NF[0])[i];
         const std::uint64_t b = bucket;
         if (c > b * 262144u && level + 3 < kOctLevels) return 4096;
         if (c > b * 4096u && level + 2 < kOctLevels) return 512;
@@ -216,8 +219,8 @@ struct BinDecider
 
     __device__ int operator()(int ab, int i, int L) const
     {
-        const Key *tree = K[ab];
-        const std::uint32_t *c = N[ab];
+        const Key *tree = ab ? K[1] : K[0];
+        const std::uint32_t *c = ab ? N[1] : N[0];
         const bool merge = i > 0 && binSiblings(tree, L, i - 1) &&
                            std::uint64_t(c[i - 1]) + c[i] <= std::uint64_t(bucket);
         if (merge) return 0;
@@ -252,22 +255,32 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
     if (base > L) return; // tiles past the sentinel index hold no operations
     const bool skip = st->converged || st->needsResize;
     const int ab = st->activeBuffer;
-    const long first = base + long(tid) * kItems;
-    int op[kItems];
-    std::uint64_t sum = 0;
+    // decisions in striped order (leaf base + k * THREADS + tid: neighbouring
+    // lanes read neighbouring leaves), staged in LDS; the scan takes each
+    // thread's run of kItems consecutive leaves from there
+    __shared__ std::uint16_t opStage[THREADS * kItems];
+    __shared__ std::uint32_t prefStage[THREADS * kItems];
     bool chg = false;
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
     {
-        const long i = first + k;
+        const long i = base + long(k) * THREADS + tid;
         int o = 0;
         if (!skip && i < L)
         {
             o = dec(ab, int(i), int(L));
             chg |= o != 1;
         }
-        op[k] = o;
-        sum += std::uint64_t(o);
+        opStage[k * THREADS + tid] = std::uint16_t(o);
+    }
+    __syncthreads();
+    std::uint32_t op[kItems];
+    std::uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+    {
+        op[k] = opStage[tid * kItems + k];
+        sum += op[k];
     }
     std::uint64_t tileSum;
     bool tileChg;
@@ -283,12 +296,22 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
             prefixChgShared = (exChg || tileChg) ? 1 : 0;
         }
     }
+    // tile-local exclusive offsets of the thread's run, read back in striped order
+    {
+        std::uint32_t r = std::uint32_t(local);
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+        {
+            prefStage[tid * kItems + k] = r;
+            r += op[k];
+        }
+    }
     __syncthreads();
-    std::uint64_t run = prefixShared + local;
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
     {
-        const long i = first + k;
+        const long i = base + long(k) * THREADS + tid;
+        const std::uint64_t run = prefixShared + prefStage[k * THREADS + tid];
         if (i <= L) offs[i] = std::int64_t(run);
         if (i == L)
         {
@@ -307,7 +330,6 @@ __device__ void decideScanBody(const Dec &dec, DevState *st, long capacity, std:
             }
             st->changed = changedAll ? 1 : 0;
         }
-        run += std::uint64_t(op[k]);
     }
 }
 
