@@ -469,39 +469,43 @@ __global__ __launch_bounds__(kSortThreads) void scanDigitKernel(const std::uint3
     if (threadIdx.x == 0) digitTotal[d] = carry;
 }
 
-// Lanes of the wave whose digit equals this lane's: for every digit bit, the
-// ballot of lanes with the bit set, complemented where this lane's bit is
-// clear. mask = 0 or ~0 from a sign-extended bit, so each step is one
-// three-input bitwise operation per 32-bit half.
-__device__ inline std::uint64_t laneMatch(unsigned dg)
+// Bit B of v as 0 or ~0 (v_bfe_i32). The extract is kept opaque so that the
+// ballot below is one compare of this value with zero: the compiler would
+// otherwise test the bit a second time with a shift and a sign compare.
+template <int B>
+__device__ inline std::uint32_t bitAsMask(unsigned v)
+{
+    std::uint32_t m;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"(v), "n"(B));
+    return m;
+}
+
+// Lanes of the wave whose BITS-bit digit equals this lane's: for every digit
+// bit, the ballot of lanes with the bit set, complemented where this lane's
+// bit is clear (a bit equal on every lane leaves the mask unchanged). With
+// mask = 0 or ~0, each step is one three-input bitwise operation (v_bitop3)
+// per 32-bit half; the steps carry no branches.
+template <int B, int BITS>
+__device__ inline void laneMatchStep(unsigned dg, std::uint32_t &lo, std::uint32_t &hi)
+{
+    if constexpr (B < BITS)
+    {
+        const std::uint32_t mask = bitAsMask<B>(dg);
+        const std::uint64_t ballot = __ballot(int(mask));
+        // x & ~(ballot ^ mask): truth table 0x90 over (x, ballot, mask)
+        lo = __builtin_amdgcn_bitop3_b32(lo, std::uint32_t(ballot), mask, 0x90);
+        hi = __builtin_amdgcn_bitop3_b32(hi, std::uint32_t(ballot >> 32), mask, 0x90);
+        laneMatchStep<B + 1, BITS>(dg, lo, hi);
+    }
+}
+template <int BITS>
+__device__ inline std::uint64_t laneMatchBits(unsigned dg)
 {
     std::uint32_t lo = ~0u, hi = ~0u;
-#pragma unroll
-    for (int b = 0; b < kRadixBits; ++b)
-    {
-        const std::uint32_t mask = std::uint32_t(__builtin_amdgcn_sbfe(int(dg), b, 1));
-        const std::uint64_t ballot = __ballot(int(mask));
-        lo &= ~(std::uint32_t(ballot) ^ mask);
-        hi &= ~(std::uint32_t(ballot >> 32) ^ mask);
-    }
+    laneMatchStep<0, BITS>(dg, lo, hi);
     return (std::uint64_t(hi) << 32) | lo;
 }
-// The same, over the digit bits set in `vary` only (bits that are equal on
-// every lane need no ballot).
-__device__ inline std::uint64_t laneMatchVarying(unsigned dg, unsigned vary)
-{
-    std::uint32_t lo = ~0u, hi = ~0u;
-#pragma unroll
-    for (int b = 0; b < kRadixBits; ++b)
-    {
-        if (!((vary >> b) & 1u)) continue; // uniform
-        const std::uint32_t mask = std::uint32_t(__builtin_amdgcn_sbfe(int(dg), b, 1));
-        const std::uint64_t ballot = __ballot(int(mask));
-        lo &= ~(std::uint32_t(ballot) ^ mask);
-        hi &= ~(std::uint32_t(ballot >> 32) ^ mask);
-    }
-    return (std::uint64_t(hi) << 32) | lo;
-}
+__device__ inline std::uint64_t laneMatch(unsigned dg) { return laneMatchBits<kRadixBits>(dg); }
 
 // Exclusive scan of the 256 per-digit values held by threads 0..255 of a
 // block of any size (all threads must call it).
@@ -838,21 +842,6 @@ constexpr int kWideTile = kWideThreads * kWideItems;
 constexpr int kWidePasses = (64 + kWideBits - 1) / kWideBits; // even: the result ends where it started
 constexpr int kWideDigitsPerThread = kWideRadix / kWideThreads;
 static_assert(kWidePasses % 2 == 0, "an even pass count leaves the result in the input buffers");
-
-template <int BITS>
-__device__ inline std::uint64_t laneMatchBits(unsigned dg)
-{
-    std::uint32_t lo = ~0u, hi = ~0u;
-#pragma unroll
-    for (int b = 0; b < BITS; ++b)
-    {
-        const std::uint32_t mask = std::uint32_t(__builtin_amdgcn_sbfe(int(dg), b, 1));
-        const std::uint64_t ballot = __ballot(int(mask));
-        lo &= ~(std::uint32_t(ballot) ^ mask);
-        hi &= ~(std::uint32_t(ballot >> 32) ^ mask);
-    }
-    return (std::uint64_t(hi) << 32) | lo;
-}
 
 // Exclusive scan of one value per thread of a kWideThreads block; total =
 // the sum over the block. All threads must call it.
@@ -1607,9 +1596,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 
     // one stable counting pass on the digit of key - base at `shift`; the
     // result is in the registers (striped order) and in stageKeys / stageVals
-    // `vary`: the digit bits that may differ within the chunk (padding keys
-    // agree with the chunk on all other bits)
-    auto pass = [&](int shift, unsigned vary) __attribute__((always_inline)) {
+    auto pass = [&](int shift) __attribute__((always_inline)) {
         for (int i = tid; i < kW * kRadix; i += kLocThreads) (&waveCount[0][0])[i] = 0;
         __syncthreads();
         std::uint32_t rank[kLocItems], old[kLocItems];
@@ -1618,7 +1605,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         for (int j = 0; j < kLocItems; ++j)
         {
             const unsigned dg = unsigned((key[j] - base) >> shift) & (kRadix - 1);
-            const std::uint64_t match = laneMatchVarying(dg, vary);
+            const std::uint64_t match = laneMatch(dg);
             const std::uint32_t lower = __popcll(match & ltMask);
             const bool lead = lower == 0;
             std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
@@ -1862,7 +1849,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         // covers, lowest first
         auto passesFrom = [&](int lo, Key mask) __attribute__((always_inline)) {
             for (int shift = lo; shift < 64; shift += kRadixBits)
-                if ((mask >> shift) & (kRadix - 1)) pass(shift, unsigned(mask >> shift) & (kRadix - 1));
+                if ((mask >> shift) & (kRadix - 1)) pass(shift);
         };
         if (span)
         {
