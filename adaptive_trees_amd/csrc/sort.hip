@@ -1592,7 +1592,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
     const std::uint32_t first = std::uint32_t(wave) * (kLocItems * 64) + lane;
     Key key[kLocItems];
     std::uint32_t val[kLocItems];
-    Key base = 0; // the chunk's smallest key; digits are taken from key - base
+    Key base = 0; // the chunk's smallest key; the registers and the stage hold key - base
 
     // one stable counting pass on the digit of key - base at `shift`; the
     // result is in the registers (striped order) and in stageKeys / stageVals
@@ -1604,7 +1604,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int j = 0; j < kLocItems; ++j)
         {
-            const unsigned dg = unsigned((key[j] - base) >> shift) & (kRadix - 1);
+            const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
             const std::uint64_t match = laneMatch(dg);
             const std::uint32_t lower = __popcll(match & ltMask);
             const bool lead = lower == 0;
@@ -1650,7 +1650,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
 #pragma unroll
         for (int j = 0; j < kLocItems; ++j)
         {
-            const std::uint32_t pos = rank[j] + waveCount[wave][unsigned((key[j] - base) >> shift) & (kRadix - 1)];
+            const std::uint32_t pos = rank[j] + waveCount[wave][unsigned(key[j] >> shift) & (kRadix - 1)];
             stageKeys[pos] = key[j];
             stageVals[pos] = val[j];
         }
@@ -1717,21 +1717,17 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         const Key span = hi - lo; // block-uniform
         const int top = span ? 63 - __builtin_clzll(span) : 0;
         const Key dmask = top == 63 ? ~Key(0) : (Key(2) << top) - 1; // the bits key - base may use
-        {
-            // padding: key - base = dmask, the largest value in every digit
-            // (it still ranks last)
-            const Key pad = base + dmask;
+        // from here on the registers hold key - base; padding: dmask, the
+        // largest value in every digit (it still ranks last)
 #pragma unroll
-            for (int j = 0; j < kLocItems; ++j)
-                if (first + std::uint32_t(j) * 64 >= ch.len) key[j] = pad;
-        }
+        for (int j = 0; j < kLocItems; ++j) key[j] = first + std::uint32_t(j) * 64 < ch.len ? key[j] - base : dmask;
         // Fix-up of the runs of keys whose key - base agree from bit `shift`
         // up, with the chunk sorted on those bits in the stage. Returns false,
         // changing nothing, when a run is longer than kWaveRun.
         auto fixRuns = [&](int shift) __attribute__((always_inline)) -> bool {
             if (tid == 0) longRun = 0, shortRuns = 0, waveRuns = 0;
             __syncthreads();
-            auto high = [&](std::uint32_t q) { return (stageKeys[q] - base) >> shift; };
+            auto high = [&](std::uint32_t q) { return stageKeys[q] >> shift; };
             // position p = t + k*kLocThreads of thread t: a wave's lanes hold
             // 64 consecutive positions, so a key's successor is the next
             // lane's (the row's last lane reads the next row's first key), and
@@ -1749,11 +1745,11 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                 const Key prevRow = uniformKey(row > 0 ? stageKeys[row - 1] : Key(0));
                 Key kn = Key(__shfl_down((unsigned long long)kp, 1, 64));
                 if (lane == 63) kn = nextRow;
-                const Key h = (kp - base) >> shift;
-                const bool eqNext = p + 1 < ch.len && ((kn - base) >> shift) == h;
+                const Key h = kp >> shift;
+                const bool eqNext = p + 1 < ch.len && (kn >> shift) == h;
                 const std::uint64_t nextMask = __ballot(eqNext);
                 const bool eqPrev = lane > 0 ? ((nextMask >> (lane - 1)) & 1u) != 0
-                                             : p > 0 && p < ch.len && ((prevRow - base) >> shift) == h;
+                                             : p > 0 && p < ch.len && (prevRow >> shift) == h;
                 if (eqNext && !eqPrev)
                 {
                     // bits lane.. of nextMask: the run's positions that equal their successor
@@ -1872,7 +1868,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                     Key vary = 0;
 #pragma unroll
                     for (int j = 0; j < kLocItems; ++j)
-                        if (first + std::uint32_t(j) * 64 < ch.len) vary |= key[j] - base;
+                        if (first + std::uint32_t(j) * 64 < ch.len) vary |= key[j];
 #pragma unroll
                     for (int o = 32; o > 0; o >>= 1) vary |= Key(__shfl_xor((unsigned long long)vary, o, 64));
                     if (lane == 0) waveLo[wave] = vary;
@@ -1882,7 +1878,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
                     for (int w = 0; w < kW; ++w) vary |= waveLo[w];
 #pragma unroll
                     for (int j = 0; j < kLocItems; ++j)
-                        if (first + std::uint32_t(j) * 64 >= ch.len) key[j] = base + vary;
+                        if (first + std::uint32_t(j) * 64 >= ch.len) key[j] = vary;
                     // the 8 bits below fix, then the high windows again
                     const int fix2 = fix > kRadixBits ? fix - kRadixBits : 0;
                     const Key below2 = (Key(1) << fix2) - 1;
@@ -1903,7 +1899,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
             const std::uint32_t idx = first + std::uint32_t(j) * 64;
             if (idx < ch.len)
             {
-                kd[ch.begin + idx] = key[j];
+                kd[ch.begin + idx] = key[j] + base;
                 vd[ch.begin + idx] = val[j];
             }
         }
