@@ -507,6 +507,12 @@ __device__ inline std::uint64_t laneMatchBits(unsigned dg)
 }
 __device__ inline std::uint64_t laneMatch(unsigned dg) { return laneMatchBits<kRadixBits>(dg); }
 
+// Number of lanes below this one in `mask` (v_mbcnt_lo / v_mbcnt_hi).
+__device__ inline std::uint32_t lanesBelow(std::uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(std::uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(std::uint32_t(mask), 0u));
+}
+
 // Exclusive scan of the 256 per-digit values held by threads 0..255 of a
 // block of any size (all threads must call it).
 __device__ inline std::uint32_t digitExclusiveScan(std::uint32_t v, std::uint32_t *waveTotals)
@@ -602,7 +608,6 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
     const Key *kr = kin + rBegin;
     const std::uint32_t *vr = vin + rBegin;
     const std::uint32_t last = rLen - 1;
-    const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
     constexpr bool kPreVals = kVals && !kIota;
     Key preKey[ITEMS];
     std::uint32_t preVal[kPreVals ? ITEMS : 1];
@@ -705,7 +710,7 @@ __global__ __launch_bounds__(THREADS, (THREADS == 512 && ITEMS > 8) || THREADS <
         {
             const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
             const std::uint64_t match = laneMatch(dg);
-            const std::uint32_t lower = __popcll(match & ltMask);
+            const std::uint32_t lower = lanesBelow(match);
             const bool lead = lower == 0;
             std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
             old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
@@ -975,7 +980,6 @@ __global__ __launch_bounds__(kWideThreads) void wideScatterKernel(const Key *__r
     // rank within the wave: the lowest lane of each digit group adds the
     // group's size to the wave's counter (the others add 0 to a slot of
     // their own; one wave's LDS operations execute in order)
-    const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
     std::uint32_t rank[kWideItems], old[kWideItems];
     int leader[kWideItems];
 #pragma unroll
@@ -983,7 +987,7 @@ __global__ __launch_bounds__(kWideThreads) void wideScatterKernel(const Key *__r
     {
         const unsigned dg = unsigned(key[j] >> shift) & (kWideRadix - 1);
         const std::uint64_t match = laneMatchBits<kWideBits>(dg);
-        const std::uint32_t lower = __popcll(match & ltMask);
+        const std::uint32_t lower = lanesBelow(match);
         const bool lead = lower == 0;
         std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
         old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
@@ -1584,7 +1588,6 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const unsigned lane = laneId();
-    const std::uint64_t ltMask = (std::uint64_t(1) << lane) - 1;
     Key *kd = finalIsY ? Y : X;
     std::uint32_t *vd = finalIsY ? vY : vX;
     const std::uint32_t count = cnt->chunks < maxChunks ? cnt->chunks : maxChunks;
@@ -1606,7 +1609,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
         {
             const unsigned dg = unsigned(key[j] >> shift) & (kRadix - 1);
             const std::uint64_t match = laneMatch(dg);
-            const std::uint32_t lower = __popcll(match & ltMask);
+            const std::uint32_t lower = lanesBelow(match);
             const bool lead = lower == 0;
             std::uint32_t *slot = lead ? &waveCount[wave][dg] : &sink[wave][lane];
             old[j] = atomicAdd(slot, lead ? std::uint32_t(__popcll(match)) : 0u);
