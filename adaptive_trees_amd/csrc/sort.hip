@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(kWideThreads) void wideScatterKernel(const Key *__r
 constexpr int kSegThreads = 512, kSegItems = 8; // tile shape of the segmented scatter
 
 template <class T, KeyKind K>
-__global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> p, const SegCoords<T> *coords,
+__global__ __launch_bounds__(kSortThreads) void segKeygenKernel(const KeygenParams<T> p, const SegCoords<T> *coords,
                                                                 const SegRange *ranges, Key *keys,
                                                                 std::uint32_t *hist, int histShift)
 {
@@ -1071,13 +1071,12 @@ __global__ __launch_bounds__(kSortThreads) void segKeygenKernel(KeygenParams<T> 
     __syncthreads();
     const SegRange d = ranges[blockIdx.x];
     const SegCoords<T> c = coords[d.seg];
-    p.x = c.x;
-    p.y = c.y;
-    p.z = c.z;
+    // p stays unmodified (in the kernel arguments): the binary key's
+    // per-depth axis table is then indexed there, not in a private copy
     for (std::uint32_t j = threadIdx.x; j < d.len; j += kSortThreads)
     {
         const std::size_t i = std::size_t(d.localBase) + j;
-        const Key k = keyOf<T, K, false>(p, nullptr, henc, p.x[i], p.y[i], p.z[i]);
+        const Key k = keyOf<T, K, false>(p, nullptr, henc, c.x[i], c.y[i], c.z[i]);
         keys[d.begin + j] = k;
         countDigit(h, unsigned(k >> histShift) & (kRadix - 1));
     }
