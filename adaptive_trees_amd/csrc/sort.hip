@@ -1737,7 +1737,7 @@ __global__ __launch_bounds__(kLocThreads, 4) void msdLocalSortKernel(const MsdCo
             // run that leaves the row walks on in LDS. Nothing is written
             // before the barrier: every key's high bits stay as read.
             std::uint32_t pairs = 0; // bit k: position t + k*kLocThreads starts a run of 2
-#pragma unroll 2
+#pragma unroll 4
             for (int k = 0; k < kLocItems; ++k)
             {
                 const std::uint32_t p = std::uint32_t(tid) + std::uint32_t(k) * kLocThreads;
